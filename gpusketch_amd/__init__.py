@@ -1,0 +1,277 @@
+"""Python view of libgpusketch's C ABI (ctypes), used by the tests and bench.py.
+
+The library is the product: every call goes through ``libgpusketch.so``'s ``gs_*`` entry points
+(declared in include/gpusketch.h). There is no Python or CPU fallback -- importing this module
+raises if the shared library is missing, and every failing call raises ``GsError`` with the
+library's message.
+
+Bucket arrays returned by ``export_buckets`` use the reference layout (src/sketch.cpp:8-37):
+shape (vertices, num_buckets, 2) of uint64 {alpha, gamma}, bucket index
+``column * bkt_per_col + depth``, deterministic bucket last.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgpusketch.so")
+
+GS_OK, GS_ERR_INVALID, GS_ERR_OOM, GS_ERR_HIP, GS_ERR_OUT_OF_SAMPLES, GS_ERR_ADMISSION, \
+    GS_ERR_UNSUPPORTED = range(7)
+
+
+class GsError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"gs error {code}: {msg}")
+        self.code = code
+
+
+class OutOfSamplesError(GsError):
+    pass
+
+
+class Geometry(ctypes.Structure):
+    _fields_ = [("num_vertices", ctypes.c_uint32), ("bkt_per_col", ctypes.c_uint32),
+                ("num_samples", ctypes.c_uint32), ("cols_per_sample", ctypes.c_uint32),
+                ("num_columns", ctypes.c_uint32), ("num_buckets", ctypes.c_uint32),
+                ("seed", ctypes.c_uint64), ("vector_len", ctypes.c_uint64),
+                ("device_bytes", ctypes.c_uint64)]
+
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built; run `python -m gpusketch_amd.build` (hipcc, gfx950)")
+
+lib = ctypes.CDLL(LIB_PATH)
+_u32, _u64, _i32, _vp, _dbl, _sz = (ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int,
+                                    ctypes.c_void_p, ctypes.c_double, ctypes.c_size_t)
+_sigs = {
+    "gs_last_error": ([], ctypes.c_char_p),
+    "gs_geometry_for": ([_u32, _dbl, _u32, _vp], _i32),
+    "gs_hash_known_answer": ([_vp, _vp, _u64, _vp], _i32),
+    "gs_store_create": ([_u32, _u64, _u32, _u32, _u32, _vp], _i32),
+    "gs_store_destroy": ([_vp], _i32),
+    "gs_store_geometry": ([_vp, _vp], _i32),
+    "gs_store_zero": ([_vp], _i32),
+    "gs_stream_create": ([_vp, _vp], _i32),
+    "gs_stream_destroy": ([_vp], _i32),
+    "gs_submit_vertex_batches": ([_vp, _vp, _vp, _vp, _u32, _vp], _i32),
+    "gs_submit_edge_pairs": ([_vp, _vp, _u64], _i32),
+    "gs_submit_edge_pairs_device": ([_vp, _vp, _u64], _i32),
+    "gs_submit_edge_pairs_filtered": ([_vp, _vp, _u64, _u32, _u32], _i32),
+    "gs_flush": ([_vp], _i32),
+    "gs_sync": ([_vp], _i32),
+    "gs_export_buckets": ([_vp, _u32, _u32, _vp], _i32),
+    "gs_import_buckets": ([_vp, _u32, _u32, _vp], _i32),
+    "gs_query_cc": ([_vp, _vp, _vp, _vp, _vp, _u64, _vp], _i32),
+    "gs_k_spanning_forests": ([_vp, _u32, _vp, _u64, _vp], _i32),
+    "gs_min_cut": ([_u32, _vp, _u64, _vp, _vp], _i32),
+    "gs_stream_set_profiling": ([_vp, _i32], _i32),
+    "gs_stream_timing": ([_vp, _vp, _vp, _vp], _i32),
+    "gs_device_alloc": ([_sz, _vp], _i32),
+    "gs_device_free": ([_vp], _i32),
+    "gs_memcpy_h2d": ([_vp, _vp, _sz], _i32),
+    "gs_memcpy_d2h": ([_vp, _vp, _sz], _i32),
+    "gs_device_synchronize": ([], _i32),
+    "gs_gen_erdos_renyi": ([_u32, _dbl, _dbl, _u64, _vp, _vp], _i32),
+    "gs_gen_rmat": ([_u32, _dbl, _dbl, _dbl, _u64, _dbl, _u64, _vp], _i32),
+    "gs_gen_planted_cut": ([_u32, _dbl, _u32, _dbl, _u64, _vp, _vp], _i32),
+}
+EXPORTED = sorted(_sigs)
+for _name, (_args, _res) in _sigs.items():
+    _f = getattr(lib, _name)
+    _f.argtypes = _args
+    _f.restype = _res
+
+
+def _check(rc: int) -> None:
+    if rc != GS_OK:
+        msg = lib.gs_last_error().decode(errors="replace")
+        if rc == GS_ERR_OUT_OF_SAMPLES:
+            raise OutOfSamplesError(rc, msg)
+        raise GsError(rc, msg)
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _pairs(pairs) -> np.ndarray:
+    return np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+
+
+def geometry_for(n: int, sketches_factor: float = 1.0, cols_per_sample: int = 1) -> Geometry:
+    g = Geometry()
+    _check(lib.gs_geometry_for(n, sketches_factor, cols_per_sample, ctypes.byref(g)))
+    return g
+
+
+def hash_known_answer(keys, seeds) -> np.ndarray:
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+    out = np.zeros_like(keys)
+    _check(lib.gs_hash_known_answer(_p(keys), _p(seeds), keys.size, _p(out)))
+    return out
+
+
+def min_cut(n: int, pairs):
+    pairs = _pairs(pairs)
+    val = ctypes.c_uint64(0)
+    side = np.zeros(n, dtype=np.uint8)
+    _check(lib.gs_min_cut(n, _p(pairs), pairs.shape[0], ctypes.byref(val), _p(side)))
+    return int(val.value), side
+
+
+def gen_erdos_renyi(n: int, p: float, del_frac: float, seed: int) -> np.ndarray:
+    cnt = ctypes.c_uint64(0)
+    _check(lib.gs_gen_erdos_renyi(n, p, del_frac, seed, None, ctypes.byref(cnt)))
+    out = np.zeros((cnt.value, 2), dtype=np.uint32)
+    _check(lib.gs_gen_erdos_renyi(n, p, del_frac, seed, _p(out), ctypes.byref(cnt)))
+    return out[: cnt.value]
+
+
+def gen_rmat(scale: int, count: int, seed: int, a=0.57, b=0.19, c=0.19,
+             del_frac: float = 0.05) -> np.ndarray:
+    out = np.zeros((count, 2), dtype=np.uint32)
+    _check(lib.gs_gen_rmat(scale, a, b, c, count, del_frac, seed, _p(out)))
+    return out
+
+
+def gen_planted_cut(n: int, p: float, cut: int, churn_frac: float, seed: int) -> np.ndarray:
+    cnt = ctypes.c_uint64(0)
+    _check(lib.gs_gen_planted_cut(n, p, cut, churn_frac, seed, None, ctypes.byref(cnt)))
+    out = np.zeros((cnt.value, 2), dtype=np.uint32)
+    _check(lib.gs_gen_planted_cut(n, p, cut, churn_frac, seed, _p(out), ctypes.byref(cnt)))
+    return out[: cnt.value]
+
+
+class DeviceBuffer:
+    """Device memory owned by the caller (inputs kept resident in HBM)."""
+
+    def __init__(self, host: np.ndarray):
+        host = np.ascontiguousarray(host)
+        self.nbytes = host.nbytes
+        self.ptr = ctypes.c_void_p()
+        _check(lib.gs_device_alloc(max(self.nbytes, 1), ctypes.byref(self.ptr)))
+        if self.nbytes:
+            _check(lib.gs_memcpy_h2d(self.ptr, _p(host), self.nbytes))
+
+    def offset(self, nbytes: int) -> ctypes.c_void_p:
+        return ctypes.c_void_p(self.ptr.value + nbytes)
+
+    def free(self):
+        if self.ptr:
+            _check(lib.gs_device_free(self.ptr))
+            self.ptr = ctypes.c_void_p()
+
+
+class Stream:
+    def __init__(self, store: "SketchStore"):
+        self.store = store
+        self.h = ctypes.c_void_p()
+        _check(lib.gs_stream_create(store.h, ctypes.byref(self.h)))
+
+    def submit_pairs(self, pairs) -> None:
+        pairs = _pairs(pairs)
+        _check(lib.gs_submit_edge_pairs(self.h, _p(pairs), pairs.shape[0]))
+
+    def submit_pairs_filtered(self, pairs, min_depth: int, depth_cap: int) -> None:
+        pairs = _pairs(pairs)
+        _check(lib.gs_submit_edge_pairs_filtered(self.h, _p(pairs), pairs.shape[0], min_depth,
+                                                 depth_cap))
+
+    def submit_pairs_device(self, dptr, nedges: int) -> None:
+        _check(lib.gs_submit_edge_pairs_device(self.h, dptr, nedges))
+
+    def submit_vertex_batches(self, src, sizes, offsets, dsts) -> None:
+        src = np.ascontiguousarray(src, dtype=np.uint32)
+        sizes = np.ascontiguousarray(sizes, dtype=np.uint64)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        dsts = np.ascontiguousarray(dsts, dtype=np.uint32)
+        _check(lib.gs_submit_vertex_batches(self.h, _p(src), _p(sizes), _p(offsets), len(src),
+                                            _p(dsts)))
+
+    def flush(self) -> None:
+        _check(lib.gs_flush(self.h))
+
+    def set_profiling(self, on: bool) -> None:
+        _check(lib.gs_stream_set_profiling(self.h, int(on)))
+
+    def timing(self):
+        ms, launches, upd = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib.gs_stream_timing(self.h, ctypes.byref(ms), ctypes.byref(launches),
+                                    ctypes.byref(upd)))
+        return ms.value, launches.value, upd.value
+
+    def close(self) -> None:
+        if self.h:
+            _check(lib.gs_stream_destroy(self.h))
+            self.h = ctypes.c_void_p()
+
+
+class SketchStore:
+    """n vertex sketches resident in HBM (one seed, reference geometry)."""
+
+    def __init__(self, n: int, seed: int, num_samples: int | None = None,
+                 sketches_factor: float = 1.0):
+        if num_samples is None:
+            num_samples = geometry_for(n, sketches_factor).num_samples
+        self.h = ctypes.c_void_p()
+        _check(lib.gs_store_create(n, seed & (2**64 - 1), num_samples, 1, 0, ctypes.byref(self.h)))
+        self.geom = Geometry()
+        _check(lib.gs_store_geometry(self.h, ctypes.byref(self.geom)))
+        self.n = n
+
+    def stream(self) -> Stream:
+        return Stream(self)
+
+    def sync(self) -> None:
+        _check(lib.gs_sync(self.h))
+
+    def zero(self) -> None:
+        _check(lib.gs_store_zero(self.h))
+
+    def export_buckets(self, v0: int = 0, count: int | None = None) -> np.ndarray:
+        count = self.n - v0 if count is None else count
+        out = np.zeros((count, self.geom.num_buckets, 2), dtype=np.uint64)
+        _check(lib.gs_export_buckets(self.h, v0, count, _p(out)))
+        return out
+
+    def import_buckets(self, data: np.ndarray, v0: int = 0) -> None:
+        data = np.ascontiguousarray(data, dtype=np.uint64)
+        _check(lib.gs_import_buckets(self.h, v0, data.shape[0], _p(data)))
+
+    def query_cc(self, want_forest: bool = False):
+        labels = np.zeros(self.n, dtype=np.uint32)
+        ncc, rounds, flen = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
+        forest = np.zeros((self.n, 2), dtype=np.uint32) if want_forest else None
+        _check(lib.gs_query_cc(self.h, _p(labels), ctypes.byref(ncc), ctypes.byref(rounds),
+                               _p(forest) if want_forest else None,
+                               self.n if want_forest else 0, ctypes.byref(flen)))
+        if want_forest:
+            return labels, int(ncc.value), int(rounds.value), forest[: flen.value].copy()
+        return labels, int(ncc.value), int(rounds.value)
+
+    def k_spanning_forests(self, k: int):
+        cap = k * self.n
+        edges = np.zeros((cap, 2), dtype=np.uint32)
+        counts = np.zeros(k, dtype=np.uint64)
+        _check(lib.gs_k_spanning_forests(self.h, k, _p(edges), cap, _p(counts)))
+        out, off = [], 0
+        for c in counts:
+            out.append(edges[off:off + int(c)].copy())
+            off += int(c)
+        return out
+
+    def close(self) -> None:
+        if self.h:
+            _check(lib.gs_store_destroy(self.h))
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -1,0 +1,87 @@
+"""Build recipe for libgpusketch.so (gfx950) and the C++ host programs.
+
+Compiles every HIP/C++ source under gpusketch_amd/csrc with hipcc into one shared library next
+to this file, and the C++ facade programs under gpusketch_amd/host into gpusketch_amd/bin.
+Run ``python -m gpusketch_amd.build`` from the repository root (or call ``build()``).
+"""
+from __future__ import annotations
+
+import glob
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+HOST = os.path.join(HERE, "host")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libgpusketch.so")
+BIN = os.path.join(HERE, "bin")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+# -Xclang -target-feature -Xclang -packed-fp32-ops keeps the compiler from forming packed FP32
+# instructions (none of this code is floating point, the flag is kept uniform for all HIP code).
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-result",
+    "-I", os.path.join(ROOT, "include"),
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError(f"compile failed: {cmd[-1] if cmd else ''}")
+    return r
+
+
+def build_library(jobs: int = 8) -> str:
+    os.makedirs(OBJ, exist_ok=True)
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
+    objs = []
+    cmds = []
+    for s in srcs:
+        o = os.path.join(OBJ, os.path.basename(s) + ".o")
+        objs.append(o)
+        lang = ["-x", "hip"] if s.endswith(".hip") else []
+        cmds.append([HIPCC, *HIP_FLAGS, *lang, "-c", s, "-o", o])
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(_run, cmds))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lpthread"])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_host_programs() -> list[str]:
+    """C++ facade test/driver programs (host code; link against libgpusketch.so)."""
+    os.makedirs(BIN, exist_ok=True)
+    outs = []
+    progs = sorted(glob.glob(os.path.join(HOST, "prog_*.cpp")))
+    lib_srcs = sorted(p for p in glob.glob(os.path.join(HOST, "*.cpp"))
+                      if not os.path.basename(p).startswith("prog_"))
+    cmds = []
+    for p in progs:
+        out = os.path.join(BIN, os.path.basename(p)[len("prog_"):-4])
+        outs.append(out)
+        cmds.append(["g++", "-O2", "-std=c++17", "-pthread", "-I", os.path.join(ROOT, "include"),
+                     p, *lib_srcs, "-o", out, f"-L{HERE}", "-lgpusketch",
+                     f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/.."])
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        list(ex.map(_run, cmds))
+    return outs
+
+
+def build() -> None:
+    build_library()
+    build_host_programs()
+
+
+if __name__ == "__main__":
+    build()
+    print(LIB)

@@ -1,0 +1,474 @@
+// C-ABI entry points of libgpusketch: store and stream lifecycle, submissions, export/import,
+// timing and device-memory helpers. Kernels live in ingest.hip and boruvka.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+
+#include "gs_internal.h"
+
+namespace gs {
+
+static thread_local std::string g_last_error;
+
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+}
+
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+void geometry(uint32_t n, double factor, uint32_t cps, int l0, uint64_t out[5]) {
+  // include/sketch.h:63-77 (vector length, samples) and :198-206 (buckets per column)
+  uint64_t len = (uint64_t)std::ceil((double)n * (uint32_t)(n - 1u) / 2);
+  uint64_t bpc = (uint64_t)(std::ceil(std::log2((double)len)) + 1);
+  if (bpc % 16 == 0) bpc += 1;
+  double div = l0 ? (std::log2(3.0) - 1) : (1 - 0.263034406);
+  uint64_t s = (uint64_t)std::ceil(factor * std::log2((double)n) / div);
+  if (s < 18) s = 18;
+  out[0] = len;
+  out[1] = bpc;
+  out[2] = s;
+  out[3] = s * cps;
+  out[4] = s * cps * bpc + 1;
+}
+
+// ---- staging helpers -------------------------------------------------------------------------
+static int ensure_stage(Stream* s, size_t bytes) {
+  if (s->stage_pending) {
+    GS_HIP(hipEventSynchronize(s->stage_free));
+    s->stage_pending = false;
+  }
+  if (s->h_stage_bytes < bytes) {
+    if (s->h_stage) hipHostFree(s->h_stage);
+    s->h_stage = nullptr;
+    size_t cap = std::max(bytes, (size_t)1 << 20);
+    if (hipHostMalloc((void**)&s->h_stage, cap, hipHostMallocDefault) != hipSuccess)
+      return fail(GS_ERR_OOM, "pinned staging of %zu bytes failed", cap);
+    s->h_stage_bytes = cap;
+  }
+  if (s->d_stage_bytes < bytes) {
+    GS_HIP(hipStreamSynchronize(s->stream));
+    if (s->d_stage) hipFree(s->d_stage);
+    s->d_stage = nullptr;
+    size_t cap = std::max(bytes, (size_t)1 << 20);
+    if (hipMalloc((void**)&s->d_stage, cap) != hipSuccess)
+      return fail(GS_ERR_OOM, "device staging of %zu bytes failed", cap);
+    s->d_stage_bytes = cap;
+  }
+  return GS_OK;
+}
+
+static int release_stage(Stream* s) {
+  GS_HIP(hipEventRecord(s->stage_free, s->stream));
+  s->stage_pending = true;
+  return GS_OK;
+}
+
+}  // namespace gs
+
+using namespace gs;
+
+extern "C" {
+
+const char* gs_last_error(void) { return g_last_error.c_str(); }
+
+int gs_geometry_for(uint32_t n, double sketches_factor, uint32_t cols_per_sample,
+                    gs_geometry* out) {
+  GS_CHECK(out && n >= 2 && cols_per_sample >= 1 && sketches_factor > 0, GS_ERR_INVALID,
+           "gs_geometry_for: bad argument");
+  uint64_t g[5];
+  geometry(n, sketches_factor, cols_per_sample, 0, g);
+  std::memset(out, 0, sizeof(*out));
+  out->num_vertices = n;
+  out->vector_len = g[0];
+  out->bkt_per_col = (uint32_t)g[1];
+  out->num_samples = (uint32_t)g[2];
+  out->cols_per_sample = cols_per_sample;
+  out->num_columns = (uint32_t)g[3];
+  out->num_buckets = (uint32_t)g[4];
+  uint64_t vstride = ((uint64_t)g[1] * g[3] + 7) & ~7ULL;
+  out->device_bytes = (uint64_t)n * (vstride + 1) * 16;
+  return GS_OK;
+}
+
+int gs_hash_known_answer(const uint64_t* keys, const uint64_t* seeds, uint64_t count,
+                         uint64_t* out) {
+  GS_CHECK((keys && seeds && out) || count == 0, GS_ERR_INVALID, "null argument");
+  return hash_known_answer(keys, seeds, count, out);
+}
+
+int gs_store_create(uint32_t n, uint64_t seed, uint32_t num_samples, uint32_t cols_per_sample,
+                    uint32_t flags, gs_store** out) {
+  GS_CHECK(out, GS_ERR_INVALID, "gs_store_create: null out");
+  *out = nullptr;
+  GS_CHECK(n >= 2, GS_ERR_INVALID, "gs_store_create: need at least 2 vertices");
+  GS_CHECK(num_samples >= 1 && cols_per_sample >= 1, GS_ERR_INVALID, "gs_store_create: bad samples");
+  GS_CHECK(cols_per_sample == 1, GS_ERR_UNSUPPORTED,
+           "gs_store_create: only the support-finding update (1 column per sample) is built");
+  uint64_t g[5];
+  geometry(n, 1.0, 1, 0, g);
+  const uint32_t bpc = (uint32_t)g[1];
+  const uint64_t cols = (uint64_t)num_samples * cols_per_sample;
+  GS_CHECK(bpc < 64 && cols <= 4096, GS_ERR_INVALID, "gs_store_create: geometry out of range");
+  gs_store* h = new gs_store();
+  Store& st = h->impl;
+  st.num_samples = num_samples;
+  st.cols_per_sample = cols_per_sample;
+  st.flags = flags;
+  hipGetDevice(&st.device);
+  st.view.n = n;
+  st.view.bpc = bpc;
+  st.view.cols = (uint32_t)cols;
+  st.view.vstride = (uint32_t)(((uint64_t)bpc * cols + 7) & ~7ULL);
+  st.view.seed = seed;
+  size_t rows_bytes = (size_t)n * st.view.vstride * sizeof(Bucket16);
+  if (hipMalloc((void**)&st.view.rows, rows_bytes) != hipSuccess ||
+      hipMalloc((void**)&st.view.det, (size_t)n * sizeof(Bucket16)) != hipSuccess) {
+    hipFree(st.view.rows);
+    hipFree(st.view.det);
+    delete h;
+    return fail(GS_ERR_OOM, "gs_store_create: %zu bytes of HBM unavailable", rows_bytes);
+  }
+  if (hipStreamCreateWithFlags(&st.qstream, hipStreamNonBlocking) != hipSuccess) {
+    hipFree(st.view.rows);
+    hipFree(st.view.det);
+    delete h;
+    return fail(GS_ERR_HIP, "gs_store_create: stream creation failed");
+  }
+  int rc = launch_zero_store(&st, st.qstream);
+  if (rc == GS_OK && hipStreamSynchronize(st.qstream) != hipSuccess)
+    rc = fail(GS_ERR_HIP, "gs_store_create: zeroing failed");
+  if (rc) {
+    hipStreamDestroy(st.qstream);
+    hipFree(st.view.rows);
+    hipFree(st.view.det);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return GS_OK;
+}
+
+int gs_store_destroy(gs_store* h) {
+  if (!h) return GS_OK;
+  Store& st = h->impl;
+  hipDeviceSynchronize();
+  std::vector<Stream*> ss;
+  {
+    std::lock_guard<std::mutex> lk(st.mu);
+    ss = st.streams;
+  }
+  for (Stream* s : ss) gs_stream_destroy(reinterpret_cast<gs_stream*>(s));
+  free_query_scratch(&st);
+  if (st.qstream) hipStreamDestroy(st.qstream);
+  hipFree(st.view.rows);
+  hipFree(st.view.det);
+  delete h;
+  return GS_OK;
+}
+
+int gs_store_geometry(const gs_store* h, gs_geometry* out) {
+  GS_CHECK(h && out, GS_ERR_INVALID, "null argument");
+  const Store& st = h->impl;
+  uint64_t g[5];
+  geometry(st.view.n, 1.0, 1, 0, g);
+  std::memset(out, 0, sizeof(*out));
+  out->num_vertices = st.view.n;
+  out->bkt_per_col = st.view.bpc;
+  out->num_samples = st.num_samples;
+  out->cols_per_sample = st.cols_per_sample;
+  out->num_columns = st.view.cols;
+  out->num_buckets = st.view.cols * st.view.bpc + 1;
+  out->seed = st.view.seed;
+  out->vector_len = g[0];
+  out->device_bytes = st.bytes();
+  return GS_OK;
+}
+
+int gs_store_zero(gs_store* h) {
+  GS_CHECK(h, GS_ERR_INVALID, "null store");
+  int rc = gs_sync(h);
+  if (rc) return rc;
+  rc = launch_zero_store(&h->impl, h->impl.qstream);
+  if (rc) return rc;
+  GS_HIP(hipStreamSynchronize(h->impl.qstream));
+  return GS_OK;
+}
+
+int gs_stream_create(gs_store* h, gs_stream** out) {
+  GS_CHECK(h && out, GS_ERR_INVALID, "null argument");
+  gs_stream* s = new gs_stream();
+  Stream& st = s->impl;
+  st.store = &h->impl;
+  if (hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&st.stage_free, hipEventDisableTiming) != hipSuccess) {
+    delete s;
+    return fail(GS_ERR_HIP, "gs_stream_create: HIP stream/event creation failed");
+  }
+  {
+    std::lock_guard<std::mutex> lk(h->impl.mu);
+    h->impl.streams.push_back(&st);
+  }
+  *out = s;
+  return GS_OK;
+}
+
+int gs_stream_destroy(gs_stream* sh) {
+  if (!sh) return GS_OK;
+  Stream& s = sh->impl;
+  hipStreamSynchronize(s.stream);
+  {
+    std::lock_guard<std::mutex> lk(s.store->mu);
+    auto& v = s.store->streams;
+    v.erase(std::remove(v.begin(), v.end(), &s), v.end());
+  }
+  free_pair_scratch(s.ws);
+  if (s.h_stage) hipHostFree(s.h_stage);
+  if (s.d_stage) hipFree(s.d_stage);
+  for (auto& p : s.ev_pairs) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
+  if (s.stage_free) hipEventDestroy(s.stage_free);
+  if (s.stream) hipStreamDestroy(s.stream);
+  delete sh;
+  return GS_OK;
+}
+
+int gs_submit_vertex_batches(gs_stream* sh, const uint32_t* src, const uint64_t* sizes,
+                             const uint64_t* offsets, uint32_t nbatches, const uint32_t* dsts) {
+  GS_CHECK(sh, GS_ERR_INVALID, "null stream");
+  if (nbatches == 0) return GS_OK;
+  GS_CHECK(src && sizes && offsets && dsts, GS_ERR_INVALID, "null batch array");
+  Stream* s = &sh->impl;
+  const uint32_t n = s->store->view.n;
+  const uint32_t chunk = 256;
+  // work items (vertex, len, start) over a compacted destination array
+  uint64_t total = 0, nitems = 0;
+  for (uint32_t b = 0; b < nbatches; b++) {
+    GS_CHECK(src[b] < n, GS_ERR_INVALID, "batch %u: source vertex %u out of range", b, src[b]);
+    total += sizes[b];
+    nitems += (sizes[b] + chunk - 1) / chunk;
+  }
+  if (total == 0) return GS_OK;
+  size_t item_bytes = nitems * sizeof(WorkItem);
+  size_t bytes = item_bytes + total * sizeof(uint32_t);
+  int rc = ensure_stage(s, bytes);
+  if (rc) return rc;
+  WorkItem* hi = reinterpret_cast<WorkItem*>(s->h_stage);
+  uint32_t* hd = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(s->h_stage) + item_bytes);
+  uint64_t pos = 0, k = 0;
+  for (uint32_t b = 0; b < nbatches; b++) {
+    for (uint64_t i = 0; i < sizes[b]; i++) {
+      uint32_t d = dsts[offsets[b] + i];
+      GS_CHECK(d < n, GS_ERR_INVALID, "batch %u: destination %u out of range", b, d);
+      hd[pos + i] = d;
+    }
+    for (uint64_t q = 0; q < sizes[b]; q += chunk) {
+      hi[k].vertex = src[b];
+      hi[k].len = (uint32_t)std::min<uint64_t>(chunk, sizes[b] - q);  // shared: atomic flush
+      hi[k].start = pos + q;
+      k++;
+    }
+    pos += sizes[b];
+  }
+  GS_HIP(hipMemcpyAsync(s->d_stage, s->h_stage, bytes, hipMemcpyHostToDevice, s->stream));
+  const WorkItem* di = reinterpret_cast<const WorkItem*>(s->d_stage);
+  const uint32_t* dd =
+      reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(s->d_stage) + item_bytes);
+  rc = launch_vertex_batches(s, nullptr, di, (uint32_t)nitems, dd);
+  if (rc) return rc;
+  return release_stage(s);
+}
+
+static int submit_pairs_host(Stream* s, const uint32_t* pairs, uint64_t nedges, uint32_t min_depth,
+                             uint32_t depth_cap) {
+  const uint32_t n = s->store->view.n;
+  const uint64_t chunk = 1ULL << 22;  // edges per staged copy
+  for (uint64_t e0 = 0; e0 < nedges; e0 += chunk) {
+    uint64_t m = std::min(chunk, nedges - e0);
+    size_t bytes = m * 2 * sizeof(uint32_t);
+    int rc = ensure_stage(s, bytes);
+    if (rc) return rc;
+    const uint32_t* p = pairs + 2 * e0;
+    for (uint64_t i = 0; i < 2 * m; i++) {
+      GS_CHECK(p[i] < n, GS_ERR_INVALID, "edge %llu: vertex %u out of range",
+               (unsigned long long)(e0 + i / 2), p[i]);
+    }
+    std::memcpy(s->h_stage, p, bytes);
+    GS_HIP(hipMemcpyAsync(s->d_stage, s->h_stage, bytes, hipMemcpyHostToDevice, s->stream));
+    rc = ingest_pairs_device(s, s->d_stage, m, min_depth, depth_cap);
+    if (rc) return rc;
+    rc = release_stage(s);
+    if (rc) return rc;
+  }
+  return GS_OK;
+}
+
+int gs_submit_edge_pairs(gs_stream* sh, const uint32_t* pairs, uint64_t nedges) {
+  GS_CHECK(sh && (pairs || nedges == 0), GS_ERR_INVALID, "null argument");
+  return submit_pairs_host(&sh->impl, pairs, nedges, 0, 0);
+}
+
+int gs_submit_edge_pairs_filtered(gs_stream* sh, const uint32_t* pairs, uint64_t nedges,
+                                  uint32_t min_depth, uint32_t depth_cap) {
+  GS_CHECK(sh && (pairs || nedges == 0), GS_ERR_INVALID, "null argument");
+  GS_CHECK(depth_cap < 64 && min_depth <= depth_cap, GS_ERR_INVALID, "bad subgraph filter");
+  return submit_pairs_host(&sh->impl, pairs, nedges, min_depth, depth_cap);
+}
+
+int gs_submit_edge_pairs_device(gs_stream* sh, const uint32_t* d_pairs, uint64_t nedges) {
+  GS_CHECK(sh && (d_pairs || nedges == 0), GS_ERR_INVALID, "null argument");
+  return ingest_pairs_device(&sh->impl, d_pairs, nedges, 0, 0);
+}
+
+int gs_flush(gs_stream* sh) {
+  GS_CHECK(sh, GS_ERR_INVALID, "null stream");
+  Stream& s = sh->impl;
+  GS_HIP(hipStreamSynchronize(s.stream));
+  if (s.ws.d_counters) {
+    uint32_t dropped = 0;
+    GS_HIP(hipMemcpy(&dropped, s.ws.d_counters + 3, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (dropped) {
+      GS_HIP(hipMemset(s.ws.d_counters + 3, 0, sizeof(uint32_t)));
+      return fail(GS_ERR_INVALID, "%u device-submitted updates named a vertex >= n and were dropped",
+                  dropped);
+    }
+  }
+  return GS_OK;
+}
+
+int gs_sync(gs_store* h) {
+  GS_CHECK(h, GS_ERR_INVALID, "null store");
+  std::vector<Stream*> ss;
+  {
+    std::lock_guard<std::mutex> lk(h->impl.mu);
+    ss = h->impl.streams;
+  }
+  for (Stream* s : ss) GS_HIP(hipStreamSynchronize(s->stream));
+  GS_HIP(hipStreamSynchronize(h->impl.qstream));
+  return GS_OK;
+}
+
+int gs_export_buckets(gs_store* h, uint32_t v0, uint32_t count, gs_bucket* out) {
+  GS_CHECK(h && (out || count == 0), GS_ERR_INVALID, "null argument");
+  Store& st = h->impl;
+  GS_CHECK((uint64_t)v0 + count <= st.view.n, GS_ERR_INVALID, "vertex range out of bounds");
+  int rc = gs_sync(h);
+  if (rc) return rc;
+  const uint32_t B = st.view.bpc, C = st.view.cols, vs = st.view.vstride;
+  const uint32_t nb = B * C + 1;
+  const uint32_t per = 1024;
+  std::vector<Bucket16> rows((size_t)per * vs), det(per);
+  for (uint32_t a = 0; a < count; a += per) {
+    uint32_t m = std::min(per, count - a);
+    GS_HIP(hipMemcpy(rows.data(), st.view.rows + (size_t)(v0 + a) * vs,
+                     (size_t)m * vs * sizeof(Bucket16), hipMemcpyDeviceToHost));
+    GS_HIP(hipMemcpy(det.data(), st.view.det + (v0 + a), (size_t)m * sizeof(Bucket16),
+                     hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < m; i++) {
+      gs_bucket* o = out + (size_t)(a + i) * nb;
+      const Bucket16* r = rows.data() + (size_t)i * vs;
+      for (uint32_t c = 0; c < C; c++)
+        for (uint32_t d = 0; d < B; d++) {
+          const Bucket16& b = r[(size_t)d * C + c];
+          o[(size_t)c * B + d] = gs_bucket{b.alpha, b.gamma};
+        }
+      o[nb - 1] = gs_bucket{det[i].alpha, det[i].gamma};
+    }
+  }
+  return GS_OK;
+}
+
+int gs_import_buckets(gs_store* h, uint32_t v0, uint32_t count, const gs_bucket* in) {
+  GS_CHECK(h && (in || count == 0), GS_ERR_INVALID, "null argument");
+  Store& st = h->impl;
+  GS_CHECK((uint64_t)v0 + count <= st.view.n, GS_ERR_INVALID, "vertex range out of bounds");
+  int rc = gs_sync(h);
+  if (rc) return rc;
+  const uint32_t B = st.view.bpc, C = st.view.cols, vs = st.view.vstride;
+  const uint32_t nb = B * C + 1;
+  const uint32_t per = 1024;
+  std::vector<Bucket16> rows((size_t)per * vs), det(per);
+  for (uint32_t a = 0; a < count; a += per) {
+    uint32_t m = std::min(per, count - a);
+    std::fill(rows.begin(), rows.end(), Bucket16{0, 0});
+    for (uint32_t i = 0; i < m; i++) {
+      const gs_bucket* src = in + (size_t)(a + i) * nb;
+      Bucket16* r = rows.data() + (size_t)i * vs;
+      for (uint32_t c = 0; c < C; c++)
+        for (uint32_t d = 0; d < B; d++) {
+          const gs_bucket& b = src[(size_t)c * B + d];
+          r[(size_t)d * C + c] = Bucket16{b.alpha, b.gamma};
+        }
+      det[i] = Bucket16{src[nb - 1].alpha, src[nb - 1].gamma};
+    }
+    GS_HIP(hipMemcpy(st.view.rows + (size_t)(v0 + a) * vs, rows.data(),
+                     (size_t)m * vs * sizeof(Bucket16), hipMemcpyHostToDevice));
+    GS_HIP(hipMemcpy(st.view.det + (v0 + a), det.data(), (size_t)m * sizeof(Bucket16),
+                     hipMemcpyHostToDevice));
+  }
+  return GS_OK;
+}
+
+int gs_stream_set_profiling(gs_stream* sh, int enable) {
+  GS_CHECK(sh, GS_ERR_INVALID, "null stream");
+  sh->impl.profile = enable != 0;
+  return GS_OK;
+}
+
+int gs_stream_timing(gs_stream* sh, double* agg_ms, uint64_t* launches, uint64_t* updates) {
+  GS_CHECK(sh, GS_ERR_INVALID, "null stream");
+  Stream& s = sh->impl;
+  GS_HIP(hipStreamSynchronize(s.stream));
+  for (auto& p : s.ev_pairs) {
+    float ms = 0;
+    GS_HIP(hipEventElapsedTime(&ms, p.first, p.second));
+    s.agg_ms += ms;
+    s.agg_launches++;
+    hipEventDestroy(p.first);
+    hipEventDestroy(p.second);
+  }
+  s.ev_pairs.clear();
+  if (agg_ms) *agg_ms = s.agg_ms;
+  if (launches) *launches = s.agg_launches;
+  if (updates) *updates = s.agg_updates;
+  s.agg_ms = 0;
+  s.agg_launches = 0;
+  s.agg_updates = 0;
+  return GS_OK;
+}
+
+int gs_device_alloc(size_t bytes, void** dptr) {
+  GS_CHECK(dptr, GS_ERR_INVALID, "null argument");
+  if (hipMalloc(dptr, bytes) != hipSuccess) return fail(GS_ERR_OOM, "hipMalloc(%zu) failed", bytes);
+  return GS_OK;
+}
+int gs_device_free(void* dptr) {
+  GS_HIP(hipFree(dptr));
+  return GS_OK;
+}
+int gs_memcpy_h2d(void* dptr, const void* src, size_t bytes) {
+  GS_HIP(hipMemcpy(dptr, src, bytes, hipMemcpyHostToDevice));
+  return GS_OK;
+}
+int gs_memcpy_d2h(void* dst, const void* dptr, size_t bytes) {
+  GS_HIP(hipMemcpy(dst, dptr, bytes, hipMemcpyDeviceToHost));
+  return GS_OK;
+}
+int gs_device_synchronize(void) {
+  GS_HIP(hipDeviceSynchronize());
+  return GS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,340 @@
+// Borůvka emulation on the device (CCSketchAlg::boruvka_emulation, src/cc_sketch_alg.cpp:491-540,
+// and the k-forest variant of src/mc_sketch_alg.cpp:738-920).
+//
+// Per round r:
+//   r == 0 : every vertex samples its own sketch at column col0 (run_round_zero :250-271);
+//   r >= 1 : acc_kernel XORs column r + the deterministic bucket of every vertex into the
+//            accumulator of its supernode root (Sketch::range_merge, src/sketch.cpp:165-190),
+//            sample_kernel samples each root's accumulator (Sketch::sample :110-132) and clears
+//            it for the next round;
+//   a GOOD sample is united into a lock-free union-find (larger root linked under the smaller,
+//   so the root of every set is its smallest vertex); FAIL or a successful union marks the round
+//   modified (sample_supernode :164-193); compress_kernel publishes the new supernode roots.
+// The partition after every round is a function of the sketch contents only, so it equals the
+// reference's; which of two equivalent sampled edges enters the spanning forest may differ.
+#include "gs_internal.h"
+
+namespace gs {
+
+constexpr int kQThreads = 256;
+
+__device__ __forceinline__ uint32_t load_parent(uint32_t* p, uint32_t x) {
+  return __hip_atomic_load(&p[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t find_root(uint32_t* parent, uint32_t x) {
+  uint32_t p = load_parent(parent, x);
+  while (p != x) {
+    x = p;
+    p = load_parent(parent, x);
+  }
+  return x;
+}
+
+__device__ __forceinline__ bool unite(uint32_t* parent, uint32_t a, uint32_t b) {
+  for (;;) {
+    a = find_root(parent, a);
+    b = find_root(parent, b);
+    if (a == b) return false;
+    if (a > b) { uint32_t t = a; a = b; b = t; }
+    uint32_t expected = b;
+    if (__hip_atomic_compare_exchange_strong(&parent[b], &expected, a, __ATOMIC_RELAXED,
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      return true;
+  }
+}
+
+// result of sampling: 0 GOOD, 1 ZERO, 2 FAIL
+__device__ __forceinline__ int sample_column(const Bucket16 det, const Bucket16* col, uint32_t stride,
+                                             uint32_t bpc, uint64_t seed, uint64_t* idx) {
+  if (det.alpha == 0 && det.gamma == 0) return 1;
+  if (det.gamma == xxh3_len8(det.alpha, seed)) { *idx = det.alpha; return 0; }
+  for (uint32_t d = 0; d < bpc; d++) {
+    Bucket16 b = col[(size_t)d * stride];
+    if (b.gamma == xxh3_len8(b.alpha, seed)) { *idx = b.alpha; return 0; }
+  }
+  return 2;
+}
+
+__device__ __forceinline__ void handle_sample(int res, uint64_t idx, uint32_t n, uint32_t* parent,
+                                              uint32_t* flags, uint32_t* forest,
+                                              uint64_t forest_cap) {
+  if (res == 2) {
+    flags[0] = 1;
+  } else if (res == 0) {
+    uint32_t a = (uint32_t)(idx >> 32), b = (uint32_t)idx;
+    if (a < n && b < n && a != b && unite(parent, a, b)) {
+      flags[0] = 1;
+      uint32_t pos = atomicAdd(&flags[2], 1u);
+      if (pos < forest_cap) {
+        forest[2 * (uint64_t)pos] = min(a, b);
+        forest[2 * (uint64_t)pos + 1] = max(a, b);
+      }
+    }
+  }
+}
+
+__global__ void init_query_kernel(uint32_t n, uint32_t* parent, uint32_t* comp) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    parent[v] = v;
+    comp[v] = v;
+  }
+}
+
+__global__ void clear_round_flags_kernel(uint32_t* flags) {
+  if (threadIdx.x == 0) { flags[0] = 0; flags[1] = 0; }
+}
+
+__global__ void clear_all_flags_kernel(uint32_t* flags) {
+  if (threadIdx.x < 4) flags[threadIdx.x] = 0;
+}
+
+__global__ void zero_acc_kernel(Bucket16* acc, uint64_t count) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    acc[i] = Bucket16{0, 0};
+}
+
+__global__ __launch_bounds__(kQThreads) void round0_kernel(StoreView st, uint32_t col0,
+                                                           uint32_t* parent, uint32_t* flags,
+                                                           uint32_t* forest, uint64_t forest_cap) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= st.n) return;
+  uint64_t idx = 0;
+  int res = sample_column(st.det[v], st.rows + (size_t)v * st.vstride + col0, st.cols, st.bpc,
+                          st.seed, &idx);
+  handle_sample(res, idx, st.n, parent, flags, forest, forest_cap);
+}
+
+// one thread per (vertex, depth) plus one for the deterministic bucket
+__global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st, uint32_t col,
+                                                        const uint32_t* __restrict__ comp,
+                                                        Bucket16* __restrict__ acc) {
+  const uint32_t w = st.bpc + 1;
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)st.n * w) return;
+  uint32_t v = (uint32_t)(i / w), j = (uint32_t)(i - (uint64_t)v * w);
+  Bucket16 b = j < st.bpc ? st.rows[row_index(st, v, j, col)] : st.det[v];
+  if ((b.alpha | b.gamma) == 0) return;
+  uint32_t r = comp[v];
+  Bucket16* a = &acc[(uint64_t)r * w + j];
+  if (r == v) {
+    // a root's own contribution races only with atomics from its members
+    __hip_atomic_fetch_xor(&a->alpha, b.alpha, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_xor(&a->gamma, b.gamma, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    __hip_atomic_fetch_xor(&a->alpha, b.alpha, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_xor(&a->gamma, b.gamma, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+__global__ __launch_bounds__(kQThreads) void sample_kernel(StoreView st,
+                                                           const uint32_t* __restrict__ comp,
+                                                           Bucket16* __restrict__ acc,
+                                                           uint32_t* parent, uint32_t* flags,
+                                                           uint32_t* forest, uint64_t forest_cap) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= st.n || comp[v] != v) return;
+  const uint32_t w = st.bpc + 1;
+  Bucket16* a = acc + (uint64_t)v * w;
+  uint64_t idx = 0;
+  int res = sample_column(a[st.bpc], a, 1, st.bpc, st.seed, &idx);
+  for (uint32_t j = 0; j < w; j++) a[j] = Bucket16{0, 0};
+  handle_sample(res, idx, st.n, parent, flags, forest, forest_cap);
+}
+
+__global__ __launch_bounds__(kQThreads) void compress_kernel(uint32_t n, uint32_t* parent,
+                                                             uint32_t* comp) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  uint32_t r = find_root(parent, v);
+  comp[v] = r;
+}
+
+__global__ __launch_bounds__(kQThreads) void finalize_parent_kernel(uint32_t n, uint32_t* parent,
+                                                                    const uint32_t* comp) {
+  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) parent[v] = comp[v];
+}
+
+void free_query_scratch(Store* st) {
+  hipFree(st->d_parent);
+  hipFree(st->d_comp);
+  hipFree(st->d_acc);
+  hipFree(st->d_flags);
+  hipFree(st->d_forest);
+  if (st->h_flags) hipHostFree(st->h_flags);
+  st->d_parent = st->d_comp = st->d_flags = st->d_forest = st->h_flags = nullptr;
+  st->d_acc = nullptr;
+  st->forest_cap = 0;
+  if (st->qs) {
+    Stream* s = st->qs;
+    hipStreamSynchronize(s->stream);
+    free_pair_scratch(s->ws);
+    delete s;  // its HIP stream is the store's query stream, destroyed by the store
+    st->qs = nullptr;
+  }
+}
+
+static int ensure_query_scratch(Store* st, uint64_t forest_pairs) {
+  const uint32_t n = st->view.n;
+  if (!st->d_parent) {
+    GS_HIP(hipMalloc(&st->d_parent, sizeof(uint32_t) * n));
+    GS_HIP(hipMalloc(&st->d_comp, sizeof(uint32_t) * n));
+    GS_HIP(hipMalloc(&st->d_acc, sizeof(Bucket16) * (uint64_t)n * (st->view.bpc + 1)));
+    GS_HIP(hipMalloc(&st->d_flags, sizeof(uint32_t) * 4));
+    GS_HIP(hipHostMalloc((void**)&st->h_flags, sizeof(uint32_t) * 4, hipHostMallocDefault));
+    uint64_t cnt = (uint64_t)n * (st->view.bpc + 1);
+    hipLaunchKernelGGL(zero_acc_kernel, dim3(2048), dim3(256), 0, st->qstream, st->d_acc, cnt);
+    GS_HIP(hipGetLastError());
+  }
+  if (st->forest_cap < forest_pairs) {
+    GS_HIP(hipStreamSynchronize(st->qstream));
+    hipFree(st->d_forest);
+    st->d_forest = nullptr;
+    GS_HIP(hipMalloc(&st->d_forest, sizeof(uint32_t) * 2 * forest_pairs));
+    st->forest_cap = forest_pairs;
+  }
+  if (!st->qs) {
+    Stream* s = new Stream();
+    s->store = st;
+    s->stream = st->qstream;  // queries and their forest toggles share one in-order stream
+    st->qs = s;
+  }
+  return GS_OK;
+}
+
+// One Borůvka pass. Forest edges are appended at d_forest[2*forest_base...]; returns the pass's
+// forest size in *forest_len.
+static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t* rounds_out,
+                        uint64_t* forest_len) {
+  const StoreView& v = st->view;
+  const uint32_t n = v.n;
+  const dim3 grid((n + kQThreads - 1) / kQThreads), block(kQThreads);
+  hipStream_t q = st->qstream;
+  uint32_t* forest = st->d_forest + 2 * forest_base;
+  uint64_t cap = st->forest_cap - forest_base;
+  hipLaunchKernelGGL(init_query_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp);
+  hipLaunchKernelGGL(clear_all_flags_kernel, dim3(1), dim3(64), 0, q, st->d_flags);
+  GS_HIP(hipGetLastError());
+  uint32_t round = 0;
+  for (;;) {
+    hipLaunchKernelGGL(clear_round_flags_kernel, dim3(1), dim3(64), 0, q, st->d_flags);
+    if (round == 0) {
+      if (col0 >= st->num_samples) return fail(GS_ERR_OUT_OF_SAMPLES, "sketch out of samples");
+      hipLaunchKernelGGL(round0_kernel, grid, block, 0, q, v, col0, st->d_parent, st->d_flags,
+                         forest, cap);
+    } else {
+      if (round >= st->num_samples)
+        return fail(GS_ERR_OUT_OF_SAMPLES,
+                    "This sketch (seed=%llu, max samples=%u) cannot be sampled more times (cur idx=%u)!",
+                    (unsigned long long)v.seed, st->num_samples, round);
+      uint64_t threads = (uint64_t)n * (v.bpc + 1);
+      hipLaunchKernelGGL(acc_kernel, dim3((unsigned)((threads + kQThreads - 1) / kQThreads)), block,
+                         0, q, v, round, st->d_comp, st->d_acc);
+      hipLaunchKernelGGL(sample_kernel, grid, block, 0, q, v, st->d_comp, st->d_acc, st->d_parent,
+                         st->d_flags, forest, cap);
+    }
+    hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp);
+    GS_HIP(hipGetLastError());
+    GS_HIP(hipMemcpyAsync(st->h_flags, st->d_flags, sizeof(uint32_t) * 4, hipMemcpyDeviceToHost, q));
+    GS_HIP(hipStreamSynchronize(q));
+    if (!st->h_flags[0]) break;
+    hipLaunchKernelGGL(finalize_parent_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp);
+    round++;
+  }
+  if (rounds_out) *rounds_out = round;
+  uint64_t fl = st->h_flags[2];
+  GS_CHECK(fl <= cap, GS_ERR_INVALID, "forest capacity exceeded");
+  *forest_len = fl;
+  return GS_OK;
+}
+
+static int sync_all_streams(Store* st) {
+  std::vector<Stream*> ss;
+  {
+    std::lock_guard<std::mutex> lk(st->mu);
+    ss = st->streams;
+  }
+  for (Stream* s : ss) GS_HIP(hipStreamSynchronize(s->stream));
+  return GS_OK;
+}
+
+int boruvka_query(Store* st, const uint32_t*, uint32_t* label_out, uint32_t* num_cc,
+                  uint32_t* rounds, uint32_t* forest_out, uint64_t forest_cap,
+                  uint64_t* forest_len) {
+  int rc = sync_all_streams(st);
+  if (rc) return rc;
+  rc = ensure_query_scratch(st, st->view.n);
+  if (rc) return rc;
+  uint64_t fl = 0;
+  rc = boruvka_pass(st, 0, 0, rounds, &fl);
+  if (rc) return rc;
+  const uint32_t n = st->view.n;
+  if (label_out) GS_HIP(hipMemcpy(label_out, st->d_comp, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+  if (num_cc) {
+    std::vector<uint32_t> lab;
+    const uint32_t* l = label_out;
+    if (!l) {
+      lab.resize(n);
+      GS_HIP(hipMemcpy(lab.data(), st->d_comp, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+      l = lab.data();
+    }
+    uint32_t c = 0;
+    for (uint32_t i = 0; i < n; i++) c += l[i] == i;
+    *num_cc = c;
+  }
+  if (forest_len) *forest_len = fl;
+  if (forest_out && forest_cap) {
+    uint64_t m = std::min(fl, forest_cap);
+    GS_HIP(hipMemcpy(forest_out, st->d_forest, sizeof(uint32_t) * 2 * m, hipMemcpyDeviceToHost));
+  }
+  return GS_OK;
+}
+
+int k_forests(Store* st, uint32_t k, uint32_t* edges_out, uint64_t edge_cap, uint64_t* counts) {
+  int rc = sync_all_streams(st);
+  if (rc) return rc;
+  const uint32_t n = st->view.n;
+  rc = ensure_query_scratch(st, (uint64_t)k * n);
+  if (rc) return rc;
+  uint64_t total = 0;
+  int err = GS_OK;
+  for (uint32_t i = 0; i < k; i++) {
+    uint64_t fl = 0;
+    err = boruvka_pass(st, i, total, nullptr, &fl);
+    if (err) break;
+    counts[i] = fl;
+    // toggle this forest out of both endpoints' sketches (filter_sf_edges, src/mc_sketch_alg.cpp:837-888)
+    err = ingest_pairs_device(st->qs, st->d_forest + 2 * total, fl, 0, 0);
+    if (err) break;
+    total += fl;
+  }
+  // toggle every forest found back in, also after a failure, so the sketches are unchanged
+  if (total) {
+    int rc2 = ingest_pairs_device(st->qs, st->d_forest, total, 0, 0);
+    if (!err) err = rc2;
+  }
+  GS_HIP(hipStreamSynchronize(st->qstream));
+  if (err) return err;
+  GS_CHECK(total <= edge_cap, GS_ERR_INVALID, "edge_cap %llu too small for %llu forest edges",
+           (unsigned long long)edge_cap, (unsigned long long)total);
+  if (total) GS_HIP(hipMemcpy(edges_out, st->d_forest, sizeof(uint32_t) * 2 * total, hipMemcpyDeviceToHost));
+  return GS_OK;
+}
+
+}  // namespace gs
+
+extern "C" int gs_query_cc(gs_store* h, uint32_t* label_out, uint32_t* num_cc, uint32_t* rounds,
+                           uint32_t* forest_out, uint64_t forest_cap, uint64_t* forest_len) {
+  GS_CHECK(h, GS_ERR_INVALID, "null store");
+  return gs::boruvka_query(&h->impl, nullptr, label_out, num_cc, rounds,
+                           forest_out, forest_cap, forest_len);
+}
+
+extern "C" int gs_k_spanning_forests(gs_store* h, uint32_t k, uint32_t* edges_out,
+                                     uint64_t edge_cap, uint64_t* counts) {
+  GS_CHECK(h && counts && (edges_out || edge_cap == 0), GS_ERR_INVALID, "null argument");
+  GS_CHECK(k >= 1, GS_ERR_INVALID, "k must be >= 1");
+  return gs::k_forests(&h->impl, k, edges_out, edge_cap, counts);
+}

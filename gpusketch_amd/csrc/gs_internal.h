@@ -1,0 +1,218 @@
+// Internal definitions shared by the HIP translation units of libgpusketch.
+//
+// Device layout of a sketch store (gfx950-first, see the store section below):
+//   rows : per vertex, bkt_per_col depth rows, each num_columns buckets of {alpha, gamma}
+//          (depth-major, so the low depths that almost every update touches are contiguous and
+//          a wave's 64 lanes flush consecutive 16-byte buckets); the per-vertex stride is padded
+//          to a multiple of 8 buckets (128 bytes) so every vertex starts on an L2 line.
+//   det  : the deterministic bucket of every vertex, in a separate dense array.
+// Reference numbering (src/sketch.cpp:8-37) is bucket [column*bkt_per_col + depth] with the
+// deterministic bucket last; export/import convert.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gpusketch.h"
+
+namespace gs {
+
+// ------------------------------------------------------------------------------------------
+// XXH3_64bits_withSeed for an 8-byte key (the hash the reference binds at include/types.h:8-9),
+// restated from the published XXH3 short-input path (len 4..8 followed by rrmxmx). Host and
+// device share this code.
+//
+// Split form used by the update kernels: with mix(x) = x ^ rotl(x,49) ^ rotl(x,24), which is
+// linear over GF(2), mix(swap(key) ^ bitflip(seed)) = mix(swap(key)) ^ mix(bitflip(seed)).
+// mix(swap(key)) is computed once per update and mix(bitflip(seed+5c)) once per column.
+// ------------------------------------------------------------------------------------------
+constexpr uint64_t kSecret8 = 0x1cad21f72c81017cULL;   // LE64 of secret bytes 8..15
+constexpr uint64_t kSecret16 = 0xdb979083e96dd4deULL;  // LE64 of secret bytes 16..23
+constexpr uint64_t kRrmxmxMul = 0x9FB21C651E98DF25ULL;
+
+__host__ __device__ __forceinline__ uint64_t rotl64(uint64_t x, unsigned r) {
+  return (x << r) | (x >> (64u - r));
+}
+__host__ __device__ __forceinline__ uint32_t bswap32(uint32_t x) {
+  return (x >> 24) | ((x >> 8) & 0xff00u) | ((x << 8) & 0xff0000u) | (x << 24);
+}
+__host__ __device__ __forceinline__ uint64_t xxh3_bitflip(uint64_t seed) {
+  uint64_t s = seed ^ ((uint64_t)bswap32((uint32_t)seed) << 32);
+  return (kSecret8 ^ kSecret16) - s;
+}
+__host__ __device__ __forceinline__ uint64_t xxh3_mix(uint64_t x) {
+  return x ^ rotl64(x, 49) ^ rotl64(x, 24);
+}
+__host__ __device__ __forceinline__ uint64_t xxh3_swap(uint64_t key) {
+  return (key >> 32) | (key << 32);
+}
+__host__ __device__ __forceinline__ uint64_t xxh3_finish(uint64_t x) {
+  x *= kRrmxmxMul;
+  x ^= (x >> 35) + 8u;
+  x *= kRrmxmxMul;
+  x ^= x >> 28;
+  return x;
+}
+__host__ __device__ __forceinline__ uint64_t xxh3_len8(uint64_t key, uint64_t seed) {
+  return xxh3_finish(xxh3_mix(xxh3_swap(key) ^ xxh3_bitflip(seed)));
+}
+// per-column constant of the split form
+__host__ __device__ __forceinline__ uint64_t xxh3_seed_key(uint64_t seed) {
+  return xxh3_mix(xxh3_bitflip(seed));
+}
+// ctz(hash | 1<<max_depth): Bucket_Boruvka::get_index_depth (include/bucket.h:54-59)
+__host__ __device__ __forceinline__ uint32_t depth_of(uint64_t h, uint32_t max_depth) {
+  return (uint32_t)__builtin_ctzll(h | (1ULL << max_depth));
+}
+// concat_pairing_fn (src/util.cpp:45-51)
+__host__ __device__ __forceinline__ uint64_t concat_pair(uint32_t i, uint32_t j) {
+  return i < j ? (((uint64_t)i << 32) | j) : (((uint64_t)j << 32) | i);
+}
+
+// ------------------------------------------------------------------------------------------
+// error reporting
+// ------------------------------------------------------------------------------------------
+void set_error(const char* fmt, ...);
+int fail(int code, const char* fmt, ...);
+
+#define GS_HIP(call)                                                                         \
+  do {                                                                                       \
+    hipError_t _e = (call);                                                                  \
+    if (_e != hipSuccess)                                                                    \
+      return ::gs::fail(GS_ERR_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #call,               \
+                        hipGetErrorString(_e));                                              \
+  } while (0)
+
+#define GS_CHECK(cond, code, ...)                                                            \
+  do {                                                                                       \
+    if (!(cond)) return ::gs::fail((code), __VA_ARGS__);                                     \
+  } while (0)
+
+// ------------------------------------------------------------------------------------------
+// store geometry and device view
+// ------------------------------------------------------------------------------------------
+struct Bucket16 {
+  uint64_t alpha, gamma;
+};
+
+struct StoreView {
+  Bucket16* rows;       // n * vstride buckets
+  Bucket16* det;        // n buckets
+  uint32_t n;           // vertices
+  uint32_t bpc;         // depths per column (B)
+  uint32_t cols;        // columns (C)
+  uint32_t vstride;     // buckets per vertex in rows (>= B*C, multiple of 8)
+  uint64_t seed;
+};
+
+__host__ __device__ __forceinline__ size_t row_index(const StoreView& s, uint32_t v, uint32_t d,
+                                                     uint32_t c) {
+  return (size_t)v * s.vstride + (size_t)d * s.cols + c;
+}
+
+// work item of the aggregation kernel: `len` destinations of vertex `vertex` starting at
+// `start` in the destination array; bit 31 of `flags` = the vertex is owned by this item alone.
+struct WorkItem {
+  uint32_t vertex;
+  uint32_t len;
+  uint64_t start;
+};
+constexpr uint32_t kItemExclusive = 0x80000000u;
+
+struct Store;
+
+// per-stream scratch for the edge-pair pipeline, grown on demand
+struct PairScratch {
+  uint64_t cap_edges = 0;       // capacity in undirected edges
+  uint32_t* d_pairs = nullptr;  // staged input pairs (2 x cap)
+  uint32_t* d_rank = nullptr;   // rank of each directed entry within its vertex (2 x cap)
+  uint32_t* d_sorted = nullptr; // destinations grouped by vertex (2 x cap)
+  uint64_t cap_items = 0;
+  WorkItem* d_items = nullptr;
+  uint32_t cap_n = 0;
+  uint32_t* d_deg = nullptr;    // per-vertex count (zero between batches)
+  uint32_t* d_off = nullptr;    // per-vertex exclusive offset
+  uint64_t* d_tile_status = nullptr;  // decoupled look-back status
+  uint32_t* d_counters = nullptr;     // [0]=tile ticket, [1]=item count, [2]=entry count,
+                                      // [3]=dropped out-of-range entries since the last flush
+};
+
+struct Stream {
+  Store* store = nullptr;
+  hipStream_t stream = nullptr;
+  hipStream_t copy_stream = nullptr;
+  PairScratch ws;
+  // host staging (pinned) and matching device buffers for vertex-batch submissions
+  uint32_t* h_stage = nullptr;
+  size_t h_stage_bytes = 0;
+  uint32_t* d_stage = nullptr;
+  size_t d_stage_bytes = 0;
+  hipEvent_t stage_free = nullptr;  // recorded when the kernels that read the staging are done
+  bool stage_pending = false;
+  // optional kernel timing of the aggregation kernel
+  bool profile = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pairs;
+  double agg_ms = 0;
+  uint64_t agg_launches = 0;
+  uint64_t agg_updates = 0;
+};
+
+struct Store {
+  StoreView view{};
+  uint32_t num_samples = 0;
+  uint32_t cols_per_sample = 1;
+  uint32_t flags = 0;
+  int device = 0;
+  std::mutex mu;
+  std::vector<Stream*> streams;
+  // Borůvka scratch, allocated on first query
+  uint32_t* d_parent = nullptr;      // union-find being merged this round
+  uint32_t* d_comp = nullptr;        // supernode root of every vertex at round start
+  Bucket16* d_acc = nullptr;         // per-root XOR of one column + deterministic bucket
+  uint32_t* d_flags = nullptr;       // [0]=modified, [1]=out of samples, [2]=forest count
+  uint32_t* d_forest = nullptr;      // forest edges (min,max) pairs, capacity k * n
+  uint64_t forest_cap = 0;           // pairs
+  uint32_t* h_flags = nullptr;       // pinned mirror of d_flags
+  hipStream_t qstream = nullptr;
+  Stream* qs = nullptr;              // internal stream for forest toggles (not in `streams`)
+  size_t bytes() const {
+    return (size_t)view.n * view.vstride * sizeof(Bucket16) + (size_t)view.n * sizeof(Bucket16);
+  }
+};
+
+// kernels/launchers implemented in ingest.hip
+int launch_vertex_batches(Stream* s, const uint32_t* d_src_unused, const WorkItem* d_items,
+                          uint32_t nitems, const uint32_t* d_dsts);
+int ingest_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uint32_t min_depth,
+                        uint32_t depth_cap);
+int ensure_pair_scratch(Stream* s, uint64_t nedges);
+void free_pair_scratch(PairScratch& ws);
+int launch_zero_store(Store* st, hipStream_t stream);
+int hash_known_answer(const uint64_t* keys, const uint64_t* seeds, uint64_t count, uint64_t* out);
+
+// launchers implemented in boruvka.hip
+int boruvka_query(Store* st, const uint32_t* start_idx_host, uint32_t* label_out,
+                  uint32_t* num_cc, uint32_t* rounds, uint32_t* forest_out, uint64_t forest_cap,
+                  uint64_t* forest_len);
+void free_query_scratch(Store* st);
+int k_forests(Store* st, uint32_t k, uint32_t* edges_out, uint64_t edge_cap, uint64_t* counts);
+
+// geometry helper shared with the host façade (include/sketch.h:63-77,198-216)
+void geometry(uint32_t n, double factor, uint32_t cols_per_sample, int l0, uint64_t out[5]);
+
+}  // namespace gs
+
+// opaque handles of the C ABI wrap the internal objects
+struct gs_store {
+  gs::Store impl;
+};
+struct gs_stream {
+  gs::Stream impl;
+};

@@ -1,0 +1,177 @@
+/*
+ * libgpusketch -- C ABI of the MI355X (gfx950) l0-sampling sketch engine.
+ *
+ * Plain pointers and sizes only. Every function returns a gs_status (0 = GS_OK) and never exits
+ * the process; gs_last_error() returns a message for the calling thread's last failure.
+ * Host pointers are caller-owned and may be reused as soon as a call returns (the library copies
+ * them into its own pinned staging first). Functions named *_device take device pointers.
+ *
+ * Each declaration names the reference interface it replaces (paths relative to the reference
+ * repository).
+ */
+#ifndef GPUSKETCH_H
+#define GPUSKETCH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gs_store gs_store;
+typedef struct gs_stream gs_stream;
+
+/* One bucket, byte-identical to the reference Bucket {vec_t alpha; vec_hash_t gamma}
+ * (include/bucket.h:7-12) with vec_hash_t 64-bit. */
+typedef struct gs_bucket {
+  uint64_t alpha;
+  uint64_t gamma;
+} gs_bucket;
+
+enum gs_status {
+  GS_OK = 0,
+  GS_ERR_INVALID = 1,         /* bad argument or state */
+  GS_ERR_OOM = 2,             /* device or pinned allocation failed */
+  GS_ERR_HIP = 3,             /* HIP runtime error (message has the call and code) */
+  GS_ERR_OUT_OF_SAMPLES = 4,  /* Sketch::sample past num_samples (include/sketch.h:219-230) */
+  GS_ERR_ADMISSION = 5,       /* serving: request exceeds the sketch-memory budget */
+  GS_ERR_UNSUPPORTED = 6
+};
+
+/* Sketch geometry; mirrors SketchParams (include/cuda_kernel.cuh:23-37) plus the derived sizes
+ * of include/sketch.h:63-77,198-216. */
+typedef struct gs_geometry {
+  uint32_t num_vertices;
+  uint32_t bkt_per_col;     /* depths per column, B */
+  uint32_t num_samples;     /* S */
+  uint32_t cols_per_sample; /* 1 (CameoSketch) */
+  uint32_t num_columns;     /* C = S * cols_per_sample */
+  uint32_t num_buckets;     /* C * B + 1, the reference per-vertex bucket count */
+  uint64_t seed;
+  uint64_t vector_len;      /* n(n-1)/2 */
+  uint64_t device_bytes;    /* HBM held by the store */
+} gs_geometry;
+
+/* Thread-local message of the last failing call ("" if none). */
+const char* gs_last_error(void);
+
+/* Sizing: Sketch::calc_vector_length / calc_cc_samples / calc_bkt_per_col
+ * (include/sketch.h:63-77,198-206) for n vertices and a CCAlgConfiguration sketches_factor. */
+int gs_geometry_for(uint32_t n, double sketches_factor, uint32_t cols_per_sample,
+                    gs_geometry* out);
+
+/* Device XXH3 known-answer check: out[i] = XXH3_64bits_withSeed(&keys[i], 8, seeds[i]) computed
+ * by a device kernel (replaces CUDA_XXH64, include/cuda_xxhash64.cuh:587-591). Host arrays. */
+int gs_hash_known_answer(const uint64_t* keys, const uint64_t* seeds, uint64_t count,
+                         uint64_t* out);
+
+/* Create a zeroed sketch store of n vertex sketches resident in HBM, all with the same seed
+ * (the reference gives every vertex the same seed, src/cc_sketch_alg.cpp:11-22). Replaces the
+ * cudaMallocManaged sketch buffer + SketchParams set-up (tools/cuda_process_stream.cu:97-116,
+ * include/cc_gpu_sketch_alg.h:36-97). flags: reserved, pass 0. */
+int gs_store_create(uint32_t n, uint64_t seed, uint32_t num_samples, uint32_t cols_per_sample,
+                    uint32_t flags, gs_store** out);
+int gs_store_destroy(gs_store* st);
+int gs_store_geometry(const gs_store* st, gs_geometry* out);
+/* Sketch::zero_contents for every vertex (src/sketch.cpp:102-108). */
+int gs_store_zero(gs_store* st);
+
+/* A submission stream (one HIP stream + staging + scratch). Mirrors CudaStream<Alg>
+ * (include/cuda_stream.h:51-97): one per worker thread id. Streams of one store may be used
+ * from different host threads concurrently; a single stream must not. */
+int gs_stream_create(gs_store* st, gs_stream** out);
+int gs_stream_destroy(gs_stream* s);
+
+/* Pre-grouped vertex batches: batch b toggles concat(src[b], dsts[offsets[b] + i]) for
+ * i < sizes[b] in the sketch of src[b]. The reference contract of CudaKernel::sketchUpdate /
+ * sketchUpdate_UVM_kernel (include/cuda_kernel.cuh:46, src/cuda_kernel.cu:152-257) as driven by
+ * CudaStream::process_batch (include/cuda_stream.h:99-213). Asynchronous: returns after the
+ * host arrays are staged. */
+int gs_submit_vertex_batches(gs_stream* s, const uint32_t* src, const uint64_t* sizes,
+                             const uint64_t* offsets, uint32_t nbatches, const uint32_t* dsts);
+
+/* Raw undirected updates: pairs[2i], pairs[2i+1] toggles edge {u,v} in both endpoints'
+ * sketches, as GraphSketchDriver inserts {u,v} and {v,u} (include/graph_sketch_driver.h:171-172)
+ * and CCSketchAlg::update applies them (src/cc_sketch_alg.cpp:154-160). Grouping by vertex
+ * happens on the device. Host memory; asynchronous after staging. */
+int gs_submit_edge_pairs(gs_stream* s, const uint32_t* pairs, uint64_t nedges);
+/* Same, input already resident in device memory (must stay valid until gs_flush). */
+int gs_submit_edge_pairs_device(gs_stream* s, const uint32_t* d_pairs, uint64_t nedges);
+/* Subgraph-filtered variant for the min-cut path: only edges whose subgraph depth
+ * get_index_depth(idx, seed, depth_cap) (src/mc_gpu_sketch_alg.cpp:160) is >= min_depth are
+ * applied. Host memory. */
+int gs_submit_edge_pairs_filtered(gs_stream* s, const uint32_t* pairs, uint64_t nedges,
+                                  uint32_t min_depth, uint32_t depth_cap);
+
+/* Wait until all work submitted on the stream is finished (CudaStream::flush_buffers,
+ * include/cuda_stream.h:215-276). */
+int gs_flush(gs_stream* s);
+/* Wait for every stream of the store (CCGPUSketchAlg::flush_buffers,
+ * src/cc_gpu_sketch_alg.cpp:21-31). */
+int gs_sync(gs_store* st);
+
+/* Copy `count` vertex sketches starting at v0 to host memory in reference bucket order
+ * [column * bkt_per_col + depth], deterministic bucket last (src/sketch.cpp:8-37); count *
+ * num_buckets buckets. The byte image equals Sketch::serialize (src/sketch.cpp:204-206). */
+int gs_export_buckets(gs_store* st, uint32_t v0, uint32_t count, gs_bucket* out);
+/* Overwrite `count` vertex sketches from reference-ordered buckets (the istream constructor,
+ * src/sketch.cpp:39-51, used by CCSketchAlg::construct_from_serialized_data :57-91). */
+int gs_import_buckets(gs_store* st, uint32_t v0, uint32_t count, const gs_bucket* in);
+
+/* Connected components by Borůvka emulation on the device (CCSketchAlg::boruvka_emulation +
+ * connected_components, src/cc_sketch_alg.cpp:491-584). label_out[v] = smallest vertex id of
+ * v's component (n entries). forest_out (may be NULL) receives up to forest_cap spanning-forest
+ * edges as (min, max) pairs; *forest_len is the forest size. Returns GS_ERR_OUT_OF_SAMPLES when
+ * the sketches run out of columns (OutOfSamplesException). */
+int gs_query_cc(gs_store* st, uint32_t* label_out, uint32_t* num_cc, uint32_t* rounds,
+                uint32_t* forest_out, uint64_t forest_cap, uint64_t* forest_len);
+
+/* k edge-disjoint spanning forests (MCSketchAlg::calc_disjoint_spanning_forests,
+ * src/mc_sketch_alg.cpp:889-920): forest i samples column i in round 0, its edges are toggled
+ * out before forest i+1, and all are toggled back in before return. edges_out holds
+ * sum(counts) (min,max) pairs, forest after forest; capacity edge_cap pairs. */
+int gs_k_spanning_forests(gs_store* st, uint32_t k, uint32_t* edges_out, uint64_t edge_cap,
+                          uint64_t* counts);
+
+/* Exact global minimum cut of an undirected multigraph given as an edge list (replaces the
+ * VieCut call of MCSketchAlg::calc_minimum_cut, src/mc_sketch_alg.cpp:966-1003). side_out[v]
+ * = 1 for the vertices of one side. Host-only computation. */
+int gs_min_cut(uint32_t n, const uint32_t* pairs, uint64_t m, uint64_t* value,
+               uint8_t* side_out);
+
+/* Kernel timing of the update path, for measurement. When enabled, the aggregation kernel's
+ * launches on this stream are bracketed by HIP events; gs_stream_timing returns the summed
+ * milliseconds, the launch count and the directed updates those launches processed. */
+int gs_stream_set_profiling(gs_stream* s, int enable);
+int gs_stream_timing(gs_stream* s, double* agg_ms, uint64_t* launches, uint64_t* updates);
+
+/* Device memory for callers that keep inputs resident (bench, batched and serving modes). */
+int gs_device_alloc(size_t bytes, void** dptr);
+int gs_device_free(void* dptr);
+int gs_memcpy_h2d(void* dptr, const void* src, size_t bytes);
+int gs_memcpy_d2h(void* dst, const void* dptr, size_t bytes);
+int gs_device_synchronize(void);
+
+/* Synthetic seeded streams (the reference's tools/stream_generators write similar binary
+ * streams). Each returns undirected updates as (u, v) pairs with toggle semantics: every
+ * deletion removes a previously inserted, still live edge; no self loops; no duplicate live
+ * edges. Call with pairs == NULL to get the count first. */
+/* Erdős–Rényi G(n, p) inserts in seeded random order, then `del_frac` of them re-deleted at
+ * random later positions. */
+int gs_gen_erdos_renyi(uint32_t n, double p, double del_frac, uint64_t seed, uint32_t* pairs,
+                       uint64_t* count);
+/* R-MAT(a, b, c, 1-a-b-c) with n = 2^scale: `count` updates, `del_frac` of which delete a
+ * random earlier live edge. */
+int gs_gen_rmat(uint32_t scale, double a, double b, double c, uint64_t count, double del_frac,
+                uint64_t seed, uint32_t* pairs);
+/* Two ER(p) halves of n/2 vertices joined by exactly `cut` crossing edges, shuffled, plus
+ * churn_frac * m insert/delete churn pairs of random non-edges. */
+int gs_gen_planted_cut(uint32_t n, double p, uint32_t cut, double churn_frac, uint64_t seed,
+                       uint32_t* pairs, uint64_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUSKETCH_H */

@@ -1,0 +1,126 @@
+"""GPU parity of the query path: device Borůvka components equal the oracle's Borůvka and a DSU
+over the net graph; spanning forests are valid; k disjoint forests preserve the planted min cut,
+which equals exact Stoer–Wagner."""
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import gpusketch_amd as gs
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def net_edges(pairs):
+    keys = (pairs.min(1).astype(np.uint64) << 32) | pairs.max(1)
+    uniq, cnt = np.unique(keys, return_counts=True)
+    live = uniq[cnt % 2 == 1]
+    return np.stack([live >> 32, live & 0xFFFFFFFF], 1).astype(np.uint32)
+
+
+def check_forest(n, forest, net, labels):
+    netset = set(map(tuple, net.tolist()))
+    assert all(tuple(e) in netset for e in forest.tolist())
+    assert len(forest) == n - len(np.unique(labels))
+    assert np.array_equal(orc.dsu_components(n, forest), labels)
+
+
+def test_fixture_stream_78_components():
+    raw = open(os.path.join(GOLD, "multiples_graph_1024_stream.data"), "rb").read()
+    n, m = struct.unpack_from("<IQ", raw, 0)
+    rec = np.frombuffer(raw, dtype=np.dtype([("t", "u1"), ("s", "<u4"), ("d", "<u4")]),
+                        count=m, offset=12)
+    pairs = np.stack([rec["s"], rec["d"]], 1).astype(np.uint32)
+    with gs.SketchStore(n, 0x5EED) as st:
+        s = st.stream()
+        s.submit_pairs(pairs)
+        s.flush()
+        labels, ncc, rounds, forest = st.query_cc(want_forest=True)
+    assert ncc == 78  # test/cc_alg_test.cpp:56
+    net = net_edges(pairs)
+    assert np.array_equal(labels, orc.dsu_components(n, net))
+    check_forest(n, forest, net, labels)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_cc_matches_oracle_boruvka_and_dsu(seed):
+    n = 8192 if seed == 1 else 3000
+    pairs = gs.gen_erdos_renyi(n, 2.0 / n if seed == 1 else 1.2 / n, 0.1, seed)
+    key = 1000 + seed
+    g = orc.Geometry(n)
+    ref = g.empty_store()
+    orc.apply_edges(ref, g, key, pairs)
+    rc, olab, orounds, _ = orc.boruvka(ref, g, key)
+    assert rc == 0
+    with gs.SketchStore(n, key) as st:
+        s = st.stream()
+        s.submit_pairs(pairs)
+        s.flush()
+        labels, ncc, rounds, forest = st.query_cc(want_forest=True)
+        # a second query on the same sketches gives the same answer
+        labels2, _, rounds2 = st.query_cc()
+    net = net_edges(pairs)
+    assert np.array_equal(labels, olab)
+    assert rounds == orounds
+    assert np.array_equal(labels, orc.dsu_components(n, net))
+    assert np.array_equal(labels2, labels) and rounds2 == rounds
+    check_forest(n, forest, net, labels)
+
+
+def test_cc_config1_shape_reduced():
+    # the ER(8192, 0.25) config at n=1024 (same density, seeded order, 10% re-deleted)
+    n = 1024
+    pairs = gs.gen_erdos_renyi(n, 0.25, 0.1, 20261203)
+    with gs.SketchStore(n, 8) as st:
+        s = st.stream()
+        s.submit_pairs(pairs)
+        s.flush()
+        got = st.export_buckets()
+        labels, ncc, rounds = st.query_cc()
+    g = orc.Geometry(n)
+    ref = g.empty_store()
+    orc.apply_edges(ref, g, 8, pairs)
+    assert np.array_equal(got, ref)
+    assert ncc == 1
+    assert np.array_equal(labels, orc.dsu_components(n, net_edges(pairs)))
+
+
+def test_out_of_samples_reported():
+    n = 64
+    path = np.array([[i, i + 1] for i in range(n - 1)], dtype=np.uint32)
+    with gs.SketchStore(n, 5, num_samples=1) as st:
+        s = st.stream()
+        s.submit_pairs(path)
+        s.flush()
+        with pytest.raises(gs.OutOfSamplesError):
+            st.query_cc()
+
+
+def test_k_forests_and_planted_min_cut():
+    n = 400
+    pairs = gs.gen_planted_cut(n, 0.15, 3, 0.1, 17)
+    net = net_edges(pairs)
+    exact, _ = orc.stoer_wagner(n, net)
+    assert exact == 3
+    k = 5
+    with gs.SketchStore(n, 0xFACE) as st:
+        s = st.stream()
+        s.submit_pairs(pairs)
+        s.flush()
+        before = st.export_buckets()
+        forests = st.k_spanning_forests(k)
+        after = st.export_buckets()
+    assert np.array_equal(before, after)  # forests toggled back in
+    netset = set(map(tuple, net.tolist()))
+    seen = set()
+    for f in forests:
+        fs = set(map(tuple, f.tolist()))
+        assert fs <= netset and not (fs & seen)
+        seen |= fs
+    cert = np.concatenate(forests)
+    val, side = gs.min_cut(n, cert)
+    assert val == exact
+    assert orc.stoer_wagner(n, cert)[0] == exact
