@@ -42,7 +42,7 @@ class Geometry(ctypes.Structure):
 
 
 if not os.path.exists(LIB_PATH):
-    raise ImportError(f"{LIB_PATH} is not built; run `python -m gpusketch_amd.build` (hipcc, gfx950)")
+    raise ImportError(f"{LIB_PATH} is not built; run `python gpusketch_amd/build.py` (hipcc, gfx950)")
 
 lib = ctypes.CDLL(LIB_PATH)
 _u32, _u64, _i32, _vp, _dbl, _sz = (ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int,
@@ -75,6 +75,7 @@ _sigs = {
     "gs_memcpy_h2d": ([_vp, _vp, _sz], _i32),
     "gs_memcpy_d2h": ([_vp, _vp, _sz], _i32),
     "gs_device_synchronize": ([], _i32),
+    "gs_device_copy": ([_vp, _vp, _sz], _i32),
     "gs_gen_erdos_renyi": ([_u32, _dbl, _dbl, _u64, _vp, _vp], _i32),
     "gs_gen_rmat": ([_u32, _dbl, _dbl, _dbl, _u64, _dbl, _u64, _vp], _i32),
     "gs_gen_planted_cut": ([_u32, _dbl, _u32, _dbl, _u64, _vp, _vp], _i32),

@@ -2,7 +2,7 @@
 
 Compiles every HIP/C++ source under gpusketch_amd/csrc with hipcc into one shared library next
 to this file, and the C++ facade programs under gpusketch_amd/host into gpusketch_amd/bin.
-Run ``python -m gpusketch_amd.build`` from the repository root (or call ``build()``).
+Run ``python gpusketch_amd/build.py`` from the repository root (or call ``build()``).
 """
 from __future__ import annotations
 

@@ -123,6 +123,9 @@ int gs_store_create(uint32_t n, uint64_t seed, uint32_t num_samples, uint32_t co
   const uint32_t bpc = (uint32_t)g[1];
   const uint64_t cols = (uint64_t)num_samples * cols_per_sample;
   GS_CHECK(bpc < 64 && cols <= 4096, GS_ERR_INVALID, "gs_store_create: geometry out of range");
+  GS_CHECK(n <= (1u << 26), GS_ERR_UNSUPPORTED, "gs_store_create: at most 2^26 vertices per store");
+  GS_CHECK(cols <= 64, GS_ERR_UNSUPPORTED,
+           "gs_store_create: at most 64 columns (one wave64 lane per column) are supported");
   gs_store* h = new gs_store();
   Store& st = h->impl;
   st.num_samples = num_samples;
@@ -134,6 +137,7 @@ int gs_store_create(uint32_t n, uint64_t seed, uint32_t num_samples, uint32_t co
   st.view.cols = (uint32_t)cols;
   st.view.vstride = (uint32_t)(((uint64_t)bpc * cols + 7) & ~7ULL);
   st.view.seed = seed;
+  st.part = make_part_params(n);
   size_t rows_bytes = (size_t)n * st.view.vstride * sizeof(Bucket16);
   if (hipMalloc((void**)&st.view.rows, rows_bytes) != hipSuccess ||
       hipMalloc((void**)&st.view.det, (size_t)n * sizeof(Bucket16)) != hipSuccess) {
@@ -465,6 +469,10 @@ int gs_memcpy_h2d(void* dptr, const void* src, size_t bytes) {
 int gs_memcpy_d2h(void* dst, const void* dptr, size_t bytes) {
   GS_HIP(hipMemcpy(dst, dptr, bytes, hipMemcpyDeviceToHost));
   return GS_OK;
+}
+int gs_device_copy(void* dst, const void* src, size_t bytes) {
+  GS_CHECK(dst && src, GS_ERR_INVALID, "null argument");
+  return device_copy(dst, src, bytes);
 }
 int gs_device_synchronize(void) {
   GS_HIP(hipDeviceSynchronize());

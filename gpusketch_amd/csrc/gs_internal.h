@@ -127,19 +127,24 @@ constexpr uint32_t kItemExclusive = 0x80000000u;
 
 struct Store;
 
+// vertex -> bucket mapping of the edge-pair grouping: hashed = (v * mul) mod 2^nbits (a
+// bijection), bucket = hashed >> local_bits, local id = low local_bits bits
+struct PartParams {
+  uint32_t nbits, local_bits, nbuckets, mask, mul, inv;
+};
+
 // per-stream scratch for the edge-pair pipeline, grown on demand
 struct PairScratch {
-  uint64_t cap_edges = 0;       // capacity in undirected edges
-  uint32_t* d_pairs = nullptr;  // staged input pairs (2 x cap)
-  uint32_t* d_rank = nullptr;   // rank of each directed entry within its vertex (2 x cap)
-  uint32_t* d_sorted = nullptr; // destinations grouped by vertex (2 x cap)
+  uint64_t cap_edges = 0;        // capacity in undirected edges
+  uint2* d_entries = nullptr;    // directed entries (vertex, other) grouped by bucket (2 x cap)
+  uint32_t* d_sorted = nullptr;  // destinations grouped by vertex (2 x cap)
   uint64_t cap_items = 0;
   WorkItem* d_items = nullptr;
-  uint32_t cap_n = 0;
-  uint32_t* d_deg = nullptr;    // per-vertex count (zero between batches)
-  uint32_t* d_off = nullptr;    // per-vertex exclusive offset
-  uint64_t* d_tile_status = nullptr;  // decoupled look-back status
-  uint32_t* d_counters = nullptr;     // [0]=tile ticket, [1]=item count, [2]=entry count,
+  uint32_t cap_buckets = 0;
+  uint32_t* d_counts = nullptr;   // 2 x nbuckets (double-buffered per batch)
+  uint32_t* d_cursors = nullptr;  // nbuckets
+  uint32_t parity = 0;
+  uint32_t* d_counters = nullptr;     // [0] unused, [1]=item count, [2] unused,
                                       // [3]=dropped out-of-range entries since the last flush
 };
 
@@ -166,6 +171,7 @@ struct Stream {
 
 struct Store {
   StoreView view{};
+  PartParams part{};
   uint32_t num_samples = 0;
   uint32_t cols_per_sample = 1;
   uint32_t flags = 0;
@@ -193,8 +199,10 @@ int launch_vertex_batches(Stream* s, const uint32_t* d_src_unused, const WorkIte
 int ingest_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uint32_t min_depth,
                         uint32_t depth_cap);
 int ensure_pair_scratch(Stream* s, uint64_t nedges);
+PartParams make_part_params(uint32_t n);
 void free_pair_scratch(PairScratch& ws);
 int launch_zero_store(Store* st, hipStream_t stream);
+int device_copy(void* dst, const void* src, size_t bytes);
 int hash_known_answer(const uint64_t* keys, const uint64_t* seeds, uint64_t count, uint64_t* out);
 
 // launchers implemented in boruvka.hip

@@ -4,7 +4,7 @@
 //   * pre-grouped vertex batches (the contract of the reference's sketchUpdate kernels,
 //     src/cuda_kernel.cu:152-257): the host turns each batch into work items;
 //   * raw undirected pairs (edge batches): the device groups directed entries by vertex in three
-//     kernels (bin_rank, scan_items, scatter) and emits the work items itself.
+//     kernels (bucket_count, bucket_scatter, bucket_sort) and emits the work items itself.
 //
 // aggregate_kernel: one wave64 per work item (<= kChunk destinations of one vertex). For each
 // destination the wave hashes the edge index once for the checksum (XXH3 with the sketch seed)
@@ -20,38 +20,37 @@ namespace gs {
 constexpr int kWave = 64;
 constexpr int kAggWaves = 4;                  // waves per aggregation workgroup
 constexpr uint32_t kChunk = 256;              // max destinations per work item
-constexpr uint32_t kSmallItem = 0;            // items this short skip the LDS delta (0 = never)
-constexpr int kBinThreads = 256;
-constexpr int kBinEdgesPerThread = 4;         // 1024 edges = 2048 entries per bin workgroup
-constexpr int kBinHash = 4096;                // LDS hash slots (> entries per workgroup)
-constexpr int kScanThreads = 256;
-constexpr int kScanPerThread = 8;             // 2048 vertices per scan tile
+constexpr int kPartThreads = 512;             // K1/K2 workgroup size
+constexpr int kPartEdges = 4096;              // undirected edges per K1/K2 workgroup
+constexpr int kSortThreads = 1024;            // K3 workgroup size (one workgroup per bucket)
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
-constexpr uint32_t kNoRank = 0xFFFFFFFFu;
-constexpr uint64_t kFlagAgg = 1ULL << 62;
-constexpr uint64_t kFlagPrefix = 2ULL << 62;
-constexpr uint64_t kValueMask = (1ULL << 62) - 1;
-constexpr int kEntryBits = 34;                // packed scan value: items << 34 | entries
 
+// Orders this wave's LDS accesses across lanes: a wave's LDS instructions execute in program
+// order, so only the compiler has to be kept from moving accesses across this point (a memory
+// fence would also wait for the wave's outstanding global loads).
 __device__ __forceinline__ void wave_sync_lds() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  asm volatile("" ::: "memory");
 }
 
 __device__ __forceinline__ void atomic_xor_u64(uint64_t* p, uint64_t v) {
   __hip_atomic_fetch_xor(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// XOR of a 32-bit value over the 64 lanes with DPP (quad swaps, half-row and row mirrors, then
+// row broadcasts); the total ends in lane 63 and is returned as a uniform value.
+__device__ __forceinline__ uint32_t wave_xor_u32(uint32_t v) {
+  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
+  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);   // quad_perm 2,3,0,1
+  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false);  // row_mirror
+  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast15
+  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast31
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 __device__ __forceinline__ uint64_t wave_xor_reduce(uint64_t x) {
-  // butterfly over the 64 lanes with ds_swizzle-free shuffles
-  for (int off = 32; off >= 1; off >>= 1) {
-    uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
-    lo ^= (uint32_t)__shfl_xor((int)lo, off, kWave);
-    hi ^= (uint32_t)__shfl_xor((int)hi, off, kWave);
-    x = ((uint64_t)hi << 32) | lo;
-  }
-  return x;
+  return ((uint64_t)wave_xor_u32((uint32_t)(x >> 32)) << 32) | wave_xor_u32((uint32_t)x);
 }
 
 // subgraph filter of the min-cut path (src/mc_gpu_sketch_alg.cpp:139-178): an edge belongs to
@@ -75,181 +74,214 @@ __global__ void fill_u64_kernel(uint64_t* p, uint64_t n, uint64_t v) {
 }
 
 // ------------------------------------------------------------------------------------------
-// K1: rank every directed entry within its vertex. A workgroup aggregates its 2048 entries per
-// vertex in an LDS hash table first, so a hub vertex costs one global atomic per workgroup
-// instead of one per entry. Workgroup 0 also resets the scan state used by K2.
+// Vertex grouping for raw pairs, in two levels with few global atomics:
+//   K1 bucket_count  : directed entries per bucket of hashed vertex ids (LDS histogram, one
+//                      non-returning global add per workgroup and bucket);
+//   K2 bucket_scatter: entries written into their bucket's region (LDS rank + one returning
+//                      global add per workgroup and bucket);
+//   K3 bucket_sort   : one workgroup per bucket sorts its entries by vertex (LDS counting sort
+//                      over the bucket's local ids) and emits the work items.
+// Vertex ids are spread over buckets by a bijective multiplicative hash on nbits bits, so the
+// skewed bit patterns of R-MAT ids do not pile up in one bucket.
 // ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kBinThreads) void bin_rank_kernel(
-    const uint32_t* __restrict__ pairs, uint64_t nedges, uint32_t n, uint32_t* __restrict__ deg,
-    uint32_t* __restrict__ rank, uint64_t seed, uint32_t min_depth, uint32_t depth_cap,
-    uint64_t* __restrict__ tile_status, uint32_t ntiles, uint32_t* __restrict__ counters) {
-  __shared__ uint32_t hkey[kBinHash];
-  __shared__ uint32_t hcnt[kBinHash];
+__device__ __forceinline__ uint32_t hashed_id(const PartParams& pp, uint32_t v) {
+  return (v * pp.mul) & pp.mask;
+}
+__device__ __forceinline__ uint32_t bucket_of(const PartParams& pp, uint32_t v) {
+  return hashed_id(pp, v) >> pp.local_bits;
+}
+__device__ __forceinline__ bool entry_ok(uint2 uv, uint32_t n, uint64_t seed, uint32_t min_depth,
+                                         uint32_t depth_cap) {
+  if (uv.x >= n || uv.y >= n || uv.x == uv.y) return false;
+  return edge_selected(concat_pair(uv.x, uv.y), seed, min_depth, depth_cap);
+}
+
+__global__ __launch_bounds__(kPartThreads) void bucket_count_kernel(
+    const uint32_t* __restrict__ pairs, uint64_t nedges, uint32_t n, PartParams pp, uint64_t seed,
+    uint32_t min_depth, uint32_t depth_cap, uint32_t* __restrict__ counts_cur,
+    uint32_t* __restrict__ counts_next, uint32_t* __restrict__ cursors,
+    uint32_t* __restrict__ counters) {
+  extern __shared__ uint32_t hist[];
   const uint32_t t = threadIdx.x;
   if (blockIdx.x == 0) {
-    for (uint32_t i = t; i < ntiles; i += kBinThreads) tile_status[i] = 0;
+    for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads) { counts_next[i] = 0; cursors[i] = 0; }
     if (t == 0) { counters[0] = 0; counters[1] = 0; counters[2] = 0; }
   }
-  for (uint32_t i = t; i < kBinHash; i += kBinThreads) { hkey[i] = kEmpty; hcnt[i] = 0; }
+  for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads) hist[i] = 0;
   __syncthreads();
-
-  const uint64_t e0 = (uint64_t)blockIdx.x * (kBinThreads * kBinEdgesPerThread);
-  uint32_t slot[2 * kBinEdgesPerThread];
-  uint32_t lrank[2 * kBinEdgesPerThread];
+  const uint64_t e0 = (uint64_t)blockIdx.x * kPartEdges;
 #pragma unroll
-  for (int k = 0; k < kBinEdgesPerThread; k++) {
-    uint64_t e = e0 + t + (uint64_t)k * kBinThreads;
-    slot[2 * k] = slot[2 * k + 1] = kEmpty;
+  for (int k = 0; k < kPartEdges / kPartThreads; k++) {
+    uint64_t e = e0 + t + (uint64_t)k * kPartThreads;
     if (e >= nedges) continue;
     uint2 uv = reinterpret_cast<const uint2*>(pairs)[e];
     if (uv.x >= n || uv.y >= n) {  // never index past the store: drop and report at flush
       atomicAdd(&counters[3], 1u);
       continue;
     }
-    if (uv.x == uv.y) continue;  // a self loop toggles the same index twice: no effect
-    if (!edge_selected(concat_pair(uv.x, uv.y), seed, min_depth, depth_cap)) continue;
-    uint32_t ends[2] = {uv.x, uv.y};
-#pragma unroll
-    for (int s = 0; s < 2; s++) {
-      uint32_t v = ends[s];
-      uint32_t h = (v * 0x9E3779B1u) >> 20;  // 12-bit slot
-      for (;;) {
-        uint32_t k0 = hkey[h];
-        if (k0 == v) break;
-        if (k0 == kEmpty) {
-          uint32_t old = atomicCAS(&hkey[h], kEmpty, v);
-          if (old == kEmpty || old == v) break;
-        }
-        h = (h + 1) & (kBinHash - 1);
-      }
-      slot[2 * k + s] = h;
-      lrank[2 * k + s] = atomicAdd(&hcnt[h], 1u);
-    }
+    if (!entry_ok(uv, n, seed, min_depth, depth_cap)) continue;
+    atomicAdd(&hist[bucket_of(pp, uv.x)], 1u);
+    atomicAdd(&hist[bucket_of(pp, uv.y)], 1u);
   }
   __syncthreads();
-  for (uint32_t i = t; i < kBinHash; i += kBinThreads) {
-    uint32_t key = hkey[i];
-    if (key != kEmpty) hcnt[i] = atomicAdd(&deg[key], hcnt[i]);
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < kBinEdgesPerThread; k++) {
-    uint64_t e = e0 + t + (uint64_t)k * kBinThreads;
-    if (e >= nedges) continue;
-    uint2 r;
-    r.x = slot[2 * k] == kEmpty ? kNoRank : hcnt[slot[2 * k]] + lrank[2 * k];
-    r.y = slot[2 * k + 1] == kEmpty ? kNoRank : hcnt[slot[2 * k + 1]] + lrank[2 * k + 1];
-    reinterpret_cast<uint2*>(rank)[e] = r;
-  }
+  for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads)
+    if (hist[i]) __hip_atomic_fetch_add(&counts_cur[i], hist[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// ------------------------------------------------------------------------------------------
-// K2: single-pass exclusive scan of the per-vertex counts (decoupled look-back over tiles of
-// 2048 vertices) producing each vertex's offset and its work items; resets the counts.
-// The scanned value packs (items << 34) | entries.
-// ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kScanThreads) void scan_items_kernel(
-    uint32_t n, uint32_t* __restrict__ deg, uint32_t* __restrict__ off,
-    WorkItem* __restrict__ items, uint32_t chunk, uint64_t* tile_status, uint32_t ntiles,
-    uint32_t* counters) {
-  __shared__ uint32_t s_tile;
-  __shared__ uint64_t s_scan[kScanThreads];
-  __shared__ uint64_t s_excl;
+// exclusive prefix of counts[0..nb) into s_off (LDS), returns the total; blockDim threads
+__device__ uint32_t block_exclusive_scan(const uint32_t* counts, uint32_t nb, uint32_t* s_off,
+                                         uint32_t* s_part) {
+  const uint32_t t = threadIdx.x, nt = blockDim.x;
+  const uint32_t per = (nb + nt - 1) / nt;
+  const uint32_t b0 = min(nb, t * per), b1 = min(nb, b0 + per);
+  uint32_t sum = 0;
+  for (uint32_t b = b0; b < b1; b++) { s_off[b] = sum; sum += counts[b]; }
+  s_part[t] = sum;
+  __syncthreads();
+  for (uint32_t o = 1; o < nt; o <<= 1) {
+    uint32_t add = t >= o ? s_part[t - o] : 0;
+    __syncthreads();
+    s_part[t] += add;
+    __syncthreads();
+  }
+  uint32_t base = t ? s_part[t - 1] : 0;
+  for (uint32_t b = b0; b < b1; b++) s_off[b] += base;
+  uint32_t total = s_part[nt - 1];
+  __syncthreads();
+  return total;
+}
+
+__global__ __launch_bounds__(kPartThreads) void bucket_scatter_kernel(
+    const uint32_t* __restrict__ pairs, uint64_t nedges, uint32_t n, PartParams pp, uint64_t seed,
+    uint32_t min_depth, uint32_t depth_cap, const uint32_t* __restrict__ counts,
+    uint32_t* __restrict__ cursors, uint2* __restrict__ entries) {
+  extern __shared__ uint32_t lds_u32[];
+  uint32_t* s_off = lds_u32;                   // nbuckets: region start of every bucket
+  uint32_t* s_hist = lds_u32 + pp.nbuckets;    // nbuckets: this block's count, then its base
+  __shared__ uint32_t s_part[kPartThreads];
   const uint32_t t = threadIdx.x;
-  if (t == 0) s_tile = atomicAdd(&counters[0], 1u);
-  __syncthreads();
-  const uint32_t tile = s_tile;
-  const uint64_t v0 = (uint64_t)tile * (kScanThreads * kScanPerThread) + (uint64_t)t * kScanPerThread;
+  for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads) s_hist[i] = 0;
+  block_exclusive_scan(counts, pp.nbuckets, s_off, s_part);
 
-  uint32_t d[kScanPerThread];
-  uint64_t mine = 0;
+  const uint64_t e0 = (uint64_t)blockIdx.x * kPartEdges;
+  constexpr int K = kPartEdges / kPartThreads;
+  uint32_t bk[2 * K], rk[2 * K];
+  uint2 uvs[K];
 #pragma unroll
-  for (int k = 0; k < kScanPerThread; k++) {
-    uint64_t v = v0 + k;
-    d[k] = v < n ? deg[v] : 0;
-    uint64_t it = d[k] ? (d[k] + chunk - 1) / chunk : 0;
-    mine += (it << kEntryBits) | d[k];
-  }
-  // block-wide inclusive scan (Hillis-Steele in LDS)
-  s_scan[t] = mine;
-  __syncthreads();
-  for (uint32_t o = 1; o < kScanThreads; o <<= 1) {
-    uint64_t add = t >= o ? s_scan[t - o] : 0;
-    __syncthreads();
-    s_scan[t] += add;
-    __syncthreads();
-  }
-  const uint64_t agg = s_scan[kScanThreads - 1];
-  if (t == 0) {
-    uint64_t excl = 0;
-    if (tile == 0) {
-      __hip_atomic_store(&tile_status[0], kFlagPrefix | agg, __ATOMIC_RELEASE,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      __hip_atomic_store(&tile_status[tile], kFlagAgg | agg, __ATOMIC_RELEASE,
-                         __HIP_MEMORY_SCOPE_AGENT);
-      int64_t j = (int64_t)tile - 1;
-      for (;;) {
-        uint64_t st;
-        do {
-          st = __hip_atomic_load(&tile_status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        } while ((st >> 62) == 0);
-        excl += st & kValueMask;
-        if ((st & kFlagPrefix) != 0) break;
-        j--;
-      }
-      __hip_atomic_store(&tile_status[tile], kFlagPrefix | (excl + agg), __ATOMIC_RELEASE,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    }
-    s_excl = excl;
-    if (tile == ntiles - 1) {
-      uint64_t total = excl + agg;
-      counters[1] = (uint32_t)(total >> kEntryBits);
-      counters[2] = (uint32_t)(total & ((1ULL << kEntryBits) - 1));
-    }
+  for (int k = 0; k < K; k++) {
+    uint64_t e = e0 + t + (uint64_t)k * kPartThreads;
+    bk[2 * k] = bk[2 * k + 1] = kEmpty;
+    uvs[k] = make_uint2(0, 0);
+    if (e >= nedges) continue;
+    uint2 uv = reinterpret_cast<const uint2*>(pairs)[e];
+    if (!entry_ok(uv, n, seed, min_depth, depth_cap)) continue;
+    uvs[k] = uv;
+    bk[2 * k] = bucket_of(pp, uv.x);
+    bk[2 * k + 1] = bucket_of(pp, uv.y);
+    rk[2 * k] = atomicAdd(&s_hist[bk[2 * k]], 1u);
+    rk[2 * k + 1] = atomicAdd(&s_hist[bk[2 * k + 1]], 1u);
   }
   __syncthreads();
-  uint64_t run = s_excl + s_scan[t] - mine;
-  uint64_t e_run = run & ((1ULL << kEntryBits) - 1);
-  uint64_t i_run = run >> kEntryBits;
+  for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads) {
+    uint32_t c = s_hist[i];
+    if (c) s_hist[i] = s_off[i] + __hip_atomic_fetch_add(&cursors[i], c, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
 #pragma unroll
-  for (int k = 0; k < kScanPerThread; k++) {
-    uint64_t v = v0 + k;
-    if (v >= n) break;
-    off[v] = (uint32_t)e_run;
-    uint32_t dv = d[k];
-    if (dv) {
-      deg[v] = 0;
-      uint32_t nit = (dv + chunk - 1) / chunk;
-      for (uint32_t q = 0; q < nit; q++) {
-        WorkItem w;
-        w.vertex = (uint32_t)v;
-        uint32_t len = min(chunk, dv - q * chunk);
-        w.len = len | (nit == 1 ? kItemExclusive : 0u);
-        w.start = e_run + (uint64_t)q * chunk;
-        items[i_run + q] = w;
-      }
-      i_run += nit;
-      e_run += dv;
-    }
+  for (int k = 0; k < K; k++) {
+    if (bk[2 * k] == kEmpty) continue;
+    const uint2 uv = uvs[k];
+    entries[s_hist[bk[2 * k]] + rk[2 * k]] = uv;
+    entries[s_hist[bk[2 * k + 1]] + rk[2 * k + 1]] = make_uint2(uv.y, uv.x);
   }
 }
 
-// ------------------------------------------------------------------------------------------
-// K3: place every directed entry at off[vertex] + rank.
-// ------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void scatter_kernel(const uint32_t* __restrict__ pairs,
-                                                      uint64_t nedges,
-                                                      const uint32_t* __restrict__ rank,
-                                                      const uint32_t* __restrict__ off,
-                                                      uint32_t* __restrict__ sorted) {
-  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < nedges;
-       e += (uint64_t)gridDim.x * blockDim.x) {
-    uint2 r = reinterpret_cast<const uint2*>(rank)[e];
-    if (r.x == kNoRank) continue;
-    uint2 uv = reinterpret_cast<const uint2*>(pairs)[e];
-    sorted[(uint64_t)off[uv.x] + r.x] = uv.y;
-    sorted[(uint64_t)off[uv.y] + r.y] = uv.x;
+__global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
+    const uint2* __restrict__ entries, PartParams pp, const uint32_t* __restrict__ counts,
+    uint32_t* __restrict__ sorted, WorkItem* __restrict__ items, uint32_t chunk,
+    uint32_t* __restrict__ counters) {
+  extern __shared__ uint32_t lds_u32[];
+  const uint32_t bins = 1u << pp.local_bits;
+  uint32_t* s_cnt = lds_u32;              // per local id: count, later the insertion cursor
+  uint32_t* s_off = lds_u32 + bins;       // per local id: offset inside the bucket
+  uint32_t* s_ioff = lds_u32 + 2 * bins;  // per local id: first item
+  __shared__ uint32_t s_part[kSortThreads];
+  __shared__ uint32_t s_base, s_ibase;
+  const uint32_t t = threadIdx.x, b = blockIdx.x;
+  // region start of this bucket = sum of the counts of the buckets before it
+  uint32_t acc = 0;
+  for (uint32_t i = t; i < b; i += kSortThreads) acc += counts[i];
+  s_part[t] = acc;
+  for (uint32_t i = t; i < bins; i += kSortThreads) s_cnt[i] = 0;
+  __syncthreads();
+  for (uint32_t o = kSortThreads / 2; o > 0; o >>= 1) {
+    if (t < o) s_part[t] += s_part[t + o];
+    __syncthreads();
+  }
+  const uint32_t base = s_part[0];
+  const uint32_t cnt = counts[b];
+  __syncthreads();
+  const uint32_t lmask = bins - 1;
+  for (uint32_t i = t; i < cnt; i += kSortThreads)
+    atomicAdd(&s_cnt[hashed_id(pp, entries[base + i].x) & lmask], 1u);
+  __syncthreads();
+  // offsets (entries) and first items per local id, in one pass over per-thread ranges
+  const uint32_t per = (bins + kSortThreads - 1) / kSortThreads;
+  const uint32_t l0 = min(bins, t * per), l1 = min(bins, l0 + per);
+  uint32_t se = 0, si = 0;
+  for (uint32_t l = l0; l < l1; l++) {
+    uint32_t d = s_cnt[l];
+    s_off[l] = se;
+    s_ioff[l] = si;
+    se += d;
+    si += d ? (d + chunk - 1) / chunk : 0;
+  }
+  __syncthreads();
+  s_part[t] = si;
+  __syncthreads();
+  for (uint32_t o = 1; o < kSortThreads; o <<= 1) {
+    uint32_t add = t >= o ? s_part[t - o] : 0;
+    __syncthreads();
+    s_part[t] += add;
+    __syncthreads();
+  }
+  const uint32_t ibefore = t ? s_part[t - 1] : 0;
+  if (t == kSortThreads - 1) s_ibase = atomicAdd(&counters[1], s_part[t]);
+  // entry offsets need the same prefix over `se`
+  __syncthreads();
+  s_part[t] = se;
+  __syncthreads();
+  for (uint32_t o = 1; o < kSortThreads; o <<= 1) {
+    uint32_t add = t >= o ? s_part[t - o] : 0;
+    __syncthreads();
+    s_part[t] += add;
+    __syncthreads();
+  }
+  const uint32_t ebefore = t ? s_part[t - 1] : 0;
+  __syncthreads();
+  const uint32_t ibase = s_ibase;
+  for (uint32_t l = l0; l < l1; l++) {
+    uint32_t d = s_cnt[l];
+    uint32_t eo = base + ebefore + s_off[l];
+    s_off[l] = ebefore + s_off[l];
+    if (!d) continue;
+    uint32_t v = (((b << pp.local_bits) | l) * pp.inv) & pp.mask;
+    uint32_t nit = (d + chunk - 1) / chunk;
+    uint32_t io = ibase + ibefore + s_ioff[l];
+    for (uint32_t q = 0; q < nit; q++) {
+      WorkItem w;
+      w.vertex = v;
+      w.len = min(chunk, d - q * chunk) | (nit == 1 ? kItemExclusive : 0u);
+      w.start = (uint64_t)eo + (uint64_t)q * chunk;
+      items[io + q] = w;
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = t; i < cnt; i += kSortThreads) {
+    uint2 e = entries[base + i];
+    uint32_t pos = atomicAdd(&s_off[hashed_id(pp, e.x) & lmask], 1u);
+    sorted[base + pos] = e.y;
   }
 }
 
@@ -257,8 +289,17 @@ __global__ __launch_bounds__(256) void scatter_kernel(const uint32_t* __restrict
 // K4: aggregation (see the file comment). Dynamic LDS per wave: delta[dl * cols] buckets +
 // stage[64] x {idx, checksum, mix(swap(idx))}; shared table of per-column seed keys.
 // ------------------------------------------------------------------------------------------
-struct StageEntry {
-  uint64_t idx, chk, mixed;
+// Lane mapping: the (destination j, column c) pairs of a chunk are flattened, pair i = j * cols + c
+// going to lane i mod 64 (two pairs per lane per iteration), so every lane is busy whatever the
+// item length.
+//
+// Per-wave LDS (64-bit words): the delta as structure of arrays (alpha[dslots], gamma[dslots],
+// slot = depth * cols + column, depth < dl), the chunk's stage (edge index, checksum,
+// mix(swap(idx)), 64 each), the per-column seed keys, and the compacted flush list (u32 slots).
+struct AggLayout {
+  uint32_t cols, dl, dslots;
+  __host__ __device__ AggLayout(uint32_t cols_, uint32_t dl_) : cols(cols_), dl(dl_), dslots(dl_ * cols_) {}
+  __host__ __device__ uint32_t wave_words() const { return 2 * dslots + 3 * 64 + cols + (dslots + 1) / 2; }
 };
 
 __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
@@ -266,20 +307,23 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
     const uint32_t* __restrict__ nitems_dev, const uint32_t* __restrict__ dsts, uint32_t dl,
     int exclusive_ok) {
   extern __shared__ __align__(16) unsigned char lds[];
+  const AggLayout L(st.cols, dl);
   const uint32_t cols = st.cols;
   const uint32_t bpc = st.bpc;
-  const uint32_t dslots = dl * cols;
-  uint64_t* colkey = reinterpret_cast<uint64_t*>(lds);
-  unsigned char* wave_base = lds + ((cols * 8 + 15) & ~15u);
-  const uint32_t wave_bytes = dslots * 16 + kWave * sizeof(StageEntry);
+  const uint32_t dslots = L.dslots;
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t wid = threadIdx.x / kWave;
-  Bucket16* delta = reinterpret_cast<Bucket16*>(wave_base + wid * wave_bytes);
-  StageEntry* stage = reinterpret_cast<StageEntry*>(wave_base + wid * wave_bytes + dslots * 16);
+  uint64_t* da = reinterpret_cast<uint64_t*>(lds) + (size_t)wid * L.wave_words();
+  uint64_t* dg = da + dslots;
+  uint64_t* sidx = dg + dslots;
+  uint64_t* schk = sidx + kWave;
+  uint64_t* smix = schk + kWave;
+  uint64_t* colkey = smix + kWave;
+  uint32_t* clist = reinterpret_cast<uint32_t*>(colkey + cols);
 
-  for (uint32_t c = threadIdx.x; c < cols; c += blockDim.x) colkey[c] = xxh3_seed_key(st.seed + 5ULL * c);
-  for (uint32_t s = lane; s < dslots; s += kWave) delta[s] = Bucket16{0, 0};
-  __syncthreads();
+  for (uint32_t c = lane; c < cols; c += kWave) colkey[c] = xxh3_seed_key(st.seed + 5ULL * c);
+  for (uint32_t e = lane; e < dslots; e += kWave) { da[e] = 0; dg[e] = 0; }
+  wave_sync_lds();
 
   const uint32_t nitems = nitems_dev ? *nitems_dev : nitems_host;
   const uint32_t gw = blockIdx.x * kAggWaves + wid;
@@ -287,8 +331,15 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
   const uint64_t seed = st.seed;
   const uint32_t q64 = kWave / cols, r64 = kWave % cols;  // 64 = q*cols + r
 
+  // software pipeline: the next item and its first chunk of destinations are loaded while the
+  // current item is hashed and flushed
+  WorkItem wnext = gw < nitems ? items[gw] : WorkItem{0, 0, 0};
+  uint32_t first_next = (gw < nitems && lane < (wnext.len & ~kItemExclusive)) ? dsts[wnext.start + lane] : 0;
   for (uint32_t it = gw; it < nitems; it += nw) {
-    const WorkItem w = items[it];
+    const WorkItem w = wnext;
+    const uint32_t first = first_next;
+    const bool has_next = it + nw < nitems;
+    if (has_next) wnext = items[it + nw];
     const uint32_t v = __builtin_amdgcn_readfirstlane(w.vertex);
     const uint32_t len = __builtin_amdgcn_readfirstlane(w.len & ~kItemExclusive);
     const bool excl = exclusive_ok && (w.len & kItemExclusive);
@@ -299,63 +350,96 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
     for (uint32_t j0 = 0; j0 < len; j0 += kWave) {
       const uint32_t cnt = min((uint32_t)kWave, len - j0);
       if (lane < cnt) {
-        uint32_t other = dsts[start + j0 + lane];
+        uint32_t other = j0 == 0 ? first : dsts[start + j0 + lane];
         uint64_t idx = concat_pair(v, other);
         uint64_t chk = xxh3_len8(idx, seed);
-        stage[lane] = StageEntry{idx, chk, xxh3_mix(xxh3_swap(idx))};
+        sidx[lane] = idx;
+        schk[lane] = chk;
+        smix[lane] = xxh3_mix(xxh3_swap(idx));
         det_a ^= idx;
         det_g ^= chk;
       }
       wave_sync_lds();
       const uint32_t nwork = cnt * cols;
       uint32_t j = lane / cols, c = lane - j * cols;
-      for (uint32_t i = lane; i < nwork; i += kWave) {
-        const StageEntry e = stage[j];
-        const uint64_t h = xxh3_finish(e.mixed ^ colkey[c]);
-        const uint32_t d = depth_of(h, bpc);
-        if (d < dl) {
-          Bucket16* b = &delta[d * cols + c];
-          atomicXor(reinterpret_cast<unsigned long long*>(&b->alpha), (unsigned long long)e.idx);
-          atomicXor(reinterpret_cast<unsigned long long*>(&b->gamma), (unsigned long long)e.chk);
-        } else if (d < bpc) {
-          Bucket16* b = &vrows[d * cols + c];
-          atomic_xor_u64(&b->alpha, e.idx);
-          atomic_xor_u64(&b->gamma, e.chk);
+      // two independent (entry, column) hashes per lane per iteration for instruction-level
+      // parallelism; the LDS operands are read before the multiply chains start
+      for (uint32_t i = lane; i < nwork; i += 2 * kWave) {
+        uint32_t j2 = j + q64, c2 = c + r64;
+        if (c2 >= cols) { c2 -= cols; j2++; }
+        const bool second = i + kWave < nwork;
+        const uint32_t j2c = min(j2, (uint32_t)kWave - 1);
+        const uint64_t a1 = sidx[j], g1 = schk[j], m1 = smix[j], k1 = colkey[c];
+        const uint64_t a2 = sidx[j2c], g2 = schk[j2c], m2 = smix[j2c], k2 = colkey[c2];
+        const uint32_t d1 = depth_of(xxh3_finish(m1 ^ k1), bpc);
+        const uint32_t d2 = depth_of(xxh3_finish(m2 ^ k2), bpc);
+        if (d1 < dl) {
+          const uint32_t slot = d1 * cols + c;
+          atomicXor(reinterpret_cast<unsigned long long*>(&da[slot]), (unsigned long long)a1);
+          atomicXor(reinterpret_cast<unsigned long long*>(&dg[slot]), (unsigned long long)g1);
         }
-        j += q64;
-        c += r64;
+        if (second && d2 < dl) {
+          const uint32_t slot = d2 * cols + c2;
+          atomicXor(reinterpret_cast<unsigned long long*>(&da[slot]), (unsigned long long)a2);
+          atomicXor(reinterpret_cast<unsigned long long*>(&dg[slot]), (unsigned long long)g2);
+        }
+        if ((d1 >= dl && d1 < bpc) || (second && d2 >= dl && d2 < bpc)) {  // rare: deep buckets
+          if (d1 >= dl && d1 < bpc) {
+            Bucket16* b = &vrows[d1 * cols + c];
+            atomic_xor_u64(&b->alpha, a1);
+            atomic_xor_u64(&b->gamma, g1);
+          }
+          if (second && d2 >= dl && d2 < bpc) {
+            Bucket16* b = &vrows[d2 * cols + c2];
+            atomic_xor_u64(&b->alpha, a2);
+            atomic_xor_u64(&b->gamma, g2);
+          }
+        }
+        j = j2 + q64;
+        c = c2 + r64;
         if (c >= cols) { c -= cols; j++; }
       }
       wave_sync_lds();
+    }
+    if (has_next) {
+      const uint32_t nlen = wnext.len & ~kItemExclusive;
+      first_next = lane < nlen ? dsts[wnext.start + lane] : 0;
     }
     det_a = wave_xor_reduce(det_a);
     det_g = wave_xor_reduce(det_g);
     if (lane == 0 && (det_a | det_g)) {
       Bucket16* b = &st.det[v];
-      if (excl) {
-        Bucket16 cur = *b;
-        cur.alpha ^= det_a;
-        cur.gamma ^= det_g;
-        *b = cur;
-      } else {
-        atomic_xor_u64(&b->alpha, det_a);
-        atomic_xor_u64(&b->gamma, det_g);
-      }
+      atomic_xor_u64(&b->alpha, det_a);
+      atomic_xor_u64(&b->gamma, det_g);
     }
-    for (uint32_t s = lane; s < dslots; s += kWave) {
-      Bucket16 x = delta[s];
-      if (x.alpha | x.gamma) {
-        Bucket16* g = &vrows[s];
+    // flush: compact the non-zero delta slots into a list (ballot + mbcnt), then every lane takes
+    // one list entry per round, so a round's HBM reads are issued together without branches
+    uint32_t nnz = 0;
+    for (uint32_t s0 = 0; s0 < dslots; s0 += kWave) {
+      const uint32_t sl = s0 + lane;
+      const bool nz = sl < dslots && (da[sl] | dg[sl]) != 0;
+      const uint64_t mask = __ballot(nz);
+      if (nz) {
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+        clist[nnz + below] = sl;
+      }
+      nnz += (uint32_t)__popcll(mask);
+    }
+    wave_sync_lds();
+    for (uint32_t r0 = 0; r0 < nnz; r0 += kWave) {
+      if (r0 + lane < nnz) {
+        const uint32_t sl = clist[r0 + lane];
+        const uint64_t xa = da[sl], xg = dg[sl];
+        da[sl] = 0;
+        dg[sl] = 0;
         if (excl) {
-          Bucket16 cur = *g;
-          cur.alpha ^= x.alpha;
-          cur.gamma ^= x.gamma;
-          *g = cur;
+          const Bucket16 cur = vrows[sl];
+          vrows[sl] = Bucket16{cur.alpha ^ xa, cur.gamma ^ xg};
         } else {
-          atomic_xor_u64(&g->alpha, x.alpha);
-          atomic_xor_u64(&g->gamma, x.gamma);
+          atomic_xor_u64(&vrows[sl].alpha, xa);
+          atomic_xor_u64(&vrows[sl].gamma, xg);
         }
-        delta[s] = Bucket16{0, 0};
       }
     }
     wave_sync_lds();
@@ -365,21 +449,49 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
 // ------------------------------------------------------------------------------------------
 // launch helpers
 // ------------------------------------------------------------------------------------------
-static uint32_t lds_window(const StoreView& v) { return v.bpc < 16 ? v.bpc : 16; }
+// depths kept in the LDS delta; deeper updates (probability 2^-12 per column) go to HBM directly
+static uint32_t lds_window(const StoreView& v) { return v.bpc < 12 ? v.bpc : 12; }
 
 static size_t agg_lds_bytes(const StoreView& v) {
-  uint32_t dl = lds_window(v);
-  size_t wave_bytes = (size_t)dl * v.cols * 16 + kWave * sizeof(StageEntry);
-  return ((v.cols * 8 + 15) & ~15u) + kAggWaves * wave_bytes;
+  const AggLayout L(v.cols, lds_window(v));
+  return (size_t)kAggWaves * L.wave_words() * sizeof(uint64_t);
 }
 
-static int agg_grid(int device) {
+// persistent grid: as many workgroups as can be resident at once (the occupancy calculator
+// accounts for LDS and registers), so the static round-robin over work items never waits on a
+// late-starting workgroup
+static int agg_grid(int device, const void* kernel, size_t lds_bytes) {
   static int cu = 0;
   if (cu == 0) {
     if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cu <= 0)
       cu = 256;
   }
-  return cu * 8;  // persistent: 8 workgroups per CU (LDS-limited occupancy decides residency)
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kAggWaves * kWave, lds_bytes) !=
+          hipSuccess ||
+      per_cu <= 0) {
+    (void)hipGetLastError();
+    per_cu = 1;
+  }
+  return cu * per_cu;
+}
+
+// Dynamic LDS above the default 64 KB cap needs an explicit per-kernel opt-in; request exactly
+// what a launch needs (static + dynamic must fit the CU's LDS) and report a refusal as an error.
+static int ensure_dyn_lds(const void* kernel, size_t bytes) {
+  static std::mutex mu;
+  static std::vector<std::pair<const void*, size_t>> granted;
+  if (bytes <= 64 * 1024) return GS_OK;
+  std::lock_guard<std::mutex> lk(mu);
+  for (auto& g : granted)
+    if (g.first == kernel && g.second >= bytes) return GS_OK;
+  hipError_t e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(GS_ERR_UNSUPPORTED, "kernel needs %zu bytes of LDS: %s", bytes, hipGetErrorString(e));
+  }
+  granted.emplace_back(kernel, bytes);
+  return GS_OK;
 }
 
 static int launch_aggregate(Stream* s, const WorkItem* d_items, uint32_t nitems_host,
@@ -387,12 +499,11 @@ static int launch_aggregate(Stream* s, const WorkItem* d_items, uint32_t nitems_
   Store* st = s->store;
   const StoreView& v = st->view;
   size_t lds = agg_lds_bytes(v);
+  int rc_lds;
   GS_CHECK(lds <= 160 * 1024, GS_ERR_UNSUPPORTED, "sketch too wide for the LDS window (%zu B)", lds);
-  static std::once_flag once;
-  std::call_once(once, [] {
-    (void)hipFuncSetAttribute((const void*)aggregate_kernel,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  });
+  const void* kern = (const void*)aggregate_kernel;
+  rc_lds = ensure_dyn_lds(kern, lds);
+  if (rc_lds) return rc_lds;
   int exclusive_ok = 1;
   {
     std::lock_guard<std::mutex> lk(st->mu);
@@ -404,9 +515,9 @@ static int launch_aggregate(Stream* s, const WorkItem* d_items, uint32_t nitems_
     GS_HIP(hipEventCreate(&e1));
     GS_HIP(hipEventRecord(e0, s->stream));
   }
-  hipLaunchKernelGGL(aggregate_kernel, dim3(agg_grid(st->device)), dim3(kAggWaves * kWave), lds,
-                     s->stream, v, d_items, nitems_host, nitems_dev, d_dsts, lds_window(v),
-                     exclusive_ok);
+  hipLaunchKernelGGL(aggregate_kernel, dim3(agg_grid(st->device, kern, lds)),
+                     dim3(kAggWaves * kWave), lds, s->stream, v, d_items, nitems_host, nitems_dev,
+                     d_dsts, lds_window(v), exclusive_ok);
   GS_HIP(hipGetLastError());
   if (s->profile) {
     GS_HIP(hipEventRecord(e1, s->stream));
@@ -423,53 +534,59 @@ int launch_vertex_batches(Stream* s, const uint32_t*, const WorkItem* d_items, u
 }
 
 void free_pair_scratch(PairScratch& ws) {
-  hipFree(ws.d_pairs);
-  hipFree(ws.d_rank);
-  hipFree(ws.d_sorted);
-  hipFree(ws.d_items);
-  hipFree(ws.d_deg);
-  hipFree(ws.d_off);
-  hipFree(ws.d_tile_status);
-  hipFree(ws.d_counters);
+  (void)hipFree(ws.d_entries);
+  (void)hipFree(ws.d_sorted);
+  (void)hipFree(ws.d_items);
+  (void)hipFree(ws.d_counts);
+  (void)hipFree(ws.d_cursors);
+  (void)hipFree(ws.d_counters);
   ws = PairScratch{};
 }
 
-static uint32_t scan_tiles(uint32_t n) {
-  const uint32_t per = kScanThreads * kScanPerThread;
-  return (n + per - 1) / per;
+PartParams make_part_params(uint32_t n) {
+  PartParams p{};
+  uint32_t nbits = 1;
+  while (nbits < 32 && (1ULL << nbits) < n) nbits++;
+  int lb = (int)nbits - 8;
+  p.nbits = nbits;
+  p.local_bits = (uint32_t)std::min(13, std::max(0, lb));
+  p.nbuckets = 1u << (nbits - p.local_bits);
+  p.mask = nbits >= 32 ? 0xFFFFFFFFu : (uint32_t)((1ULL << nbits) - 1);
+  p.mul = 0x9E3779B1u;  // odd: a bijection modulo 2^nbits
+  uint32_t inv = p.mul;  // Newton iteration for the inverse modulo 2^32
+  for (int i = 0; i < 5; i++) inv *= 2u - p.mul * inv;
+  p.inv = inv;
+  return p;
 }
 
 int ensure_pair_scratch(Stream* s, uint64_t nedges) {
   PairScratch& ws = s->ws;
-  const uint32_t n = s->store->view.n;
-  if (ws.cap_n < n) {
-    hipFree(ws.d_deg);
-    hipFree(ws.d_off);
-    hipFree(ws.d_tile_status);
-    hipFree(ws.d_counters);
-    ws.d_deg = ws.d_off = nullptr;
-    ws.d_tile_status = nullptr;
-    ws.d_counters = nullptr;
-    GS_HIP(hipMalloc(&ws.d_deg, sizeof(uint32_t) * n));
-    GS_HIP(hipMalloc(&ws.d_off, sizeof(uint32_t) * n));
-    GS_HIP(hipMalloc(&ws.d_tile_status, sizeof(uint64_t) * scan_tiles(n)));
+  const PartParams& pp = s->store->part;
+  if (ws.cap_buckets < pp.nbuckets) {
+    GS_HIP(hipStreamSynchronize(s->stream));
+    (void)hipFree(ws.d_counts);
+    (void)hipFree(ws.d_cursors);
+    (void)hipFree(ws.d_counters);
+    ws.d_counts = ws.d_cursors = ws.d_counters = nullptr;
+    GS_HIP(hipMalloc(&ws.d_counts, sizeof(uint32_t) * 2 * pp.nbuckets));
+    GS_HIP(hipMalloc(&ws.d_cursors, sizeof(uint32_t) * pp.nbuckets));
     GS_HIP(hipMalloc(&ws.d_counters, sizeof(uint32_t) * 4));
-    GS_HIP(hipMemsetAsync(ws.d_deg, 0, sizeof(uint32_t) * n, s->stream));
+    GS_HIP(hipMemsetAsync(ws.d_counts, 0, sizeof(uint32_t) * 2 * pp.nbuckets, s->stream));
     GS_HIP(hipMemsetAsync(ws.d_counters, 0, sizeof(uint32_t) * 4, s->stream));
-    ws.cap_n = n;
+    ws.cap_buckets = pp.nbuckets;
+    ws.parity = 0;
   }
   if (ws.cap_edges < nedges) {
     GS_HIP(hipStreamSynchronize(s->stream));
-    hipFree(ws.d_pairs);
-    hipFree(ws.d_rank);
-    hipFree(ws.d_sorted);
-    hipFree(ws.d_items);
-    ws.d_pairs = ws.d_rank = ws.d_sorted = nullptr;
+    (void)hipFree(ws.d_entries);
+    (void)hipFree(ws.d_sorted);
+    (void)hipFree(ws.d_items);
+    ws.d_entries = nullptr;
+    ws.d_sorted = nullptr;
     ws.d_items = nullptr;
     uint64_t cap = nedges;
-    uint64_t icap = std::min<uint64_t>((uint64_t)n, 2 * cap) + (2 * cap) / kChunk + 16;
-    GS_HIP(hipMalloc(&ws.d_pairs, sizeof(uint32_t) * 2 * cap));
-    GS_HIP(hipMalloc(&ws.d_rank, sizeof(uint32_t) * 2 * cap));
+    uint64_t icap = std::min<uint64_t>((uint64_t)s->store->view.n, 2 * cap) + (2 * cap) / kChunk + 16;
+    GS_HIP(hipMalloc(&ws.d_entries, sizeof(uint2) * 2 * cap));
     GS_HIP(hipMalloc(&ws.d_sorted, sizeof(uint32_t) * 2 * cap));
     GS_HIP(hipMalloc(&ws.d_items, sizeof(WorkItem) * icap));
     ws.cap_edges = cap;
@@ -487,22 +604,48 @@ int ingest_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uin
   if (rc) return rc;
   PairScratch& ws = s->ws;
   const StoreView& v = s->store->view;
-  const uint32_t ntiles = scan_tiles(v.n);
-  const uint32_t per_block = kBinThreads * kBinEdgesPerThread;
-  const uint32_t nbin = (uint32_t)((nedges + per_block - 1) / per_block);
-  hipLaunchKernelGGL(bin_rank_kernel, dim3(nbin), dim3(kBinThreads), 0, s->stream, d_pairs,
-                     nedges, v.n, ws.d_deg, ws.d_rank, v.seed, min_depth, depth_cap,
-                     ws.d_tile_status, ntiles, ws.d_counters);
+  const PartParams& pp = s->store->part;
+  uint32_t* counts_cur = ws.d_counts + (size_t)ws.parity * pp.nbuckets;
+  uint32_t* counts_next = ws.d_counts + (size_t)(ws.parity ^ 1) * pp.nbuckets;
+  ws.parity ^= 1;
+  const uint32_t nblk = (uint32_t)((nedges + kPartEdges - 1) / kPartEdges);
+  const size_t sort_lds = sizeof(uint32_t) * 3 * ((size_t)1 << pp.local_bits);
+  if ((rc = ensure_dyn_lds((const void*)bucket_count_kernel, sizeof(uint32_t) * pp.nbuckets)) ||
+      (rc = ensure_dyn_lds((const void*)bucket_scatter_kernel, sizeof(uint32_t) * 2 * pp.nbuckets)) ||
+      (rc = ensure_dyn_lds((const void*)bucket_sort_kernel, sort_lds)))
+    return rc;
+  hipLaunchKernelGGL(bucket_count_kernel, dim3(nblk), dim3(kPartThreads),
+                     sizeof(uint32_t) * pp.nbuckets, s->stream, d_pairs, nedges, v.n, pp, v.seed,
+                     min_depth, depth_cap, counts_cur, counts_next, ws.d_cursors, ws.d_counters);
   GS_HIP(hipGetLastError());
-  hipLaunchKernelGGL(scan_items_kernel, dim3(ntiles), dim3(kScanThreads), 0, s->stream, v.n,
-                     ws.d_deg, ws.d_off, ws.d_items, kChunk, ws.d_tile_status, ntiles,
+  hipLaunchKernelGGL(bucket_scatter_kernel, dim3(nblk), dim3(kPartThreads),
+                     sizeof(uint32_t) * 2 * pp.nbuckets, s->stream, d_pairs, nedges, v.n, pp,
+                     v.seed, min_depth, depth_cap, counts_cur, ws.d_cursors, ws.d_entries);
+  GS_HIP(hipGetLastError());
+  hipLaunchKernelGGL(bucket_sort_kernel, dim3(pp.nbuckets), dim3(kSortThreads), sort_lds,
+                     s->stream, ws.d_entries, pp, counts_cur, ws.d_sorted, ws.d_items, kChunk,
                      ws.d_counters);
   GS_HIP(hipGetLastError());
-  const uint32_t nsc = (uint32_t)std::min<uint64_t>((nedges + 255) / 256, 65535ULL * 4);
-  hipLaunchKernelGGL(scatter_kernel, dim3(nsc), dim3(256), 0, s->stream, d_pairs, nedges,
-                     ws.d_rank, ws.d_off, ws.d_sorted);
-  GS_HIP(hipGetLastError());
   return launch_aggregate(s, ws.d_items, 0, ws.d_counters + 1, ws.d_sorted, 2 * nedges);
+}
+
+__global__ void copy16_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+int device_copy(void* dst, const void* src, size_t bytes) {
+  GS_CHECK(bytes % 16 == 0 && ((uintptr_t)dst % 16) == 0 && ((uintptr_t)src % 16) == 0,
+           GS_ERR_INVALID, "gs_device_copy: 16-byte aligned sizes and pointers only");
+  uint64_t n = bytes / 16;
+  unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, 65536);
+  if (grid == 0) return GS_OK;
+  hipLaunchKernelGGL(copy16_kernel, dim3(grid), dim3(256), 0, 0, reinterpret_cast<uint4*>(dst),
+                     reinterpret_cast<const uint4*>(src), n);
+  GS_HIP(hipGetLastError());
+  GS_HIP(hipDeviceSynchronize());
+  return GS_OK;
 }
 
 int launch_zero_store(Store* st, hipStream_t stream) {

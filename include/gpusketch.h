@@ -152,6 +152,9 @@ int gs_device_free(void* dptr);
 int gs_memcpy_h2d(void* dptr, const void* src, size_t bytes);
 int gs_memcpy_d2h(void* dst, const void* dptr, size_t bytes);
 int gs_device_synchronize(void);
+/* Device-to-device copy by a plain streaming kernel (16-byte loads and stores); also the
+ * known-size reference kernel that calibrates HBM byte counters. */
+int gs_device_copy(void* dst, const void* src, size_t bytes);
 
 /* Synthetic seeded streams (the reference's tools/stream_generators write similar binary
  * streams). Each returns undirected updates as (u, v) pairs with toggle semantics: every
