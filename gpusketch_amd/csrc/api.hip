@@ -265,12 +265,18 @@ int gs_submit_vertex_batches(gs_stream* sh, const uint32_t* src, const uint64_t*
     nitems += (sizes[b] + chunk - 1) / chunk;
   }
   if (total == 0) return GS_OK;
+  // staging: per-class item counts (all items in class 0), the items, the destinations
+  const size_t head = 64;
   size_t item_bytes = nitems * sizeof(WorkItem);
-  size_t bytes = item_bytes + total * sizeof(uint32_t);
+  size_t bytes = head + item_bytes + total * sizeof(uint32_t);
   int rc = ensure_stage(s, bytes);
   if (rc) return rc;
-  WorkItem* hi = reinterpret_cast<WorkItem*>(s->h_stage);
-  uint32_t* hd = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(s->h_stage) + item_bytes);
+  GS_CHECK(nitems < (1ULL << 32), GS_ERR_INVALID, "too many work items in one submission");
+  std::memset(s->h_stage, 0, head);
+  s->h_stage[0] = (uint32_t)nitems;
+  WorkItem* hi = reinterpret_cast<WorkItem*>(reinterpret_cast<char*>(s->h_stage) + head);
+  uint32_t* hd =
+      reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(s->h_stage) + head + item_bytes);
   uint64_t pos = 0, k = 0;
   for (uint32_t b = 0; b < nbatches; b++) {
     for (uint64_t i = 0; i < sizes[b]; i++) {
@@ -287,10 +293,10 @@ int gs_submit_vertex_batches(gs_stream* sh, const uint32_t* src, const uint64_t*
     pos += sizes[b];
   }
   GS_HIP(hipMemcpyAsync(s->d_stage, s->h_stage, bytes, hipMemcpyHostToDevice, s->stream));
-  const WorkItem* di = reinterpret_cast<const WorkItem*>(s->d_stage);
-  const uint32_t* dd =
-      reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(s->d_stage) + item_bytes);
-  rc = launch_vertex_batches(s, nullptr, di, (uint32_t)nitems, dd);
+  const char* base = reinterpret_cast<const char*>(s->d_stage);
+  const WorkItem* di = reinterpret_cast<const WorkItem*>(base + head);
+  const uint32_t* dd = reinterpret_cast<const uint32_t*>(base + head + item_bytes);
+  rc = launch_vertex_batches(s, reinterpret_cast<const uint32_t*>(base), di, (uint32_t)nitems, dd);
   if (rc) return rc;
   return release_stage(s);
 }

@@ -134,6 +134,14 @@ struct PartParams {
 };
 
 // per-stream scratch for the edge-pair pipeline, grown on demand
+// Work items are grouped into length classes (class = floor(log2(len)), len <= 256) so that
+// the aggregation kernel can hand out the longest items first.
+constexpr int kClasses = 9;
+constexpr int kClassBase = 4;  // counters[kClassBase + k]: items of class k in the current batch
+struct ItemClasses {
+  uint64_t off[kClasses];  // start of each class's region in the item array
+};
+
 struct PairScratch {
   uint64_t cap_edges = 0;        // capacity in undirected edges
   uint2* d_entries = nullptr;    // directed entries (vertex, other) grouped by bucket (2 x cap)
@@ -144,9 +152,11 @@ struct PairScratch {
   uint32_t* d_counts = nullptr;   // 2 x nbuckets (double-buffered per batch)
   uint32_t* d_cursors = nullptr;  // nbuckets
   uint32_t parity = 0;
-  uint32_t* d_counters = nullptr;     // [0] unused, [1]=item count, [2] unused,
-                                      // [3]=dropped out-of-range entries since the last flush
+  ItemClasses cls{};
+  uint32_t* d_counters = nullptr;  // [0..2] unused, [3]=dropped out-of-range entries since the
+                                   // last flush, [kClassBase + k]=items of class k
 };
+constexpr int kCounterWords = 16;
 
 struct Stream {
   Store* store = nullptr;
@@ -194,7 +204,7 @@ struct Store {
 };
 
 // kernels/launchers implemented in ingest.hip
-int launch_vertex_batches(Stream* s, const uint32_t* d_src_unused, const WorkItem* d_items,
+int launch_vertex_batches(Stream* s, const uint32_t* d_class_counts, const WorkItem* d_items,
                           uint32_t nitems, const uint32_t* d_dsts);
 int ingest_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uint32_t min_depth,
                         uint32_t depth_cap);

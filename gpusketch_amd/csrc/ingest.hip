@@ -25,6 +25,10 @@ constexpr int kPartEdges = 4096;              // undirected edges per K1/K2 work
 constexpr int kSortThreads = 1024;            // K3 workgroup size (one workgroup per bucket)
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 
+__host__ __device__ __forceinline__ uint32_t item_class(uint32_t len) {
+  return 31u - (uint32_t)__builtin_clz(len);
+}
+
 // Orders this wave's LDS accesses across lanes: a wave's LDS instructions execute in program
 // order, so only the compiler has to be kept from moving accesses across this point (a memory
 // fence would also wait for the wave's outstanding global loads).
@@ -105,7 +109,7 @@ __global__ __launch_bounds__(kPartThreads) void bucket_count_kernel(
   const uint32_t t = threadIdx.x;
   if (blockIdx.x == 0) {
     for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads) { counts_next[i] = 0; cursors[i] = 0; }
-    if (t == 0) { counters[0] = 0; counters[1] = 0; counters[2] = 0; }
+    if (t < kClassBase + kClasses && t != 3) counters[t] = 0;  // [3] persists until a flush
   }
   for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads) hist[i] = 0;
   __syncthreads();
@@ -199,15 +203,14 @@ __global__ __launch_bounds__(kPartThreads) void bucket_scatter_kernel(
 
 __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
     const uint2* __restrict__ entries, PartParams pp, const uint32_t* __restrict__ counts,
-    uint32_t* __restrict__ sorted, WorkItem* __restrict__ items, uint32_t chunk,
-    uint32_t* __restrict__ counters) {
+    uint32_t* __restrict__ sorted, WorkItem* __restrict__ items, ItemClasses cls, uint32_t chunk,
+    uint32_t* __restrict__ class_counts) {
   extern __shared__ uint32_t lds_u32[];
   const uint32_t bins = 1u << pp.local_bits;
-  uint32_t* s_cnt = lds_u32;              // per local id: count, later the insertion cursor
-  uint32_t* s_off = lds_u32 + bins;       // per local id: offset inside the bucket
-  uint32_t* s_ioff = lds_u32 + 2 * bins;  // per local id: first item
+  uint32_t* s_cnt = lds_u32;              // per local id: count
+  uint32_t* s_off = lds_u32 + bins;       // per local id: offset inside the bucket, then cursor
   __shared__ uint32_t s_part[kSortThreads];
-  __shared__ uint32_t s_base, s_ibase;
+  __shared__ uint32_t s_ccnt[kClasses], s_cbase[kClasses];
   const uint32_t t = threadIdx.x, b = blockIdx.x;
   // region start of this bucket = sum of the counts of the buckets before it
   uint32_t acc = 0;
@@ -226,29 +229,15 @@ __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
   for (uint32_t i = t; i < cnt; i += kSortThreads)
     atomicAdd(&s_cnt[hashed_id(pp, entries[base + i].x) & lmask], 1u);
   __syncthreads();
-  // offsets (entries) and first items per local id, in one pass over per-thread ranges
+  // entry offsets per local id (exclusive prefix over the bucket)
   const uint32_t per = (bins + kSortThreads - 1) / kSortThreads;
   const uint32_t l0 = min(bins, t * per), l1 = min(bins, l0 + per);
-  uint32_t se = 0, si = 0;
+  uint32_t se = 0;
   for (uint32_t l = l0; l < l1; l++) {
-    uint32_t d = s_cnt[l];
     s_off[l] = se;
-    s_ioff[l] = si;
-    se += d;
-    si += d ? (d + chunk - 1) / chunk : 0;
+    se += s_cnt[l];
   }
-  __syncthreads();
-  s_part[t] = si;
-  __syncthreads();
-  for (uint32_t o = 1; o < kSortThreads; o <<= 1) {
-    uint32_t add = t >= o ? s_part[t - o] : 0;
-    __syncthreads();
-    s_part[t] += add;
-    __syncthreads();
-  }
-  const uint32_t ibefore = t ? s_part[t - 1] : 0;
-  if (t == kSortThreads - 1) s_ibase = atomicAdd(&counters[1], s_part[t]);
-  // entry offsets need the same prefix over `se`
+  if (t < kClasses) s_ccnt[t] = 0;
   __syncthreads();
   s_part[t] = se;
   __syncthreads();
@@ -259,22 +248,33 @@ __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
     __syncthreads();
   }
   const uint32_t ebefore = t ? s_part[t - 1] : 0;
-  __syncthreads();
-  const uint32_t ibase = s_ibase;
+  // work items go to per-length-class regions (class = floor(log2(len))) so the aggregation
+  // kernel can hand them out largest first; count per class, reserve, then write
   for (uint32_t l = l0; l < l1; l++) {
-    uint32_t d = s_cnt[l];
-    uint32_t eo = base + ebefore + s_off[l];
+    const uint32_t d = s_cnt[l];
+    for (uint32_t q = 0; q < d; q += chunk) atomicAdd(&s_ccnt[item_class(min(chunk, d - q))], 1u);
+  }
+  __syncthreads();
+  if (t < kClasses) {
+    s_cbase[t] = s_ccnt[t] ? atomicAdd(&class_counts[t], s_ccnt[t]) : 0;
+    s_ccnt[t] = 0;
+  }
+  __syncthreads();
+  for (uint32_t l = l0; l < l1; l++) {
+    const uint32_t d = s_cnt[l];
+    const uint32_t eo = base + ebefore + s_off[l];
     s_off[l] = ebefore + s_off[l];
     if (!d) continue;
-    uint32_t v = (((b << pp.local_bits) | l) * pp.inv) & pp.mask;
-    uint32_t nit = (d + chunk - 1) / chunk;
-    uint32_t io = ibase + ibefore + s_ioff[l];
+    const uint32_t v = (((b << pp.local_bits) | l) * pp.inv) & pp.mask;
+    const uint32_t nit = (d + chunk - 1) / chunk;
     for (uint32_t q = 0; q < nit; q++) {
       WorkItem w;
       w.vertex = v;
-      w.len = min(chunk, d - q * chunk) | (nit == 1 ? kItemExclusive : 0u);
+      const uint32_t ln = min(chunk, d - q * chunk);
+      w.len = ln | (nit == 1 ? kItemExclusive : 0u);
       w.start = (uint64_t)eo + (uint64_t)q * chunk;
-      items[io + q] = w;
+      const uint32_t k = item_class(ln);
+      items[cls.off[k] + s_cbase[k] + atomicAdd(&s_ccnt[k], 1u)] = w;
     }
   }
   __syncthreads();
@@ -303,8 +303,8 @@ struct AggLayout {
 };
 
 __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
-    StoreView st, const WorkItem* __restrict__ items, uint32_t nitems_host,
-    const uint32_t* __restrict__ nitems_dev, const uint32_t* __restrict__ dsts, uint32_t dl,
+    StoreView st, const WorkItem* __restrict__ items, ItemClasses cls,
+    const uint32_t* __restrict__ class_counts, const uint32_t* __restrict__ dsts, uint32_t dl,
     int exclusive_ok) {
   extern __shared__ __align__(16) unsigned char lds[];
   const AggLayout L(st.cols, dl);
@@ -325,7 +325,22 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
   for (uint32_t e = lane; e < dslots; e += kWave) { da[e] = 0; dg[e] = 0; }
   wave_sync_lds();
 
-  const uint32_t nitems = nitems_dev ? *nitems_dev : nitems_host;
+  // items are handed out largest class first: virtual index -> (class, index in class)
+  uint32_t ccount[kClasses];
+  uint32_t nitems = 0;
+#pragma unroll
+  for (int k = 0; k < kClasses; k++) {
+    ccount[k] = __builtin_amdgcn_readfirstlane(class_counts[k]);
+    nitems += ccount[k];
+  }
+  auto item_at = [&](uint32_t vi) -> WorkItem {
+#pragma unroll
+    for (int k = kClasses - 1; k > 0; k--) {
+      if (vi < ccount[k]) return items[cls.off[k] + vi];
+      vi -= ccount[k];
+    }
+    return items[cls.off[0] + vi];
+  };
   const uint32_t gw = blockIdx.x * kAggWaves + wid;
   const uint32_t nw = gridDim.x * kAggWaves;
   const uint64_t seed = st.seed;
@@ -333,13 +348,13 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
 
   // software pipeline: the next item and its first chunk of destinations are loaded while the
   // current item is hashed and flushed
-  WorkItem wnext = gw < nitems ? items[gw] : WorkItem{0, 0, 0};
+  WorkItem wnext = gw < nitems ? item_at(gw) : WorkItem{0, 0, 0};
   uint32_t first_next = (gw < nitems && lane < (wnext.len & ~kItemExclusive)) ? dsts[wnext.start + lane] : 0;
   for (uint32_t it = gw; it < nitems; it += nw) {
     const WorkItem w = wnext;
     const uint32_t first = first_next;
     const bool has_next = it + nw < nitems;
-    if (has_next) wnext = items[it + nw];
+    if (has_next) wnext = item_at(it + nw);
     const uint32_t v = __builtin_amdgcn_readfirstlane(w.vertex);
     const uint32_t len = __builtin_amdgcn_readfirstlane(w.len & ~kItemExclusive);
     const bool excl = exclusive_ok && (w.len & kItemExclusive);
@@ -494,8 +509,9 @@ static int ensure_dyn_lds(const void* kernel, size_t bytes) {
   return GS_OK;
 }
 
-static int launch_aggregate(Stream* s, const WorkItem* d_items, uint32_t nitems_host,
-                            const uint32_t* nitems_dev, const uint32_t* d_dsts, uint64_t updates) {
+static int launch_aggregate(Stream* s, const WorkItem* d_items, const ItemClasses& cls,
+                            const uint32_t* class_counts, const uint32_t* d_dsts,
+                            uint64_t updates) {
   Store* st = s->store;
   const StoreView& v = st->view;
   size_t lds = agg_lds_bytes(v);
@@ -516,8 +532,8 @@ static int launch_aggregate(Stream* s, const WorkItem* d_items, uint32_t nitems_
     GS_HIP(hipEventRecord(e0, s->stream));
   }
   hipLaunchKernelGGL(aggregate_kernel, dim3(agg_grid(st->device, kern, lds)),
-                     dim3(kAggWaves * kWave), lds, s->stream, v, d_items, nitems_host, nitems_dev,
-                     d_dsts, lds_window(v), exclusive_ok);
+                     dim3(kAggWaves * kWave), lds, s->stream, v, d_items, cls, class_counts, d_dsts,
+                     lds_window(v), exclusive_ok);
   GS_HIP(hipGetLastError());
   if (s->profile) {
     GS_HIP(hipEventRecord(e1, s->stream));
@@ -527,10 +543,11 @@ static int launch_aggregate(Stream* s, const WorkItem* d_items, uint32_t nitems_
   return GS_OK;
 }
 
-int launch_vertex_batches(Stream* s, const uint32_t*, const WorkItem* d_items, uint32_t nitems,
-                          const uint32_t* d_dsts) {
+int launch_vertex_batches(Stream* s, const uint32_t* d_class_counts, const WorkItem* d_items,
+                          uint32_t nitems, const uint32_t* d_dsts) {
   if (nitems == 0) return GS_OK;
-  return launch_aggregate(s, d_items, nitems, nullptr, d_dsts, 0);
+  ItemClasses cls{};  // every item in class 0
+  return launch_aggregate(s, d_items, cls, d_class_counts, d_dsts, 0);
 }
 
 void free_pair_scratch(PairScratch& ws) {
@@ -570,9 +587,9 @@ int ensure_pair_scratch(Stream* s, uint64_t nedges) {
     ws.d_counts = ws.d_cursors = ws.d_counters = nullptr;
     GS_HIP(hipMalloc(&ws.d_counts, sizeof(uint32_t) * 2 * pp.nbuckets));
     GS_HIP(hipMalloc(&ws.d_cursors, sizeof(uint32_t) * pp.nbuckets));
-    GS_HIP(hipMalloc(&ws.d_counters, sizeof(uint32_t) * 4));
+    GS_HIP(hipMalloc(&ws.d_counters, sizeof(uint32_t) * kCounterWords));
     GS_HIP(hipMemsetAsync(ws.d_counts, 0, sizeof(uint32_t) * 2 * pp.nbuckets, s->stream));
-    GS_HIP(hipMemsetAsync(ws.d_counters, 0, sizeof(uint32_t) * 4, s->stream));
+    GS_HIP(hipMemsetAsync(ws.d_counters, 0, sizeof(uint32_t) * kCounterWords, s->stream));
     ws.cap_buckets = pp.nbuckets;
     ws.parity = 0;
   }
@@ -585,7 +602,15 @@ int ensure_pair_scratch(Stream* s, uint64_t nedges) {
     ws.d_sorted = nullptr;
     ws.d_items = nullptr;
     uint64_t cap = nedges;
-    uint64_t icap = std::min<uint64_t>((uint64_t)s->store->view.n, 2 * cap) + (2 * cap) / kChunk + 16;
+    // class k holds items of length >= 2^k, at most one shorter-than-chunk item per vertex
+    const uint64_t verts = std::min<uint64_t>((uint64_t)s->store->view.n, 2 * cap);
+    uint64_t icap = 0;
+    for (int k = 0; k < kClasses; k++) {
+      ws.cls.off[k] = icap;
+      uint64_t ck = std::min<uint64_t>(verts, (2 * cap) >> k);
+      if (k == kClasses - 1) ck = (2 * cap) / kChunk;
+      icap += ck + 16;
+    }
     GS_HIP(hipMalloc(&ws.d_entries, sizeof(uint2) * 2 * cap));
     GS_HIP(hipMalloc(&ws.d_sorted, sizeof(uint32_t) * 2 * cap));
     GS_HIP(hipMalloc(&ws.d_items, sizeof(WorkItem) * icap));
@@ -623,10 +648,11 @@ int ingest_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uin
                      v.seed, min_depth, depth_cap, counts_cur, ws.d_cursors, ws.d_entries);
   GS_HIP(hipGetLastError());
   hipLaunchKernelGGL(bucket_sort_kernel, dim3(pp.nbuckets), dim3(kSortThreads), sort_lds,
-                     s->stream, ws.d_entries, pp, counts_cur, ws.d_sorted, ws.d_items, kChunk,
-                     ws.d_counters);
+                     s->stream, ws.d_entries, pp, counts_cur, ws.d_sorted, ws.d_items, ws.cls,
+                     kChunk, ws.d_counters + kClassBase);
   GS_HIP(hipGetLastError());
-  return launch_aggregate(s, ws.d_items, 0, ws.d_counters + 1, ws.d_sorted, 2 * nedges);
+  return launch_aggregate(s, ws.d_items, ws.cls, ws.d_counters + kClassBase, ws.d_sorted,
+                          2 * nedges);
 }
 
 __global__ void copy16_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n) {
