@@ -31,6 +31,11 @@ __device__ __forceinline__ uint32_t find_root(uint32_t* parent, uint32_t x) {
   return x;
 }
 
+// per-vertex query state, meaningful at supernode roots
+constexpr uint32_t kVZero = 1;    // sampled ZERO this round (no edge leaves the supernode)
+constexpr uint32_t kVMerged = 2;  // absorbed another supernode this round
+constexpr uint32_t kVDone = 4;    // ZERO and not merged since: the supernode can never change
+
 __device__ __forceinline__ bool unite(uint32_t* parent, uint32_t a, uint32_t b) {
   for (;;) {
     a = find_root(parent, a);
@@ -56,14 +61,19 @@ __device__ __forceinline__ int sample_column(const Bucket16 det, const Bucket16*
   return 2;
 }
 
-__device__ __forceinline__ void handle_sample(int res, uint64_t idx, uint32_t n, uint32_t* parent,
+__device__ __forceinline__ void handle_sample(int res, uint64_t idx, uint32_t self, uint32_t n,
+                                              uint32_t* parent, uint32_t* vflags,
                                               uint32_t* flags, uint32_t* forest,
                                               uint64_t forest_cap) {
   if (res == 2) {
     flags[0] = 1;
-  } else if (res == 0) {
+  } else if (res == 1) {
+    atomicOr(&vflags[self], kVZero);
+  } else {
     uint32_t a = (uint32_t)(idx >> 32), b = (uint32_t)idx;
     if (a < n && b < n && a != b && unite(parent, a, b)) {
+      // the new root is the smaller of the two roots; find it again after the link
+      atomicOr(&vflags[find_root(parent, a)], kVMerged);
       flags[0] = 1;
       uint32_t pos = atomicAdd(&flags[2], 1u);
       if (pos < forest_cap) {
@@ -74,10 +84,11 @@ __device__ __forceinline__ void handle_sample(int res, uint64_t idx, uint32_t n,
   }
 }
 
-__global__ void init_query_kernel(uint32_t n, uint32_t* parent, uint32_t* comp) {
+__global__ void init_query_kernel(uint32_t n, uint32_t* parent, uint32_t* comp, uint32_t* vflags) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
     parent[v] = v;
     comp[v] = v;
+    vflags[v] = 0;
   }
 }
 
@@ -96,59 +107,94 @@ __global__ void zero_acc_kernel(Bucket16* acc, uint64_t count) {
 }
 
 __global__ __launch_bounds__(kQThreads) void round0_kernel(StoreView st, uint32_t col0,
-                                                           uint32_t* parent, uint32_t* flags,
-                                                           uint32_t* forest, uint64_t forest_cap) {
+                                                           uint32_t* parent, uint32_t* vflags,
+                                                           uint32_t* flags, uint32_t* forest,
+                                                           uint64_t forest_cap) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= st.n) return;
   uint64_t idx = 0;
   int res = sample_column(st.det[v], st.rows + (size_t)v * st.vstride + col0, st.cols, st.bpc,
                           st.seed, &idx);
-  handle_sample(res, idx, st.n, parent, flags, forest, forest_cap);
+  handle_sample(res, idx, v, st.n, parent, vflags, flags, forest, forest_cap);
 }
 
-// one thread per (vertex, depth) plus one for the deterministic bucket
+// Rounds >= 1: XOR column `col` (depths 0..B-1) and the deterministic bucket of every vertex of
+// a live supernode into the supernode's accumulator. Lanes are vertices; the lanes of a wave
+// that share the supernode of the wave's first live lane (the common case once a large
+// component has formed) are XOR-reduced with DPP and issue one atomic per bucket, the others
+// issue their own. Every lane stays active through the reductions.
+constexpr int kAccBatch = 8;
 __global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st, uint32_t col,
                                                         const uint32_t* __restrict__ comp,
+                                                        const uint32_t* __restrict__ vflags,
                                                         Bucket16* __restrict__ acc) {
   const uint32_t w = st.bpc + 1;
-  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  if (i >= (uint64_t)st.n * w) return;
-  uint32_t v = (uint32_t)(i / w), j = (uint32_t)(i - (uint64_t)v * w);
-  Bucket16 b = j < st.bpc ? st.rows[row_index(st, v, j, col)] : st.det[v];
-  if ((b.alpha | b.gamma) == 0) return;
-  uint32_t r = comp[v];
-  Bucket16* a = &acc[(uint64_t)r * w + j];
-  if (r == v) {
-    // a root's own contribution races only with atomics from its members
-    __hip_atomic_fetch_xor(&a->alpha, b.alpha, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_xor(&a->gamma, b.gamma, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  } else {
-    __hip_atomic_fetch_xor(&a->alpha, b.alpha, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_xor(&a->gamma, b.gamma, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63;
+  bool act = v < st.n;
+  uint32_t r = act ? comp[v] : 0;
+  if (act && (vflags[r] & kVDone)) act = false;
+  const uint64_t am = __ballot(act);
+  if (am == 0) return;  // wave-uniform
+  const uint32_t r0 = (uint32_t)__shfl((int)r, __builtin_ctzll(am));
+  const bool grp = act && r == r0;
+  const Bucket16* vrow = st.rows + (size_t)(act ? v : 0) * st.vstride + col;
+  Bucket16* a0 = acc + (size_t)r0 * w;
+  Bucket16* ar = acc + (size_t)r * w;
+  for (uint32_t j0 = 0; j0 < w; j0 += kAccBatch) {
+    Bucket16 b[kAccBatch];
+#pragma unroll
+    for (int k = 0; k < kAccBatch; k++) {
+      const uint32_t j = j0 + k;
+      b[k] = Bucket16{0, 0};
+      if (act && j < w) b[k] = j < st.bpc ? vrow[(size_t)j * st.cols] : st.det[v];
+    }
+#pragma unroll
+    for (int k = 0; k < kAccBatch; k++) {
+      const uint32_t j = j0 + k;
+      if (j >= w) break;
+      const uint64_t xa = wave_xor_reduce(grp ? b[k].alpha : 0);
+      const uint64_t xg = wave_xor_reduce(grp ? b[k].gamma : 0);
+      if (lane == 0 && (xa | xg)) {
+        __hip_atomic_fetch_xor(&a0[j].alpha, xa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_xor(&a0[j].gamma, xg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      if (act && !grp && (b[k].alpha | b[k].gamma)) {
+        __hip_atomic_fetch_xor(&ar[j].alpha, b[k].alpha, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_xor(&ar[j].gamma, b[k].gamma, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
   }
 }
 
 __global__ __launch_bounds__(kQThreads) void sample_kernel(StoreView st,
                                                            const uint32_t* __restrict__ comp,
                                                            Bucket16* __restrict__ acc,
-                                                           uint32_t* parent, uint32_t* flags,
-                                                           uint32_t* forest, uint64_t forest_cap) {
+                                                           uint32_t* parent, uint32_t* vflags,
+                                                           uint32_t* flags, uint32_t* forest,
+                                                           uint64_t forest_cap) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= st.n || comp[v] != v) return;
+  if (v >= st.n || comp[v] != v || (vflags[v] & kVDone)) return;
   const uint32_t w = st.bpc + 1;
   Bucket16* a = acc + (uint64_t)v * w;
   uint64_t idx = 0;
   int res = sample_column(a[st.bpc], a, 1, st.bpc, st.seed, &idx);
   for (uint32_t j = 0; j < w; j++) a[j] = Bucket16{0, 0};
-  handle_sample(res, idx, st.n, parent, flags, forest, forest_cap);
+  handle_sample(res, idx, v, st.n, parent, vflags, flags, forest, forest_cap);
 }
 
+// Publishes the roots after a round's unions and settles the root states: a root that sampled
+// ZERO and absorbed nothing has no leaving edge and stays as it is in every later round.
 __global__ __launch_bounds__(kQThreads) void compress_kernel(uint32_t n, uint32_t* parent,
-                                                             uint32_t* comp) {
+                                                             uint32_t* comp, uint32_t* vflags) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
   uint32_t r = find_root(parent, v);
   comp[v] = r;
+  if (r == v) {
+    const uint32_t f = vflags[v];
+    vflags[v] = ((f & (kVZero | kVDone)) && !(f & kVMerged)) ? kVDone : 0u;
+  }
 }
 
 __global__ __launch_bounds__(kQThreads) void finalize_parent_kernel(uint32_t n, uint32_t* parent,
@@ -158,18 +204,19 @@ __global__ __launch_bounds__(kQThreads) void finalize_parent_kernel(uint32_t n, 
 }
 
 void free_query_scratch(Store* st) {
-  hipFree(st->d_parent);
-  hipFree(st->d_comp);
-  hipFree(st->d_acc);
-  hipFree(st->d_flags);
-  hipFree(st->d_forest);
-  if (st->h_flags) hipHostFree(st->h_flags);
-  st->d_parent = st->d_comp = st->d_flags = st->d_forest = st->h_flags = nullptr;
+  (void)hipFree(st->d_parent);
+  (void)hipFree(st->d_comp);
+  (void)hipFree(st->d_vflags);
+  (void)hipFree(st->d_acc);
+  (void)hipFree(st->d_flags);
+  (void)hipFree(st->d_forest);
+  if (st->h_flags) (void)hipHostFree(st->h_flags);
+  st->d_parent = st->d_comp = st->d_vflags = st->d_flags = st->d_forest = st->h_flags = nullptr;
   st->d_acc = nullptr;
   st->forest_cap = 0;
   if (st->qs) {
     Stream* s = st->qs;
-    hipStreamSynchronize(s->stream);
+    (void)hipStreamSynchronize(s->stream);
     free_pair_scratch(s->ws);
     delete s;  // its HIP stream is the store's query stream, destroyed by the store
     st->qs = nullptr;
@@ -181,6 +228,7 @@ static int ensure_query_scratch(Store* st, uint64_t forest_pairs) {
   if (!st->d_parent) {
     GS_HIP(hipMalloc(&st->d_parent, sizeof(uint32_t) * n));
     GS_HIP(hipMalloc(&st->d_comp, sizeof(uint32_t) * n));
+    GS_HIP(hipMalloc(&st->d_vflags, sizeof(uint32_t) * n));
     GS_HIP(hipMalloc(&st->d_acc, sizeof(Bucket16) * (uint64_t)n * (st->view.bpc + 1)));
     GS_HIP(hipMalloc(&st->d_flags, sizeof(uint32_t) * 4));
     GS_HIP(hipHostMalloc((void**)&st->h_flags, sizeof(uint32_t) * 4, hipHostMallocDefault));
@@ -190,7 +238,7 @@ static int ensure_query_scratch(Store* st, uint64_t forest_pairs) {
   }
   if (st->forest_cap < forest_pairs) {
     GS_HIP(hipStreamSynchronize(st->qstream));
-    hipFree(st->d_forest);
+    (void)hipFree(st->d_forest);
     st->d_forest = nullptr;
     GS_HIP(hipMalloc(&st->d_forest, sizeof(uint32_t) * 2 * forest_pairs));
     st->forest_cap = forest_pairs;
@@ -214,7 +262,8 @@ static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t
   hipStream_t q = st->qstream;
   uint32_t* forest = st->d_forest + 2 * forest_base;
   uint64_t cap = st->forest_cap - forest_base;
-  hipLaunchKernelGGL(init_query_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp);
+  hipLaunchKernelGGL(init_query_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
+                     st->d_vflags);
   hipLaunchKernelGGL(clear_all_flags_kernel, dim3(1), dim3(64), 0, q, st->d_flags);
   GS_HIP(hipGetLastError());
   uint32_t round = 0;
@@ -222,20 +271,20 @@ static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t
     hipLaunchKernelGGL(clear_round_flags_kernel, dim3(1), dim3(64), 0, q, st->d_flags);
     if (round == 0) {
       if (col0 >= st->num_samples) return fail(GS_ERR_OUT_OF_SAMPLES, "sketch out of samples");
-      hipLaunchKernelGGL(round0_kernel, grid, block, 0, q, v, col0, st->d_parent, st->d_flags,
-                         forest, cap);
+      hipLaunchKernelGGL(round0_kernel, grid, block, 0, q, v, col0, st->d_parent, st->d_vflags,
+                         st->d_flags, forest, cap);
     } else {
       if (round >= st->num_samples)
         return fail(GS_ERR_OUT_OF_SAMPLES,
                     "This sketch (seed=%llu, max samples=%u) cannot be sampled more times (cur idx=%u)!",
                     (unsigned long long)v.seed, st->num_samples, round);
-      uint64_t threads = (uint64_t)n * (v.bpc + 1);
-      hipLaunchKernelGGL(acc_kernel, dim3((unsigned)((threads + kQThreads - 1) / kQThreads)), block,
-                         0, q, v, round, st->d_comp, st->d_acc);
+      hipLaunchKernelGGL(acc_kernel, grid, block, 0, q, v, round, st->d_comp, st->d_vflags,
+                         st->d_acc);
       hipLaunchKernelGGL(sample_kernel, grid, block, 0, q, v, st->d_comp, st->d_acc, st->d_parent,
-                         st->d_flags, forest, cap);
+                         st->d_vflags, st->d_flags, forest, cap);
     }
-    hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp);
+    hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
+                       st->d_vflags);
     GS_HIP(hipGetLastError());
     GS_HIP(hipMemcpyAsync(st->h_flags, st->d_flags, sizeof(uint32_t) * 4, hipMemcpyDeviceToHost, q));
     GS_HIP(hipStreamSynchronize(q));

@@ -111,6 +111,25 @@ struct StoreView {
   uint64_t seed;
 };
 
+#if defined(__HIP__)
+// XOR of a 32-bit value over the 64 lanes with DPP (quad swaps, half-row and row mirrors, then
+// row broadcasts); the total ends in lane 63 and is returned as a uniform value.
+__device__ __forceinline__ uint32_t wave_xor_u32(uint32_t v) {
+  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
+  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);   // quad_perm 2,3,0,1
+  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false);  // row_mirror
+  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast15
+  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast31
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+__device__ __forceinline__ uint64_t wave_xor_reduce(uint64_t x) {
+  return ((uint64_t)wave_xor_u32((uint32_t)(x >> 32)) << 32) | wave_xor_u32((uint32_t)x);
+}
+
+#endif
+
 __host__ __device__ __forceinline__ size_t row_index(const StoreView& s, uint32_t v, uint32_t d,
                                                      uint32_t c) {
   return (size_t)v * s.vstride + (size_t)d * s.cols + c;
@@ -191,6 +210,7 @@ struct Store {
   // Borůvka scratch, allocated on first query
   uint32_t* d_parent = nullptr;      // union-find being merged this round
   uint32_t* d_comp = nullptr;        // supernode root of every vertex at round start
+  uint32_t* d_vflags = nullptr;      // per-root query state (ZERO / merged / done)
   Bucket16* d_acc = nullptr;         // per-root XOR of one column + deterministic bucket
   uint32_t* d_flags = nullptr;       // [0]=modified, [1]=out of samples, [2]=forest count
   uint32_t* d_forest = nullptr;      // forest edges (min,max) pairs, capacity k * n

@@ -41,22 +41,6 @@ __device__ __forceinline__ void atomic_xor_u64(uint64_t* p, uint64_t v) {
   __hip_atomic_fetch_xor(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// XOR of a 32-bit value over the 64 lanes with DPP (quad swaps, half-row and row mirrors, then
-// row broadcasts); the total ends in lane 63 and is returned as a uniform value.
-__device__ __forceinline__ uint32_t wave_xor_u32(uint32_t v) {
-  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
-  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);   // quad_perm 2,3,0,1
-  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false);  // row_half_mirror
-  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false);  // row_mirror
-  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast15
-  v ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast31
-  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
-}
-
-__device__ __forceinline__ uint64_t wave_xor_reduce(uint64_t x) {
-  return ((uint64_t)wave_xor_u32((uint32_t)(x >> 32)) << 32) | wave_xor_u32((uint32_t)x);
-}
-
 // subgraph filter of the min-cut path (src/mc_gpu_sketch_alg.cpp:139-178): an edge belongs to
 // subgraphs 0..depth, depth = get_index_depth(idx, seed, depth_cap)
 __device__ __forceinline__ bool edge_selected(uint64_t idx, uint64_t seed, uint32_t min_depth,
@@ -704,7 +688,7 @@ int hash_known_answer(const uint64_t* keys, const uint64_t* seeds, uint64_t coun
   uint32_t* dmis = nullptr;
   uint32_t mis = 0;
   int rc = GS_OK;
-  auto cleanup = [&] { hipFree(dk); hipFree(ds); hipFree(dout); hipFree(dmis); };
+  auto cleanup = [&] { (void)hipFree(dk); (void)hipFree(ds); (void)hipFree(dout); (void)hipFree(dmis); };
   if (hipMalloc(&dk, 8 * count) != hipSuccess || hipMalloc(&ds, 8 * count) != hipSuccess ||
       hipMalloc(&dout, 8 * count) != hipSuccess || hipMalloc(&dmis, 4) != hipSuccess) {
     cleanup();
