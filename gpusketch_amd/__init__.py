@@ -58,6 +58,7 @@ _sigs = {
     "gs_stream_create": ([_vp, _vp], _i32),
     "gs_stream_destroy": ([_vp], _i32),
     "gs_submit_vertex_batches": ([_vp, _vp, _vp, _vp, _u32, _vp], _i32),
+    "gs_submit_vertex_batches_filtered": ([_vp, _vp, _vp, _vp, _u32, _vp, _u32, _u32], _i32),
     "gs_submit_edge_pairs": ([_vp, _vp, _u64], _i32),
     "gs_submit_edge_pairs_device": ([_vp, _vp, _u64], _i32),
     "gs_submit_edge_pairs_filtered": ([_vp, _vp, _u64, _u32, _u32], _i32),
@@ -70,6 +71,8 @@ _sigs = {
     "gs_min_cut": ([_u32, _vp, _u64, _vp, _vp], _i32),
     "gs_stream_set_profiling": ([_vp, _i32], _i32),
     "gs_stream_timing": ([_vp, _vp, _vp, _vp], _i32),
+    "gs_host_alloc": ([_sz, _vp], _i32),
+    "gs_host_free": ([_vp], _i32),
     "gs_device_alloc": ([_sz, _vp], _i32),
     "gs_device_free": ([_vp], _i32),
     "gs_memcpy_h2d": ([_vp, _vp, _sz], _i32),
@@ -186,13 +189,18 @@ class Stream:
     def submit_pairs_device(self, dptr, nedges: int) -> None:
         _check(lib.gs_submit_edge_pairs_device(self.h, dptr, nedges))
 
-    def submit_vertex_batches(self, src, sizes, offsets, dsts) -> None:
+    def submit_vertex_batches(self, src, sizes, offsets, dsts, min_depth: int = 0,
+                              depth_cap: int = 0) -> None:
         src = np.ascontiguousarray(src, dtype=np.uint32)
         sizes = np.ascontiguousarray(sizes, dtype=np.uint64)
         offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
         dsts = np.ascontiguousarray(dsts, dtype=np.uint32)
-        _check(lib.gs_submit_vertex_batches(self.h, _p(src), _p(sizes), _p(offsets), len(src),
-                                            _p(dsts)))
+        if min_depth:
+            _check(lib.gs_submit_vertex_batches_filtered(self.h, _p(src), _p(sizes), _p(offsets),
+                                                         len(src), _p(dsts), min_depth, depth_cap))
+        else:
+            _check(lib.gs_submit_vertex_batches(self.h, _p(src), _p(sizes), _p(offsets), len(src),
+                                                _p(dsts)))
 
     def flush(self) -> None:
         _check(lib.gs_flush(self.h))

@@ -58,20 +58,28 @@ def build_library(jobs: int = 8) -> str:
     return LIB
 
 
+HOST_LIB = os.path.join(HERE, "libgpusketch_host.so")
+HOST_FLAGS = ["-O2", "-std=c++17", "-pthread", "-Wall", "-Wextra", "-Wno-unused-parameter",
+              "-I", os.path.join(ROOT, "include")]
+
+
 def build_host_programs() -> list[str]:
-    """C++ facade test/driver programs (host code; link against libgpusketch.so)."""
+    """The C++ API (include/gpusketch/*.h, gpusketch_amd/host/*.cpp) as libgpusketch_host.so over
+    libgpusketch.so, and the C++ test programs host/prog_*.cpp linked against both."""
     os.makedirs(BIN, exist_ok=True)
-    outs = []
-    progs = sorted(glob.glob(os.path.join(HOST, "prog_*.cpp")))
     lib_srcs = sorted(p for p in glob.glob(os.path.join(HOST, "*.cpp"))
                       if not os.path.basename(p).startswith("prog_"))
+    tmp = HOST_LIB + ".tmp"
+    _run(["g++", *HOST_FLAGS, "-fPIC", "-shared", *lib_srcs, "-o", tmp, f"-L{HERE}",
+          "-lgpusketch", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, HOST_LIB)
+    outs = []
     cmds = []
-    for p in progs:
+    for p in sorted(glob.glob(os.path.join(HOST, "prog_*.cpp"))):
         out = os.path.join(BIN, os.path.basename(p)[len("prog_"):-4])
         outs.append(out)
-        cmds.append(["g++", "-O2", "-std=c++17", "-pthread", "-I", os.path.join(ROOT, "include"),
-                     p, *lib_srcs, "-o", out, f"-L{HERE}", "-lgpusketch",
-                     f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/.."])
+        cmds.append(["g++", *HOST_FLAGS, p, "-o", out, f"-L{HERE}", "-lgpusketch_host",
+                     "-lgpusketch", "-Wl,-rpath,$ORIGIN/.."])
     with ThreadPoolExecutor(max_workers=8) as ex:
         list(ex.map(_run, cmds))
     return outs

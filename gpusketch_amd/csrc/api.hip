@@ -249,9 +249,12 @@ int gs_stream_destroy(gs_stream* sh) {
   return GS_OK;
 }
 
-int gs_submit_vertex_batches(gs_stream* sh, const uint32_t* src, const uint64_t* sizes,
-                             const uint64_t* offsets, uint32_t nbatches, const uint32_t* dsts) {
+static int submit_vertex_batches(gs_stream* sh, const uint32_t* src, const uint64_t* sizes,
+                                 const uint64_t* offsets, uint32_t nbatches, const uint32_t* dsts,
+                                 uint32_t min_depth, uint32_t depth_cap) {
   GS_CHECK(sh, GS_ERR_INVALID, "null stream");
+  GS_CHECK(depth_cap < 64 && min_depth <= depth_cap, GS_ERR_INVALID,
+           "subgraph filter: need min_depth <= depth_cap < 64");
   if (nbatches == 0) return GS_OK;
   GS_CHECK(src && sizes && offsets && dsts, GS_ERR_INVALID, "null batch array");
   Stream* s = &sh->impl;
@@ -296,9 +299,22 @@ int gs_submit_vertex_batches(gs_stream* sh, const uint32_t* src, const uint64_t*
   const char* base = reinterpret_cast<const char*>(s->d_stage);
   const WorkItem* di = reinterpret_cast<const WorkItem*>(base + head);
   const uint32_t* dd = reinterpret_cast<const uint32_t*>(base + head + item_bytes);
-  rc = launch_vertex_batches(s, reinterpret_cast<const uint32_t*>(base), di, (uint32_t)nitems, dd);
+  rc = launch_vertex_batches(s, reinterpret_cast<const uint32_t*>(base), di, (uint32_t)nitems, dd,
+                             total, min_depth, depth_cap);
   if (rc) return rc;
   return release_stage(s);
+}
+
+int gs_submit_vertex_batches(gs_stream* sh, const uint32_t* src, const uint64_t* sizes,
+                             const uint64_t* offsets, uint32_t nbatches, const uint32_t* dsts) {
+  return submit_vertex_batches(sh, src, sizes, offsets, nbatches, dsts, 0, 0);
+}
+
+int gs_submit_vertex_batches_filtered(gs_stream* sh, const uint32_t* src, const uint64_t* sizes,
+                                      const uint64_t* offsets, uint32_t nbatches,
+                                      const uint32_t* dsts, uint32_t min_depth,
+                                      uint32_t depth_cap) {
+  return submit_vertex_batches(sh, src, sizes, offsets, nbatches, dsts, min_depth, depth_cap);
 }
 
 static int submit_pairs_host(Stream* s, const uint32_t* pairs, uint64_t nedges, uint32_t min_depth,
@@ -456,6 +472,17 @@ int gs_stream_timing(gs_stream* sh, double* agg_ms, uint64_t* launches, uint64_t
   s.agg_ms = 0;
   s.agg_launches = 0;
   s.agg_updates = 0;
+  return GS_OK;
+}
+
+int gs_host_alloc(size_t bytes, void** hptr) {
+  GS_CHECK(hptr, GS_ERR_INVALID, "null argument");
+  if (hipHostMalloc(hptr, bytes, hipHostMallocDefault) != hipSuccess)
+    return fail(GS_ERR_OOM, "hipHostMalloc(%zu) failed", bytes);
+  return GS_OK;
+}
+int gs_host_free(void* hptr) {
+  GS_HIP(hipHostFree(hptr));
   return GS_OK;
 }
 

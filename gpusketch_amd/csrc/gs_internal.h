@@ -20,60 +20,9 @@
 #include <vector>
 
 #include "../../include/gpusketch.h"
+#include "../../include/gpusketch/xxh3_len8.h"
 
 namespace gs {
-
-// ------------------------------------------------------------------------------------------
-// XXH3_64bits_withSeed for an 8-byte key (the hash the reference binds at include/types.h:8-9),
-// restated from the published XXH3 short-input path (len 4..8 followed by rrmxmx). Host and
-// device share this code.
-//
-// Split form used by the update kernels: with mix(x) = x ^ rotl(x,49) ^ rotl(x,24), which is
-// linear over GF(2), mix(swap(key) ^ bitflip(seed)) = mix(swap(key)) ^ mix(bitflip(seed)).
-// mix(swap(key)) is computed once per update and mix(bitflip(seed+5c)) once per column.
-// ------------------------------------------------------------------------------------------
-constexpr uint64_t kSecret8 = 0x1cad21f72c81017cULL;   // LE64 of secret bytes 8..15
-constexpr uint64_t kSecret16 = 0xdb979083e96dd4deULL;  // LE64 of secret bytes 16..23
-constexpr uint64_t kRrmxmxMul = 0x9FB21C651E98DF25ULL;
-
-__host__ __device__ __forceinline__ uint64_t rotl64(uint64_t x, unsigned r) {
-  return (x << r) | (x >> (64u - r));
-}
-__host__ __device__ __forceinline__ uint32_t bswap32(uint32_t x) {
-  return (x >> 24) | ((x >> 8) & 0xff00u) | ((x << 8) & 0xff0000u) | (x << 24);
-}
-__host__ __device__ __forceinline__ uint64_t xxh3_bitflip(uint64_t seed) {
-  uint64_t s = seed ^ ((uint64_t)bswap32((uint32_t)seed) << 32);
-  return (kSecret8 ^ kSecret16) - s;
-}
-__host__ __device__ __forceinline__ uint64_t xxh3_mix(uint64_t x) {
-  return x ^ rotl64(x, 49) ^ rotl64(x, 24);
-}
-__host__ __device__ __forceinline__ uint64_t xxh3_swap(uint64_t key) {
-  return (key >> 32) | (key << 32);
-}
-__host__ __device__ __forceinline__ uint64_t xxh3_finish(uint64_t x) {
-  x *= kRrmxmxMul;
-  x ^= (x >> 35) + 8u;
-  x *= kRrmxmxMul;
-  x ^= x >> 28;
-  return x;
-}
-__host__ __device__ __forceinline__ uint64_t xxh3_len8(uint64_t key, uint64_t seed) {
-  return xxh3_finish(xxh3_mix(xxh3_swap(key) ^ xxh3_bitflip(seed)));
-}
-// per-column constant of the split form
-__host__ __device__ __forceinline__ uint64_t xxh3_seed_key(uint64_t seed) {
-  return xxh3_mix(xxh3_bitflip(seed));
-}
-// ctz(hash | 1<<max_depth): Bucket_Boruvka::get_index_depth (include/bucket.h:54-59)
-__host__ __device__ __forceinline__ uint32_t depth_of(uint64_t h, uint32_t max_depth) {
-  return (uint32_t)__builtin_ctzll(h | (1ULL << max_depth));
-}
-// concat_pairing_fn (src/util.cpp:45-51)
-__host__ __device__ __forceinline__ uint64_t concat_pair(uint32_t i, uint32_t j) {
-  return i < j ? (((uint64_t)i << 32) | j) : (((uint64_t)j << 32) | i);
-}
 
 // ------------------------------------------------------------------------------------------
 // error reporting
@@ -225,7 +174,8 @@ struct Store {
 
 // kernels/launchers implemented in ingest.hip
 int launch_vertex_batches(Stream* s, const uint32_t* d_class_counts, const WorkItem* d_items,
-                          uint32_t nitems, const uint32_t* d_dsts);
+                          uint32_t nitems, const uint32_t* d_dsts, uint64_t updates,
+                          uint32_t min_depth, uint32_t depth_cap);
 int ingest_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uint32_t min_depth,
                         uint32_t depth_cap);
 int ensure_pair_scratch(Stream* s, uint64_t nedges);

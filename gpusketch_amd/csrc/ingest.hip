@@ -289,7 +289,7 @@ struct AggLayout {
 __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
     StoreView st, const WorkItem* __restrict__ items, ItemClasses cls,
     const uint32_t* __restrict__ class_counts, const uint32_t* __restrict__ dsts, uint32_t dl,
-    int exclusive_ok) {
+    int exclusive_ok, uint32_t min_depth, uint32_t depth_cap) {
   extern __shared__ __align__(16) unsigned char lds[];
   const AggLayout L(st.cols, dl);
   const uint32_t cols = st.cols;
@@ -352,6 +352,8 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
         uint32_t other = j0 == 0 ? first : dsts[start + j0 + lane];
         uint64_t idx = concat_pair(v, other);
         uint64_t chk = xxh3_len8(idx, seed);
+        // subgraph filter: an update outside the store's subgraph becomes the (0, 0) no-op
+        if (min_depth && depth_of(chk, depth_cap) < min_depth) idx = chk = 0;
         sidx[lane] = idx;
         schk[lane] = chk;
         smix[lane] = xxh3_mix(xxh3_swap(idx));
@@ -495,7 +497,7 @@ static int ensure_dyn_lds(const void* kernel, size_t bytes) {
 
 static int launch_aggregate(Stream* s, const WorkItem* d_items, const ItemClasses& cls,
                             const uint32_t* class_counts, const uint32_t* d_dsts,
-                            uint64_t updates) {
+                            uint64_t updates, uint32_t min_depth = 0, uint32_t depth_cap = 0) {
   Store* st = s->store;
   const StoreView& v = st->view;
   size_t lds = agg_lds_bytes(v);
@@ -517,7 +519,7 @@ static int launch_aggregate(Stream* s, const WorkItem* d_items, const ItemClasse
   }
   hipLaunchKernelGGL(aggregate_kernel, dim3(agg_grid(st->device, kern, lds)),
                      dim3(kAggWaves * kWave), lds, s->stream, v, d_items, cls, class_counts, d_dsts,
-                     lds_window(v), exclusive_ok);
+                     lds_window(v), exclusive_ok, min_depth, depth_cap);
   GS_HIP(hipGetLastError());
   if (s->profile) {
     GS_HIP(hipEventRecord(e1, s->stream));
@@ -528,10 +530,11 @@ static int launch_aggregate(Stream* s, const WorkItem* d_items, const ItemClasse
 }
 
 int launch_vertex_batches(Stream* s, const uint32_t* d_class_counts, const WorkItem* d_items,
-                          uint32_t nitems, const uint32_t* d_dsts) {
+                          uint32_t nitems, const uint32_t* d_dsts, uint64_t updates,
+                          uint32_t min_depth, uint32_t depth_cap) {
   if (nitems == 0) return GS_OK;
   ItemClasses cls{};  // every item in class 0
-  return launch_aggregate(s, d_items, cls, d_class_counts, d_dsts, 0);
+  return launch_aggregate(s, d_items, cls, d_class_counts, d_dsts, updates, min_depth, depth_cap);
 }
 
 void free_pair_scratch(PairScratch& ws) {

@@ -91,6 +91,15 @@ int gs_stream_destroy(gs_stream* s);
 int gs_submit_vertex_batches(gs_stream* s, const uint32_t* src, const uint64_t* sizes,
                              const uint64_t* offsets, uint32_t nbatches, const uint32_t* dsts);
 
+/* Subgraph-filtered vertex batches for the min-cut path: only destinations whose edge depth
+ * get_index_depth(concat(src, dst), seed, depth_cap) is >= min_depth are applied, the routing of
+ * MCGPUSketchAlg::apply_update_batch / complete_update_batch (src/mc_gpu_sketch_alg.cpp:80-178)
+ * for the sketch subgraph min_depth. min_depth = 0 applies everything. */
+int gs_submit_vertex_batches_filtered(gs_stream* s, const uint32_t* src, const uint64_t* sizes,
+                                      const uint64_t* offsets, uint32_t nbatches,
+                                      const uint32_t* dsts, uint32_t min_depth,
+                                      uint32_t depth_cap);
+
 /* Raw undirected updates: pairs[2i], pairs[2i+1] toggles edge {u,v} in both endpoints'
  * sketches, as GraphSketchDriver inserts {u,v} and {v,u} (include/graph_sketch_driver.h:171-172)
  * and CCSketchAlg::update applies them (src/cc_sketch_alg.cpp:154-160). Grouping by vertex
@@ -145,6 +154,11 @@ int gs_min_cut(uint32_t n, const uint32_t* pairs, uint64_t m, uint64_t* value,
  * milliseconds, the launch count and the directed updates those launches processed. */
 int gs_stream_set_profiling(gs_stream* s, int enable);
 int gs_stream_timing(gs_stream* s, double* agg_ms, uint64_t* launches, uint64_t* updates);
+
+/* Pinned host memory for staging buffers (replaces the cudaMallocHost calls of CudaStream and
+ * SKGPUSketchAlg, include/cuda_stream.h:51-97, include/sk_gpu_sketch_alg.h:72). */
+int gs_host_alloc(size_t bytes, void** hptr);
+int gs_host_free(void* hptr);
 
 /* Device memory for callers that keep inputs resident (bench, batched and serving modes). */
 int gs_device_alloc(size_t bytes, void** dptr);

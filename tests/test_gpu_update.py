@@ -92,6 +92,35 @@ def test_vertex_batches_match_oracle():
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("min_depth", [1, 3])
+def test_vertex_batches_subgraph_filter_match_oracle(min_depth):
+    # MC routing: a destination reaches subgraph g iff get_index_depth(idx, seed, cap) >= g
+    rng = np.random.default_rng(11 + min_depth)
+    n, seed, cap = 1500, 909, 20
+    pairs = rng.integers(0, n, size=(30000, 2), dtype=np.uint32)
+    pairs = pairs[pairs[:, 0] != pairs[:, 1]]
+    g = orc.Geometry(n)
+    src, sizes, offs, dsts = orc.group_by_vertex(pairs, n, 512)
+    keep = np.zeros(len(dsts), dtype=bool)
+    for b in range(len(src)):
+        for i in range(int(offs[b]), int(offs[b] + sizes[b])):
+            idx = orc.concat_pair(int(src[b]), int(dsts[i]))
+            keep[i] = orc.index_depth(idx, seed, cap) >= min_depth
+    assert 0 < keep.sum() < len(dsts)
+    fsrc, fsizes, foffs, fdsts = [], [], [], []
+    for b in range(len(src)):
+        sel = dsts[int(offs[b]):int(offs[b] + sizes[b])][keep[int(offs[b]):int(offs[b] + sizes[b])]]
+        fsrc.append(src[b]); fsizes.append(len(sel)); foffs.append(len(fdsts)); fdsts.extend(sel)
+    with gs.SketchStore(n, seed) as st:
+        s = st.stream()
+        s.submit_vertex_batches(src, sizes, offs, dsts, min_depth=min_depth, depth_cap=cap)
+        s.flush()
+        got = st.export_buckets()
+    ref = g.empty_store()
+    orc.apply_vertex_batches(ref, g, seed, fsrc, fsizes, foffs, np.array(fdsts, dtype=np.uint32))
+    assert np.array_equal(got, ref)
+
+
 def test_empty_self_loops_and_toggle_cancel():
     n = 300
     with gs.SketchStore(n, 5) as st:
