@@ -71,6 +71,13 @@ _sigs = {
     "gs_min_cut": ([_u32, _vp, _u64, _vp, _vp], _i32),
     "gs_stream_set_profiling": ([_vp, _i32], _i32),
     "gs_stream_timing": ([_vp, _vp, _vp, _vp], _i32),
+    "gs_group_create": ([_vp, _u32, _vp], _i32),
+    "gs_group_destroy": ([_vp], _i32),
+    "gs_group_info": ([_vp, _u32, _vp, _vp], _i32),
+    "gs_group_stream": ([_vp, _vp], _i32),
+    "gs_group_submit_pairs": ([_vp, _vp, _vp], _i32),
+    "gs_group_submit_pairs_device": ([_vp, _vp, _u64], _i32),
+    "gs_group_query_cc": ([_vp, _vp, _vp, _vp, _vp, _vp], _i32),
     "gs_host_alloc": ([_sz, _vp], _i32),
     "gs_host_free": ([_vp], _i32),
     "gs_device_alloc": ([_sz, _vp], _i32),
@@ -277,6 +284,77 @@ class SketchStore:
     def close(self) -> None:
         if self.h:
             _check(lib.gs_store_destroy(self.h))
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class _GroupStream(Stream):
+    """The stream a group owns (flush / profiling / timing; destroyed with the group)."""
+
+    def __init__(self, h):
+        self.store = None
+        self.h = h
+
+    def close(self) -> None:
+        self.h = ctypes.c_void_p()
+
+
+class StoreGroup:
+    """Stores ingested and queried together (batched streams): one launch sequence for all
+    members' edge batches, one batched Boruvka for all members' connectivity queries."""
+
+    def __init__(self, stores):
+        self.stores = list(stores)
+        arr = (ctypes.c_void_p * len(self.stores))(*[s.h.value for s in self.stores])
+        self.h = ctypes.c_void_p()
+        _check(lib.gs_group_create(arr, len(self.stores), ctypes.byref(self.h)))
+        self.bases = []
+        total = ctypes.c_uint32()
+        for i in range(len(self.stores)):
+            b = ctypes.c_uint32()
+            _check(lib.gs_group_info(self.h, i, ctypes.byref(b), ctypes.byref(total)))
+            self.bases.append(int(b.value))
+        self.total = int(total.value)
+        sh = ctypes.c_void_p()
+        _check(lib.gs_group_stream(self.h, ctypes.byref(sh)))
+        self.stream = _GroupStream(sh)
+
+    def submit_pairs(self, pairs_per_member) -> None:
+        """pairs_per_member: one (m_i, 2) uint32 array (member-local ids) or None per member."""
+        keep = [None if p is None else _pairs(p) for p in pairs_per_member]
+        ptrs = (ctypes.c_void_p * len(keep))(*[None if p is None else p.ctypes.data for p in keep])
+        counts = np.array([0 if p is None else p.shape[0] for p in keep], dtype=np.uint64)
+        _check(lib.gs_group_submit_pairs(self.h, ptrs, _p(counts)))
+
+    def submit_pairs_device(self, dptr, nedges: int) -> None:
+        _check(lib.gs_group_submit_pairs_device(self.h, dptr, nedges))
+
+    def flush(self) -> None:
+        self.stream.flush()
+
+    def query_cc(self, mask=None, allow_out_of_samples: bool = False):
+        """Returns (labels per member, num_cc, rounds, status) for the masked members."""
+        g = len(self.stores)
+        labels = np.zeros(self.total, dtype=np.uint32)
+        ncc = np.zeros(g, dtype=np.uint32)
+        rounds = np.zeros(g, dtype=np.uint32)
+        status = np.zeros(g, dtype=np.int32)
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        rc = lib.gs_group_query_cc(self.h, None if m is None else _p(m), _p(labels), _p(ncc),
+                                   _p(rounds), _p(status))
+        if rc != GS_OK and not (allow_out_of_samples and rc == GS_ERR_OUT_OF_SAMPLES):
+            _check(rc)
+        per = [labels[b:b + s.n].copy() for b, s in zip(self.bases, self.stores)]
+        return per, ncc, rounds, status
+
+    def close(self) -> None:
+        if self.h:
+            _check(lib.gs_group_destroy(self.h))
             self.h = ctypes.c_void_p()
 
     def __enter__(self):

@@ -46,7 +46,7 @@ void geometry(uint32_t n, double factor, uint32_t cps, int l0, uint64_t out[5]) 
 }
 
 // ---- staging helpers -------------------------------------------------------------------------
-static int ensure_stage(Stream* s, size_t bytes) {
+int ensure_stage(Stream* s, size_t bytes) {
   if (s->stage_pending) {
     GS_HIP(hipEventSynchronize(s->stage_free));
     s->stage_pending = false;
@@ -71,7 +71,24 @@ static int ensure_stage(Stream* s, size_t bytes) {
   return GS_OK;
 }
 
-static int release_stage(Stream* s) {
+void destroy_stream_resources(Stream& s) {
+  free_pair_scratch(s.ws);
+  if (s.h_stage) (void)hipHostFree(s.h_stage);
+  if (s.d_stage) (void)hipFree(s.d_stage);
+  for (auto& p : s.ev_pairs) {
+    (void)hipEventDestroy(p.first);
+    (void)hipEventDestroy(p.second);
+  }
+  if (s.stage_free) (void)hipEventDestroy(s.stage_free);
+  if (s.stream) (void)hipStreamDestroy(s.stream);
+  s.h_stage = nullptr;
+  s.d_stage = nullptr;
+  s.ev_pairs.clear();
+  s.stage_free = nullptr;
+  s.stream = nullptr;
+}
+
+int release_stage(Stream* s) {
   GS_HIP(hipEventRecord(s->stage_free, s->stream));
   s->stage_pending = true;
   return GS_OK;
@@ -233,18 +250,14 @@ int gs_stream_create(gs_store* h, gs_stream** out) {
 int gs_stream_destroy(gs_stream* sh) {
   if (!sh) return GS_OK;
   Stream& s = sh->impl;
+  GS_CHECK(!s.group, GS_ERR_INVALID, "a group's stream is destroyed with the group");
   hipStreamSynchronize(s.stream);
   {
     std::lock_guard<std::mutex> lk(s.store->mu);
     auto& v = s.store->streams;
     v.erase(std::remove(v.begin(), v.end(), &s), v.end());
   }
-  free_pair_scratch(s.ws);
-  if (s.h_stage) hipHostFree(s.h_stage);
-  if (s.d_stage) hipFree(s.d_stage);
-  for (auto& p : s.ev_pairs) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
-  if (s.stage_free) hipEventDestroy(s.stage_free);
-  if (s.stream) hipStreamDestroy(s.stream);
+  destroy_stream_resources(s);
   delete sh;
   return GS_OK;
 }
@@ -274,7 +287,7 @@ static int submit_vertex_batches(gs_stream* sh, const uint32_t* src, const uint6
   size_t bytes = head + item_bytes + total * sizeof(uint32_t);
   int rc = ensure_stage(s, bytes);
   if (rc) return rc;
-  GS_CHECK(nitems < (1ULL << 32), GS_ERR_INVALID, "too many work items in one submission");
+  GS_CHECK(total < (1ULL << 32), GS_ERR_INVALID, "too many destinations in one submission");
   std::memset(s->h_stage, 0, head);
   s->h_stage[0] = (uint32_t)nitems;
   WorkItem* hi = reinterpret_cast<WorkItem*>(reinterpret_cast<char*>(s->h_stage) + head);
@@ -290,7 +303,8 @@ static int submit_vertex_batches(gs_stream* sh, const uint32_t* src, const uint6
     for (uint64_t q = 0; q < sizes[b]; q += chunk) {
       hi[k].vertex = src[b];
       hi[k].len = (uint32_t)std::min<uint64_t>(chunk, sizes[b] - q);  // shared: atomic flush
-      hi[k].start = pos + q;
+      hi[k].start = (uint32_t)(pos + q);
+      hi[k].graph = 0;
       k++;
     }
     pos += sizes[b];

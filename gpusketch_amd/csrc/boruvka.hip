@@ -12,6 +12,9 @@
 //   modified (sample_supernode :164-193); compress_kernel publishes the new supernode roots.
 // The partition after every round is a function of the sketch contents only, so it equals the
 // reference's; which of two equivalent sampled edges enters the spanning forest may differ.
+#include <algorithm>
+#include <cstring>
+
 #include "gs_internal.h"
 
 namespace gs {
@@ -369,6 +372,254 @@ int k_forests(Store* st, uint32_t k, uint32_t* edges_out, uint64_t edge_cap, uin
   GS_CHECK(total <= edge_cap, GS_ERR_INVALID, "edge_cap %llu too small for %llu forest edges",
            (unsigned long long)edge_cap, (unsigned long long)total);
   if (total) GS_HIP(hipMemcpy(edges_out, st->d_forest, sizeof(uint32_t) * 2 * total, hipMemcpyDeviceToHost));
+  return GS_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Batched Boruvka over the members of a group (virtual vertex space, member of global vertex V
+// in vgraph[V]). Every member runs its own rounds: a member whose round modified nothing is done
+// (its partition is final), one that still changes after its last column is out of samples.
+// Rounds >= 1 are one captured HIP graph: the round number lives in device memory, so the same
+// graph replays round after round and the host only reads back whether any member is active.
+// gstate: [0,G) modified this round, [G,2G) status (0 active, 1 done, 2 out of samples),
+// [2G,3G) rounds used, [3G] round counter, [3G+1] any member active after this round.
+// ------------------------------------------------------------------------------------------
+constexpr uint32_t kGActive = 0, kGDone = 1, kGOutOfSamples = 2;
+
+__device__ __forceinline__ void handle_group_sample(int res, uint64_t idx, uint32_t self,
+                                                    const GraphDesc& d, uint32_t* parent,
+                                                    uint32_t* vflags, uint32_t* gmod) {
+  if (res == 2) {
+    *gmod = 1;
+  } else if (res == 1) {
+    atomicOr(&vflags[self], kVZero);
+  } else {
+    const uint32_t a = (uint32_t)(idx >> 32), b = (uint32_t)idx;
+    if (a < d.n && b < d.n && a != b && unite(parent, a + d.base, b + d.base)) {
+      atomicOr(&vflags[find_root(parent, a + d.base)], kVMerged);
+      *gmod = 1;
+    }
+  }
+}
+
+__global__ void g_begin_kernel(uint32_t* gstate, uint32_t G) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) gstate[3 * G + 1] = 0;
+}
+
+__global__ __launch_bounds__(kQThreads) void g_round0_kernel(
+    const GraphDesc* __restrict__ descs, const uint32_t* __restrict__ vgraph, uint32_t N,
+    uint32_t G, uint32_t col0, uint32_t* parent, uint32_t* vflags, uint32_t* gstate) {
+  const uint32_t V = blockIdx.x * blockDim.x + threadIdx.x;
+  if (V >= N) return;
+  const uint32_t g = vgraph[V];
+  if (gstate[G + g] != kGActive) return;
+  const GraphDesc d = descs[g];
+  const uint32_t v = V - d.base;
+  uint64_t idx = 0;
+  const int res = sample_column(d.det[v], d.rows + (size_t)v * d.vstride + col0, d.cols, d.bpc,
+                                d.seed, &idx);
+  handle_group_sample(res, idx, V, d, parent, vflags, &gstate[g]);
+}
+
+// the group form of acc_kernel: lanes are global vertices of possibly different members
+__global__ __launch_bounds__(kQThreads) void g_acc_kernel(
+    const GraphDesc* __restrict__ descs, const uint32_t* __restrict__ vgraph, uint32_t N,
+    uint32_t G, const uint32_t* __restrict__ comp, const uint32_t* __restrict__ vflags,
+    const uint32_t* __restrict__ gstate, Bucket16* __restrict__ acc, uint32_t acc_stride,
+    uint32_t max_w) {
+  const uint32_t col = gstate[3 * G];
+  const uint32_t V = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63;
+  bool act = V < N;
+  uint32_t g = 0, r = 0;
+  if (act) {
+    g = vgraph[V];
+    act = gstate[G + g] == kGActive;
+  }
+  if (act) {
+    r = comp[V];
+    act = !(vflags[r] & kVDone);
+  }
+  GraphDesc d{};
+  if (act) d = descs[g];
+  const uint64_t am = __ballot(act);
+  if (am == 0) return;  // wave-uniform
+  const uint32_t r0 = (uint32_t)__shfl((int)r, __builtin_ctzll(am));
+  const bool grp = act && r == r0;
+  const uint32_t w = act ? d.bpc + 1 : 0;
+  const uint32_t v = act ? V - d.base : 0;
+  const Bucket16* vrow = act ? d.rows + (size_t)v * d.vstride + col : nullptr;
+  Bucket16* a0 = acc + (size_t)r0 * acc_stride;
+  Bucket16* ar = acc + (size_t)r * acc_stride;
+  for (uint32_t j0 = 0; j0 < max_w; j0 += kAccBatch) {
+    Bucket16 b[kAccBatch];
+#pragma unroll
+    for (int k = 0; k < kAccBatch; k++) {
+      const uint32_t j = j0 + k;
+      b[k] = Bucket16{0, 0};
+      if (j < w) b[k] = j < d.bpc ? vrow[(size_t)j * d.cols] : d.det[v];
+    }
+#pragma unroll
+    for (int k = 0; k < kAccBatch; k++) {
+      const uint32_t j = j0 + k;
+      if (j >= max_w) break;
+      const uint64_t xa = wave_xor_reduce(grp ? b[k].alpha : 0);
+      const uint64_t xg = wave_xor_reduce(grp ? b[k].gamma : 0);
+      if (lane == 0 && (xa | xg)) {
+        __hip_atomic_fetch_xor(&a0[j].alpha, xa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_xor(&a0[j].gamma, xg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      if (act && !grp && (b[k].alpha | b[k].gamma)) {
+        __hip_atomic_fetch_xor(&ar[j].alpha, b[k].alpha, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_xor(&ar[j].gamma, b[k].gamma, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(kQThreads) void g_sample_kernel(
+    const GraphDesc* __restrict__ descs, const uint32_t* __restrict__ vgraph, uint32_t N,
+    uint32_t G, const uint32_t* __restrict__ comp, Bucket16* __restrict__ acc, uint32_t acc_stride,
+    uint32_t* parent, uint32_t* vflags, uint32_t* gstate) {
+  const uint32_t V = blockIdx.x * blockDim.x + threadIdx.x;
+  if (V >= N || comp[V] != V || (vflags[V] & kVDone)) return;
+  const uint32_t g = vgraph[V];
+  if (gstate[G + g] != kGActive) return;
+  const GraphDesc d = descs[g];
+  Bucket16* a = acc + (size_t)V * acc_stride;
+  uint64_t idx = 0;
+  const int res = sample_column(a[d.bpc], a, 1, d.bpc, d.seed, &idx);
+  for (uint32_t j = 0; j <= d.bpc; j++) a[j] = Bucket16{0, 0};
+  handle_group_sample(res, idx, V, d, parent, vflags, &gstate[g]);
+}
+
+// after a round: settle every active member and advance the round counter (one workgroup)
+__global__ void g_settle_kernel(const GraphDesc* __restrict__ descs, uint32_t G,
+                                uint32_t* gstate) {
+  const uint32_t round = gstate[3 * G];
+  uint32_t active = 0;
+  for (uint32_t g = threadIdx.x; g < G; g += blockDim.x) {
+    if (gstate[G + g] != kGActive) continue;
+    if (!gstate[g]) {
+      gstate[G + g] = kGDone;  // nothing changed: the partition is final
+      gstate[2 * G + g] = round;
+    } else if (round + 1 >= descs[g].samples) {
+      gstate[G + g] = kGOutOfSamples;
+      gstate[2 * G + g] = round + 1;
+    } else {
+      active = 1;
+    }
+    gstate[g] = 0;
+  }
+  if (active) atomicOr(&gstate[3 * G + 1], 1u);
+  __syncthreads();
+  if (threadIdx.x == 0) gstate[3 * G] = round + 1;
+}
+
+void free_group_query(Group* g) {
+  (void)hipFree(g->d_parent);
+  (void)hipFree(g->d_comp);
+  (void)hipFree(g->d_vflags);
+  (void)hipFree(g->d_gstate);
+  (void)hipFree(g->d_acc);
+  if (g->h_gstate) (void)hipHostFree(g->h_gstate);
+  if (g->round_exec) (void)hipGraphExecDestroy(g->round_exec);
+  g->d_parent = g->d_comp = g->d_vflags = g->d_gstate = g->h_gstate = nullptr;
+  g->d_acc = nullptr;
+  g->round_exec = nullptr;
+}
+
+// the kernels of one round >= 1, launched on `q` (eagerly or under capture)
+static void launch_group_round(Group* g, hipStream_t q) {
+  const uint32_t N = g->n_total, G = (uint32_t)g->members.size();
+  const dim3 grid((N + kQThreads - 1) / kQThreads), block(kQThreads);
+  hipLaunchKernelGGL(g_begin_kernel, dim3(1), dim3(64), 0, q, g->d_gstate, G);
+  hipLaunchKernelGGL(g_acc_kernel, grid, block, 0, q, g->d_descs, g->d_vgraph, N, G, g->d_comp,
+                     g->d_vflags, g->d_gstate, g->d_acc, (uint32_t)g->acc_stride,
+                     g->max_bpc + 1);
+  hipLaunchKernelGGL(g_sample_kernel, grid, block, 0, q, g->d_descs, g->d_vgraph, N, G, g->d_comp,
+                     g->d_acc, (uint32_t)g->acc_stride, g->d_parent, g->d_vflags, g->d_gstate);
+  hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, N, g->d_parent, g->d_comp, g->d_vflags);
+  hipLaunchKernelGGL(g_settle_kernel, dim3(1), dim3(1024), 0, q, g->d_descs, G, g->d_gstate);
+  hipLaunchKernelGGL(finalize_parent_kernel, grid, block, 0, q, N, g->d_parent, g->d_comp);
+}
+
+int group_query_cc(Group* g, const uint8_t* mask, uint32_t* labels, uint32_t* num_cc,
+                   uint32_t* rounds, int32_t* status) {
+  const uint32_t N = g->n_total, G = (uint32_t)g->members.size();
+  hipStream_t q = g->s->stream;
+  if (!g->d_parent) {
+    g->acc_stride = g->max_bpc + 1;
+    GS_HIP(hipMalloc(&g->d_parent, sizeof(uint32_t) * N));
+    GS_HIP(hipMalloc(&g->d_comp, sizeof(uint32_t) * N));
+    GS_HIP(hipMalloc(&g->d_vflags, sizeof(uint32_t) * N));
+    GS_HIP(hipMalloc(&g->d_gstate, sizeof(uint32_t) * (3 * (size_t)G + 2)));
+    GS_HIP(hipHostMalloc((void**)&g->h_gstate, sizeof(uint32_t) * (3 * (size_t)G + 2),
+                         hipHostMallocDefault));
+    GS_HIP(hipMalloc(&g->d_acc, sizeof(Bucket16) * (size_t)N * g->acc_stride));
+    hipLaunchKernelGGL(zero_acc_kernel, dim3(2048), dim3(256), 0, q, g->d_acc,
+                       (uint64_t)N * g->acc_stride);
+    GS_HIP(hipGetLastError());
+    GS_HIP(hipStreamSynchronize(q));
+    // capture one round once; replays reuse the same scratch pointers
+    hipGraph_t graph = nullptr;
+    GS_HIP(hipStreamBeginCapture(q, hipStreamCaptureModeThreadLocal));
+    launch_group_round(g, q);
+    GS_HIP(hipStreamEndCapture(q, &graph));
+    hipError_t e = hipGraphInstantiate(&g->round_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    GS_HIP(e);
+  }
+  // host-initialised state: nothing modified, members outside the mask already done, round 0
+  std::vector<uint32_t> init(3 * (size_t)G + 2, 0);
+  for (uint32_t i = 0; i < G; i++) init[G + i] = (mask && !mask[i]) ? kGDone : kGActive;
+  std::memcpy(g->h_gstate, init.data(), init.size() * sizeof(uint32_t));
+  GS_HIP(hipMemcpyAsync(g->d_gstate, g->h_gstate, init.size() * sizeof(uint32_t),
+                        hipMemcpyHostToDevice, q));
+  const dim3 grid((N + kQThreads - 1) / kQThreads), block(kQThreads);
+  hipLaunchKernelGGL(init_query_kernel, grid, block, 0, q, N, g->d_parent, g->d_comp, g->d_vflags);
+  hipLaunchKernelGGL(g_begin_kernel, dim3(1), dim3(64), 0, q, g->d_gstate, G);
+  hipLaunchKernelGGL(g_round0_kernel, grid, block, 0, q, g->d_descs, g->d_vgraph, N, G, 0u,
+                     g->d_parent, g->d_vflags, g->d_gstate);
+  hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, N, g->d_parent, g->d_comp, g->d_vflags);
+  hipLaunchKernelGGL(g_settle_kernel, dim3(1), dim3(1024), 0, q, g->d_descs, G, g->d_gstate);
+  hipLaunchKernelGGL(finalize_parent_kernel, grid, block, 0, q, N, g->d_parent, g->d_comp);
+  GS_HIP(hipGetLastError());
+  uint32_t max_samples = 0;
+  for (Store* m : g->members) max_samples = std::max(max_samples, m->num_samples);
+  // replay rounds, two per host check: a round with no active member does nothing
+  for (uint32_t done = 0; done <= max_samples + 1; done += 2) {
+    GS_HIP(hipMemcpyAsync(&g->h_gstate[3 * G + 1], &g->d_gstate[3 * G + 1], sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, q));
+    GS_HIP(hipStreamSynchronize(q));
+    if (!g->h_gstate[3 * G + 1]) break;
+    GS_HIP(hipGraphLaunch(g->round_exec, q));
+    GS_HIP(hipGraphLaunch(g->round_exec, q));
+  }
+  GS_HIP(hipMemcpyAsync(g->h_gstate, g->d_gstate, sizeof(uint32_t) * (3 * (size_t)G + 2),
+                        hipMemcpyDeviceToHost, q));
+  GS_HIP(hipMemcpyAsync(labels, g->d_comp, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, q));
+  GS_HIP(hipStreamSynchronize(q));
+  int rc = GS_OK;
+  for (uint32_t i = 0; i < G; i++) {
+    const uint32_t base = g->bases[i], n = g->members[i]->view.n;
+    uint32_t cc = 0;
+    for (uint32_t v = 0; v < n; v++) {
+      labels[base + v] -= base;  // local smallest vertex of the component
+      cc += labels[base + v] == v;
+    }
+    const uint32_t st = g->h_gstate[G + i];
+    if (num_cc) num_cc[i] = cc;
+    if (rounds) rounds[i] = g->h_gstate[2 * G + i];
+    const int32_t code = st == kGOutOfSamples ? GS_ERR_OUT_OF_SAMPLES
+                         : st == kGDone       ? GS_OK
+                                              : GS_ERR_INVALID;
+    if (status) status[i] = code;
+    if (code != GS_OK && !(mask && !mask[i]) && rc == GS_OK) rc = code;
+  }
+  if (rc == GS_ERR_OUT_OF_SAMPLES)
+    return fail(rc, "a member of the group ran out of samples (see the per-member status)");
+  if (rc) return fail(rc, "batched query did not converge");
   return GS_OK;
 }
 

@@ -1,0 +1,209 @@
+// Groups of stores ingested and queried together (the batched-streams mode): many independent
+// graphs of different sizes share one virtual vertex space (member i's vertex v is global vertex
+// bases[i] + v), so one launch sequence of the grouping and aggregation kernels serves all of
+// them, and one batched Boruvka answers connectivity for all of them. Each member keeps its own
+// geometry (B, C, seed) and its own HBM store; nothing of one member is visible to another.
+#include <algorithm>
+#include <cstring>
+
+#include "gs_internal.h"
+
+namespace gs {
+
+static void unregister_from_members(Group& g) {
+  for (Store* m : g.members) {
+    std::lock_guard<std::mutex> lk(m->mu);
+    auto& v = m->streams;
+    v.erase(std::remove(v.begin(), v.end(), g.s), v.end());
+  }
+}
+
+static void destroy_group(gs_group* h) {
+  Group& g = h->impl;
+  if (g.s && g.s->stream) (void)hipStreamSynchronize(g.s->stream);
+  free_group_query(&g);
+  if (g.s) {
+    unregister_from_members(g);
+    destroy_stream_resources(*g.s);
+  }
+  delete g.handle;
+  (void)hipFree(g.d_descs);
+  (void)hipFree(g.d_bases);
+  (void)hipFree(g.d_vgraph);
+  delete h;
+}
+
+}  // namespace gs
+
+using namespace gs;
+
+extern "C" {
+
+int gs_group_create(gs_store* const* stores, uint32_t count, gs_group** out) {
+  GS_CHECK(stores && out && count >= 1, GS_ERR_INVALID, "gs_group_create: bad argument");
+  gs_group* h = new gs_group();
+  Group& g = h->impl;
+  uint64_t total = 0;
+  std::vector<GraphDesc> descs(count);
+  for (uint32_t i = 0; i < count; i++) {
+    if (!stores[i]) {
+      delete h;
+      return fail(GS_ERR_INVALID, "gs_group_create: member %u is null", i);
+    }
+    Store* st = &stores[i]->impl;
+    for (uint32_t j = 0; j < i; j++)
+      if (&stores[j]->impl == st) {
+        delete h;
+        return fail(GS_ERR_INVALID, "gs_group_create: store given twice (members %u, %u)", j, i);
+      }
+    g.members.push_back(st);
+    g.bases.push_back((uint32_t)total);
+    const StoreView& v = st->view;
+    descs[i] = GraphDesc{v.rows, v.det, v.n, v.bpc, v.cols, v.vstride, v.seed, (uint32_t)total,
+                         st->num_samples};
+    total += v.n;
+    g.max_cols = std::max(g.max_cols, v.cols);
+    g.max_bpc = std::max(g.max_bpc, v.bpc);
+    g.min_bpc = i ? std::min(g.min_bpc, v.bpc) : v.bpc;
+  }
+  if (total > (1ULL << 26)) {
+    delete h;
+    return fail(GS_ERR_INVALID, "gs_group_create: %llu vertices in total exceed 2^26",
+                (unsigned long long)total);
+  }
+  g.n_total = (uint32_t)total;
+  g.part = make_part_params(g.n_total);
+  std::vector<uint32_t> vgraph(total);
+  for (uint32_t i = 0; i < count; i++)
+    std::fill(vgraph.begin() + g.bases[i], vgraph.begin() + g.bases[i] + descs[i].n, i);
+  int rc = GS_OK;
+  if (hipMalloc(&g.d_descs, sizeof(GraphDesc) * count) != hipSuccess ||
+      hipMalloc(&g.d_bases, sizeof(uint32_t) * count) != hipSuccess ||
+      hipMalloc(&g.d_vgraph, sizeof(uint32_t) * total) != hipSuccess)
+    rc = fail(GS_ERR_OOM, "gs_group_create: device allocation failed");
+  if (!rc && (hipMemcpy(g.d_descs, descs.data(), sizeof(GraphDesc) * count,
+                        hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy(g.d_bases, g.bases.data(), sizeof(uint32_t) * count,
+                        hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy(g.d_vgraph, vgraph.data(), sizeof(uint32_t) * total,
+                        hipMemcpyHostToDevice) != hipSuccess))
+    rc = fail(GS_ERR_HIP, "gs_group_create: copy of the member table failed");
+  if (!rc) {
+    g.handle = new gs_stream();
+    g.s = &g.handle->impl;
+    g.s->group = &g;
+    if (hipStreamCreateWithFlags(&g.s->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&g.s->stage_free, hipEventDisableTiming) != hipSuccess)
+      rc = fail(GS_ERR_HIP, "gs_group_create: HIP stream/event creation failed");
+  }
+  if (rc) {
+    destroy_group(h);
+    return rc;
+  }
+  for (Store* m : g.members) {
+    std::lock_guard<std::mutex> lk(m->mu);
+    m->streams.push_back(g.s);
+  }
+  *out = h;
+  return GS_OK;
+}
+
+int gs_group_destroy(gs_group* h) {
+  if (h) destroy_group(h);
+  return GS_OK;
+}
+
+int gs_group_info(const gs_group* h, uint32_t member, uint32_t* base, uint32_t* total) {
+  GS_CHECK(h, GS_ERR_INVALID, "null group");
+  const Group& g = h->impl;
+  GS_CHECK(member < g.members.size(), GS_ERR_INVALID, "member %u out of range", member);
+  if (base) *base = g.bases[member];
+  if (total) *total = g.n_total;
+  return GS_OK;
+}
+
+int gs_group_stream(gs_group* h, gs_stream** out) {
+  GS_CHECK(h && out, GS_ERR_INVALID, "null argument");
+  *out = h->impl.handle;
+  return GS_OK;
+}
+
+int gs_group_submit_pairs(gs_group* h, const uint32_t* const* pairs, const uint64_t* nedges) {
+  GS_CHECK(h && pairs && nedges, GS_ERR_INVALID, "null argument");
+  Group& g = h->impl;
+  Stream* s = g.s;
+  const uint32_t count = (uint32_t)g.members.size();
+  for (uint32_t i = 0; i < count; i++)
+    GS_CHECK(pairs[i] || nedges[i] == 0, GS_ERR_INVALID, "member %u: null pairs", i);
+  const uint64_t chunk = 1ULL << 22;  // edges per staged copy
+  uint32_t mi = 0;
+  uint64_t ei = 0;  // next edge of member mi
+  for (;;) {
+    while (mi < count && ei >= nedges[mi]) {
+      mi++;
+      ei = 0;
+    }
+    if (mi >= count) break;
+    int rc = ensure_stage(s, chunk * 2 * sizeof(uint32_t));
+    if (rc) return rc;
+    uint32_t* out = s->h_stage;
+    uint64_t m = 0;
+    while (m < chunk && mi < count) {
+      if (ei >= nedges[mi]) {
+        mi++;
+        ei = 0;
+        continue;
+      }
+      const uint32_t n = g.members[mi]->view.n, base = g.bases[mi];
+      const uint64_t take = std::min(chunk - m, nedges[mi] - ei);
+      const uint32_t* p = pairs[mi] + 2 * ei;
+      for (uint64_t i = 0; i < take; i++) {
+        const uint32_t u = p[2 * i], v = p[2 * i + 1];
+        GS_CHECK(u < n && v < n, GS_ERR_INVALID, "member %u edge %llu: vertex out of range", mi,
+                 (unsigned long long)(ei + i));
+        out[2 * (m + i)] = u + base;
+        out[2 * (m + i) + 1] = v + base;
+      }
+      m += take;
+      ei += take;
+    }
+    if (m == 0) break;
+    GS_HIP(hipMemcpyAsync(s->d_stage, s->h_stage, m * 2 * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, s->stream));
+    rc = ingest_pairs_device(s, s->d_stage, m, 0, 0);
+    if (rc) return rc;
+    rc = release_stage(s);
+    if (rc) return rc;
+  }
+  return GS_OK;
+}
+
+int gs_group_submit_pairs_device(gs_group* h, const uint32_t* d_pairs, uint64_t nedges) {
+  GS_CHECK(h, GS_ERR_INVALID, "null group");
+  GS_CHECK(d_pairs || nedges == 0, GS_ERR_INVALID, "null pairs");
+  const uint64_t chunk = 1ULL << 30;
+  for (uint64_t e0 = 0; e0 < nedges; e0 += chunk) {
+    int rc = ingest_pairs_device(h->impl.s, d_pairs + 2 * e0, std::min(chunk, nedges - e0), 0, 0);
+    if (rc) return rc;
+  }
+  return GS_OK;
+}
+
+int gs_group_query_cc(gs_group* h, const uint8_t* member_mask, uint32_t* labels,
+                      uint32_t* num_cc, uint32_t* rounds, int32_t* status) {
+  GS_CHECK(h && labels, GS_ERR_INVALID, "null argument");
+  Group& g = h->impl;
+  // every writer of every member must be done before the sketches are read
+  for (Store* m : g.members) {
+    std::vector<Stream*> ss;
+    {
+      std::lock_guard<std::mutex> lk(m->mu);
+      ss = m->streams;
+    }
+    for (Stream* s : ss) GS_HIP(hipStreamSynchronize(s->stream));
+    GS_HIP(hipStreamSynchronize(m->qstream));
+  }
+  return group_query_cc(&g, member_mask, labels, num_cc, rounds, status);
+}
+
+}  // extern "C"

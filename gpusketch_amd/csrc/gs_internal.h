@@ -85,15 +85,29 @@ __host__ __device__ __forceinline__ size_t row_index(const StoreView& s, uint32_
 }
 
 // work item of the aggregation kernel: `len` destinations of vertex `vertex` starting at
-// `start` in the destination array; bit 31 of `flags` = the vertex is owned by this item alone.
+// `start` in the destination array; bit 31 of `len` = the vertex is owned by this item alone;
+// `graph` = member index when a group of stores is ingested together (0 otherwise).
 struct WorkItem {
   uint32_t vertex;
   uint32_t len;
-  uint64_t start;
+  uint32_t start;
+  uint32_t graph;
 };
 constexpr uint32_t kItemExclusive = 0x80000000u;
 
 struct Store;
+struct Group;
+
+// one member of a group of stores that share a virtual vertex space: global vertex
+// base + v is vertex v of this store
+struct GraphDesc {
+  Bucket16* rows;
+  Bucket16* det;
+  uint32_t n, bpc, cols, vstride;
+  uint64_t seed;
+  uint32_t base;
+  uint32_t samples;  // num_samples: Boruvka rounds available
+};
 
 // vertex -> bucket mapping of the edge-pair grouping: hashed = (v * mul) mod 2^nbits (a
 // bijection), bucket = hashed >> local_bits, local id = low local_bits bits
@@ -127,7 +141,8 @@ struct PairScratch {
 constexpr int kCounterWords = 16;
 
 struct Stream {
-  Store* store = nullptr;
+  Store* store = nullptr;  // the store this stream updates, or
+  Group* group = nullptr;  // the group of stores it updates together
   hipStream_t stream = nullptr;
   hipStream_t copy_stream = nullptr;
   PairScratch ws;
@@ -172,6 +187,44 @@ struct Store {
   }
 };
 
+// A group of stores ingested (and queried) together in one virtual vertex space.
+struct Group {
+  std::vector<Store*> members;
+  std::vector<uint32_t> bases;  // host copy; bases[i] = sum of the members' n before i
+  uint32_t n_total = 0;
+  uint32_t max_cols = 0, min_bpc = 0, max_bpc = 0;
+  PartParams part{};
+  GraphDesc* d_descs = nullptr;
+  uint32_t* d_bases = nullptr;
+  Stream* s = nullptr;          // the group's stream (registered as a writer of every member)
+  gs_stream* handle = nullptr;   // owns `s`
+  uint32_t* d_vgraph = nullptr;  // member of every global vertex
+  // batched Boruvka scratch (virtual vertex space), allocated on the first query
+  uint32_t* d_parent = nullptr;
+  uint32_t* d_comp = nullptr;
+  uint32_t* d_vflags = nullptr;
+  uint32_t* d_gstate = nullptr;  // [0,G) modified this round, [G,2G) status, [2G,3G) rounds,
+                                 // [3G] round counter, [3G+1] any member still active
+  uint32_t* h_gstate = nullptr;  // pinned mirror
+  Bucket16* d_acc = nullptr;
+  uint64_t acc_stride = 0;       // buckets per vertex in d_acc (max_bpc + 1)
+  hipGraphExec_t round_exec = nullptr;  // one captured Boruvka round over every member
+};
+
+// vertex space and partition of whatever a stream updates
+inline const PartParams& stream_part(const Stream* s) {
+  return s->group ? s->group->part : s->store->part;
+}
+inline uint32_t stream_vertices(const Stream* s) {
+  return s->group ? s->group->n_total : s->store->view.n;
+}
+
+// pinned + device staging of a stream (api.hip): ensure_stage waits until the previous staged
+// submission has been consumed; release_stage marks the staging busy until the stream gets there
+int ensure_stage(Stream* s, size_t bytes);
+int release_stage(Stream* s);
+void destroy_stream_resources(Stream& s);
+
 // kernels/launchers implemented in ingest.hip
 int launch_vertex_batches(Stream* s, const uint32_t* d_class_counts, const WorkItem* d_items,
                           uint32_t nitems, const uint32_t* d_dsts, uint64_t updates,
@@ -190,6 +243,10 @@ int boruvka_query(Store* st, const uint32_t* start_idx_host, uint32_t* label_out
                   uint32_t* num_cc, uint32_t* rounds, uint32_t* forest_out, uint64_t forest_cap,
                   uint64_t* forest_len);
 void free_query_scratch(Store* st);
+// batched Boruvka over the members of a group (status per member: GS_OK or out of samples)
+int group_query_cc(Group* g, const uint8_t* member_mask, uint32_t* labels, uint32_t* num_cc,
+                   uint32_t* rounds, int32_t* status);
+void free_group_query(Group* g);
 int k_forests(Store* st, uint32_t k, uint32_t* edges_out, uint64_t edge_cap, uint64_t* counts);
 
 // geometry helper shared with the host façade (include/sketch.h:63-77,198-216)
@@ -203,4 +260,7 @@ struct gs_store {
 };
 struct gs_stream {
   gs::Stream impl;
+};
+struct gs_group {
+  gs::Group impl;
 };

@@ -185,10 +185,22 @@ __global__ __launch_bounds__(kPartThreads) void bucket_scatter_kernel(
   }
 }
 
+// member of a group that owns global vertex v: the last base <= v (bases ascending, bases[0] = 0)
+__device__ __forceinline__ uint32_t graph_of(const uint32_t* __restrict__ bases, uint32_t ngraphs,
+                                             uint32_t v) {
+  uint32_t lo = 0, hi = ngraphs;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (bases[mid] <= v) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
     const uint2* __restrict__ entries, PartParams pp, const uint32_t* __restrict__ counts,
     uint32_t* __restrict__ sorted, WorkItem* __restrict__ items, ItemClasses cls, uint32_t chunk,
-    uint32_t* __restrict__ class_counts) {
+    uint32_t* __restrict__ class_counts, const uint32_t* __restrict__ bases, uint32_t ngraphs) {
   extern __shared__ uint32_t lds_u32[];
   const uint32_t bins = 1u << pp.local_bits;
   uint32_t* s_cnt = lds_u32;              // per local id: count
@@ -251,12 +263,14 @@ __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
     if (!d) continue;
     const uint32_t v = (((b << pp.local_bits) | l) * pp.inv) & pp.mask;
     const uint32_t nit = (d + chunk - 1) / chunk;
+    const uint32_t g = ngraphs > 1 ? graph_of(bases, ngraphs, v) : 0u;
     for (uint32_t q = 0; q < nit; q++) {
       WorkItem w;
       w.vertex = v;
+      w.graph = g;
       const uint32_t ln = min(chunk, d - q * chunk);
       w.len = ln | (nit == 1 ? kItemExclusive : 0u);
-      w.start = (uint64_t)eo + (uint64_t)q * chunk;
+      w.start = eo + q * chunk;
       const uint32_t k = item_class(ln);
       items[cls.off[k] + s_cbase[k] + atomicAdd(&s_ccnt[k], 1u)] = w;
     }
@@ -286,28 +300,37 @@ struct AggLayout {
   __host__ __device__ uint32_t wave_words() const { return 2 * dslots + 3 * 64 + cols + (dslots + 1) / 2; }
 };
 
+// kGroup: items belong to the members of a group (descs[item.graph]); the LDS layout is sized
+// for the widest member (max_cols) and the per-member geometry is reloaded when a wave moves to
+// an item of another member. Otherwise every item belongs to the store `st`.
+template <bool kGroup>
 __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
-    StoreView st, const WorkItem* __restrict__ items, ItemClasses cls,
+    StoreView st, const GraphDesc* __restrict__ descs, uint32_t max_cols,
+    const WorkItem* __restrict__ items, ItemClasses cls,
     const uint32_t* __restrict__ class_counts, const uint32_t* __restrict__ dsts, uint32_t dl,
     int exclusive_ok, uint32_t min_depth, uint32_t depth_cap) {
   extern __shared__ __align__(16) unsigned char lds[];
-  const AggLayout L(st.cols, dl);
-  const uint32_t cols = st.cols;
-  const uint32_t bpc = st.bpc;
-  const uint32_t dslots = L.dslots;
+  const AggLayout L(kGroup ? max_cols : st.cols, dl);
+  uint32_t cols = st.cols;
+  uint32_t bpc = st.bpc;
+  uint32_t dslots = dl * cols;
+  uint64_t seed = st.seed;
+  uint32_t vbase = 0;
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t wid = threadIdx.x / kWave;
   uint64_t* da = reinterpret_cast<uint64_t*>(lds) + (size_t)wid * L.wave_words();
-  uint64_t* dg = da + dslots;
-  uint64_t* sidx = dg + dslots;
+  uint64_t* dg = da + L.dslots;
+  uint64_t* sidx = dg + L.dslots;
   uint64_t* schk = sidx + kWave;
   uint64_t* smix = schk + kWave;
   uint64_t* colkey = smix + kWave;
-  uint32_t* clist = reinterpret_cast<uint32_t*>(colkey + cols);
+  uint32_t* clist = reinterpret_cast<uint32_t*>(colkey + L.cols);
 
-  for (uint32_t c = lane; c < cols; c += kWave) colkey[c] = xxh3_seed_key(st.seed + 5ULL * c);
-  for (uint32_t e = lane; e < dslots; e += kWave) { da[e] = 0; dg[e] = 0; }
+  if (!kGroup)
+    for (uint32_t c = lane; c < cols; c += kWave) colkey[c] = xxh3_seed_key(seed + 5ULL * c);
+  for (uint32_t e = lane; e < L.dslots; e += kWave) { da[e] = 0; dg[e] = 0; }
   wave_sync_lds();
+  uint32_t cur_graph = 0xFFFFFFFFu;
 
   // items are handed out largest class first: virtual index -> (class, index in class)
   uint32_t ccount[kClasses];
@@ -327,8 +350,7 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
   };
   const uint32_t gw = blockIdx.x * kAggWaves + wid;
   const uint32_t nw = gridDim.x * kAggWaves;
-  const uint64_t seed = st.seed;
-  const uint32_t q64 = kWave / cols, r64 = kWave % cols;  // 64 = q*cols + r
+  uint32_t q64 = kWave / cols, r64 = kWave % cols;  // 64 = q*cols + r
 
   // software pipeline: the next item and its first chunk of destinations are loaded while the
   // current item is hashed and flushed
@@ -339,7 +361,27 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
     const uint32_t first = first_next;
     const bool has_next = it + nw < nitems;
     if (has_next) wnext = item_at(it + nw);
-    const uint32_t v = __builtin_amdgcn_readfirstlane(w.vertex);
+    if (kGroup) {
+      const uint32_t g = __builtin_amdgcn_readfirstlane(w.graph);
+      if (g != cur_graph) {  // the wave moves to another member: its geometry and column keys
+        cur_graph = g;
+        const GraphDesc& gd = descs[g];
+        st.rows = gd.rows;
+        st.det = gd.det;
+        st.n = gd.n;
+        st.vstride = gd.vstride;
+        cols = gd.cols;
+        bpc = gd.bpc;
+        seed = gd.seed;
+        vbase = gd.base;
+        dslots = dl * cols;
+        q64 = kWave / cols;
+        r64 = kWave % cols;
+        for (uint32_t c = lane; c < cols; c += kWave) colkey[c] = xxh3_seed_key(seed + 5ULL * c);
+        wave_sync_lds();
+      }
+    }
+    const uint32_t v = __builtin_amdgcn_readfirstlane(w.vertex) - vbase;
     const uint32_t len = __builtin_amdgcn_readfirstlane(w.len & ~kItemExclusive);
     const bool excl = exclusive_ok && (w.len & kItemExclusive);
     const uint64_t start = w.start;
@@ -349,7 +391,7 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
     for (uint32_t j0 = 0; j0 < len; j0 += kWave) {
       const uint32_t cnt = min((uint32_t)kWave, len - j0);
       if (lane < cnt) {
-        uint32_t other = j0 == 0 ? first : dsts[start + j0 + lane];
+        uint32_t other = (j0 == 0 ? first : dsts[start + j0 + lane]) - vbase;
         uint64_t idx = concat_pair(v, other);
         uint64_t chk = xxh3_len8(idx, seed);
         // subgraph filter: an update outside the store's subgraph becomes the (0, 0) no-op
@@ -453,8 +495,8 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
 // depths kept in the LDS delta; deeper updates (probability 2^-12 per column) go to HBM directly
 static uint32_t lds_window(const StoreView& v) { return v.bpc < 12 ? v.bpc : 12; }
 
-static size_t agg_lds_bytes(const StoreView& v) {
-  const AggLayout L(v.cols, lds_window(v));
+static size_t agg_lds_bytes(uint32_t cols, uint32_t dl) {
+  const AggLayout L(cols, dl);
   return (size_t)kAggWaves * L.wave_words() * sizeof(uint64_t);
 }
 
@@ -498,28 +540,46 @@ static int ensure_dyn_lds(const void* kernel, size_t bytes) {
 static int launch_aggregate(Stream* s, const WorkItem* d_items, const ItemClasses& cls,
                             const uint32_t* class_counts, const uint32_t* d_dsts,
                             uint64_t updates, uint32_t min_depth = 0, uint32_t depth_cap = 0) {
-  Store* st = s->store;
-  const StoreView& v = st->view;
-  size_t lds = agg_lds_bytes(v);
-  int rc_lds;
-  GS_CHECK(lds <= 160 * 1024, GS_ERR_UNSUPPORTED, "sketch too wide for the LDS window (%zu B)", lds);
-  const void* kern = (const void*)aggregate_kernel;
-  rc_lds = ensure_dyn_lds(kern, lds);
-  if (rc_lds) return rc_lds;
+  Group* grp = s->group;
+  StoreView v{};
+  uint32_t cols, dl;
+  int device = 0;
   int exclusive_ok = 1;
-  {
-    std::lock_guard<std::mutex> lk(st->mu);
-    exclusive_ok = st->streams.size() == 1 ? 1 : 0;
+  if (grp) {
+    cols = grp->max_cols;
+    dl = grp->min_bpc < 12 ? grp->min_bpc : 12;
+    device = grp->members[0]->device;
+    for (Store* m : grp->members) {  // plain stores only if the group is every member's only writer
+      std::lock_guard<std::mutex> lk(m->mu);
+      if (m->streams.size() != 1) exclusive_ok = 0;
+    }
+  } else {
+    v = s->store->view;
+    cols = v.cols;
+    dl = lds_window(v);
+    device = s->store->device;
+    std::lock_guard<std::mutex> lk(s->store->mu);
+    exclusive_ok = s->store->streams.size() == 1 ? 1 : 0;
   }
+  const size_t lds = agg_lds_bytes(cols, dl);
+  GS_CHECK(lds <= 160 * 1024, GS_ERR_UNSUPPORTED, "sketch too wide for the LDS window (%zu B)", lds);
+  const void* kern = grp ? (const void*)aggregate_kernel<true> : (const void*)aggregate_kernel<false>;
+  int rc_lds = ensure_dyn_lds(kern, lds);
+  if (rc_lds) return rc_lds;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (s->profile) {
     GS_HIP(hipEventCreate(&e0));
     GS_HIP(hipEventCreate(&e1));
     GS_HIP(hipEventRecord(e0, s->stream));
   }
-  hipLaunchKernelGGL(aggregate_kernel, dim3(agg_grid(st->device, kern, lds)),
-                     dim3(kAggWaves * kWave), lds, s->stream, v, d_items, cls, class_counts, d_dsts,
-                     lds_window(v), exclusive_ok, min_depth, depth_cap);
+  const dim3 grid(agg_grid(device, kern, lds)), block(kAggWaves * kWave);
+  if (grp)
+    hipLaunchKernelGGL(aggregate_kernel<true>, grid, block, lds, s->stream, v, grp->d_descs, cols,
+                       d_items, cls, class_counts, d_dsts, dl, exclusive_ok, 0u, 0u);
+  else
+    hipLaunchKernelGGL(aggregate_kernel<false>, grid, block, lds, s->stream, v,
+                       (const GraphDesc*)nullptr, cols, d_items, cls, class_counts, d_dsts, dl,
+                       exclusive_ok, min_depth, depth_cap);
   GS_HIP(hipGetLastError());
   if (s->profile) {
     GS_HIP(hipEventRecord(e1, s->stream));
@@ -565,7 +625,7 @@ PartParams make_part_params(uint32_t n) {
 
 int ensure_pair_scratch(Stream* s, uint64_t nedges) {
   PairScratch& ws = s->ws;
-  const PartParams& pp = s->store->part;
+  const PartParams& pp = stream_part(s);
   if (ws.cap_buckets < pp.nbuckets) {
     GS_HIP(hipStreamSynchronize(s->stream));
     (void)hipFree(ws.d_counts);
@@ -590,7 +650,7 @@ int ensure_pair_scratch(Stream* s, uint64_t nedges) {
     ws.d_items = nullptr;
     uint64_t cap = nedges;
     // class k holds items of length >= 2^k, at most one shorter-than-chunk item per vertex
-    const uint64_t verts = std::min<uint64_t>((uint64_t)s->store->view.n, 2 * cap);
+    const uint64_t verts = std::min<uint64_t>((uint64_t)stream_vertices(s), 2 * cap);
     uint64_t icap = 0;
     for (int k = 0; k < kClasses; k++) {
       ws.cls.off[k] = icap;
@@ -615,8 +675,10 @@ int ingest_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uin
   int rc = ensure_pair_scratch(s, nedges);
   if (rc) return rc;
   PairScratch& ws = s->ws;
-  const StoreView& v = s->store->view;
-  const PartParams& pp = s->store->part;
+  const PartParams& pp = stream_part(s);
+  const uint32_t nverts = stream_vertices(s);
+  const uint64_t seed = s->group ? 0 : s->store->view.seed;
+  GS_CHECK(!s->group || min_depth == 0, GS_ERR_INVALID, "no subgraph filter on group streams");
   uint32_t* counts_cur = ws.d_counts + (size_t)ws.parity * pp.nbuckets;
   uint32_t* counts_next = ws.d_counts + (size_t)(ws.parity ^ 1) * pp.nbuckets;
   ws.parity ^= 1;
@@ -627,16 +689,18 @@ int ingest_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uin
       (rc = ensure_dyn_lds((const void*)bucket_sort_kernel, sort_lds)))
     return rc;
   hipLaunchKernelGGL(bucket_count_kernel, dim3(nblk), dim3(kPartThreads),
-                     sizeof(uint32_t) * pp.nbuckets, s->stream, d_pairs, nedges, v.n, pp, v.seed,
+                     sizeof(uint32_t) * pp.nbuckets, s->stream, d_pairs, nedges, nverts, pp, seed,
                      min_depth, depth_cap, counts_cur, counts_next, ws.d_cursors, ws.d_counters);
   GS_HIP(hipGetLastError());
   hipLaunchKernelGGL(bucket_scatter_kernel, dim3(nblk), dim3(kPartThreads),
-                     sizeof(uint32_t) * 2 * pp.nbuckets, s->stream, d_pairs, nedges, v.n, pp,
-                     v.seed, min_depth, depth_cap, counts_cur, ws.d_cursors, ws.d_entries);
+                     sizeof(uint32_t) * 2 * pp.nbuckets, s->stream, d_pairs, nedges, nverts, pp,
+                     seed, min_depth, depth_cap, counts_cur, ws.d_cursors, ws.d_entries);
   GS_HIP(hipGetLastError());
   hipLaunchKernelGGL(bucket_sort_kernel, dim3(pp.nbuckets), dim3(kSortThreads), sort_lds,
                      s->stream, ws.d_entries, pp, counts_cur, ws.d_sorted, ws.d_items, ws.cls,
-                     kChunk, ws.d_counters + kClassBase);
+                     kChunk, ws.d_counters + kClassBase,
+                     s->group ? (const uint32_t*)s->group->d_bases : (const uint32_t*)nullptr,
+                     s->group ? (uint32_t)s->group->members.size() : 1u);
   GS_HIP(hipGetLastError());
   return launch_aggregate(s, ws.d_items, ws.cls, ws.d_counters + kClassBase, ws.d_sorted,
                           2 * nedges);

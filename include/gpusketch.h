@@ -155,6 +155,33 @@ int gs_min_cut(uint32_t n, const uint32_t* pairs, uint64_t m, uint64_t* value,
 int gs_stream_set_profiling(gs_stream* s, int enable);
 int gs_stream_timing(gs_stream* s, double* agg_ms, uint64_t* launches, uint64_t* updates);
 
+/* ---- Batched streams: groups of stores ------------------------------------------------------
+ * Many independent graphs (stores of any sizes, seeds and geometries) updated by one launch
+ * sequence of the grouping and aggregation kernels and queried by one batched Borůvka. Member i's
+ * vertex v is global vertex base_i + v of one virtual vertex space (at most 2^26 vertices). The
+ * group owns a stream registered as a writer of every member; destroy the group before its
+ * members; calls on one group are not concurrent. The reference holds one graph per algorithm
+ * object (include/cc_gpu_sketch_alg.h:8-110); the per-member results equal what that graph's own
+ * store would give (each member's sketches bit-exact, partitions identical). */
+typedef struct gs_group gs_group;
+int gs_group_create(gs_store* const* stores, uint32_t count, gs_group** out);
+int gs_group_destroy(gs_group* g);
+int gs_group_info(const gs_group* g, uint32_t member, uint32_t* base, uint32_t* total);
+/* The group's stream, for gs_flush / gs_stream_set_profiling / gs_stream_timing (not for
+ * gs_stream_destroy). */
+int gs_group_stream(gs_group* g, gs_stream** out);
+/* Host edge pairs of every member (member-local ids; NULL with nedges[i] = 0 skips a member),
+ * ingested as one stream of the group's vertex space. */
+int gs_group_submit_pairs(gs_group* g, const uint32_t* const* pairs, const uint64_t* nedges);
+/* Device-resident pairs already in global ids (base_i added); must stay valid until gs_flush. */
+int gs_group_submit_pairs_device(gs_group* g, const uint32_t* d_pairs, uint64_t nedges);
+/* Connected components of the members selected by member_mask (NULL: all) in one batched
+ * Borůvka whose rounds >= 1 replay a captured HIP graph. labels: total entries, member-local
+ * smallest vertex of each component; num_cc / rounds / status: one entry per member, status
+ * GS_OK or GS_ERR_OUT_OF_SAMPLES (the call then also returns GS_ERR_OUT_OF_SAMPLES). */
+int gs_group_query_cc(gs_group* g, const uint8_t* member_mask, uint32_t* labels,
+                      uint32_t* num_cc, uint32_t* rounds, int32_t* status);
+
 /* Pinned host memory for staging buffers (replaces the cudaMallocHost calls of CudaStream and
  * SKGPUSketchAlg, include/cuda_stream.h:51-97, include/sk_gpu_sketch_alg.h:72). */
 int gs_host_alloc(size_t bytes, void** hptr);

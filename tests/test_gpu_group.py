@@ -1,0 +1,132 @@
+"""Batched streams: a group of stores of different sizes and seeds ingested by one launch
+sequence and queried by one batched Boruvka. Every member must end bit-exact with the oracle and
+with its own single-store run, and every member's partition must equal its single-store query."""
+import numpy as np
+import pytest
+
+import gpusketch_amd as gs
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [100, 1000, 4096, 777, 20000, 65]
+
+
+def member_pairs(rng, n, m):
+    p = rng.integers(0, n, size=(m, 2), dtype=np.uint32)
+    # some toggles twice (a delete) and a few self loops, which the device drops
+    p = np.concatenate([p, p[: m // 10], np.array([[1, 1]], dtype=np.uint32)])
+    rng.shuffle(p)
+    return p
+
+
+def net_edges(pairs):
+    """Edges present an odd number of times (toggle semantics), self loops dropped."""
+    p = pairs[pairs[:, 0] != pairs[:, 1]]
+    lo, hi = np.minimum(p[:, 0], p[:, 1]), np.maximum(p[:, 0], p[:, 1])
+    key = lo.astype(np.uint64) << np.uint64(32) | hi.astype(np.uint64)
+    uniq, cnt = np.unique(key, return_counts=True)
+    odd = uniq[cnt % 2 == 1]
+    return np.stack([(odd >> np.uint64(32)).astype(np.uint32),
+                     (odd & np.uint64(0xFFFFFFFF)).astype(np.uint32)], axis=1)
+
+
+def oracle_store(n, seed, pairs):
+    g = orc.Geometry(n)
+    st = g.empty_store()
+    orc.apply_edges(st, g, seed, pairs[pairs[:, 0] != pairs[:, 1]])
+    return st
+
+
+def test_group_ingest_and_query_match_single_stores():
+    rng = np.random.default_rng(41)
+    seeds = [1000 + 17 * i for i in range(len(SIZES))]
+    stores = [gs.SketchStore(n, s) for n, s in zip(SIZES, seeds)]
+    singles = [gs.SketchStore(n, s) for n, s in zip(SIZES, seeds)]
+    pairs = [member_pairs(rng, n, 6 * n) for n in SIZES]
+    pairs[4] = gs.gen_rmat(14, 150000, 9, del_frac=0.05)  # hubs: the shared-vertex flush path
+    pairs[4] = pairs[4][np.all(pairs[4] < SIZES[4], axis=1)]
+    try:
+        with gs.StoreGroup(stores) as grp:
+            half = [p[: len(p) // 2] for p in pairs]
+            rest = [p[len(p) // 2:] for p in pairs]
+            grp.submit_pairs(half)
+            grp.submit_pairs([None, rest[1], rest[2], rest[3], rest[4], rest[5]])
+            grp.submit_pairs([rest[0], None, None, None, None, None])
+            grp.flush()
+            for i, (n, s) in enumerate(zip(SIZES, seeds)):
+                got = stores[i].export_buckets()
+                assert np.array_equal(got, oracle_store(n, s, pairs[i])), f"member {i}"
+                st = singles[i].stream()
+                st.submit_pairs(pairs[i])
+                st.flush()
+                st.close()
+            labels, ncc, rounds, status = grp.query_cc()
+            for i in range(len(SIZES)):
+                lab1, ncc1, r1 = singles[i].query_cc()
+                assert np.array_equal(labels[i], lab1), f"member {i}"
+                assert ncc[i] == ncc1 and rounds[i] == r1 and status[i] == 0
+                assert np.array_equal(labels[i], orc.dsu_components(SIZES[i], net_edges(pairs[i])))
+            # more updates, then the captured rounds replay for a second query
+            more = [member_pairs(rng, n, n) for n in SIZES]
+            grp.submit_pairs(more)
+            grp.flush()
+            for i in range(len(SIZES)):
+                st = singles[i].stream()
+                st.submit_pairs(more[i])
+                st.flush()
+                st.close()
+                assert np.array_equal(stores[i].export_buckets(), singles[i].export_buckets())
+            labels2, ncc2, _, _ = grp.query_cc()
+            for i in range(len(SIZES)):
+                lab1, ncc1, _ = singles[i].query_cc()
+                assert np.array_equal(labels2[i], lab1) and ncc2[i] == ncc1
+            # a subset of members
+            mask = np.array([1, 0, 1, 0, 0, 1], dtype=np.uint8)
+            labels3, _, _, status3 = grp.query_cc(mask)
+            for i in range(len(SIZES)):
+                if mask[i]:
+                    assert np.array_equal(labels3[i], labels2[i])
+    finally:
+        for s in stores + singles:
+            s.close()
+
+
+def test_group_device_resident_global_pairs():
+    rng = np.random.default_rng(5)
+    sizes = [3000, 500, 9000]
+    seeds = [7, 8, 9]
+    stores = [gs.SketchStore(n, s) for n, s in zip(sizes, seeds)]
+    pairs = [member_pairs(rng, n, 5 * n) for n in sizes]
+    try:
+        with gs.StoreGroup(stores) as grp:
+            glob = np.concatenate([p + np.uint32(b) for p, b in zip(pairs, grp.bases)])
+            glob = glob[rng.permutation(len(glob))]  # members interleaved in one stream
+            buf = gs.DeviceBuffer(glob)
+            grp.submit_pairs_device(buf.offset(0), len(glob))
+            grp.flush()
+            buf.free()
+            for i, (n, s) in enumerate(zip(sizes, seeds)):
+                assert np.array_equal(stores[i].export_buckets(), oracle_store(n, s, pairs[i]))
+    finally:
+        for s in stores:
+            s.close()
+
+
+def test_group_member_out_of_samples():
+    rng = np.random.default_rng(3)
+    n = 2000
+    ok = gs.SketchStore(n, 1)
+    tiny = gs.SketchStore(n, 2, num_samples=1)  # one column: cannot finish a dense graph
+    p = rng.integers(0, n, size=(20 * n, 2), dtype=np.uint32)
+    try:
+        with gs.StoreGroup([ok, tiny]) as grp:
+            grp.submit_pairs([p, p])
+            grp.flush()
+            _, _, _, status = grp.query_cc(allow_out_of_samples=True)
+            assert status[0] == 0 and status[1] == gs.GS_ERR_OUT_OF_SAMPLES
+            with pytest.raises(gs.OutOfSamplesError):
+                grp.query_cc()
+    finally:
+        ok.close()
+        tiny.close()
