@@ -73,6 +73,119 @@ def copy_calibration(mib: int, reps: int = 5) -> None:
     print(json.dumps({"copy_calibration_bytes_per_dispatch": nbytes, "dispatches": reps}))
 
 
+def batched_workload(graphs: int, seed: int):
+    """Independent graphs for the batched-streams workload: n = 2^s with s uniform in [10, 16]
+    (log-uniform n in [2^10, 2^16]), 4..64 updates per vertex, 5% deletions of live edges; even
+    graphs Erdos-Renyi, odd graphs R-MAT(0.57, 0.19, 0.19, 0.05)."""
+    import gpusketch_amd as gs
+    rng = np.random.default_rng(seed)
+    scales = rng.integers(10, 17, size=graphs)
+    upv = rng.integers(4, 65, size=graphs)
+    out = []
+    for g in range(graphs):
+        n = 1 << int(scales[g])
+        m = int(upv[g]) * n // 2  # updates: each touches two vertices
+        if g % 2 == 0:
+            p = min(1.0, m / (1 + DEL_FRAC) / (n * (n - 1) / 2))
+            pairs = gs.gen_erdos_renyi(n, p, DEL_FRAC, seed + g)
+        else:
+            pairs = gs.gen_rmat(int(scales[g]), m, seed + g, *RMAT, del_frac=DEL_FRAC)
+        out.append((n, 0xB0000000 + 7919 * g, pairs))
+    return out
+
+
+def bench_batched(args) -> dict:
+    """Config: 256 independent graphs. Plain = every graph through its own store stream, one
+    after another; batched = one group: one autotuned ingest launch sequence over all graphs'
+    interleaved updates and one batched Boruvka (captured rounds) for all connectivity queries.
+    Every member must end bit-exact with its plain run and with the same partition."""
+    import gpusketch_amd as gs
+    t0 = time.perf_counter()
+    work = batched_workload(args.graphs, SEED_STREAM)
+    total_upd = sum(len(p) for _, _, p in work)
+    log(f"generated {len(work)} graphs, {sum(n for n, _, _ in work)} vertices, {total_upd} "
+        f"updates in {time.perf_counter() - t0:.1f}s")
+    stores = [gs.SketchStore(n, sd) for n, sd, _ in work]
+    sketch_gb = sum(s.geom.device_bytes for s in stores) / 1e9
+    # plain: graph after graph, each from its own device-resident array, 2^20-edge batches
+    bufs = [gs.DeviceBuffer(p) for _, _, p in work]
+    streams = [s.stream() for s in stores]
+    gs.lib.gs_device_synchronize()
+    t0 = time.perf_counter()
+    for (n, sd, p), st, buf in zip(work, streams, bufs):
+        for b in range(0, len(p), BATCH):
+            st.submit_pairs_device(buf.offset(b * 8), min(BATCH, len(p) - b))
+        st.flush()
+    gs.lib.gs_device_synchronize()
+    plain_s = time.perf_counter() - t0
+    for b in bufs:
+        b.free()
+    sample = list(range(0, len(work), max(1, len(work) // 8)))
+    ref = {i: stores[i].export_buckets() for i in sample}
+    lat_plain = []
+    plain_labels = {}
+    for _ in range(args.query_reps):
+        q0 = time.perf_counter()
+        for i, s in enumerate(stores):
+            lab, _, _ = s.query_cc()
+            plain_labels[i] = lab
+        lat_plain.append((time.perf_counter() - q0) * 1e3)
+    for st in streams:
+        st.close()
+    for s in stores:
+        s.zero()
+    log(f"plain: {total_upd / plain_s:.3e} updates/s, all-graph CC {np.median(lat_plain):.1f} ms")
+
+    with gs.StoreGroup(stores) as grp:
+        glob = np.concatenate([p + np.uint32(b) for (_, _, p), b in zip(work, grp.bases)])
+        glob = glob[np.random.default_rng(1).permutation(len(glob))]  # graphs interleaved
+        gbuf = gs.DeviceBuffer(glob)
+        del glob
+        # one launch sequence covers 2^25 updates of all graphs at once: a vertex's updates
+        # aggregate in LDS across the whole submission
+        chunk = 1 << 25
+        sample_edges = min(chunk, total_upd)
+        best = grp.stream.autotune(gbuf.offset(0), sample_edges)  # leaves the sketches as they are
+        log(f"autotune: {best.as_dict()}")
+        gs.lib.gs_device_synchronize()
+        t0 = time.perf_counter()
+        for b in range(0, total_upd, chunk):
+            grp.submit_pairs_device(gbuf.offset(b * 8), min(chunk, total_upd - b))
+        grp.flush()
+        gs.lib.gs_device_synchronize()
+        batched_s = time.perf_counter() - t0
+        gbuf.free()
+        exact = all(np.array_equal(stores[i].export_buckets(), ref[i]) for i in sample)
+        lat_batched = []
+        same = True
+        for _ in range(args.query_reps):
+            q0 = time.perf_counter()
+            labels, ncc, rounds, status = grp.query_cc()
+            lat_batched.append((time.perf_counter() - q0) * 1e3)
+            same = same and all(np.array_equal(labels[i], plain_labels[i]) for i in range(len(work)))
+    for s in stores:
+        s.close()
+    log(f"batched: {total_upd / batched_s:.3e} updates/s, all-graph CC {np.median(lat_batched):.1f} ms, "
+        f"bit-exact sample {exact}, partitions equal {same}")
+    return {
+        "metric": "batched sketch updates/s (256 independent graphs, one group)",
+        "value": total_upd / batched_s, "unit": "edges/s", "n_gpus": 1,
+        "steps": int((total_upd + (1 << 25) - 1) // (1 << 25)), "warmup": 0,
+        "ms_per_step": batched_s * 1e3 / max(1, (total_upd + (1 << 25) - 1) // (1 << 25)),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "batched_streams", "graphs": len(work),
+                   "vertices": int(sum(n for n, _, _ in work)), "updates": int(total_upd),
+                   "sketch_gb": sketch_gb, "tuning": best.as_dict()},
+        "plain": {"value": total_upd / plain_s, "unit": "edges/s",
+                  "all_graph_cc_ms_p50": float(np.median(lat_plain))},
+        "batched": {"all_graph_cc_ms_p50": float(np.median(lat_batched)),
+                    "all_graph_cc_ms_p99": float(np.percentile(lat_batched, 99))},
+        "checks": {"members_bit_exact_sample": exact, "sample_members": len(sample),
+                   "partitions_equal_all": same},
+    }
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +196,9 @@ def main() -> None:
     ap.add_argument("--d-sample", type=int, default=2, help="batches used for the bytes model")
     ap.add_argument("--pcie-batches", type=int, default=16,
                     help="batches submitted from host memory for the PCIe-inclusive rate")
+    ap.add_argument("--mode", choices=["single", "batched"], default="single")
+    ap.add_argument("--graphs", type=int, default=256, help="batched mode: independent graphs")
+    ap.add_argument("--query-reps", type=int, default=3, help="batched mode: query repetitions")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the known-size copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -93,6 +209,10 @@ def main() -> None:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the sketch engine "
               "runs on a single MI355X in this build", file=sys.stderr)
         sys.exit(2)
+
+    if args.mode == "batched":
+        print(json.dumps(bench_batched(args)), flush=True)
+        return
 
     import gpusketch_amd as gs
     from oracle import oracle as orc

@@ -41,6 +41,15 @@ class Geometry(ctypes.Structure):
                 ("device_bytes", ctypes.c_uint64)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("lds_depths", ctypes.c_uint32), ("item_updates", ctypes.c_uint32),
+                ("bucket_bits", ctypes.c_uint32), ("kernel_ms", ctypes.c_float)]
+
+    def as_dict(self):
+        return {"lds_depths": self.lds_depths, "item_updates": self.item_updates,
+                "bucket_bits": self.bucket_bits, "kernel_ms": self.kernel_ms}
+
+
 if not os.path.exists(LIB_PATH):
     raise ImportError(f"{LIB_PATH} is not built; run `python gpusketch_amd/build.py` (hipcc, gfx950)")
 
@@ -71,6 +80,9 @@ _sigs = {
     "gs_min_cut": ([_u32, _vp, _u64, _vp, _vp], _i32),
     "gs_stream_set_profiling": ([_vp, _i32], _i32),
     "gs_stream_timing": ([_vp, _vp, _vp, _vp], _i32),
+    "gs_stream_set_tuning": ([_vp, _vp], _i32),
+    "gs_stream_get_tuning": ([_vp, _vp], _i32),
+    "gs_stream_autotune": ([_vp, _vp, _u64, _vp], _i32),
     "gs_group_create": ([_vp, _u32, _vp], _i32),
     "gs_group_destroy": ([_vp], _i32),
     "gs_group_info": ([_vp, _u32, _vp, _vp], _i32),
@@ -220,6 +232,20 @@ class Stream:
         _check(lib.gs_stream_timing(self.h, ctypes.byref(ms), ctypes.byref(launches),
                                     ctypes.byref(upd)))
         return ms.value, launches.value, upd.value
+
+    def set_tuning(self, lds_depths: int = 0, item_updates: int = 0, bucket_bits: int = 0):
+        t = Tuning(lds_depths, item_updates, bucket_bits, 0.0)
+        _check(lib.gs_stream_set_tuning(self.h, ctypes.byref(t)))
+
+    def get_tuning(self) -> Tuning:
+        t = Tuning()
+        _check(lib.gs_stream_get_tuning(self.h, ctypes.byref(t)))
+        return t
+
+    def autotune(self, dptr, nedges: int) -> Tuning:
+        t = Tuning()
+        _check(lib.gs_stream_autotune(self.h, dptr, nedges, ctypes.byref(t)))
+        return t
 
     def close(self) -> None:
         if self.h:

@@ -461,6 +461,40 @@ int gs_import_buckets(gs_store* h, uint32_t v0, uint32_t count, const gs_bucket*
   return GS_OK;
 }
 
+int gs_stream_set_tuning(gs_stream* sh, const gs_tuning* t) {
+  GS_CHECK(sh && t, GS_ERR_INVALID, "null argument");
+  GS_CHECK(t->lds_depths <= 32, GS_ERR_INVALID, "lds_depths must be <= 32");
+  GS_CHECK(t->item_updates == 0 || (t->item_updates >= 64 && t->item_updates <= 512),
+           GS_ERR_INVALID, "item_updates must be in [64, 512]");
+  GS_CHECK(t->bucket_bits <= 13, GS_ERR_INVALID, "bucket_bits must be <= 13");
+  Stream& s = sh->impl;
+  GS_HIP(hipStreamSynchronize(s.stream));
+  s.tune_dl = t->lds_depths;
+  s.tune_chunk = t->item_updates;
+  s.tune_bucket_bits = t->bucket_bits;
+  if (t->bucket_bits) {
+    s.tuned_part = make_part_params(stream_vertices(&s), (int)t->bucket_bits);
+    GS_CHECK(s.tuned_part.nbits - s.tuned_part.local_bits <= 14, GS_ERR_INVALID,
+             "bucket_bits too small for %u vertices", stream_vertices(&s));
+  }
+  return GS_OK;
+}
+
+int gs_stream_get_tuning(gs_stream* sh, gs_tuning* t) {
+  GS_CHECK(sh && t, GS_ERR_INVALID, "null argument");
+  const Stream& s = sh->impl;
+  t->lds_depths = s.tune_dl;
+  t->item_updates = s.tune_chunk;
+  t->bucket_bits = s.tune_bucket_bits;
+  t->kernel_ms = 0;
+  return GS_OK;
+}
+
+int gs_stream_autotune(gs_stream* sh, const uint32_t* d_pairs, uint64_t nedges, gs_tuning* best) {
+  GS_CHECK(sh && d_pairs, GS_ERR_INVALID, "null argument");
+  return autotune_stream(&sh->impl, d_pairs, nedges, best);
+}
+
 int gs_stream_set_profiling(gs_stream* sh, int enable) {
   GS_CHECK(sh, GS_ERR_INVALID, "null stream");
   sh->impl.profile = enable != 0;

@@ -116,9 +116,9 @@ struct PartParams {
 };
 
 // per-stream scratch for the edge-pair pipeline, grown on demand
-// Work items are grouped into length classes (class = floor(log2(len)), len <= 256) so that
+// Work items are grouped into length classes (class = floor(log2(len)), len <= 1023) so that
 // the aggregation kernel can hand out the longest items first.
-constexpr int kClasses = 9;
+constexpr int kClasses = 10;
 constexpr int kClassBase = 4;  // counters[kClassBase + k]: items of class k in the current batch
 struct ItemClasses {
   uint64_t off[kClasses];  // start of each class's region in the item array
@@ -131,6 +131,7 @@ struct PairScratch {
   uint64_t cap_items = 0;
   WorkItem* d_items = nullptr;
   uint32_t cap_buckets = 0;
+  uint32_t cur_buckets = 0;       // bucket count the count buffers were last used with
   uint32_t* d_counts = nullptr;   // 2 x nbuckets (double-buffered per batch)
   uint32_t* d_cursors = nullptr;  // nbuckets
   uint32_t parity = 0;
@@ -153,6 +154,10 @@ struct Stream {
   size_t d_stage_bytes = 0;
   hipEvent_t stage_free = nullptr;  // recorded when the kernels that read the staging are done
   bool stage_pending = false;
+  // tuning (0 = default): depths kept in the aggregation kernel's LDS delta, destinations per
+  // work item, bits of the vertex id resolved inside one grouping bucket
+  uint32_t tune_dl = 0, tune_chunk = 0, tune_bucket_bits = 0;
+  PartParams tuned_part{};
   // optional kernel timing of the aggregation kernel
   bool profile = false;
   std::vector<hipEvent_t> ev_pool;
@@ -213,6 +218,7 @@ struct Group {
 
 // vertex space and partition of whatever a stream updates
 inline const PartParams& stream_part(const Stream* s) {
+  if (s->tune_bucket_bits) return s->tuned_part;
   return s->group ? s->group->part : s->store->part;
 }
 inline uint32_t stream_vertices(const Stream* s) {
@@ -232,7 +238,8 @@ int launch_vertex_batches(Stream* s, const uint32_t* d_class_counts, const WorkI
 int ingest_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uint32_t min_depth,
                         uint32_t depth_cap);
 int ensure_pair_scratch(Stream* s, uint64_t nedges);
-PartParams make_part_params(uint32_t n);
+PartParams make_part_params(uint32_t n, int local_bits = -1);
+int autotune_stream(Stream* s, const uint32_t* d_pairs, uint64_t nedges, gs_tuning* best);
 void free_pair_scratch(PairScratch& ws);
 int launch_zero_store(Store* st, hipStream_t stream);
 int device_copy(void* dst, const void* src, size_t bytes);

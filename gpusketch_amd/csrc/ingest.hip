@@ -547,7 +547,7 @@ static int launch_aggregate(Stream* s, const WorkItem* d_items, const ItemClasse
   int exclusive_ok = 1;
   if (grp) {
     cols = grp->max_cols;
-    dl = grp->min_bpc < 12 ? grp->min_bpc : 12;
+    dl = std::min(grp->min_bpc, s->tune_dl ? s->tune_dl : 12u);
     device = grp->members[0]->device;
     for (Store* m : grp->members) {  // plain stores only if the group is every member's only writer
       std::lock_guard<std::mutex> lk(m->mu);
@@ -556,7 +556,7 @@ static int launch_aggregate(Stream* s, const WorkItem* d_items, const ItemClasse
   } else {
     v = s->store->view;
     cols = v.cols;
-    dl = lds_window(v);
+    dl = s->tune_dl ? std::min(s->tune_dl, v.bpc) : lds_window(v);
     device = s->store->device;
     std::lock_guard<std::mutex> lk(s->store->mu);
     exclusive_ok = s->store->streams.size() == 1 ? 1 : 0;
@@ -607,13 +607,13 @@ void free_pair_scratch(PairScratch& ws) {
   ws = PairScratch{};
 }
 
-PartParams make_part_params(uint32_t n) {
+PartParams make_part_params(uint32_t n, int local_bits) {
   PartParams p{};
   uint32_t nbits = 1;
   while (nbits < 32 && (1ULL << nbits) < n) nbits++;
-  int lb = (int)nbits - 8;
+  int lb = local_bits >= 0 ? local_bits : (int)nbits - 8;
   p.nbits = nbits;
-  p.local_bits = (uint32_t)std::min(13, std::max(0, lb));
+  p.local_bits = (uint32_t)std::min(13, std::min((int)nbits, std::max(0, lb)));
   p.nbuckets = 1u << (nbits - p.local_bits);
   p.mask = nbits >= 32 ? 0xFFFFFFFFu : (uint32_t)((1ULL << nbits) - 1);
   p.mul = 0x9E3779B1u;  // odd: a bijection modulo 2^nbits
@@ -638,6 +638,12 @@ int ensure_pair_scratch(Stream* s, uint64_t nedges) {
     GS_HIP(hipMemsetAsync(ws.d_counts, 0, sizeof(uint32_t) * 2 * pp.nbuckets, s->stream));
     GS_HIP(hipMemsetAsync(ws.d_counters, 0, sizeof(uint32_t) * kCounterWords, s->stream));
     ws.cap_buckets = pp.nbuckets;
+    ws.cur_buckets = pp.nbuckets;
+    ws.parity = 0;
+  }
+  if (ws.cur_buckets != pp.nbuckets) {  // re-tuned: both count buffers must start at zero
+    GS_HIP(hipMemsetAsync(ws.d_counts, 0, sizeof(uint32_t) * 2 * ws.cap_buckets, s->stream));
+    ws.cur_buckets = pp.nbuckets;
     ws.parity = 0;
   }
   if (ws.cap_edges < nedges) {
@@ -652,10 +658,12 @@ int ensure_pair_scratch(Stream* s, uint64_t nedges) {
     // class k holds items of length >= 2^k, at most one shorter-than-chunk item per vertex
     const uint64_t verts = std::min<uint64_t>((uint64_t)stream_vertices(s), 2 * cap);
     uint64_t icap = 0;
+    // classes of full items (len >= 128, items of one vertex may repeat) are bounded by the
+    // entries alone; shorter items are a vertex's last, at most one per vertex
     for (int k = 0; k < kClasses; k++) {
       ws.cls.off[k] = icap;
-      uint64_t ck = std::min<uint64_t>(verts, (2 * cap) >> k);
-      if (k == kClasses - 1) ck = (2 * cap) / kChunk;
+      uint64_t ck = (2 * cap) >> k;
+      if (k < 7) ck = std::min<uint64_t>(verts, ck);
       icap += ck + 16;
     }
     GS_HIP(hipMalloc(&ws.d_entries, sizeof(uint2) * 2 * cap));
@@ -698,12 +706,72 @@ int ingest_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uin
   GS_HIP(hipGetLastError());
   hipLaunchKernelGGL(bucket_sort_kernel, dim3(pp.nbuckets), dim3(kSortThreads), sort_lds,
                      s->stream, ws.d_entries, pp, counts_cur, ws.d_sorted, ws.d_items, ws.cls,
-                     kChunk, ws.d_counters + kClassBase,
+                     s->tune_chunk ? s->tune_chunk : kChunk, ws.d_counters + kClassBase,
                      s->group ? (const uint32_t*)s->group->d_bases : (const uint32_t*)nullptr,
                      s->group ? (uint32_t)s->group->members.size() : 1u);
   GS_HIP(hipGetLastError());
   return launch_aggregate(s, ws.d_items, ws.cls, ws.d_counters + kClassBase, ws.d_sorted,
                           2 * nedges);
+}
+
+// ------------------------------------------------------------------------------------------
+// Autotuner: times the whole ingest (grouping + aggregation) of a device-resident sample batch
+// for every candidate (LDS depth window, destinations per work item, bucket bits). Each
+// candidate applies the batch twice, which toggles every update back out, so the sketches end
+// as they started. The fastest candidate (best of two timings) stays set on the stream.
+// ------------------------------------------------------------------------------------------
+int autotune_stream(Stream* s, const uint32_t* d_pairs, uint64_t nedges, gs_tuning* best) {
+  GS_CHECK(nedges > 0, GS_ERR_INVALID, "autotune needs a sample batch");
+  const uint32_t min_bpc = s->group ? s->group->min_bpc : s->store->view.bpc;
+  const PartParams base = s->group ? s->group->part : s->store->part;
+  std::vector<uint32_t> dls, chunks = {128, 256, 512}, bits;
+  for (uint32_t d : {8u, 10u, 12u, 14u, 16u})
+    if (d <= min_bpc) dls.push_back(d);
+  if (dls.empty()) dls.push_back(min_bpc);
+  for (int b = (int)base.local_bits - 1; b <= (int)base.local_bits + 1; b++)
+    if (b >= 0 && b <= 13 && b <= (int)base.nbits && (base.nbits - b) <= 14) bits.push_back((uint32_t)b);
+  hipEvent_t e0, e1;
+  GS_HIP(hipEventCreate(&e0));
+  GS_HIP(hipEventCreate(&e1));
+  const bool prof = s->profile;
+  s->profile = false;
+  gs_tuning win{0, 0, 0, 1e30f};
+  int rc = GS_OK;
+  for (uint32_t b : bits)
+    for (uint32_t c : chunks)
+      for (uint32_t d : dls) {
+        s->tune_dl = d;
+        s->tune_chunk = c;
+        s->tune_bucket_bits = b;
+        s->tuned_part = make_part_params(stream_vertices(s), (int)b);
+        float t_best = 1e30f;
+        for (int rep = 0; rep < 2 && rc == GS_OK; rep++) {
+          if (hipEventRecord(e0, s->stream) != hipSuccess) rc = fail(GS_ERR_HIP, "event record");
+          if (!rc) rc = ingest_pairs_device(s, d_pairs, nedges, 0, 0);
+          if (!rc && hipEventRecord(e1, s->stream) != hipSuccess) rc = fail(GS_ERR_HIP, "event record");
+          if (!rc) rc = ingest_pairs_device(s, d_pairs, nedges, 0, 0);  // toggles it back out
+          if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = fail(GS_ERR_HIP, "event sync");
+          float ms = 0;
+          if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = fail(GS_ERR_HIP, "elapsed");
+          if (!rc) t_best = std::min(t_best, ms);
+        }
+        if (rc) break;
+        if (t_best < win.kernel_ms) win = gs_tuning{d, c, b, t_best};
+      }
+  (void)hipStreamSynchronize(s->stream);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  s->profile = prof;
+  if (rc) {
+    s->tune_dl = s->tune_chunk = s->tune_bucket_bits = 0;
+    return rc;
+  }
+  s->tune_dl = win.lds_depths;
+  s->tune_chunk = win.item_updates;
+  s->tune_bucket_bits = win.bucket_bits;
+  s->tuned_part = make_part_params(stream_vertices(s), (int)win.bucket_bits);
+  if (best) *best = win;
+  return GS_OK;
 }
 
 __global__ void copy16_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n) {

@@ -155,6 +155,24 @@ int gs_min_cut(uint32_t n, const uint32_t* pairs, uint64_t m, uint64_t* value,
 int gs_stream_set_profiling(gs_stream* s, int enable);
 int gs_stream_timing(gs_stream* s, double* agg_ms, uint64_t* launches, uint64_t* updates);
 
+/* ---- Tuning ----------------------------------------------------------------------------------
+ * Launch parameters of a stream's update path (0 = default): lds_depths = depths of every column
+ * kept in the aggregation kernel's LDS delta (default 12), item_updates = destinations per work
+ * item (128..512, default 256), bucket_bits = vertex-id bits resolved inside one grouping
+ * bucket (default log2(n) - 8, at most 13). The reference fixes its launch shape at compile time
+ * (num_device_threads = 1024, num_batch_per_buffer = 540, include/cc_gpu_sketch_alg.h:16-23). */
+typedef struct gs_tuning {
+  uint32_t lds_depths;
+  uint32_t item_updates;
+  uint32_t bucket_bits;
+  float kernel_ms; /* autotune: the winner's time for the sample batch */
+} gs_tuning;
+int gs_stream_set_tuning(gs_stream* s, const gs_tuning* t);
+int gs_stream_get_tuning(gs_stream* s, gs_tuning* t);
+/* Time every candidate on a device-resident sample batch (each applied twice, which leaves the
+ * sketches unchanged) and keep the fastest on the stream. */
+int gs_stream_autotune(gs_stream* s, const uint32_t* d_pairs, uint64_t nedges, gs_tuning* best);
+
 /* ---- Batched streams: groups of stores ------------------------------------------------------
  * Many independent graphs (stores of any sizes, seeds and geometries) updated by one launch
  * sequence of the grouping and aggregation kernels and queried by one batched Borůvka. Member i's

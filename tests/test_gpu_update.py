@@ -200,3 +200,41 @@ def test_full_size_rmat_sampled_vertices_bit_exact():
         others = np.concatenate([pairs[pairs[:, 0] == v, 1], pairs[pairs[:, 1] == v, 0]])
         others = others[others != v]
         assert np.array_equal(got[i], orc.vertex_sketch(g, 0xABCDEF, v, others)), v
+
+
+@pytest.mark.parametrize("lds_depths,item_updates,bucket_bits", [(8, 128, 4), (16, 512, 6), (12, 64, 0)])
+def test_tuned_launch_parameters_match_oracle(lds_depths, item_updates, bucket_bits):
+    pairs = gs.gen_rmat(12, 120000, 21, del_frac=0.05)
+    n = 1 << 12
+    with gs.SketchStore(n, 99) as st:
+        s = st.stream()
+        s.set_tuning(lds_depths, item_updates, bucket_bits)
+        for b in range(0, len(pairs), 40000):
+            s.submit_pairs(pairs[b:b + 40000])
+        s.flush()
+        got = st.export_buckets()
+    assert np.array_equal(got, oracle_store(n, 99, pairs))
+
+
+def test_autotune_leaves_sketches_unchanged_and_keeps_parity():
+    pairs = gs.gen_rmat(13, 300000, 33, del_frac=0.05)
+    n = 1 << 13
+    with gs.SketchStore(n, 5150) as st:
+        s = st.stream()
+        s.submit_pairs(pairs[:100000])
+        s.flush()
+        before = st.export_buckets()
+        buf = gs.DeviceBuffer(pairs[100000:200000])
+        best = s.autotune(buf.offset(0), 100000)
+        s.flush()
+        assert np.array_equal(st.export_buckets(), before)  # every candidate applied twice
+        assert best.item_updates in (128, 256, 512) and best.lds_depths in (8, 10, 12, 14, 16)
+        t = s.get_tuning()
+        assert (t.lds_depths, t.item_updates, t.bucket_bits) == (
+            best.lds_depths, best.item_updates, best.bucket_bits)
+        s.submit_pairs_device(buf.offset(0), 100000)
+        s.submit_pairs(pairs[200000:])
+        s.flush()
+        buf.free()
+        got = st.export_buckets()
+    assert np.array_equal(got, oracle_store(n, 5150, pairs))
