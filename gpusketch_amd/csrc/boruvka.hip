@@ -598,11 +598,24 @@ int group_query_cc(Group* g, const uint8_t* mask, uint32_t* labels, uint32_t* nu
   }
   GS_HIP(hipMemcpyAsync(g->h_gstate, g->d_gstate, sizeof(uint32_t) * (3 * (size_t)G + 2),
                         hipMemcpyDeviceToHost, q));
-  GS_HIP(hipMemcpyAsync(labels, g->d_comp, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, q));
+  // labels of the queried members only
+  for (uint32_t i = 0; i < G; i++) {
+    if (mask && !mask[i]) continue;
+    const uint32_t base = g->bases[i], n = g->members[i]->view.n;
+    GS_HIP(hipMemcpyAsync(labels + base, g->d_comp + base, sizeof(uint32_t) * n,
+                          hipMemcpyDeviceToHost, q));
+  }
   GS_HIP(hipStreamSynchronize(q));
   int rc = GS_OK;
   for (uint32_t i = 0; i < G; i++) {
     const uint32_t base = g->bases[i], n = g->members[i]->view.n;
+    if (mask && !mask[i]) {  // not queried: identity labels
+      for (uint32_t v = 0; v < n; v++) labels[base + v] = v;
+      if (num_cc) num_cc[i] = n;
+      if (rounds) rounds[i] = 0;
+      if (status) status[i] = GS_OK;
+      continue;
+    }
     uint32_t cc = 0;
     for (uint32_t v = 0; v < n; v++) {
       labels[base + v] -= base;  // local smallest vertex of the component

@@ -14,6 +14,7 @@
 #include "gpusketch/graph_sketch_driver.h"
 #include "gpusketch/mc_gpu_sketch_alg.h"
 #include "gpusketch/sk_gpu_sketch_alg.h"
+#include "gpusketch/sketch_server.h"
 
 static int failures = 0;
 #define CHECK(cond)                                                           \
@@ -289,6 +290,60 @@ static void test_mc_planted_cut() {
   std::remove(path.c_str());
 }
 
+// admission by sketch memory: a tenant that does not fit waits until another is removed, then
+// its queued requests are answered; answers equal those of a plain server
+static void test_server_admission() {
+  TenantSpec a{3000, 11, 1}, b{5000, 12, 1}, c{4000, 13, 1};
+  gs_geometry ga{}, gb{}, gc{};
+  gs_throw_if(gs_geometry_for(a.num_vertices, 1, 1, &ga));
+  gs_throw_if(gs_geometry_for(b.num_vertices, 1, 1, &gb));
+  gs_throw_if(gs_geometry_for(c.num_vertices, 1, 1, &gc));
+  ServerConfig cfg;
+  cfg.memory_budget_bytes = ga.device_bytes + gb.device_bytes + gc.device_bytes / 2;
+  cfg.keep_labels = true;
+  std::mt19937_64 rng(77);
+  auto batch = [&](uint32_t n, size_t m) {
+    std::vector<uint32_t> p(2 * m);
+    for (size_t i = 0; i < m; i++) {
+      p[2 * i] = rng() % n;
+      do p[2 * i + 1] = rng() % n; while (p[2 * i + 1] == p[2 * i]);
+    }
+    return p;
+  };
+  auto pc = batch(c.num_vertices, 3000), pa = batch(a.num_vertices, 2000);
+  std::vector<std::vector<RequestResult>> answers;
+  for (int batching = 0; batching < 2; batching++) {
+    cfg.batching = batching != 0;
+    SketchServer srv(cfg);
+    uint32_t ta = srv.add_tenant(a), tb = srv.add_tenant(b), tc = srv.add_tenant(c);
+    srv.wait_admissions();
+    CHECK(srv.is_admitted(ta) && srv.is_admitted(tb) && !srv.is_admitted(tc));
+    srv.submit_update(ta, pa.data(), pa.size() / 2);
+    srv.submit_update(tc, pc.data(), pc.size() / 2);  // queued until tc is admitted
+    srv.submit_cc_query(ta);
+    srv.submit_cc_query(tc);
+    srv.remove_tenant(tb);  // frees b's memory: c gets in
+    srv.drain();
+    CHECK(srv.is_admitted(tc) && srv.error().empty());
+    auto res = srv.results();
+    std::sort(res.begin(), res.end(),
+              [](const RequestResult& x, const RequestResult& y) { return x.id < y.id; });
+    CHECK(res.size() == 4);
+    answers.push_back(res);
+    // the CC answers are the exact partitions of the net graphs
+    std::vector<Edge> ea, ec;
+    for (size_t i = 0; i < pa.size(); i += 2) ea.push_back({pa[i], pa[i + 1]});
+    for (size_t i = 0; i < pc.size(); i += 2) ec.push_back({pc[i], pc[i + 1]});
+    ConnectedComponents cca(a.num_vertices, res[2].labels.data());
+    ConnectedComponents ccc(c.num_vertices, res[3].labels.data());
+    CHECK(same_partition(cca, dsu_labels(a.num_vertices, net_edges(ea))));
+    CHECK(same_partition(ccc, dsu_labels(c.num_vertices, net_edges(ec))));
+  }
+  for (size_t i = 0; i < 4; i++)
+    CHECK(answers[0][i].num_cc == answers[1][i].num_cc &&
+          answers[0][i].partition_hash == answers[1][i].partition_hash);
+}
+
 int main(int argc, char** argv) {
   if (argc != 3) {
     std::fprintf(stderr, "usage: facade_gpu <fixture.data> <work_dir>\n");
@@ -300,6 +355,7 @@ int main(int argc, char** argv) {
   run("eager_dsu_matches_device", test_eager_dsu_matches_device);
   run("sk_paths_match_cc", test_sk_paths_match_cc);
   run("mc_planted_cut", test_mc_planted_cut);
+  run("server_admission", test_server_admission);
   std::printf("%d failure(s)\n", failures);
   return failures ? 1 : 0;
 }

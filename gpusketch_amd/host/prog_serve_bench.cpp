@@ -1,0 +1,252 @@
+// Serving benchmark and mix-versus-alone check for SketchServer.
+//   serve_bench [--tenants 64] [--requests 1000] [--rate 2000] [--budget-gb 200] [--seed 7]
+//               [--mc-tenants 8] [--k 4] [--min-n 4096] [--max-n 262144]
+//               [--max-edges 1000000] [--alone-check 0|1]
+// Tenants: n log-uniform in [min-n, max-n]. Requests arrive as a Poisson process of the given
+// rate (requests/s): 1% connectivity queries on random tenants, 0.5% min-cut queries (k) on the
+// first mc-tenants tenants, the rest update batches of log-uniform 1k..max-edges edges, 5% of
+// which delete an earlier inserted edge of the tenant. The same trace runs twice: plain (one
+// request at a time on per-tenant streams) and served (continuous batching). Prints one JSON
+// line: updates/s and p50/p99 latencies per mode, and whether every answer and every final
+// sketch is identical in both modes (and, with --alone-check, for a tenant served alone).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <random>
+#include <string>
+#include <thread>
+
+#include "gpusketch/sketch_server.h"
+
+struct Req {
+  double t;  // arrival offset (s)
+  uint32_t tenant;
+  RequestKind kind;
+  std::vector<uint32_t> pairs;
+};
+
+struct Trace {
+  std::vector<TenantSpec> tenants;
+  std::vector<Req> reqs;
+  uint64_t total_updates = 0;
+};
+
+static Trace make_trace(int ntenants, int nreq, double rate, uint64_t seed, int mc_tenants,
+                        uint32_t min_n, uint32_t max_n, uint64_t max_edges) {
+  std::mt19937_64 rng(seed);
+  std::uniform_real_distribution<double> U(0.0, 1.0);
+  Trace tr;
+  for (int i = 0; i < ntenants; i++) {
+    const double lg = std::log2((double)min_n) + U(rng) * (std::log2((double)max_n) - std::log2((double)min_n));
+    TenantSpec s;
+    s.num_vertices = (node_id_t)std::floor(std::pow(2.0, lg));
+    s.seed = 0xC0DE0000ull + 7919ull * i;
+    tr.tenants.push_back(s);
+  }
+  std::vector<std::vector<uint32_t>> history(ntenants);
+  double t = 0;
+  std::exponential_distribution<double> gap(rate > 0 ? rate : 1.0);
+  for (int q = 0; q < nreq; q++) {
+    if (rate > 0) t += gap(rng);
+    Req r;
+    r.t = t;
+    const double x = U(rng);
+    if (x < 0.01) {
+      r.kind = RequestKind::CC_QUERY;
+      r.tenant = (uint32_t)(rng() % ntenants);
+    } else if (x < 0.015 && mc_tenants > 0) {
+      r.kind = RequestKind::MINCUT_QUERY;
+      r.tenant = (uint32_t)(rng() % std::min(mc_tenants, ntenants));
+    } else {
+      r.kind = RequestKind::UPDATE;
+      r.tenant = (uint32_t)(rng() % ntenants);
+      const uint32_t n = tr.tenants[r.tenant].num_vertices;
+      const uint64_t m = (uint64_t)std::floor(std::exp(std::log(1000.0) + U(rng) * (std::log((double)max_edges) - std::log(1000.0))));
+      auto& h = history[r.tenant];
+      r.pairs.resize(2 * m);
+      for (uint64_t e = 0; e < m; e++) {
+        uint32_t u, v;
+        if (!h.empty() && U(rng) < 0.05) {  // delete an earlier inserted edge
+          const size_t j = rng() % (h.size() / 2);
+          u = h[2 * j];
+          v = h[2 * j + 1];
+        } else {
+          do {
+            u = (uint32_t)(rng() % n);
+            v = (uint32_t)(rng() % n);
+          } while (u == v);
+          h.push_back(u);
+          h.push_back(v);
+        }
+        r.pairs[2 * e] = u;
+        r.pairs[2 * e + 1] = v;
+      }
+      tr.total_updates += m;
+    }
+    tr.reqs.push_back(std::move(r));
+  }
+  return tr;
+}
+
+struct RunStats {
+  double makespan = 0;
+  double upd_per_s = 0;
+  std::map<std::string, std::pair<double, double>> p50p99;  // kind -> (p50 ms, p99 ms)
+  uint64_t steps = 0;
+  std::map<uint64_t, RequestResult> by_index;  // trace index -> result
+  std::vector<std::vector<uint64_t>> exports;
+  std::string error;
+};
+
+static double pct(std::vector<double> v, double p) {
+  if (v.empty()) return 0;
+  std::sort(v.begin(), v.end());
+  const double idx = p / 100.0 * (v.size() - 1);
+  const size_t lo = (size_t)std::floor(idx), hi = (size_t)std::ceil(idx);
+  return v[lo] + (v[hi] - v[lo]) * (idx - lo);
+}
+
+static RunStats run(const Trace& tr, bool batching, uint64_t budget, uint32_t k,
+                    const std::vector<uint32_t>& only_tenants, bool keep_exports) {
+  ServerConfig cfg;
+  cfg.batching = batching;
+  cfg.memory_budget_bytes = budget;
+  SketchServer srv(cfg);
+  std::vector<int> tid(tr.tenants.size(), -1);
+  for (size_t i = 0; i < tr.tenants.size(); i++)
+    if (only_tenants.empty() ||
+        std::find(only_tenants.begin(), only_tenants.end(), (uint32_t)i) != only_tenants.end())
+      tid[i] = (int)srv.add_tenant(tr.tenants[i]);
+  srv.wait_admissions();  // the clock then measures requests only
+  std::map<uint64_t, uint64_t> id_to_index;
+  const double start = srv.now_s();
+  for (size_t q = 0; q < tr.reqs.size(); q++) {
+    const Req& r = tr.reqs[q];
+    if (tid[r.tenant] < 0 || !srv.is_admitted((uint32_t)tid[r.tenant])) continue;
+    const double due = start + r.t;
+    double now = srv.now_s();
+    if (due > now) std::this_thread::sleep_for(std::chrono::duration<double>(due - now));
+    uint64_t id = 0;
+    const uint32_t t = (uint32_t)tid[r.tenant];
+    if (r.kind == RequestKind::UPDATE) id = srv.submit_update(t, r.pairs.data(), r.pairs.size() / 2);
+    else if (r.kind == RequestKind::CC_QUERY) id = srv.submit_cc_query(t);
+    else id = srv.submit_mincut_query(t, k);
+    id_to_index[id] = q;
+  }
+  srv.drain();
+  RunStats st;
+  st.error = srv.error();
+  auto res = srv.results();
+  double last = start, first = 1e300;
+  std::map<std::string, std::vector<double>> lat;
+  uint64_t upd = 0;
+  for (auto& r : res) {
+    last = std::max(last, r.done_s);
+    first = std::min(first, r.arrival_s);
+    const char* kind = r.kind == RequestKind::UPDATE ? "update"
+                       : r.kind == RequestKind::CC_QUERY ? "cc_query" : "mincut_query";
+    lat[kind].push_back((r.done_s - r.arrival_s) * 1e3);
+    upd += r.updates;
+    st.by_index[id_to_index[r.id]] = r;
+  }
+  st.makespan = last - std::min(first, start);
+  st.upd_per_s = upd / st.makespan;
+  for (auto& kv : lat) st.p50p99[kv.first] = {pct(kv.second, 50), pct(kv.second, 99)};
+  st.steps = srv.steps();
+  if (keep_exports)
+    for (size_t i = 0; i < tr.tenants.size(); i++)
+      if (tid[i] >= 0 && srv.is_admitted((uint32_t)tid[i]))
+        st.exports.push_back(srv.export_tenant((uint32_t)tid[i]));
+  return st;
+}
+
+static bool same_answers(const RunStats& a, const RunStats& b, std::string* why) {
+  for (auto& kv : b.by_index) {
+    auto it = a.by_index.find(kv.first);
+    if (it == a.by_index.end()) continue;
+    const RequestResult &x = it->second, &y = kv.second;
+    if (x.status != y.status || x.num_cc != y.num_cc || x.partition_hash != y.partition_hash ||
+        x.mincut != y.mincut) {
+      *why = "request " + std::to_string(kv.first) + " differs";
+      return false;
+    }
+  }
+  return true;
+}
+
+static std::string stats_json(const RunStats& s) {
+  std::string j = "{\"updates_per_s\": " + std::to_string(s.upd_per_s) +
+                  ", \"makespan_s\": " + std::to_string(s.makespan) +
+                  ", \"steps\": " + std::to_string(s.steps);
+  for (auto& kv : s.p50p99)
+    j += ", \"" + kv.first + "_ms\": {\"p50\": " + std::to_string(kv.second.first) +
+         ", \"p99\": " + std::to_string(kv.second.second) + "}";
+  return j + "}";
+}
+
+int main(int argc, char** argv) {
+  int ntenants = 64, nreq = 1000, mc = 8, alone = 1;
+  double rate = 2000, budget_gb = 200;
+  uint64_t seed = 7, max_edges = 1000000;
+  uint32_t k = 4, min_n = 4096, max_n = 262144;
+  for (int i = 1; i + 1 < argc; i += 2) {
+    std::string a = argv[i];
+    const char* v = argv[i + 1];
+    if (a == "--tenants") ntenants = std::atoi(v);
+    else if (a == "--requests") nreq = std::atoi(v);
+    else if (a == "--rate") rate = std::atof(v);
+    else if (a == "--budget-gb") budget_gb = std::atof(v);
+    else if (a == "--seed") seed = std::stoull(v);
+    else if (a == "--mc-tenants") mc = std::atoi(v);
+    else if (a == "--k") k = (uint32_t)std::atoi(v);
+    else if (a == "--min-n") min_n = (uint32_t)std::atoi(v);
+    else if (a == "--max-n") max_n = (uint32_t)std::atoi(v);
+    else if (a == "--max-edges") max_edges = std::stoull(v);
+    else if (a == "--alone-check") alone = std::atoi(v);
+    else {
+      std::fprintf(stderr, "unknown option %s\n", a.c_str());
+      return 2;
+    }
+  }
+  const uint64_t budget = (uint64_t)(budget_gb * 1e9);
+  auto t0 = std::chrono::steady_clock::now();
+  Trace tr = make_trace(ntenants, nreq, rate, seed, mc, min_n, max_n, max_edges);
+  std::fprintf(stderr, "[serve] trace: %d tenants, %zu requests, %llu updates (%.1fs)\n",
+               ntenants, tr.reqs.size(), (unsigned long long)tr.total_updates,
+               std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+  RunStats plain = run(tr, false, budget, k, {}, true);
+  std::fprintf(stderr, "[serve] plain: %.3e updates/s over %.3fs\n", plain.upd_per_s, plain.makespan);
+  RunStats served = run(tr, true, budget, k, {}, true);
+  std::fprintf(stderr, "[serve] served: %.3e updates/s over %.3fs in %llu steps\n",
+               served.upd_per_s, served.makespan, (unsigned long long)served.steps);
+  std::string why;
+  bool same = same_answers(plain, served, &why);
+  bool same_sketch = plain.exports == served.exports;
+  bool alone_ok = true;
+  if (alone) {
+    // the first tenant with a query, served alone, must answer exactly as in the mix
+    uint32_t t = 0;
+    for (auto& r : tr.reqs)
+      if (r.kind != RequestKind::UPDATE) {
+        t = r.tenant;
+        break;
+      }
+    RunStats solo = run(tr, true, budget, k, {t}, false);
+    alone_ok = same_answers(served, solo, &why);
+  }
+  uint64_t nq = 0;
+  for (auto& r : tr.reqs) nq += r.kind != RequestKind::UPDATE;
+  std::printf("{\"tenants\": %d, \"requests\": %zu, \"queries\": %llu, \"updates\": %llu, "
+              "\"rate_req_per_s\": %g, \"budget_gb\": %g, \"k\": %u, \"plain\": %s, "
+              "\"served\": %s, \"answers_identical\": %s, \"sketches_identical\": %s, "
+              "\"alone_equals_mix\": %s, \"error\": \"%s\"}\n",
+              ntenants, tr.reqs.size(), (unsigned long long)nq,
+              (unsigned long long)tr.total_updates, rate, budget_gb, k,
+              stats_json(plain).c_str(), stats_json(served).c_str(), same ? "true" : "false",
+              same_sketch ? "true" : "false", alone_ok ? "true" : "false",
+              (plain.error + served.error).c_str());
+  return (same && same_sketch && alone_ok && plain.error.empty() && served.error.empty()) ? 0 : 1;
+}

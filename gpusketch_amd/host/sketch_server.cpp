@@ -1,0 +1,406 @@
+// SketchServer (see sketch_server.h): admission, arrival-ordered scheduling, continuous batching
+// of many tenants' updates into one store-group submission and of their connectivity queries
+// into one batched Boruvka. Every device call is made from the server's worker thread.
+#include "gpusketch/sketch_server.h"
+
+#include <algorithm>
+#include <chrono>
+#include <stdexcept>
+#include <string>
+
+#include "gpusketch/device_sketch_store.h"
+#include "gpusketch/sketch.h"
+
+namespace {
+uint64_t fnv1a(const uint32_t* p, size_t n) {
+  uint64_t h = 1469598103934665603ull;
+  for (size_t i = 0; i < n; i++) {
+    h ^= p[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+}  // namespace
+
+SketchServer::SketchServer(ServerConfig c) : cfg(c), t0(std::chrono::steady_clock::now()) {
+  worker = std::thread([this] { loop(); });
+}
+
+SketchServer::~SketchServer() {
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    stop = true;
+  }
+  cv.notify_all();
+  worker.join();
+  if (group) gs_group_destroy(group);
+  for (auto& t : tenants) {
+    if (t->stream) gs_stream_destroy(t->stream);
+    if (t->store) gs_store_destroy(t->store);
+  }
+}
+
+double SketchServer::now_s() const {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+uint32_t SketchServer::add_tenant(const TenantSpec& spec) {
+  gs_geometry g{};
+  gs_throw_if(gs_geometry_for(spec.num_vertices, spec.sketches_factor, 1, &g));
+  if (g.device_bytes > cfg.memory_budget_bytes)
+    throw std::invalid_argument("tenant needs " + std::to_string(g.device_bytes) +
+                                " bytes, more than the whole budget");
+  std::lock_guard<std::mutex> lk(mu);
+  auto t = std::make_unique<Tenant>();
+  t->spec = spec;
+  t->bytes = g.device_bytes;
+  tenants.push_back(std::move(t));
+  const uint32_t id = (uint32_t)tenants.size() - 1;
+  waiting.push_back(id);
+  cv.notify_all();
+  return id;
+}
+
+void SketchServer::remove_tenant(uint32_t t) {
+  std::lock_guard<std::mutex> lk(mu);
+  tenants.at(t)->removing = true;
+  cv.notify_all();
+}
+
+bool SketchServer::is_admitted(uint32_t t) {
+  std::lock_guard<std::mutex> lk(mu);
+  return tenants.at(t)->admitted;
+}
+
+void SketchServer::wait_admissions() {
+  std::unique_lock<std::mutex> lk(mu);
+  cv.notify_all();
+  idle_cv.wait(lk, [&] {
+    return !busy && (waiting.empty() ||
+                     admitted + tenants[waiting.front()]->bytes > cfg.memory_budget_bytes);
+  });
+}
+
+uint64_t SketchServer::admitted_bytes() {
+  std::lock_guard<std::mutex> lk(mu);
+  return admitted;
+}
+
+uint64_t SketchServer::submit_update(uint32_t tenant, const uint32_t* pairs, uint64_t nedges) {
+  Request r{0, tenant, RequestKind::UPDATE, 0, std::vector<uint32_t>(pairs, pairs + 2 * nedges),
+            now_s()};
+  std::lock_guard<std::mutex> lk(mu);
+  if (tenant >= tenants.size() || tenants[tenant]->removing)
+    throw std::invalid_argument("unknown or removed tenant");
+  r.id = next_id++;
+  tenants[tenant]->queued++;
+  queue.push_back(std::move(r));
+  cv.notify_all();
+  return next_id - 1;
+}
+
+uint64_t SketchServer::submit_cc_query(uint32_t tenant) {
+  Request r{0, tenant, RequestKind::CC_QUERY, 0, {}, now_s()};
+  std::lock_guard<std::mutex> lk(mu);
+  if (tenant >= tenants.size() || tenants[tenant]->removing)
+    throw std::invalid_argument("unknown or removed tenant");
+  r.id = next_id++;
+  tenants[tenant]->queued++;
+  queue.push_back(std::move(r));
+  cv.notify_all();
+  return next_id - 1;
+}
+
+uint64_t SketchServer::submit_mincut_query(uint32_t tenant, uint32_t k) {
+  Request r{0, tenant, RequestKind::MINCUT_QUERY, k, {}, now_s()};
+  std::lock_guard<std::mutex> lk(mu);
+  if (tenant >= tenants.size() || tenants[tenant]->removing)
+    throw std::invalid_argument("unknown or removed tenant");
+  r.id = next_id++;
+  tenants[tenant]->queued++;
+  queue.push_back(std::move(r));
+  cv.notify_all();
+  return next_id - 1;
+}
+
+void SketchServer::drain() {
+  std::unique_lock<std::mutex> lk(mu);
+  idle_cv.wait(lk, [&] { return answered == next_id && !busy; });
+}
+
+std::vector<RequestResult> SketchServer::results() {
+  std::lock_guard<std::mutex> lk(mu);
+  return done;
+}
+
+std::vector<uint64_t> SketchServer::export_tenant(uint32_t t) {
+  drain();
+  std::lock_guard<std::mutex> lk(mu);
+  Tenant& tn = *tenants.at(t);
+  if (!tn.store) throw std::invalid_argument("tenant has no store");
+  gs_geometry g{};
+  gs_throw_if(gs_store_geometry(tn.store, &g));
+  std::vector<uint64_t> out((size_t)g.num_vertices * g.num_buckets * 2);
+  gs_throw_if(gs_export_buckets(tn.store, 0, g.num_vertices, reinterpret_cast<gs_bucket*>(out.data())));
+  return out;
+}
+
+// admit waiting tenants in arrival order while their memory fits (mu held, worker thread)
+void SketchServer::admit_locked() {
+  while (!waiting.empty()) {
+    Tenant& t = *tenants[waiting.front()];
+    if (admitted + t.bytes > cfg.memory_budget_bytes) break;
+    const uint32_t samples =
+        (uint32_t)Sketch::calc_cc_samples(t.spec.num_vertices, t.spec.sketches_factor);
+    gs_throw_if(gs_store_create(t.spec.num_vertices, t.spec.seed, samples, 1, 0, &t.store));
+    if (!cfg.batching) gs_throw_if(gs_stream_create(t.store, &t.stream));
+    t.admitted = true;
+    admitted += t.bytes;
+    waiting.erase(waiting.begin());
+  }
+}
+
+void SketchServer::ensure_group() {
+  std::vector<uint32_t> members;
+  std::vector<gs_store*> stores;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    for (uint32_t i = 0; i < tenants.size(); i++)
+      if (tenants[i]->admitted && !tenants[i]->removed) {
+        members.push_back(i);
+        stores.push_back(tenants[i]->store);
+      }
+  }
+  if (group && members == group_members) return;
+  if (group) gs_throw_if(gs_group_destroy(group));
+  group = nullptr;
+  group_members = members;
+  if (!members.empty())
+    gs_throw_if(gs_group_create(stores.data(), (uint32_t)stores.size(), &group));
+}
+
+void SketchServer::finish(Request& r, RequestResult res) {
+  res.id = r.id;
+  res.tenant = r.tenant;
+  res.kind = r.kind;
+  res.arrival_s = r.arrival_s;
+  res.done_s = now_s();
+  std::lock_guard<std::mutex> lk(mu);
+  tenants[r.tenant]->queued--;
+  answered++;
+  done.push_back(std::move(res));
+}
+
+void SketchServer::apply_updates(std::vector<Request*>& upd) {
+  if (upd.empty()) return;
+  if (cfg.batching) {
+    ensure_group();
+    const size_t G = group_members.size();
+    std::vector<std::vector<uint32_t>> per(G);
+    std::vector<size_t> slot(tenants.size(), (size_t)-1);
+    for (size_t i = 0; i < G; i++) slot[group_members[i]] = i;
+    for (Request* r : upd) {
+      auto& v = per[slot[r->tenant]];
+      v.insert(v.end(), r->pairs.begin(), r->pairs.end());
+    }
+    std::vector<const uint32_t*> ptr(G);
+    std::vector<uint64_t> cnt(G);
+    for (size_t i = 0; i < G; i++) {
+      ptr[i] = per[i].empty() ? nullptr : per[i].data();
+      cnt[i] = per[i].size() / 2;
+    }
+    gs_stream* gsm = nullptr;
+    gs_throw_if(gs_group_stream(group, &gsm));
+    gs_throw_if(gs_group_submit_pairs(group, ptr.data(), cnt.data()));
+    gs_throw_if(gs_flush(gsm));
+  } else {
+    for (Request* r : upd) {
+      Tenant& t = *tenants[r->tenant];
+      gs_throw_if(gs_submit_edge_pairs(t.stream, r->pairs.data(), r->pairs.size() / 2));
+      gs_throw_if(gs_flush(t.stream));
+    }
+  }
+  for (Request* r : upd) {
+    RequestResult res;
+    res.updates = r->pairs.size() / 2;
+    finish(*r, std::move(res));
+  }
+}
+
+void SketchServer::run_step(std::vector<Request>& step) {
+  num_steps++;
+  if (cfg.batching) ensure_group();
+  std::vector<Request*> upd_q, upd_o, cc, mc;
+  std::vector<char> queried(tenants.size(), 0);
+  for (auto& r : step)
+    if (r.kind != RequestKind::UPDATE) queried[r.tenant] = 1;
+  for (auto& r : step) {
+    if (r.kind == RequestKind::UPDATE) (queried[r.tenant] ? upd_q : upd_o).push_back(&r);
+    else (r.kind == RequestKind::CC_QUERY ? cc : mc).push_back(&r);
+  }
+  // 1. the updates the step's queries depend on: those of the queried tenants
+  apply_updates(upd_q);
+  // 2. connectivity queries: one batched Boruvka over the queried tenants
+  if (!cc.empty()) {
+    if (cfg.batching) {
+      const size_t G = group_members.size();
+      std::vector<uint8_t> mask(G, 0);
+      std::vector<size_t> slot(tenants.size(), (size_t)-1);
+      for (size_t i = 0; i < G; i++) slot[group_members[i]] = i;
+      for (Request* r : cc) mask[slot[r->tenant]] = 1;
+      uint32_t total = 0, base = 0;
+      gs_throw_if(gs_group_info(group, 0, &base, &total));
+      std::vector<uint32_t> labels(total), ncc(G), rounds(G);
+      std::vector<int32_t> status(G);
+      const int rc = gs_group_query_cc(group, mask.data(), labels.data(), ncc.data(),
+                                       rounds.data(), status.data());
+      if (rc != GS_OK && rc != GS_ERR_OUT_OF_SAMPLES) gs_throw_if(rc);
+      for (Request* r : cc) {
+        const size_t i = slot[r->tenant];
+        gs_throw_if(gs_group_info(group, (uint32_t)i, &base, nullptr));
+        const uint32_t n = tenants[r->tenant]->spec.num_vertices;
+        RequestResult res;
+        res.status = status[i];
+        res.num_cc = ncc[i];
+        res.partition_hash = fnv1a(labels.data() + base, n);
+        if (cfg.keep_labels) res.labels.assign(labels.begin() + base, labels.begin() + base + n);
+        finish(*r, std::move(res));
+      }
+    } else {
+      for (Request* r : cc) {
+        Tenant& t = *tenants[r->tenant];
+        const uint32_t n = t.spec.num_vertices;
+        std::vector<uint32_t> labels(n);
+        uint32_t ncc = 0, rounds = 0;
+        RequestResult res;
+        res.status = gs_query_cc(t.store, labels.data(), &ncc, &rounds, nullptr, 0, nullptr);
+        if (res.status != GS_OK && res.status != GS_ERR_OUT_OF_SAMPLES) gs_throw_if(res.status);
+        res.num_cc = ncc;
+        res.partition_hash = fnv1a(labels.data(), n);
+        if (cfg.keep_labels) res.labels = std::move(labels);
+        finish(*r, std::move(res));
+      }
+    }
+  }
+  // 3. min-cut queries: k spanning forests on the device, exact cut of their union
+  for (Request* r : mc) {
+    Tenant& t = *tenants[r->tenant];
+    const uint32_t n = t.spec.num_vertices, k = std::max<uint32_t>(1, r->k);
+    std::vector<uint32_t> edges(2 * (size_t)k * n);
+    std::vector<uint64_t> counts(k);
+    RequestResult res;
+    res.status = gs_k_spanning_forests(t.store, k, edges.data(), (uint64_t)k * n, counts.data());
+    if (res.status == GS_OK) {
+      uint64_t m = 0;
+      for (uint64_t c : counts) m += c;
+      uint64_t value = 0;
+      std::vector<uint8_t> side(n);
+      gs_throw_if(gs_min_cut(n, edges.data(), m, &value, side.data()));
+      res.mincut = std::min<uint64_t>(value, k);
+    } else if (res.status != GS_ERR_OUT_OF_SAMPLES) {
+      gs_throw_if(res.status);
+    }
+    finish(*r, std::move(res));
+  }
+  // 4. every other tenant's updates
+  apply_updates(upd_o);
+}
+
+void SketchServer::loop() {
+  for (;;) {
+    std::vector<Request> step;
+    std::vector<uint32_t> release;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      auto runnable = [&] {
+        for (auto& r : queue)
+          if (tenants[r.tenant]->admitted) return true;
+        return false;
+      };
+      auto removable = [&] {
+        for (auto& t : tenants)
+          if (t->removing && !t->removed && t->queued == 0) return true;
+        return false;
+      };
+      cv.wait(lk, [&] {
+        return stop || runnable() || removable() ||
+               (!waiting.empty() && admitted + tenants[waiting.front()]->bytes <= cfg.memory_budget_bytes);
+      });
+      if (stop && queue.empty()) return;
+      for (uint32_t i = 0; i < tenants.size(); i++)
+        if (tenants[i]->removing && !tenants[i]->removed && tenants[i]->queued == 0) {
+          tenants[i]->removed = true;
+          release.push_back(i);
+        }
+      if (release.empty()) {
+        admit_locked();
+        std::vector<char> blocked(tenants.size(), 0);
+        uint64_t nupd = 0;
+        for (auto it = queue.begin(); it != queue.end();) {
+          const uint32_t t = it->tenant;
+          if (!tenants[t]->admitted || blocked[t]) {
+            ++it;
+            continue;
+          }
+          if (it->kind == RequestKind::UPDATE) {
+            const uint64_t m = it->pairs.size() / 2;
+            if (nupd > 0 && nupd + m > cfg.max_step_updates) break;  // step full
+            nupd += m;
+          } else {
+            blocked[t] = 1;  // this tenant's later requests wait for the next step
+          }
+          step.push_back(std::move(*it));
+          it = queue.erase(it);
+          if (!cfg.batching) break;  // plain: one request per step
+        }
+      }
+      busy = true;
+    }
+    if (!release.empty()) {
+      // drop the group before the stores it refers to
+      if (group) gs_throw_if(gs_group_destroy(group));
+      group = nullptr;
+      group_members.clear();
+      std::lock_guard<std::mutex> lk(mu);
+      for (uint32_t i : release) {
+        Tenant& t = *tenants[i];
+        if (t.stream) gs_stream_destroy(t.stream);
+        if (t.store) gs_store_destroy(t.store);
+        t.stream = nullptr;
+        t.store = nullptr;
+        if (t.admitted) admitted -= t.bytes;
+        t.admitted = false;
+      }
+      admit_locked();
+    } else {
+      try {
+        run_step(step);
+      } catch (const std::exception& e) {
+        // a device error answers every unanswered request of the step with the error
+        std::lock_guard<std::mutex> lk(mu);
+        last_error = e.what();
+        for (auto& r : step) {
+          bool answered_already = false;
+          for (auto& d : done)
+            if (d.id == r.id) answered_already = true;
+          if (answered_already) continue;
+          RequestResult res;
+          res.id = r.id;
+          res.tenant = r.tenant;
+          res.kind = r.kind;
+          res.arrival_s = r.arrival_s;
+          res.done_s = now_s();
+          res.status = GS_ERR_HIP;
+          tenants[r.tenant]->queued--;
+          answered++;
+          done.push_back(std::move(res));
+        }
+      }
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      busy = false;
+    }
+    idle_cv.notify_all();
+  }
+}

@@ -1,0 +1,133 @@
+// SketchServer: many tenant graphs kept resident in device memory and served from one request
+// queue. Tenants are admitted while their sketch memory fits the budget (later ones wait until
+// memory is released). Requests (update batches, connectivity queries, min-cut queries) are
+// taken in arrival order; with batching on, one scheduling step gathers every pending request of
+// every admitted tenant up to and including that tenant's first pending query, applies all
+// gathered updates of the queried tenants in one device submission (a store group), answers all
+// gathered connectivity queries with one batched Boruvka, then the min-cut queries, then applies
+// every other tenant's gathered updates in one more submission. A query
+// therefore sees exactly the updates of its tenant that arrived before it, so every answer is
+// the one the tenant would get alone. With batching off, requests are served one at a time on
+// per-tenant streams (the plain single-stream path). No counterpart in the reference, whose
+// algorithm objects each hold one graph and are driven by one stream.
+#pragma once
+#include <atomic>
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../gpusketch.h"
+#include "types.h"
+
+struct ServerConfig {
+  uint64_t memory_budget_bytes = 200ull * 1000 * 1000 * 1000;  // sketch memory of admitted tenants
+  uint64_t max_step_updates = 1ull << 25;  // updates per device submission when batching
+  bool batching = true;
+  bool keep_labels = false;  // keep the labels of every CC answer (tests)
+};
+
+struct TenantSpec {
+  node_id_t num_vertices = 0;
+  uint64_t seed = 0;
+  double sketches_factor = 1;
+};
+
+enum class RequestKind { UPDATE, CC_QUERY, MINCUT_QUERY };
+
+struct RequestResult {
+  uint64_t id = 0;
+  uint32_t tenant = 0;
+  RequestKind kind = RequestKind::UPDATE;
+  uint64_t updates = 0;      // UPDATE: edges in the batch
+  double arrival_s = 0;      // seconds since the server started
+  double done_s = 0;
+  int status = 0;            // gs_status of the answer (e.g. out of samples)
+  uint32_t num_cc = 0;       // CC_QUERY
+  uint64_t partition_hash = 0;
+  uint64_t mincut = 0;       // MINCUT_QUERY: the cut if below k, else k
+  std::vector<node_id_t> labels;  // CC_QUERY with keep_labels
+};
+
+class SketchServer {
+ public:
+  explicit SketchServer(ServerConfig cfg);
+  ~SketchServer();
+  SketchServer(const SketchServer&) = delete;
+  SketchServer& operator=(const SketchServer&) = delete;
+
+  // register a tenant; it is admitted once its sketch memory fits the budget
+  uint32_t add_tenant(const TenantSpec& spec);
+  // release a tenant after its queued requests; its memory admits waiting tenants
+  void remove_tenant(uint32_t tenant);
+  bool is_admitted(uint32_t tenant);
+  // wait until every waiting tenant that fits the budget has been admitted
+  void wait_admissions();
+  uint64_t admitted_bytes();
+
+  // enqueue (thread-safe, returns the request id); pairs are copied
+  uint64_t submit_update(uint32_t tenant, const uint32_t* pairs, uint64_t nedges);
+  uint64_t submit_cc_query(uint32_t tenant);
+  uint64_t submit_mincut_query(uint32_t tenant, uint32_t k);
+
+  // wait until every request submitted so far is answered
+  void drain();
+  std::vector<RequestResult> results();
+  // reference-ordered buckets of a tenant (after drain), for checks
+  std::vector<uint64_t> export_tenant(uint32_t tenant);
+  double now_s() const;
+  uint64_t steps() const { return num_steps.load(); }
+  // message of the last device error that failed a step ("" if none)
+  std::string error() {
+    std::lock_guard<std::mutex> lk(mu);
+    return last_error;
+  }
+
+ private:
+  struct Request {
+    uint64_t id;
+    uint32_t tenant;
+    RequestKind kind;
+    uint32_t k;
+    std::vector<uint32_t> pairs;
+    double arrival_s;
+  };
+  struct Tenant {
+    TenantSpec spec;
+    uint64_t bytes = 0;
+    gs_store* store = nullptr;
+    gs_stream* stream = nullptr;  // plain mode
+    bool admitted = false, removing = false, removed = false;
+    uint64_t queued = 0;          // requests not yet answered
+  };
+
+  void loop();
+  void admit_locked();
+  void run_step(std::vector<Request>& step);
+  void apply_updates(std::vector<Request*>& upd);
+  void ensure_group();
+  void finish(Request& r, RequestResult res);
+
+  ServerConfig cfg;
+  std::chrono::steady_clock::time_point t0;
+  std::mutex mu;
+  std::condition_variable cv, idle_cv;
+  std::deque<Request> queue;
+  std::vector<std::unique_ptr<Tenant>> tenants;
+  std::vector<uint32_t> waiting;  // tenants not yet admitted, in arrival order
+  uint64_t admitted = 0;
+  uint64_t next_id = 0, answered = 0;
+  bool stop = false;
+  bool busy = false;
+  std::vector<RequestResult> done;
+  std::atomic<uint64_t> num_steps{0};
+  std::string last_error;
+  // store group over the admitted tenants (batching mode), rebuilt when membership changes
+  gs_group* group = nullptr;
+  std::vector<uint32_t> group_members;
+  std::thread worker;
+};
