@@ -186,6 +186,82 @@ def bench_batched(args) -> dict:
     }
 
 
+def bench_mincut(args) -> dict:
+    """Min-cut / k-connectivity: n = 2^15, two Erdos-Renyi(0.1) halves joined by exactly 5
+    crossing edges, the stream shuffled with 10% insert/delete churn of non-edges; k = 8 edge-
+    disjoint spanning forests from the sketches (device), exact minimum cut of their union
+    (host), checked against the exact minimum cut of the whole net graph."""
+    import gpusketch_amd as gs
+    n, k, cut = 1 << 15, 8, 5
+    t0 = time.perf_counter()
+    pairs = gs.gen_planted_cut(n, 0.1, cut, 0.1, SEED_STREAM)
+    log(f"generated {len(pairs)} updates in {time.perf_counter() - t0:.1f}s")
+    factor = k / np.log2(n) * 1.5  # the sketch factor of the reference's min-cut tool
+    with gs.SketchStore(n, SEED_SKETCH, sketches_factor=factor) as st:
+        s = st.stream()
+        buf = gs.DeviceBuffer(pairs)
+        gs.lib.gs_device_synchronize()
+        t0 = time.perf_counter()
+        for b in range(0, len(pairs), BATCH):
+            s.submit_pairs_device(buf.offset(b * 8), min(BATCH, len(pairs) - b))
+        s.flush()
+        gs.lib.gs_device_synchronize()
+        ingest_s = time.perf_counter() - t0
+        buf.free()
+        lat = []
+        for _ in range(3):
+            q0 = time.perf_counter()
+            forests = st.k_spanning_forests(k)
+            lat.append((time.perf_counter() - q0) * 1e3)
+        cert = np.concatenate(forests)
+        c0 = time.perf_counter()
+        value, _ = gs.min_cut(n, cert)
+        cert_ms = (time.perf_counter() - c0) * 1e3
+        geo = st.geom
+        s.close()
+    keys = (pairs.min(1).astype(np.uint64) << np.uint64(32)) | pairs.max(1).astype(np.uint64)
+    uniq, cnt = np.unique(keys, return_counts=True)
+    live = uniq[cnt % 2 == 1]
+    net = np.stack([live >> np.uint64(32), live & np.uint64(0xFFFFFFFF)], 1).astype(np.uint32)
+    e0 = time.perf_counter()
+    exact, _ = gs.min_cut(n, net)
+    exact_s = time.perf_counter() - e0
+    log(f"min cut: certificate {value}, exact {exact}; forests p50 {np.median(lat):.1f} ms")
+    return {
+        "metric": "min-cut stream: sketch updates/s and k-forest query latency (n=2^15, k=8)",
+        "value": len(pairs) / ingest_s, "unit": "edges/s", "n_gpus": 1,
+        "steps": int((len(pairs) + BATCH - 1) // BATCH), "warmup": 0,
+        "ms_per_step": ingest_s * 1e3 / ((len(pairs) + BATCH - 1) // BATCH),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "planted_cut_n32768", "n": n, "k": k, "planted_cut": cut,
+                   "p": 0.1, "churn": 0.1, "updates": int(len(pairs)),
+                   "net_edges": int(len(net)), "num_columns": geo.num_columns,
+                   "bkt_per_col": geo.bkt_per_col, "sketches_factor": factor},
+        "k_forests_ms": {"p50": float(np.median(lat)), "max": float(np.max(lat))},
+        "certificate_edges": int(len(cert)), "certificate_cut_ms": cert_ms,
+        "min_cut": int(value), "exact_min_cut": int(exact), "exact_host_s": exact_s,
+        "equal": int(value) == int(exact) == cut,
+    }
+
+
+def run_serving(rate: float, requests: int = 1000, tenants: int = 64) -> dict:
+    """The serving trace (C++ program serve_bench): tenants with n log-uniform in
+    [2^12, 2^18], Poisson arrivals of update batches of 1k..1M edges (5% deletions), 1% CC
+    queries, min-cut queries (k = 4) on 8 tenants, 200 GB admission budget; served one request
+    at a time (plain) and with continuous batching (served)."""
+    import subprocess
+    prog = os.path.join(ROOT, "gpusketch_amd", "bin", "serve_bench")
+    r = subprocess.run([prog, "--tenants", str(tenants), "--requests", str(requests),
+                        "--rate", str(rate), "--budget-gb", "200", "--k", "4",
+                        "--mc-tenants", "8"], capture_output=True, text=True, timeout=900)
+    for line in r.stderr.splitlines():
+        log(line)
+    if r.returncode != 0:
+        raise RuntimeError(f"serve_bench failed ({r.returncode}): {r.stdout} {r.stderr}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,7 +272,9 @@ def main() -> None:
     ap.add_argument("--d-sample", type=int, default=2, help="batches used for the bytes model")
     ap.add_argument("--pcie-batches", type=int, default=16,
                     help="batches submitted from host memory for the PCIe-inclusive rate")
-    ap.add_argument("--mode", choices=["single", "batched"], default="single")
+    ap.add_argument("--mode", choices=["single", "batched", "serving", "mincut"], default="single")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="single mode: skip the served-vs-plain trace")
     ap.add_argument("--graphs", type=int, default=256, help="batched mode: independent graphs")
     ap.add_argument("--query-reps", type=int, default=3, help="batched mode: query repetitions")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -212,6 +290,21 @@ def main() -> None:
 
     if args.mode == "batched":
         print(json.dumps(bench_batched(args)), flush=True)
+        return
+    if args.mode == "mincut":
+        print(json.dumps(bench_mincut(args)), flush=True)
+        return
+    if args.mode == "serving":
+        lat = run_serving(2000)
+        cap = run_serving(0)  # every request queued at once: capacity
+        print(json.dumps({
+            "metric": "served vs plain: updates/s and p50/p99 query latency (64 tenants)",
+            "value": cap["served"]["updates_per_s"], "unit": "edges/s", "n_gpus": 1,
+            "steps": cap["served"]["steps"], "warmup": 0,
+            "ms_per_step": cap["served"]["makespan_s"] * 1e3 / max(1, cap["served"]["steps"]),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+            "data": "synthetic", "config": {"workload": "serving_64_tenants"},
+            "poisson_2000_req_per_s": lat, "saturation": cap}), flush=True)
         return
 
     import gpusketch_amd as gs
@@ -290,7 +383,7 @@ def main() -> None:
         traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
 
     result = {
-        "metric": "edge updates/s at n=2^17 (single-stream GPU CC, R-MAT, 2^20-edge batches)",
+        "metric": "edge updates/s at n=2^17 + p50/p99 CC query latency, served vs plain",
         "value": args.steps * BATCH / elapsed,
         "unit": "edges/s",
         "n_gpus": 1,
@@ -319,6 +412,15 @@ def main() -> None:
     stream.close()
     store.close()
     dbuf.free()
+    if not args.no_serving:
+        sv = run_serving(2000)
+        result["serving"] = {
+            "workload": f"{sv['tenants']} tenants, {sv['requests']} Poisson requests at "
+                        f"{sv['rate_req_per_s']:g}/s, {sv['updates']} updates, {sv['queries']} queries",
+            "plain": sv["plain"], "served": sv["served"],
+            "answers_identical": sv["answers_identical"],
+            "sketches_identical": sv["sketches_identical"],
+            "alone_equals_mix": sv["alone_equals_mix"]}
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
         threads = max(1, min(threads, 16))

@@ -216,28 +216,28 @@ extern "C" int gs_gen_planted_cut(uint32_t n, double p, uint32_t cut, double chu
   if ((uint64_t)cut > (uint64_t)h * (n - h)) return GS_ERR_INVALID;
   Rng rng(seed);
   std::vector<uint64_t> base;
-  bernoulli_pairs(0, h, p, rng, base);
+  bernoulli_pairs(0, h, p, rng, base);  // keys come out in increasing order
   bernoulli_pairs(h, n - h, p, rng, base);
-  std::unordered_set<uint64_t> present(base.begin(), base.end());
-  for (uint32_t k = 0; k < cut;) {
+  auto in_base = [&](uint64_t key) { return std::binary_search(base.begin(), base.end(), key); };
+  // crossing edges are never inside a half, so only their own duplicates need a check
+  std::unordered_set<uint64_t> crossing;
+  while (crossing.size() < cut) {
     uint32_t x = (uint32_t)rng.below(h), y = h + (uint32_t)rng.below(n - h);
-    uint64_t key = key_of(x, y);
-    if (present.insert(key).second) { base.push_back(key); k++; }
+    crossing.insert(key_of(x, y));
   }
-  shuffle(base, rng);
-  const uint64_t m = base.size();
+  const uint64_t m = base.size() + crossing.size();
   const uint64_t nchurn = (uint64_t)std::llround(churn_frac * (double)m);
-  std::vector<std::pair<uint64_t, uint64_t>> extra;
-  extra.reserve(2 * nchurn);
   const uint64_t all_pairs = (uint64_t)n * (n - 1) / 2;
-  const uint64_t room = all_pairs > present.size() ? all_pairs - present.size() : 0;
-  const uint64_t churn = std::min(nchurn, room);
+  const uint64_t churn = std::min(nchurn, all_pairs > m ? all_pairs - m : 0);
+  std::vector<std::pair<uint64_t, uint64_t>> extra;
+  extra.reserve(2 * churn);
   std::unordered_set<uint64_t> used;
+  used.reserve(2 * churn);
   for (uint64_t k = 0; k < churn;) {
     uint32_t x = (uint32_t)rng.below(n), y = (uint32_t)rng.below(n);
     if (x == y) continue;
     uint64_t key = key_of(x, y);
-    if (present.count(key) || !used.insert(key).second) continue;
+    if (in_base(key) || crossing.count(key) || !used.insert(key).second) continue;
     uint64_t p1 = rng.below(m + 1), p2 = rng.below(m + 1);
     if (p1 > p2) std::swap(p1, p2);
     extra.push_back({p1, key});  // insert ...
@@ -248,6 +248,8 @@ extern "C" int gs_gen_planted_cut(uint32_t n, double p, uint32_t cut, double chu
     *count = m + extra.size();
     return GS_OK;
   }
+  for (uint64_t key : crossing) base.push_back(key);
+  shuffle(base, rng);
   if (*count < m + extra.size()) return GS_ERR_INVALID;
   std::vector<uint64_t> all = interleave(base, extra);
   emit(all, pairs);

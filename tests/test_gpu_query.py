@@ -88,6 +88,28 @@ def test_cc_config1_shape_reduced():
     assert np.array_equal(labels, orc.dsu_components(n, net_edges(pairs)))
 
 
+def test_cc_config1_full_size():
+    # the plain-CC config at full size: ER(8192, 0.25) in seeded order, 10% of the inserted
+    # edges re-deleted later (about 9.2 M updates); every bucket bit-exact, components vs DSU
+    n = 8192
+    pairs = gs.gen_erdos_renyi(n, 0.25, 0.1, 20261204)
+    assert len(pairs) > 9_000_000
+    with gs.SketchStore(n, 0xC0FFEE) as st:
+        s = st.stream()
+        for b in range(0, len(pairs), 1 << 20):
+            s.submit_pairs(pairs[b:b + (1 << 20)])
+        s.flush()
+        got = st.export_buckets()
+        labels, ncc, rounds = st.query_cc()
+    g = orc.Geometry(n)
+    assert (g.bkt_per_col, g.num_columns, g.num_buckets) == (26, 18, 469)
+    ref = g.empty_store()
+    orc.apply_edges(ref, g, 0xC0FFEE, pairs)
+    assert np.array_equal(got, ref)
+    assert ncc == 1
+    assert np.array_equal(labels, orc.dsu_components(n, net_edges(pairs)))
+
+
 def test_out_of_samples_reported():
     n = 64
     path = np.array([[i, i + 1] for i in range(n - 1)], dtype=np.uint32)
