@@ -43,11 +43,13 @@ class Geometry(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("lds_depths", ctypes.c_uint32), ("item_updates", ctypes.c_uint32),
-                ("bucket_bits", ctypes.c_uint32), ("kernel_ms", ctypes.c_float)]
+                ("bucket_bits", ctypes.c_uint32), ("launch_edges", ctypes.c_uint32),
+                ("kernel_ms", ctypes.c_float)]
 
     def as_dict(self):
         return {"lds_depths": self.lds_depths, "item_updates": self.item_updates,
-                "bucket_bits": self.bucket_bits, "kernel_ms": self.kernel_ms}
+                "bucket_bits": self.bucket_bits, "launch_edges": self.launch_edges,
+                "kernel_ms": self.kernel_ms}
 
 
 if not os.path.exists(LIB_PATH):
@@ -233,8 +235,9 @@ class Stream:
                                     ctypes.byref(upd)))
         return ms.value, launches.value, upd.value
 
-    def set_tuning(self, lds_depths: int = 0, item_updates: int = 0, bucket_bits: int = 0):
-        t = Tuning(lds_depths, item_updates, bucket_bits, 0.0)
+    def set_tuning(self, lds_depths: int = 0, item_updates: int = 0, bucket_bits: int = 0,
+                   launch_edges: int = 0):
+        t = Tuning(lds_depths, item_updates, bucket_bits, launch_edges, 0.0)
         _check(lib.gs_stream_set_tuning(self.h, ctypes.byref(t)))
 
     def get_tuning(self) -> Tuning:

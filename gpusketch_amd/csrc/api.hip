@@ -46,7 +46,7 @@ void geometry(uint32_t n, double factor, uint32_t cps, int l0, uint64_t out[5]) 
 }
 
 // ---- staging helpers -------------------------------------------------------------------------
-int ensure_stage(Stream* s, size_t bytes) {
+int ensure_stage(Stream* s, size_t bytes, bool device) {
   if (s->stage_pending) {
     GS_HIP(hipEventSynchronize(s->stage_free));
     s->stage_pending = false;
@@ -59,7 +59,7 @@ int ensure_stage(Stream* s, size_t bytes) {
       return fail(GS_ERR_OOM, "pinned staging of %zu bytes failed", cap);
     s->h_stage_bytes = cap;
   }
-  if (s->d_stage_bytes < bytes) {
+  if (device && s->d_stage_bytes < bytes) {
     GS_HIP(hipStreamSynchronize(s->stream));
     if (s->d_stage) hipFree(s->d_stage);
     s->d_stage = nullptr;
@@ -71,21 +71,64 @@ int ensure_stage(Stream* s, size_t bytes) {
   return GS_OK;
 }
 
+int init_stream_objects(Stream* s) {
+  if (!s->stream) {
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+      return fail(GS_ERR_HIP, "HIP stream creation failed");
+    s->own_stream = true;
+  }
+  if (hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(GS_ERR_HIP, "HIP copy stream creation failed");
+  hipEvent_t* evs[] = {&s->stage_free, &s->pend_free[0], &s->pend_free[1], &s->copied};
+  for (hipEvent_t* e : evs)
+    if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess)
+      return fail(GS_ERR_HIP, "HIP event creation failed");
+  return GS_OK;
+}
+
 void destroy_stream_resources(Stream& s) {
+  if (s.stream) (void)hipStreamSynchronize(s.stream);
+  if (s.copy_stream) (void)hipStreamSynchronize(s.copy_stream);
   free_pair_scratch(s.ws);
   if (s.h_stage) (void)hipHostFree(s.h_stage);
   if (s.d_stage) (void)hipFree(s.d_stage);
+  for (int b = 0; b < 2; b++) {
+    if (s.d_pend[b]) (void)hipFree(s.d_pend[b]);
+    if (s.pend_free[b]) (void)hipEventDestroy(s.pend_free[b]);
+    s.d_pend[b] = nullptr;
+    s.pend_free[b] = nullptr;
+  }
+  if (s.copied) (void)hipEventDestroy(s.copied);
   for (auto& p : s.ev_pairs) {
     (void)hipEventDestroy(p.first);
     (void)hipEventDestroy(p.second);
   }
   if (s.stage_free) (void)hipEventDestroy(s.stage_free);
-  if (s.stream) (void)hipStreamDestroy(s.stream);
+  if (s.copy_stream) (void)hipStreamDestroy(s.copy_stream);
+  if (s.stream && s.own_stream) (void)hipStreamDestroy(s.stream);
   s.h_stage = nullptr;
   s.d_stage = nullptr;
   s.ev_pairs.clear();
   s.stage_free = nullptr;
+  s.copied = nullptr;
+  s.copy_stream = nullptr;
   s.stream = nullptr;
+  s.pend_cap = s.pend_n = 0;
+}
+
+int sync_store(Store* st) {
+  std::vector<Stream*> ss;
+  {
+    std::lock_guard<std::mutex> lk(st->mu);
+    ss = st->streams;
+  }
+  for (Stream* s : ss) {
+    int rc = drain_stream(s);
+    if (rc) return rc;
+  }
+  for (Stream* s : ss) GS_HIP(hipStreamSynchronize(s->stream));
+  GS_HIP(hipStreamSynchronize(st->qstream));
+  return GS_OK;
 }
 
 int release_stage(Stream* s) {
@@ -234,10 +277,11 @@ int gs_stream_create(gs_store* h, gs_stream** out) {
   gs_stream* s = new gs_stream();
   Stream& st = s->impl;
   st.store = &h->impl;
-  if (hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&st.stage_free, hipEventDisableTiming) != hipSuccess) {
+  int rc = init_stream_objects(&st);
+  if (rc) {
+    destroy_stream_resources(st);
     delete s;
-    return fail(GS_ERR_HIP, "gs_stream_create: HIP stream/event creation failed");
+    return rc;
   }
   {
     std::lock_guard<std::mutex> lk(h->impl.mu);
@@ -251,6 +295,7 @@ int gs_stream_destroy(gs_stream* sh) {
   if (!sh) return GS_OK;
   Stream& s = sh->impl;
   GS_CHECK(!s.group, GS_ERR_INVALID, "a group's stream is destroyed with the group");
+  (void)drain_stream(&s);  // pending updates are applied before the stream goes away
   hipStreamSynchronize(s.stream);
   {
     std::lock_guard<std::mutex> lk(s.store->mu);
@@ -331,30 +376,6 @@ int gs_submit_vertex_batches_filtered(gs_stream* sh, const uint32_t* src, const 
   return submit_vertex_batches(sh, src, sizes, offsets, nbatches, dsts, min_depth, depth_cap);
 }
 
-static int submit_pairs_host(Stream* s, const uint32_t* pairs, uint64_t nedges, uint32_t min_depth,
-                             uint32_t depth_cap) {
-  const uint32_t n = s->store->view.n;
-  const uint64_t chunk = 1ULL << 22;  // edges per staged copy
-  for (uint64_t e0 = 0; e0 < nedges; e0 += chunk) {
-    uint64_t m = std::min(chunk, nedges - e0);
-    size_t bytes = m * 2 * sizeof(uint32_t);
-    int rc = ensure_stage(s, bytes);
-    if (rc) return rc;
-    const uint32_t* p = pairs + 2 * e0;
-    for (uint64_t i = 0; i < 2 * m; i++) {
-      GS_CHECK(p[i] < n, GS_ERR_INVALID, "edge %llu: vertex %u out of range",
-               (unsigned long long)(e0 + i / 2), p[i]);
-    }
-    std::memcpy(s->h_stage, p, bytes);
-    GS_HIP(hipMemcpyAsync(s->d_stage, s->h_stage, bytes, hipMemcpyHostToDevice, s->stream));
-    rc = ingest_pairs_device(s, s->d_stage, m, min_depth, depth_cap);
-    if (rc) return rc;
-    rc = release_stage(s);
-    if (rc) return rc;
-  }
-  return GS_OK;
-}
-
 int gs_submit_edge_pairs(gs_stream* sh, const uint32_t* pairs, uint64_t nedges) {
   GS_CHECK(sh && (pairs || nedges == 0), GS_ERR_INVALID, "null argument");
   return submit_pairs_host(&sh->impl, pairs, nedges, 0, 0);
@@ -369,12 +390,14 @@ int gs_submit_edge_pairs_filtered(gs_stream* sh, const uint32_t* pairs, uint64_t
 
 int gs_submit_edge_pairs_device(gs_stream* sh, const uint32_t* d_pairs, uint64_t nedges) {
   GS_CHECK(sh && (d_pairs || nedges == 0), GS_ERR_INVALID, "null argument");
-  return ingest_pairs_device(&sh->impl, d_pairs, nedges, 0, 0);
+  return submit_pairs_device(&sh->impl, d_pairs, nedges, 0, 0);
 }
 
 int gs_flush(gs_stream* sh) {
   GS_CHECK(sh, GS_ERR_INVALID, "null stream");
   Stream& s = sh->impl;
+  int rc = drain_stream(&s);
+  if (rc) return rc;
   GS_HIP(hipStreamSynchronize(s.stream));
   if (s.ws.d_counters) {
     uint32_t dropped = 0;
@@ -390,14 +413,7 @@ int gs_flush(gs_stream* sh) {
 
 int gs_sync(gs_store* h) {
   GS_CHECK(h, GS_ERR_INVALID, "null store");
-  std::vector<Stream*> ss;
-  {
-    std::lock_guard<std::mutex> lk(h->impl.mu);
-    ss = h->impl.streams;
-  }
-  for (Stream* s : ss) GS_HIP(hipStreamSynchronize(s->stream));
-  GS_HIP(hipStreamSynchronize(h->impl.qstream));
-  return GS_OK;
+  return sync_store(&h->impl);
 }
 
 int gs_export_buckets(gs_store* h, uint32_t v0, uint32_t count, gs_bucket* out) {
@@ -464,14 +480,19 @@ int gs_import_buckets(gs_store* h, uint32_t v0, uint32_t count, const gs_bucket*
 int gs_stream_set_tuning(gs_stream* sh, const gs_tuning* t) {
   GS_CHECK(sh && t, GS_ERR_INVALID, "null argument");
   GS_CHECK(t->lds_depths <= 32, GS_ERR_INVALID, "lds_depths must be <= 32");
-  GS_CHECK(t->item_updates == 0 || (t->item_updates >= 64 && t->item_updates <= 512),
-           GS_ERR_INVALID, "item_updates must be in [64, 512]");
+  GS_CHECK(t->item_updates == 0 || (t->item_updates >= 64 && t->item_updates <= 1024),
+           GS_ERR_INVALID, "item_updates must be in [64, 1024]");
   GS_CHECK(t->bucket_bits <= 13, GS_ERR_INVALID, "bucket_bits must be <= 13");
+  GS_CHECK(t->launch_edges == 0 || (t->launch_edges >= (1u << 12) && t->launch_edges <= (1u << 28)),
+           GS_ERR_INVALID, "launch_edges must be in [2^12, 2^28]");
   Stream& s = sh->impl;
+  int rc = drain_stream(&s);  // pending edges were counted under the old parameters
+  if (rc) return rc;
   GS_HIP(hipStreamSynchronize(s.stream));
   s.tune_dl = t->lds_depths;
   s.tune_chunk = t->item_updates;
   s.tune_bucket_bits = t->bucket_bits;
+  s.tune_launch = t->launch_edges;
   if (t->bucket_bits) {
     s.tuned_part = make_part_params(stream_vertices(&s), (int)t->bucket_bits);
     GS_CHECK(s.tuned_part.nbits - s.tuned_part.local_bits <= 14, GS_ERR_INVALID,
@@ -486,6 +507,7 @@ int gs_stream_get_tuning(gs_stream* sh, gs_tuning* t) {
   t->lds_depths = s.tune_dl;
   t->item_updates = s.tune_chunk;
   t->bucket_bits = s.tune_bucket_bits;
+  t->launch_edges = s.tune_launch;
   t->kernel_ms = 0;
   return GS_OK;
 }

@@ -219,9 +219,8 @@ void free_query_scratch(Store* st) {
   st->forest_cap = 0;
   if (st->qs) {
     Stream* s = st->qs;
-    (void)hipStreamSynchronize(s->stream);
-    free_pair_scratch(s->ws);
-    delete s;  // its HIP stream is the store's query stream, destroyed by the store
+    destroy_stream_resources(*s);  // its HIP stream is the store's query stream, kept
+    delete s;
     st->qs = nullptr;
   }
 }
@@ -250,6 +249,13 @@ static int ensure_query_scratch(Store* st, uint64_t forest_pairs) {
     Stream* s = new Stream();
     s->store = st;
     s->stream = st->qstream;  // queries and their forest toggles share one in-order stream
+    s->own_stream = false;
+    int rc = init_stream_objects(s);
+    if (rc) {
+      destroy_stream_resources(*s);
+      delete s;
+      return rc;
+    }
     st->qs = s;
   }
   return GS_OK;
@@ -302,15 +308,7 @@ static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t
   return GS_OK;
 }
 
-static int sync_all_streams(Store* st) {
-  std::vector<Stream*> ss;
-  {
-    std::lock_guard<std::mutex> lk(st->mu);
-    ss = st->streams;
-  }
-  for (Stream* s : ss) GS_HIP(hipStreamSynchronize(s->stream));
-  return GS_OK;
-}
+static int sync_all_streams(Store* st) { return sync_store(st); }
 
 int boruvka_query(Store* st, const uint32_t*, uint32_t* label_out, uint32_t* num_cc,
                   uint32_t* rounds, uint32_t* forest_out, uint64_t forest_cap,

@@ -92,9 +92,7 @@ int gs_group_create(gs_store* const* stores, uint32_t count, gs_group** out) {
     g.handle = new gs_stream();
     g.s = &g.handle->impl;
     g.s->group = &g;
-    if (hipStreamCreateWithFlags(&g.s->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&g.s->stage_free, hipEventDisableTiming) != hipSuccess)
-      rc = fail(GS_ERR_HIP, "gs_group_create: HIP stream/event creation failed");
+    rc = init_stream_objects(g.s);
   }
   if (rc) {
     destroy_group(h);
@@ -168,11 +166,8 @@ int gs_group_submit_pairs(gs_group* h, const uint32_t* const* pairs, const uint6
       ei += take;
     }
     if (m == 0) break;
-    GS_HIP(hipMemcpyAsync(s->d_stage, s->h_stage, m * 2 * sizeof(uint32_t),
-                          hipMemcpyHostToDevice, s->stream));
-    rc = ingest_pairs_device(s, s->d_stage, m, 0, 0);
-    if (rc) return rc;
-    rc = release_stage(s);
+    // the staging is pinned: copied straight into the stream's fill buffer, read before return
+    rc = submit_pairs_host(s, s->h_stage, m, 0, 0);
     if (rc) return rc;
   }
   return GS_OK;
@@ -181,12 +176,7 @@ int gs_group_submit_pairs(gs_group* h, const uint32_t* const* pairs, const uint6
 int gs_group_submit_pairs_device(gs_group* h, const uint32_t* d_pairs, uint64_t nedges) {
   GS_CHECK(h, GS_ERR_INVALID, "null group");
   GS_CHECK(d_pairs || nedges == 0, GS_ERR_INVALID, "null pairs");
-  const uint64_t chunk = 1ULL << 30;
-  for (uint64_t e0 = 0; e0 < nedges; e0 += chunk) {
-    int rc = ingest_pairs_device(h->impl.s, d_pairs + 2 * e0, std::min(chunk, nedges - e0), 0, 0);
-    if (rc) return rc;
-  }
-  return GS_OK;
+  return submit_pairs_device(h->impl.s, d_pairs, nedges, 0, 0);
 }
 
 int gs_group_query_cc(gs_group* h, const uint8_t* member_mask, uint32_t* labels,
@@ -195,13 +185,8 @@ int gs_group_query_cc(gs_group* h, const uint8_t* member_mask, uint32_t* labels,
   Group& g = h->impl;
   // every writer of every member must be done before the sketches are read
   for (Store* m : g.members) {
-    std::vector<Stream*> ss;
-    {
-      std::lock_guard<std::mutex> lk(m->mu);
-      ss = m->streams;
-    }
-    for (Stream* s : ss) GS_HIP(hipStreamSynchronize(s->stream));
-    GS_HIP(hipStreamSynchronize(m->qstream));
+    int rc = sync_store(m);
+    if (rc) return rc;
   }
   return group_query_cc(&g, member_mask, labels, num_cc, rounds, status);
 }

@@ -118,11 +118,14 @@ struct PartParams {
 // per-stream scratch for the edge-pair pipeline, grown on demand
 // Work items are grouped into length classes (class = floor(log2(len)), len <= 1023) so that
 // the aggregation kernel can hand out the longest items first.
-constexpr int kClasses = 10;
+constexpr int kClasses = 11;
 constexpr int kClassBase = 4;  // counters[kClassBase + k]: items of class k in the current batch
 struct ItemClasses {
   uint64_t off[kClasses];  // start of each class's region in the item array
 };
+
+// edges coalesced per launch of the grouping and aggregation kernels (default)
+constexpr uint64_t kLaunchEdges = 1ULL << 22;
 
 struct PairScratch {
   uint64_t cap_edges = 0;        // capacity in undirected edges
@@ -145,6 +148,7 @@ struct Stream {
   Store* store = nullptr;  // the store this stream updates, or
   Group* group = nullptr;  // the group of stores it updates together
   hipStream_t stream = nullptr;
+  bool own_stream = true;  // false: `stream` belongs to someone else (a store's query stream)
   hipStream_t copy_stream = nullptr;
   PairScratch ws;
   // host staging (pinned) and matching device buffers for vertex-batch submissions
@@ -154,9 +158,22 @@ struct Stream {
   size_t d_stage_bytes = 0;
   hipEvent_t stage_free = nullptr;  // recorded when the kernels that read the staging are done
   bool stage_pending = false;
+  // Coalescing: submitted edge pairs are appended (range-checked and filtered) to the fill
+  // buffer d_pend[pend_buf]; the grouping and aggregation kernels run once the buffer holds
+  // pend_cap edges, or at a flush. Host batches are copied on copy_stream into the fill buffer
+  // while the kernels read the other one; pend_free[b] is recorded after the kernel that read
+  // buffer b. `mu` serializes submissions and drains of one stream.
+  std::mutex mu;
+  uint32_t* d_pend[2] = {nullptr, nullptr};
+  uint64_t pend_cap = 0;  // edges per buffer
+  uint64_t pend_n = 0;    // edges waiting in the fill buffer
+  int pend_buf = 0;
+  hipEvent_t pend_free[2] = {nullptr, nullptr};
+  bool pend_used[2] = {false, false};
+  hipEvent_t copied = nullptr;  // recorded on copy_stream after a host batch has landed
   // tuning (0 = default): depths kept in the aggregation kernel's LDS delta, destinations per
   // work item, bits of the vertex id resolved inside one grouping bucket
-  uint32_t tune_dl = 0, tune_chunk = 0, tune_bucket_bits = 0;
+  uint32_t tune_dl = 0, tune_chunk = 0, tune_bucket_bits = 0, tune_launch = 0;
   PartParams tuned_part{};
   // optional kernel timing of the aggregation kernel
   bool profile = false;
@@ -225,18 +242,32 @@ inline uint32_t stream_vertices(const Stream* s) {
   return s->group ? s->group->n_total : s->store->view.n;
 }
 
-// pinned + device staging of a stream (api.hip): ensure_stage waits until the previous staged
-// submission has been consumed; release_stage marks the staging busy until the stream gets there
-int ensure_stage(Stream* s, size_t bytes);
+// pinned (+ device, unless `device` is false) staging of a stream (api.hip): ensure_stage waits
+// until the previous staged submission has been consumed; release_stage marks the staging busy
+// until the stream gets there
+int ensure_stage(Stream* s, size_t bytes, bool device = true);
 int release_stage(Stream* s);
+// HIP objects of a stream (its own HIP stream unless `stream` is already set, the copy stream,
+// the staging and fill-buffer events) and their release
+int init_stream_objects(Stream* s);
 void destroy_stream_resources(Stream& s);
+// every pending submission of every writer of a store applied and finished, queries included
+int sync_store(Store* st);
 
 // kernels/launchers implemented in ingest.hip
 int launch_vertex_batches(Stream* s, const uint32_t* d_class_counts, const WorkItem* d_items,
                           uint32_t nitems, const uint32_t* d_dsts, uint64_t updates,
                           uint32_t min_depth, uint32_t depth_cap);
+// edge-pair submissions: append to the stream's fill buffer (processing it whenever it is full);
+// drain_stream processes whatever is pending; ingest_pairs_device appends and drains at once
+int submit_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uint32_t min_depth,
+                        uint32_t depth_cap);
+int submit_pairs_host(Stream* s, const uint32_t* pairs, uint64_t nedges, uint32_t min_depth,
+                      uint32_t depth_cap);
+int drain_stream(Stream* s);
 int ingest_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uint32_t min_depth,
                         uint32_t depth_cap);
+uint64_t stream_launch_edges(const Stream* s);
 int ensure_pair_scratch(Stream* s, uint64_t nedges);
 PartParams make_part_params(uint32_t n, int local_bits = -1);
 int autotune_stream(Stream* s, const uint32_t* d_pairs, uint64_t nedges, gs_tuning* best);

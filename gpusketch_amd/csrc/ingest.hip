@@ -19,7 +19,7 @@ namespace gs {
 
 constexpr int kWave = 64;
 constexpr int kAggWaves = 4;                  // waves per aggregation workgroup
-constexpr uint32_t kChunk = 256;              // max destinations per work item
+constexpr uint32_t kChunk = 512;              // max destinations per work item
 constexpr int kPartThreads = 512;             // K1/K2 workgroup size
 constexpr int kPartEdges = 4096;              // undirected edges per K1/K2 workgroup
 constexpr int kSortThreads = 1024;            // K3 workgroup size (one workgroup per bucket)
@@ -63,10 +63,14 @@ __global__ void fill_u64_kernel(uint64_t* p, uint64_t n, uint64_t v) {
 
 // ------------------------------------------------------------------------------------------
 // Vertex grouping for raw pairs, in two levels with few global atomics:
-//   K1 bucket_count  : directed entries per bucket of hashed vertex ids (LDS histogram, one
+//   K1 append_count  : one per submission: copies the pairs into the stream's fill buffer (in
+//                      place for host batches), drops out-of-range pairs (counted, reported at
+//                      the flush), self loops and pairs outside the subgraph filter (written as
+//                      the no-op pair (0, 0)), and adds the batch's directed entries per bucket of
+//                      hashed vertex ids to the fill buffer's bucket counts (LDS histogram, one
 //                      non-returning global add per workgroup and bucket);
-//   K2 bucket_scatter: entries written into their bucket's region (LDS rank + one returning
-//                      global add per workgroup and bucket);
+//   K2 bucket_scatter: once per fill buffer: entries written into their bucket's region (LDS
+//                      rank + one returning global add per workgroup and bucket);
 //   K3 bucket_sort   : one workgroup per bucket sorts its entries by vertex (LDS counting sort
 //                      over the bucket's local ids) and emits the work items.
 // Vertex ids are spread over buckets by a bijective multiplicative hash on nbits bits, so the
@@ -78,20 +82,21 @@ __device__ __forceinline__ uint32_t hashed_id(const PartParams& pp, uint32_t v) 
 __device__ __forceinline__ uint32_t bucket_of(const PartParams& pp, uint32_t v) {
   return hashed_id(pp, v) >> pp.local_bits;
 }
-__device__ __forceinline__ bool entry_ok(uint2 uv, uint32_t n, uint64_t seed, uint32_t min_depth,
-                                         uint32_t depth_cap) {
-  if (uv.x >= n || uv.y >= n || uv.x == uv.y) return false;
-  return edge_selected(concat_pair(uv.x, uv.y), seed, min_depth, depth_cap);
+__device__ __forceinline__ bool entry_ok(uint2 uv, uint32_t n) {
+  return uv.x < n && uv.y < n && uv.x != uv.y;
 }
 
-__global__ __launch_bounds__(kPartThreads) void bucket_count_kernel(
-    const uint32_t* __restrict__ pairs, uint64_t nedges, uint32_t n, PartParams pp, uint64_t seed,
+// src and dst may be the same buffer (each pair is read and written by one thread)
+__global__ __launch_bounds__(kPartThreads) void append_count_kernel(
+    const uint2* src, uint2* dst, uint64_t nedges, uint32_t n, PartParams pp, uint64_t seed,
     uint32_t min_depth, uint32_t depth_cap, uint32_t* __restrict__ counts_cur,
     uint32_t* __restrict__ counts_next, uint32_t* __restrict__ cursors,
     uint32_t* __restrict__ counters) {
   extern __shared__ uint32_t hist[];
   const uint32_t t = threadIdx.x;
   if (blockIdx.x == 0) {
+    // the next fill buffer's counts and this one's scatter cursors start at zero; the class
+    // counters belong to the launch that will process this fill buffer
     for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads) { counts_next[i] = 0; cursors[i] = 0; }
     if (t < kClassBase + kClasses && t != 3) counters[t] = 0;  // [3] persists until a flush
   }
@@ -102,12 +107,16 @@ __global__ __launch_bounds__(kPartThreads) void bucket_count_kernel(
   for (int k = 0; k < kPartEdges / kPartThreads; k++) {
     uint64_t e = e0 + t + (uint64_t)k * kPartThreads;
     if (e >= nedges) continue;
-    uint2 uv = reinterpret_cast<const uint2*>(pairs)[e];
+    uint2 uv = src[e];
+    bool ok = true;
     if (uv.x >= n || uv.y >= n) {  // never index past the store: drop and report at flush
       atomicAdd(&counters[3], 1u);
-      continue;
+      ok = false;
+    } else if (uv.x == uv.y || !edge_selected(concat_pair(uv.x, uv.y), seed, min_depth, depth_cap)) {
+      ok = false;
     }
-    if (!entry_ok(uv, n, seed, min_depth, depth_cap)) continue;
+    dst[e] = ok ? uv : make_uint2(0, 0);
+    if (!ok) continue;
     atomicAdd(&hist[bucket_of(pp, uv.x)], 1u);
     atomicAdd(&hist[bucket_of(pp, uv.y)], 1u);
   }
@@ -140,9 +149,9 @@ __device__ uint32_t block_exclusive_scan(const uint32_t* counts, uint32_t nb, ui
 }
 
 __global__ __launch_bounds__(kPartThreads) void bucket_scatter_kernel(
-    const uint32_t* __restrict__ pairs, uint64_t nedges, uint32_t n, PartParams pp, uint64_t seed,
-    uint32_t min_depth, uint32_t depth_cap, const uint32_t* __restrict__ counts,
-    uint32_t* __restrict__ cursors, uint2* __restrict__ entries) {
+    const uint32_t* __restrict__ pairs, uint64_t nedges, uint32_t n, PartParams pp,
+    const uint32_t* __restrict__ counts, uint32_t* __restrict__ cursors,
+    uint2* __restrict__ entries) {
   extern __shared__ uint32_t lds_u32[];
   uint32_t* s_off = lds_u32;                   // nbuckets: region start of every bucket
   uint32_t* s_hist = lds_u32 + pp.nbuckets;    // nbuckets: this block's count, then its base
@@ -162,7 +171,7 @@ __global__ __launch_bounds__(kPartThreads) void bucket_scatter_kernel(
     uvs[k] = make_uint2(0, 0);
     if (e >= nedges) continue;
     uint2 uv = reinterpret_cast<const uint2*>(pairs)[e];
-    if (!entry_ok(uv, n, seed, min_depth, depth_cap)) continue;
+    if (!entry_ok(uv, n)) continue;
     uvs[k] = uv;
     bk[2 * k] = bucket_of(pp, uv.x);
     bk[2 * k + 1] = bucket_of(pp, uv.y);
@@ -284,26 +293,35 @@ __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
 }
 
 // ------------------------------------------------------------------------------------------
-// K4: aggregation (see the file comment). Dynamic LDS per wave: delta[dl * cols] buckets +
-// stage[64] x {idx, checksum, mix(swap(idx))}; shared table of per-column seed keys.
-// ------------------------------------------------------------------------------------------
-// Lane mapping: the (destination j, column c) pairs of a chunk are flattened, pair i = j * cols + c
-// going to lane i mod 64 (two pairs per lane per iteration), so every lane is busy whatever the
-// item length.
+// K4: aggregation, one wave64 per work item (see the file comment).
+//
+// Lane layout: lane = 8*e + s with e = entry slot of a group of 8 consecutive destinations and
+// s = column slot; the lane handles columns s, s+8, ..., s+8(T-1) (those < cols) of its entry.
+// Its column keys therefore stay in registers for the whole item, and an entry's staged values
+// are read from LDS once for T columns (8 lanes read one address: a broadcast).
+//
+// Depth 0 receives half of all (update, column) pairs; those are XORed into per-lane registers
+// and merged into the LDS delta once per item. Depths 1..dl-1 are XORed into the LDS delta, so
+// the 8 lanes sharing a column rarely hit one LDS address in the same instruction. Depths >= dl
+// (probability 2^-dl per pair) go to HBM with 64-bit atomics.
 //
 // Per-wave LDS (64-bit words): the delta as structure of arrays (alpha[dslots], gamma[dslots],
-// slot = depth * cols + column, depth < dl), the chunk's stage (edge index, checksum,
-// mix(swap(idx)), 64 each), the per-column seed keys, and the compacted flush list (u32 slots).
+// slot = depth * cols + column) and the chunk's stage (edge index, checksum, mix(swap(idx)), 64
+// each).
+// ------------------------------------------------------------------------------------------
 struct AggLayout {
   uint32_t cols, dl, dslots;
   __host__ __device__ AggLayout(uint32_t cols_, uint32_t dl_) : cols(cols_), dl(dl_), dslots(dl_ * cols_) {}
-  __host__ __device__ uint32_t wave_words() const { return 2 * dslots + 3 * 64 + cols + (dslots + 1) / 2; }
+  __host__ __device__ uint32_t wave_words() const { return 2 * dslots + 3 * 64; }
 };
+constexpr int kMaxColSteps = 8;  // T <= 8: up to 64 columns
+constexpr int kFlushRounds = 4;  // 64-slot rounds per flush step
 
 // kGroup: items belong to the members of a group (descs[item.graph]); the LDS layout is sized
 // for the widest member (max_cols) and the per-member geometry is reloaded when a wave moves to
 // an item of another member. Otherwise every item belongs to the store `st`.
-template <bool kGroup>
+// T = ceil(columns / 8) of the widest store the launch covers.
+template <bool kGroup, int T>
 __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
     StoreView st, const GraphDesc* __restrict__ descs, uint32_t max_cols,
     const WorkItem* __restrict__ items, ItemClasses cls,
@@ -313,21 +331,25 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
   const AggLayout L(kGroup ? max_cols : st.cols, dl);
   uint32_t cols = st.cols;
   uint32_t bpc = st.bpc;
-  uint32_t dslots = dl * cols;
   uint64_t seed = st.seed;
   uint32_t vbase = 0;
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t wid = threadIdx.x / kWave;
+  const uint32_t le = lane >> 3, ls = lane & 7;
   uint64_t* da = reinterpret_cast<uint64_t*>(lds) + (size_t)wid * L.wave_words();
   uint64_t* dg = da + L.dslots;
   uint64_t* sidx = dg + L.dslots;
   uint64_t* schk = sidx + kWave;
   uint64_t* smix = schk + kWave;
-  uint64_t* colkey = smix + kWave;
-  uint32_t* clist = reinterpret_cast<uint32_t*>(colkey + L.cols);
 
-  if (!kGroup)
-    for (uint32_t c = lane; c < cols; c += kWave) colkey[c] = xxh3_seed_key(seed + 5ULL * c);
+  uint64_t colk[T];
+  uint64_t acc_a[T], acc_g[T];
+#pragma unroll
+  for (int t = 0; t < T; t++) {
+    const uint32_t c = ls + 8 * t;
+    colk[t] = (!kGroup && c < cols) ? xxh3_seed_key(seed + 5ULL * c) : 0;
+    acc_a[t] = acc_g[t] = 0;
+  }
   for (uint32_t e = lane; e < L.dslots; e += kWave) { da[e] = 0; dg[e] = 0; }
   wave_sync_lds();
   uint32_t cur_graph = 0xFFFFFFFFu;
@@ -340,29 +362,51 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
     ccount[k] = __builtin_amdgcn_readfirstlane(class_counts[k]);
     nitems += ccount[k];
   }
-  auto item_at = [&](uint32_t vi) -> WorkItem {
+  // A descriptor is fetched as one word per lane (lanes 0..3 hold vertex, len, start, graph), a
+  // vector load whose wait falls where a field is first read (readlane), one item later. The
+  // class holding virtual index vi is found with wave-uniform selects, no branches.
+  auto item_word = [&](uint32_t vi) -> uint32_t {
+    uint64_t off = 0;
+    bool found = false;
 #pragma unroll
-    for (int k = kClasses - 1; k > 0; k--) {
-      if (vi < ccount[k]) return items[cls.off[k] + vi];
-      vi -= ccount[k];
+    for (int k = kClasses - 1; k >= 0; k--) {
+      const bool here = !found && (k == 0 || vi < ccount[k]);
+      off = here ? cls.off[k] + vi : off;
+      found = found || here;
+      vi = found ? vi : vi - ccount[k];
     }
-    return items[cls.off[0] + vi];
+    return reinterpret_cast<const uint32_t*>(items + off)[lane & 3];
   };
-  const uint32_t gw = blockIdx.x * kAggWaves + wid;
+  auto field = [](uint32_t word, int k) -> uint32_t {
+    return (uint32_t)__builtin_amdgcn_readlane((int)word, k);
+  };
+  const uint32_t gw = __builtin_amdgcn_readfirstlane(blockIdx.x * kAggWaves + wid);
   const uint32_t nw = gridDim.x * kAggWaves;
-  uint32_t q64 = kWave / cols, r64 = kWave % cols;  // 64 = q*cols + r
 
-  // software pipeline: the next item and its first chunk of destinations are loaded while the
-  // current item is hashed and flushed
-  WorkItem wnext = gw < nitems ? item_at(gw) : WorkItem{0, 0, 0};
-  uint32_t first_next = (gw < nitems && lane < (wnext.len & ~kItemExclusive)) ? dsts[wnext.start + lane] : 0;
+  // software pipeline: while item i is hashed and flushed, the descriptor of item i+2 and the
+  // first chunk of destinations of item i+1 are in flight; inside an item the next chunk of
+  // destinations is loaded while the current one is processed
+  // the first chunk of an item's destinations; lanes past its length load its first entry (an
+  // unconditional load, so no branch joins force an early wait; those values are never used)
+  auto first_chunk = [&](uint32_t ln, uint32_t sp) -> uint32_t {
+    return dsts[(uint64_t)sp + (lane < ln ? lane : 0u)];
+  };
+  const uint32_t last = nitems ? nitems - 1 : 0;
+  uint32_t wcur = item_word(min(gw, last));
+  uint32_t wnext = item_word(min(gw + nw, last));
+  uint32_t dcur = gw < nitems ? first_chunk(field(wcur, 1) & ~kItemExclusive, field(wcur, 2)) : 0u;
   for (uint32_t it = gw; it < nitems; it += nw) {
-    const WorkItem w = wnext;
-    const uint32_t first = first_next;
-    const bool has_next = it + nw < nitems;
-    if (has_next) wnext = item_at(it + nw);
+    const uint32_t w = wcur;
+    // fields of this item (loaded two items ago) and of the next one (loaded one item ago), read
+    // before any new load is issued
+    const uint32_t v_g = field(w, 0);
+    const uint32_t wlen = field(w, 1);
+    const uint64_t start = field(w, 2);
+    const uint32_t next_len = field(wnext, 1) & ~kItemExclusive, next_start = field(wnext, 2);
+    const uint32_t wafter = item_word(min(it + 2 * nw, last));
+    const uint32_t dfirst_next = first_chunk(next_len, next_start);
     if (kGroup) {
-      const uint32_t g = __builtin_amdgcn_readfirstlane(w.graph);
+      const uint32_t g = field(w, 3);
       if (g != cur_graph) {  // the wave moves to another member: its geometry and column keys
         cur_graph = g;
         const GraphDesc& gd = descs[g];
@@ -374,24 +418,29 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
         bpc = gd.bpc;
         seed = gd.seed;
         vbase = gd.base;
-        dslots = dl * cols;
-        q64 = kWave / cols;
-        r64 = kWave % cols;
-        for (uint32_t c = lane; c < cols; c += kWave) colkey[c] = xxh3_seed_key(seed + 5ULL * c);
-        wave_sync_lds();
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+          const uint32_t c = ls + 8 * t;
+          colk[t] = c < cols ? xxh3_seed_key(seed + 5ULL * c) : 0;
+        }
       }
     }
-    const uint32_t v = __builtin_amdgcn_readfirstlane(w.vertex) - vbase;
-    const uint32_t len = __builtin_amdgcn_readfirstlane(w.len & ~kItemExclusive);
-    const bool excl = exclusive_ok && (w.len & kItemExclusive);
-    const uint64_t start = w.start;
+    const uint32_t v = v_g - vbase;
+    const uint32_t len = wlen & ~kItemExclusive;
+    const bool excl = exclusive_ok && (wlen & kItemExclusive);
+    // LDS depth window of this item: floor(log2 len) + 4 depths (at most dl); the deeper pairs,
+    // about 1.5 per item, go to HBM atomics, and the flush scans only the window
+    const uint32_t dli = min(dl, 35u - (uint32_t)__builtin_clz(len));
+    const uint32_t islots = dli * cols;
     uint64_t det_a = 0, det_g = 0;
     Bucket16* vrows = st.rows + (size_t)v * st.vstride;
 
     for (uint32_t j0 = 0; j0 < len; j0 += kWave) {
       const uint32_t cnt = min((uint32_t)kWave, len - j0);
+      // next chunk (clamped to the item's last entry: an unconditional load)
+      const uint32_t dn = dsts[start + min(j0 + kWave + lane, len - 1)];
       if (lane < cnt) {
-        uint32_t other = (j0 == 0 ? first : dsts[start + j0 + lane]) - vbase;
+        const uint32_t other = dcur - vbase;
         uint64_t idx = concat_pair(v, other);
         uint64_t chk = xxh3_len8(idx, seed);
         // subgraph filter: an update outside the store's subgraph becomes the (0, 0) no-op
@@ -402,51 +451,49 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
         det_a ^= idx;
         det_g ^= chk;
       }
+      dcur = dn;
       wave_sync_lds();
-      const uint32_t nwork = cnt * cols;
-      uint32_t j = lane / cols, c = lane - j * cols;
-      // two independent (entry, column) hashes per lane per iteration for instruction-level
-      // parallelism; the LDS operands are read before the multiply chains start
-      for (uint32_t i = lane; i < nwork; i += 2 * kWave) {
-        uint32_t j2 = j + q64, c2 = c + r64;
-        if (c2 >= cols) { c2 -= cols; j2++; }
-        const bool second = i + kWave < nwork;
-        const uint32_t j2c = min(j2, (uint32_t)kWave - 1);
-        const uint64_t a1 = sidx[j], g1 = schk[j], m1 = smix[j], k1 = colkey[c];
-        const uint64_t a2 = sidx[j2c], g2 = schk[j2c], m2 = smix[j2c], k2 = colkey[c2];
-        const uint32_t d1 = depth_of(xxh3_finish(m1 ^ k1), bpc);
-        const uint32_t d2 = depth_of(xxh3_finish(m2 ^ k2), bpc);
-        if (d1 < dl) {
-          const uint32_t slot = d1 * cols + c;
-          atomicXor(reinterpret_cast<unsigned long long*>(&da[slot]), (unsigned long long)a1);
-          atomicXor(reinterpret_cast<unsigned long long*>(&dg[slot]), (unsigned long long)g1);
-        }
-        if (second && d2 < dl) {
-          const uint32_t slot = d2 * cols + c2;
-          atomicXor(reinterpret_cast<unsigned long long*>(&da[slot]), (unsigned long long)a2);
-          atomicXor(reinterpret_cast<unsigned long long*>(&dg[slot]), (unsigned long long)g2);
-        }
-        if ((d1 >= dl && d1 < bpc) || (second && d2 >= dl && d2 < bpc)) {  // rare: deep buckets
-          if (d1 >= dl && d1 < bpc) {
-            Bucket16* b = &vrows[d1 * cols + c];
-            atomic_xor_u64(&b->alpha, a1);
-            atomic_xor_u64(&b->gamma, g1);
-          }
-          if (second && d2 >= dl && d2 < bpc) {
-            Bucket16* b = &vrows[d2 * cols + c2];
-            atomic_xor_u64(&b->alpha, a2);
-            atomic_xor_u64(&b->gamma, g2);
+      const uint32_t ngroups = (cnt + 7) >> 3;
+      for (uint32_t g = 0; g < ngroups; g++) {
+        const uint32_t j = 8 * g + le;
+        if (j < cnt) {
+          const uint64_t a = sidx[j], gm = schk[j], m = smix[j];
+#pragma unroll
+          for (int t = 0; t < T; t++) {
+            const uint32_t c = ls + 8 * t;
+            if (c < cols) {
+              const uint32_t d = depth_fast(m ^ colk[t], bpc);
+              if (d == 0) {
+                acc_a[t] ^= a;
+                acc_g[t] ^= gm;
+              } else if (d < dli) {
+                const uint32_t slot = mul24_lo(d, cols) + c;
+                atomicXor(reinterpret_cast<unsigned long long*>(&da[slot]), (unsigned long long)a);
+                atomicXor(reinterpret_cast<unsigned long long*>(&dg[slot]), (unsigned long long)gm);
+              } else if (d < bpc) {  // rare: deep buckets straight to HBM
+                Bucket16* b = &vrows[(size_t)d * cols + c];
+                atomic_xor_u64(&b->alpha, a);
+                atomic_xor_u64(&b->gamma, gm);
+              }
+            }
           }
         }
-        j = j2 + q64;
-        c = c2 + r64;
-        if (c >= cols) { c -= cols; j++; }
       }
       wave_sync_lds();
     }
-    if (has_next) {
-      const uint32_t nlen = wnext.len & ~kItemExclusive;
-      first_next = lane < nlen ? dsts[wnext.start + lane] : 0;
+    dcur = dfirst_next;
+    wcur = wnext;
+    wnext = wafter;
+    // depth-0 registers into the LDS delta (slot = column): the 8 lanes of a column slot XOR into
+    // one address, once per item
+#pragma unroll
+    for (int t = 0; t < T; t++) {
+      const uint32_t c = ls + 8 * t;
+      if (c < cols && (acc_a[t] | acc_g[t])) {
+        atomicXor(reinterpret_cast<unsigned long long*>(&da[c]), (unsigned long long)acc_a[t]);
+        atomicXor(reinterpret_cast<unsigned long long*>(&dg[c]), (unsigned long long)acc_g[t]);
+      }
+      acc_a[t] = acc_g[t] = 0;
     }
     det_a = wave_xor_reduce(det_a);
     det_g = wave_xor_reduce(det_g);
@@ -455,34 +502,45 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
       atomic_xor_u64(&b->alpha, det_a);
       atomic_xor_u64(&b->gamma, det_g);
     }
-    // flush: compact the non-zero delta slots into a list (ballot + mbcnt), then every lane takes
-    // one list entry per round, so a round's HBM reads are issued together without branches
-    uint32_t nnz = 0;
-    for (uint32_t s0 = 0; s0 < dslots; s0 += kWave) {
-      const uint32_t sl = s0 + lane;
-      const bool nz = sl < dslots && (da[sl] | dg[sl]) != 0;
-      const uint64_t mask = __ballot(nz);
-      if (nz) {
-        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-        clist[nnz + below] = sl;
-      }
-      nnz += (uint32_t)__popcll(mask);
-    }
     wave_sync_lds();
-    for (uint32_t r0 = 0; r0 < nnz; r0 += kWave) {
-      if (r0 + lane < nnz) {
-        const uint32_t sl = clist[r0 + lane];
-        const uint64_t xa = da[sl], xg = dg[sl];
-        da[sl] = 0;
-        dg[sl] = 0;
-        if (excl) {
-          const Bucket16 cur = vrows[sl];
-          vrows[sl] = Bucket16{cur.alpha ^ xa, cur.gamma ^ xg};
-        } else {
-          atomic_xor_u64(&vrows[sl].alpha, xa);
-          atomic_xor_u64(&vrows[sl].gamma, xg);
+    // flush, kFlushRounds x 64 slots at a time: all LDS reads first, then the HBM reads of the
+    // non-zero slots together, one wait, then the stores (exclusive item: plain read-xor-write;
+    // otherwise 64-bit atomics, which need no read)
+    for (uint32_t s0 = 0; s0 < islots; s0 += kFlushRounds * kWave) {
+      uint64_t xa[kFlushRounds], xg[kFlushRounds];
+#pragma unroll
+      for (int r = 0; r < kFlushRounds; r++) {
+        const uint32_t sl = s0 + r * kWave + lane;
+        xa[r] = sl < islots ? da[sl] : 0;
+        xg[r] = sl < islots ? dg[sl] : 0;
+      }
+#pragma unroll
+      for (int r = 0; r < kFlushRounds; r++) {
+        const uint32_t sl = s0 + r * kWave + lane;
+        if (sl < islots) {
+          da[sl] = 0;
+          dg[sl] = 0;
         }
+      }
+      if (excl) {
+        // unconditional loads (lanes with nothing to flush read the vertex's first bucket) so the
+        // reads issue together and one wait covers them
+        Bucket16 cur[kFlushRounds];
+#pragma unroll
+        for (int r = 0; r < kFlushRounds; r++)
+          cur[r] = vrows[(xa[r] | xg[r]) ? s0 + r * kWave + lane : 0u];
+#pragma unroll
+        for (int r = 0; r < kFlushRounds; r++)
+          if (xa[r] | xg[r])
+            vrows[s0 + r * kWave + lane] = Bucket16{cur[r].alpha ^ xa[r], cur[r].gamma ^ xg[r]};
+      } else {
+#pragma unroll
+        for (int r = 0; r < kFlushRounds; r++)
+          if (xa[r] | xg[r]) {
+            Bucket16* b = &vrows[s0 + r * kWave + lane];
+            atomic_xor_u64(&b->alpha, xa[r]);
+            atomic_xor_u64(&b->gamma, xg[r]);
+          }
       }
     }
     wave_sync_lds();
@@ -537,6 +595,44 @@ static int ensure_dyn_lds(const void* kernel, size_t bytes) {
   return GS_OK;
 }
 
+// one instantiation of the aggregation kernel per (group, T = ceil(columns / 8))
+struct AggArgs {
+  StoreView v;
+  const GraphDesc* descs;
+  uint32_t cols;
+  const WorkItem* items;
+  ItemClasses cls;
+  const uint32_t* class_counts;
+  const uint32_t* dsts;
+  uint32_t dl;
+  int exclusive_ok;
+  uint32_t min_depth, depth_cap;
+};
+struct AggKernel {
+  const void* fn;
+  void (*launch)(dim3, dim3, size_t, hipStream_t, const AggArgs&);
+};
+template <bool G, int T>
+static void agg_launch(dim3 grid, dim3 block, size_t lds, hipStream_t s, const AggArgs& a) {
+  hipLaunchKernelGGL((aggregate_kernel<G, T>), grid, block, lds, s, a.v, a.descs, a.cols, a.items,
+                     a.cls, a.class_counts, a.dsts, a.dl, a.exclusive_ok, a.min_depth, a.depth_cap);
+}
+template <bool G, int T>
+static AggKernel agg_kernel_entry() {
+  return AggKernel{reinterpret_cast<const void*>(&aggregate_kernel<G, T>), &agg_launch<G, T>};
+}
+static const AggKernel& agg_kernel_for(bool group, uint32_t cols) {
+  static const AggKernel table[2][kMaxColSteps] = {
+      {agg_kernel_entry<false, 1>(), agg_kernel_entry<false, 2>(), agg_kernel_entry<false, 3>(),
+       agg_kernel_entry<false, 4>(), agg_kernel_entry<false, 5>(), agg_kernel_entry<false, 6>(),
+       agg_kernel_entry<false, 7>(), agg_kernel_entry<false, 8>()},
+      {agg_kernel_entry<true, 1>(), agg_kernel_entry<true, 2>(), agg_kernel_entry<true, 3>(),
+       agg_kernel_entry<true, 4>(), agg_kernel_entry<true, 5>(), agg_kernel_entry<true, 6>(),
+       agg_kernel_entry<true, 7>(), agg_kernel_entry<true, 8>()}};
+  const uint32_t t = (cols + 7) / 8;
+  return table[group ? 1 : 0][t - 1];
+}
+
 static int launch_aggregate(Stream* s, const WorkItem* d_items, const ItemClasses& cls,
                             const uint32_t* class_counts, const uint32_t* d_dsts,
                             uint64_t updates, uint32_t min_depth = 0, uint32_t depth_cap = 0) {
@@ -561,10 +657,13 @@ static int launch_aggregate(Stream* s, const WorkItem* d_items, const ItemClasse
     std::lock_guard<std::mutex> lk(s->store->mu);
     exclusive_ok = s->store->streams.size() == 1 ? 1 : 0;
   }
+  GS_CHECK(cols >= 1 && cols <= 8 * kMaxColSteps, GS_ERR_UNSUPPORTED,
+           "sketches with %u columns (more than %d) are not supported by the update kernel", cols,
+           8 * kMaxColSteps);
   const size_t lds = agg_lds_bytes(cols, dl);
   GS_CHECK(lds <= 160 * 1024, GS_ERR_UNSUPPORTED, "sketch too wide for the LDS window (%zu B)", lds);
-  const void* kern = grp ? (const void*)aggregate_kernel<true> : (const void*)aggregate_kernel<false>;
-  int rc_lds = ensure_dyn_lds(kern, lds);
+  const AggKernel& kern = agg_kernel_for(grp != nullptr, cols);
+  int rc_lds = ensure_dyn_lds(kern.fn, lds);
   if (rc_lds) return rc_lds;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (s->profile) {
@@ -572,14 +671,10 @@ static int launch_aggregate(Stream* s, const WorkItem* d_items, const ItemClasse
     GS_HIP(hipEventCreate(&e1));
     GS_HIP(hipEventRecord(e0, s->stream));
   }
-  const dim3 grid(agg_grid(device, kern, lds)), block(kAggWaves * kWave);
-  if (grp)
-    hipLaunchKernelGGL(aggregate_kernel<true>, grid, block, lds, s->stream, v, grp->d_descs, cols,
-                       d_items, cls, class_counts, d_dsts, dl, exclusive_ok, 0u, 0u);
-  else
-    hipLaunchKernelGGL(aggregate_kernel<false>, grid, block, lds, s->stream, v,
-                       (const GraphDesc*)nullptr, cols, d_items, cls, class_counts, d_dsts, dl,
-                       exclusive_ok, min_depth, depth_cap);
+  const dim3 grid(agg_grid(device, kern.fn, lds)), block(kAggWaves * kWave);
+  AggArgs a{v, grp ? grp->d_descs : nullptr, cols, d_items, cls, class_counts, d_dsts, dl,
+            exclusive_ok, grp ? 0u : min_depth, grp ? 0u : depth_cap};
+  kern.launch(grid, block, lds, s->stream, a);
   GS_HIP(hipGetLastError());
   if (s->profile) {
     GS_HIP(hipEventRecord(e1, s->stream));
@@ -658,12 +753,13 @@ int ensure_pair_scratch(Stream* s, uint64_t nedges) {
     // class k holds items of length >= 2^k, at most one shorter-than-chunk item per vertex
     const uint64_t verts = std::min<uint64_t>((uint64_t)stream_vertices(s), 2 * cap);
     uint64_t icap = 0;
-    // classes of full items (len >= 128, items of one vertex may repeat) are bounded by the
-    // entries alone; shorter items are a vertex's last, at most one per vertex
+    // classes that may hold full items (len >= 64, the smallest item size a stream can be tuned
+    // to; items of one vertex may repeat) are bounded by the entries alone; shorter items are a
+    // vertex's last, at most one per vertex
     for (int k = 0; k < kClasses; k++) {
       ws.cls.off[k] = icap;
       uint64_t ck = (2 * cap) >> k;
-      if (k < 7) ck = std::min<uint64_t>(verts, ck);
+      if (k < 6) ck = std::min<uint64_t>(verts, ck);
       icap += ck + 16;
     }
     GS_HIP(hipMalloc(&ws.d_entries, sizeof(uint2) * 2 * cap));
@@ -675,13 +771,33 @@ int ensure_pair_scratch(Stream* s, uint64_t nedges) {
   return GS_OK;
 }
 
-int ingest_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uint32_t min_depth,
+uint64_t stream_launch_edges(const Stream* s) {
+  return s->tune_launch ? (uint64_t)s->tune_launch : kLaunchEdges;
+}
+
+// the fill buffers, sized for the stream's launch size (grown on demand, never while pending)
+static int ensure_pending(Stream* s) {
+  const uint64_t want = stream_launch_edges(s);
+  if (s->pend_cap >= want && s->d_pend[0]) return ensure_pair_scratch(s, s->pend_cap);
+  GS_CHECK(s->pend_n == 0, GS_ERR_INVALID, "launch size changed with updates pending");
+  GS_HIP(hipStreamSynchronize(s->stream));
+  if (s->copy_stream) GS_HIP(hipStreamSynchronize(s->copy_stream));
+  for (int b = 0; b < 2; b++) {
+    (void)hipFree(s->d_pend[b]);
+    s->d_pend[b] = nullptr;
+    s->pend_used[b] = false;
+  }
+  for (int b = 0; b < 2; b++)
+    if (hipMalloc(&s->d_pend[b], want * 2 * sizeof(uint32_t)) != hipSuccess)
+      return fail(GS_ERR_OOM, "edge buffers of %llu edges unavailable", (unsigned long long)want);
+  s->pend_cap = want;
+  s->pend_buf = 0;
+  return ensure_pair_scratch(s, want);
+}
+
+// K1 over `m` pairs at src, into the fill buffer behind what is pending
+static int append_pairs(Stream* s, const uint32_t* src, uint64_t m, uint32_t min_depth,
                         uint32_t depth_cap) {
-  if (nedges == 0) return GS_OK;
-  GS_CHECK(nedges < (1ULL << 31), GS_ERR_INVALID, "edge batch too large (%llu)",
-           (unsigned long long)nedges);
-  int rc = ensure_pair_scratch(s, nedges);
-  if (rc) return rc;
   PairScratch& ws = s->ws;
   const PartParams& pp = stream_part(s);
   const uint32_t nverts = stream_vertices(s);
@@ -689,29 +805,136 @@ int ingest_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uin
   GS_CHECK(!s->group || min_depth == 0, GS_ERR_INVALID, "no subgraph filter on group streams");
   uint32_t* counts_cur = ws.d_counts + (size_t)ws.parity * pp.nbuckets;
   uint32_t* counts_next = ws.d_counts + (size_t)(ws.parity ^ 1) * pp.nbuckets;
-  ws.parity ^= 1;
+  int rc = ensure_dyn_lds((const void*)append_count_kernel, sizeof(uint32_t) * pp.nbuckets);
+  if (rc) return rc;
+  uint2* dst = reinterpret_cast<uint2*>(s->d_pend[s->pend_buf]) + s->pend_n;
+  const uint32_t nblk = (uint32_t)((m + kPartEdges - 1) / kPartEdges);
+  hipLaunchKernelGGL(append_count_kernel, dim3(nblk), dim3(kPartThreads),
+                     sizeof(uint32_t) * pp.nbuckets, s->stream, reinterpret_cast<const uint2*>(src),
+                     dst, m, nverts, pp, seed, min_depth, depth_cap, counts_cur, counts_next,
+                     ws.d_cursors, ws.d_counters);
+  GS_HIP(hipGetLastError());
+  s->pend_n += m;
+  return GS_OK;
+}
+
+// K2..K4 over the fill buffer; the other buffer becomes the fill buffer
+static int process_pending(Stream* s) {
+  if (s->pend_n == 0) return GS_OK;
+  PairScratch& ws = s->ws;
+  const PartParams& pp = stream_part(s);
+  const uint32_t nverts = stream_vertices(s);
+  const uint64_t nedges = s->pend_n;
+  const int b = s->pend_buf;
+  uint32_t* counts_cur = ws.d_counts + (size_t)ws.parity * pp.nbuckets;
   const uint32_t nblk = (uint32_t)((nedges + kPartEdges - 1) / kPartEdges);
   const size_t sort_lds = sizeof(uint32_t) * 3 * ((size_t)1 << pp.local_bits);
-  if ((rc = ensure_dyn_lds((const void*)bucket_count_kernel, sizeof(uint32_t) * pp.nbuckets)) ||
-      (rc = ensure_dyn_lds((const void*)bucket_scatter_kernel, sizeof(uint32_t) * 2 * pp.nbuckets)) ||
+  int rc;
+  if ((rc = ensure_dyn_lds((const void*)bucket_scatter_kernel, sizeof(uint32_t) * 2 * pp.nbuckets)) ||
       (rc = ensure_dyn_lds((const void*)bucket_sort_kernel, sort_lds)))
     return rc;
-  hipLaunchKernelGGL(bucket_count_kernel, dim3(nblk), dim3(kPartThreads),
-                     sizeof(uint32_t) * pp.nbuckets, s->stream, d_pairs, nedges, nverts, pp, seed,
-                     min_depth, depth_cap, counts_cur, counts_next, ws.d_cursors, ws.d_counters);
-  GS_HIP(hipGetLastError());
   hipLaunchKernelGGL(bucket_scatter_kernel, dim3(nblk), dim3(kPartThreads),
-                     sizeof(uint32_t) * 2 * pp.nbuckets, s->stream, d_pairs, nedges, nverts, pp,
-                     seed, min_depth, depth_cap, counts_cur, ws.d_cursors, ws.d_entries);
+                     sizeof(uint32_t) * 2 * pp.nbuckets, s->stream, s->d_pend[b], nedges, nverts,
+                     pp, counts_cur, ws.d_cursors, ws.d_entries);
   GS_HIP(hipGetLastError());
+  GS_HIP(hipEventRecord(s->pend_free[b], s->stream));  // the fill buffer has been read
+  s->pend_used[b] = true;
   hipLaunchKernelGGL(bucket_sort_kernel, dim3(pp.nbuckets), dim3(kSortThreads), sort_lds,
                      s->stream, ws.d_entries, pp, counts_cur, ws.d_sorted, ws.d_items, ws.cls,
                      s->tune_chunk ? s->tune_chunk : kChunk, ws.d_counters + kClassBase,
                      s->group ? (const uint32_t*)s->group->d_bases : (const uint32_t*)nullptr,
                      s->group ? (uint32_t)s->group->members.size() : 1u);
   GS_HIP(hipGetLastError());
+  ws.parity ^= 1;
+  s->pend_n = 0;
+  s->pend_buf ^= 1;
   return launch_aggregate(s, ws.d_items, ws.cls, ws.d_counters + kClassBase, ws.d_sorted,
                           2 * nedges);
+}
+
+int submit_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uint32_t min_depth,
+                        uint32_t depth_cap) {
+  if (nedges == 0) return GS_OK;
+  std::lock_guard<std::mutex> lk(s->mu);
+  int rc = ensure_pending(s);
+  if (rc) return rc;
+  // (the launches that read the fill buffer earlier ran on this same stream)
+  for (uint64_t e0 = 0; e0 < nedges;) {
+    const uint64_t m = std::min(nedges - e0, s->pend_cap - s->pend_n);
+    if ((rc = append_pairs(s, d_pairs + 2 * e0, m, min_depth, depth_cap))) return rc;
+    e0 += m;
+    if (s->pend_n == s->pend_cap && (rc = process_pending(s))) return rc;
+  }
+  return GS_OK;
+}
+
+// Host batches: copied on the copy stream straight into the fill buffer (from the caller's
+// memory when it is pinned, else through the stream's pinned staging, range-checked on the way),
+// then appended in place on the compute stream. The call returns once the caller's memory has
+// been read.
+int submit_pairs_host(Stream* s, const uint32_t* pairs, uint64_t nedges, uint32_t min_depth,
+                      uint32_t depth_cap) {
+  if (nedges == 0) return GS_OK;
+  std::lock_guard<std::mutex> lk(s->mu);
+  int rc = ensure_pending(s);
+  if (rc) return rc;
+  hipPointerAttribute_t attr{};
+  bool pinned = false;
+  if (hipPointerGetAttributes(&attr, pairs) == hipSuccess)
+    pinned = attr.type == hipMemoryTypeHost;
+  else
+    (void)hipGetLastError();
+  const uint32_t n = stream_vertices(s);
+  for (uint64_t e0 = 0; e0 < nedges;) {
+    const uint64_t m = std::min(nedges - e0, s->pend_cap - s->pend_n);
+    const size_t bytes = m * 2 * sizeof(uint32_t);
+    const uint32_t* src = pairs + 2 * e0;
+    if (!pinned) {
+      if ((rc = ensure_stage(s, bytes, false))) return rc;
+      uint32_t bad = 0;
+      for (uint64_t i = 0; i < 2 * m; i++) {
+        const uint32_t v = src[i];
+        bad |= v >= n;
+        s->h_stage[i] = v;
+      }
+      if (bad) {
+        for (uint64_t i = 0; i < 2 * m; i++)
+          GS_CHECK(src[i] < n, GS_ERR_INVALID, "edge %llu: vertex %u out of range",
+                   (unsigned long long)(e0 + i / 2), src[i]);
+      }
+      src = s->h_stage;
+    }
+    // the fill buffer may still be read by the launch that processed it two buffers back
+    if (s->pend_used[s->pend_buf])
+      GS_HIP(hipStreamWaitEvent(s->copy_stream, s->pend_free[s->pend_buf], 0));
+    uint32_t* dst = s->d_pend[s->pend_buf] + 2 * s->pend_n;
+    GS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->copy_stream));
+    GS_HIP(hipEventRecord(s->copied, s->copy_stream));
+    GS_HIP(hipStreamWaitEvent(s->stream, s->copied, 0));
+    if (!pinned) {
+      GS_HIP(hipEventRecord(s->stage_free, s->copy_stream));
+      s->stage_pending = true;
+    }
+    if ((rc = append_pairs(s, dst, m, min_depth, depth_cap))) return rc;
+    e0 += m;
+    if (s->pend_n == s->pend_cap && (rc = process_pending(s))) return rc;
+  }
+  if (pinned) GS_HIP(hipEventSynchronize(s->copied));  // the caller may reuse its buffer
+  return GS_OK;
+}
+
+int drain_stream(Stream* s) {
+  std::lock_guard<std::mutex> lk(s->mu);
+  return process_pending(s);
+}
+
+int ingest_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uint32_t min_depth,
+                        uint32_t depth_cap) {
+  GS_CHECK(nedges < (1ULL << 31), GS_ERR_INVALID, "edge batch too large (%llu)",
+           (unsigned long long)nedges);
+  int rc = submit_pairs_device(s, d_pairs, nedges, min_depth, depth_cap);
+  if (rc) return rc;
+  return drain_stream(s);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -724,7 +947,7 @@ int autotune_stream(Stream* s, const uint32_t* d_pairs, uint64_t nedges, gs_tuni
   GS_CHECK(nedges > 0, GS_ERR_INVALID, "autotune needs a sample batch");
   const uint32_t min_bpc = s->group ? s->group->min_bpc : s->store->view.bpc;
   const PartParams base = s->group ? s->group->part : s->store->part;
-  std::vector<uint32_t> dls, chunks = {128, 256, 512}, bits;
+  std::vector<uint32_t> dls, chunks = {256, 512, 1024}, bits;
   for (uint32_t d : {8u, 10u, 12u, 14u, 16u})
     if (d <= min_bpc) dls.push_back(d);
   if (dls.empty()) dls.push_back(min_bpc);
@@ -735,7 +958,7 @@ int autotune_stream(Stream* s, const uint32_t* d_pairs, uint64_t nedges, gs_tuni
   GS_HIP(hipEventCreate(&e1));
   const bool prof = s->profile;
   s->profile = false;
-  gs_tuning win{0, 0, 0, 1e30f};
+  gs_tuning win{0, 0, 0, s->tune_launch, 1e30f};
   int rc = GS_OK;
   for (uint32_t b : bits)
     for (uint32_t c : chunks)
@@ -756,7 +979,7 @@ int autotune_stream(Stream* s, const uint32_t* d_pairs, uint64_t nedges, gs_tuni
           if (!rc) t_best = std::min(t_best, ms);
         }
         if (rc) break;
-        if (t_best < win.kernel_ms) win = gs_tuning{d, c, b, t_best};
+        if (t_best < win.kernel_ms) win = gs_tuning{d, c, b, s->tune_launch, t_best};
       }
   (void)hipStreamSynchronize(s->stream);
   (void)hipEventDestroy(e0);

@@ -158,13 +158,16 @@ int gs_stream_timing(gs_stream* s, double* agg_ms, uint64_t* launches, uint64_t*
 /* ---- Tuning ----------------------------------------------------------------------------------
  * Launch parameters of a stream's update path (0 = default): lds_depths = depths of every column
  * kept in the aggregation kernel's LDS delta (default 12), item_updates = destinations per work
- * item (128..512, default 256), bucket_bits = vertex-id bits resolved inside one grouping
- * bucket (default log2(n) - 8, at most 13). The reference fixes its launch shape at compile time
+ * item (64..1024, default 512), bucket_bits = vertex-id bits resolved inside one grouping
+ * bucket (default log2(n) - 8, at most 13), launch_edges = submitted edge pairs coalesced into
+ * one launch of the grouping and aggregation kernels (2^12..2^28, default 2^22; a flush or a
+ * query applies a partial launch). The reference fixes its launch shape at compile time
  * (num_device_threads = 1024, num_batch_per_buffer = 540, include/cc_gpu_sketch_alg.h:16-23). */
 typedef struct gs_tuning {
   uint32_t lds_depths;
   uint32_t item_updates;
   uint32_t bucket_bits;
+  uint32_t launch_edges;
   float kernel_ms; /* autotune: the winner's time for the sample batch */
 } gs_tuning;
 int gs_stream_set_tuning(gs_stream* s, const gs_tuning* t);

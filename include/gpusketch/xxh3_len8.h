@@ -47,6 +47,47 @@ GS_HD uint64_t xxh3_seed_key(uint64_t seed) { return xxh3_mix(xxh3_bitflip(seed)
 GS_HD uint32_t depth_of(uint64_t h, uint32_t max_depth) {
   return (uint32_t)__builtin_ctzll(h | (1ULL << max_depth));
 }
+// 32x32 -> low 32 bits of the product of the low 24 bits (full-rate v_mul_u32_u24 on gfx950)
+// The device form is written as the instruction itself: with only the low bits of the product in
+// use, the compiler otherwise turns __umul24 back into a quarter-rate v_mul_lo_u32.
+GS_HD uint32_t mul24_lo(uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t r;
+  asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "s"(b), "v"(a));
+  return r;
+#else
+  return (a & 0xFFFFFFu) * (b & 0xFFFFFFu);
+#endif
+}
+// low 32 bits of (hi:lo) >> s, 0 < s < 32 (v_alignbit_b32)
+GS_HD uint32_t align_bits(uint32_t hi, uint32_t lo, uint32_t s) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_alignbit(hi, lo, s);
+#else
+  return (uint32_t)((((uint64_t)hi << 32) | lo) >> s);
+#endif
+}
+// depth_of(xxh3_finish(x), max_depth) with a cheaper second multiply. The depth only needs the
+// low bits of h = x3 ^ (x3 >> 28): bits 0..15 of h come from bits 0..43 of x3 = x2 * M, whose
+// high word is needed mod 2^12 only, so its two cross products run as 24-bit multiplies. The first
+// product x1 = x * M is needed in full (x2 = x1 ^ ((x1 >> 35) + 8) feeds x1's high word into x2's
+// low word; the added term is below 2^30, so x2's high word equals x1's). When those 16 bits are
+// all zero (probability 2^-16) and max_depth > 16 the full hash decides.
+GS_HD uint32_t depth_fast(uint64_t x, uint32_t max_depth) {
+  constexpr uint32_t kMlo = (uint32_t)kRrmxmxMul, kMhi = (uint32_t)(kRrmxmxMul >> 32);
+  const uint32_t xlo = (uint32_t)x, xhi = (uint32_t)(x >> 32);
+  const uint64_t p = (uint64_t)xlo * kMlo;
+  const uint32_t x1hi = (uint32_t)(p >> 32) + xlo * kMhi + xhi * kMlo;
+  const uint32_t x2lo = (uint32_t)p ^ ((x1hi >> 3) + 8u);
+  const uint64_t q = (uint64_t)x2lo * kMlo;
+  const uint32_t x3hi = (uint32_t)(q >> 32) + mul24_lo(x2lo, kMhi) + mul24_lo(x1hi, kMlo);
+  const uint32_t h16 = ((uint32_t)q ^ align_bits(x3hi, (uint32_t)q, 28)) & 0xFFFFu;
+  const uint32_t cap = max_depth < 16u ? max_depth : 16u;
+  const uint32_t d = (uint32_t)__builtin_ctz(h16 | (1u << cap));
+  if (d < 16u || max_depth <= 16u) return d;
+  return depth_of(xxh3_finish(x), max_depth);
+}
+
 // concat_pairing_fn (src/util.cpp:45-51)
 GS_HD uint64_t concat_pair(uint32_t i, uint32_t j) {
   return i < j ? (((uint64_t)i << 32) | j) : (((uint64_t)j << 32) | i);
