@@ -82,6 +82,7 @@ _sigs = {
     "gs_min_cut": ([_u32, _vp, _u64, _vp, _vp], _i32),
     "gs_stream_set_profiling": ([_vp, _i32], _i32),
     "gs_stream_timing": ([_vp, _vp, _vp, _vp], _i32),
+    "gs_stream_traffic": ([_vp, _vp, _vp], _i32),
     "gs_stream_set_tuning": ([_vp, _vp], _i32),
     "gs_stream_get_tuning": ([_vp, _vp], _i32),
     "gs_stream_autotune": ([_vp, _vp, _u64, _vp], _i32),
@@ -172,6 +173,25 @@ def gen_planted_cut(n: int, p: float, cut: int, churn_frac: float, seed: int) ->
     return out[: cnt.value]
 
 
+class PinnedBuffer:
+    """Page-locked host memory (gs_host_alloc) with a numpy view: submissions from it are copied
+    to the device by DMA without staging."""
+
+    def __init__(self, shape, dtype=np.uint32):
+        dt = np.dtype(dtype)
+        self.nbytes = int(np.prod(shape)) * dt.itemsize
+        self.ptr = ctypes.c_void_p()
+        _check(lib.gs_host_alloc(max(1, self.nbytes), ctypes.byref(self.ptr)))
+        raw = (ctypes.c_char * max(1, self.nbytes)).from_address(self.ptr.value)
+        self.array = np.frombuffer(raw, dtype=dt, count=int(np.prod(shape))).reshape(shape)
+
+    def free(self) -> None:
+        if self.ptr:
+            self.array = None
+            _check(lib.gs_host_free(self.ptr))
+            self.ptr = ctypes.c_void_p()
+
+
 class DeviceBuffer:
     """Device memory owned by the caller (inputs kept resident in HBM)."""
 
@@ -234,6 +254,13 @@ class Stream:
         _check(lib.gs_stream_timing(self.h, ctypes.byref(ms), ctypes.byref(launches),
                                     ctypes.byref(upd)))
         return ms.value, launches.value, upd.value
+
+    def traffic(self):
+        """(device buckets read-modify-written by flushes, updates XORed straight into HBM) of
+        the profiled launches since the last call."""
+        f, d = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib.gs_stream_traffic(self.h, ctypes.byref(f), ctypes.byref(d)))
+        return f.value, d.value
 
     def set_tuning(self, lds_depths: int = 0, item_updates: int = 0, bucket_bits: int = 0,
                    launch_edges: int = 0):

@@ -99,6 +99,8 @@ void destroy_stream_resources(Stream& s) {
     s.pend_free[b] = nullptr;
   }
   if (s.copied) (void)hipEventDestroy(s.copied);
+  if (s.d_traffic) (void)hipFree(s.d_traffic);
+  s.d_traffic = nullptr;
   for (auto& p : s.ev_pairs) {
     (void)hipEventDestroy(p.first);
     (void)hipEventDestroy(p.second);
@@ -479,7 +481,7 @@ int gs_import_buckets(gs_store* h, uint32_t v0, uint32_t count, const gs_bucket*
 
 int gs_stream_set_tuning(gs_stream* sh, const gs_tuning* t) {
   GS_CHECK(sh && t, GS_ERR_INVALID, "null argument");
-  GS_CHECK(t->lds_depths <= 32, GS_ERR_INVALID, "lds_depths must be <= 32");
+  GS_CHECK(t->lds_depths <= 16, GS_ERR_INVALID, "lds_depths must be <= 16");
   GS_CHECK(t->item_updates == 0 || (t->item_updates >= 64 && t->item_updates <= 1024),
            GS_ERR_INVALID, "item_updates must be in [64, 1024]");
   GS_CHECK(t->bucket_bits <= 13, GS_ERR_INVALID, "bucket_bits must be <= 13");
@@ -489,15 +491,17 @@ int gs_stream_set_tuning(gs_stream* sh, const gs_tuning* t) {
   int rc = drain_stream(&s);  // pending edges were counted under the old parameters
   if (rc) return rc;
   GS_HIP(hipStreamSynchronize(s.stream));
+  PartParams part{};
+  if (t->bucket_bits) {
+    part = make_part_params(stream_vertices(&s), (int)t->bucket_bits);
+    GS_CHECK(part.nbits - part.local_bits <= 13, GS_ERR_INVALID,
+             "bucket_bits too small for %u vertices", stream_vertices(&s));
+  }
   s.tune_dl = t->lds_depths;
   s.tune_chunk = t->item_updates;
   s.tune_bucket_bits = t->bucket_bits;
   s.tune_launch = t->launch_edges;
-  if (t->bucket_bits) {
-    s.tuned_part = make_part_params(stream_vertices(&s), (int)t->bucket_bits);
-    GS_CHECK(s.tuned_part.nbits - s.tuned_part.local_bits <= 14, GS_ERR_INVALID,
-             "bucket_bits too small for %u vertices", stream_vertices(&s));
-  }
+  if (t->bucket_bits) s.tuned_part = part;
   return GS_OK;
 }
 
@@ -542,6 +546,20 @@ int gs_stream_timing(gs_stream* sh, double* agg_ms, uint64_t* launches, uint64_t
   s.agg_ms = 0;
   s.agg_launches = 0;
   s.agg_updates = 0;
+  return GS_OK;
+}
+
+int gs_stream_traffic(gs_stream* sh, uint64_t* flushed_buckets, uint64_t* direct_updates) {
+  GS_CHECK(sh, GS_ERR_INVALID, "null stream");
+  Stream& s = sh->impl;
+  unsigned long long h[2] = {0, 0};
+  if (s.d_traffic) {
+    GS_HIP(hipStreamSynchronize(s.stream));
+    GS_HIP(hipMemcpy(h, s.d_traffic, sizeof(h), hipMemcpyDeviceToHost));
+    GS_HIP(hipMemset(s.d_traffic, 0, sizeof(h)));
+  }
+  if (flushed_buckets) *flushed_buckets = h[0];
+  if (direct_updates) *direct_updates = h[1];
   return GS_OK;
 }
 

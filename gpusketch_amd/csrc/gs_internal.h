@@ -125,7 +125,7 @@ struct ItemClasses {
 };
 
 // edges coalesced per launch of the grouping and aggregation kernels (default)
-constexpr uint64_t kLaunchEdges = 1ULL << 22;
+constexpr uint64_t kLaunchEdges = 1ULL << 23;
 
 struct PairScratch {
   uint64_t cap_edges = 0;        // capacity in undirected edges
@@ -133,11 +133,10 @@ struct PairScratch {
   uint32_t* d_sorted = nullptr;  // destinations grouped by vertex (2 x cap)
   uint64_t cap_items = 0;
   WorkItem* d_items = nullptr;
+  uint64_t cap_hist = 0;          // words of d_hist
+  uint32_t* d_hist = nullptr;     // [bucket][partition block]: entries, then their offsets
   uint32_t cap_buckets = 0;
-  uint32_t cur_buckets = 0;       // bucket count the count buffers were last used with
-  uint32_t* d_counts = nullptr;   // 2 x nbuckets (double-buffered per batch)
-  uint32_t* d_cursors = nullptr;  // nbuckets
-  uint32_t parity = 0;
+  uint32_t* d_totals = nullptr;   // [bucket]: entries of the launch
   ItemClasses cls{};
   uint32_t* d_counters = nullptr;  // [0..2] unused, [3]=dropped out-of-range entries since the
                                    // last flush, [kClassBase + k]=items of class k
@@ -171,6 +170,8 @@ struct Stream {
   hipEvent_t pend_free[2] = {nullptr, nullptr};
   bool pend_used[2] = {false, false};
   hipEvent_t copied = nullptr;  // recorded on copy_stream after a host batch has landed
+  bool copies_pending = false;  // host batches in the fill buffer the compute stream has not
+                                // waited for yet
   // tuning (0 = default): depths kept in the aggregation kernel's LDS delta, destinations per
   // work item, bits of the vertex id resolved inside one grouping bucket
   uint32_t tune_dl = 0, tune_chunk = 0, tune_bucket_bits = 0, tune_launch = 0;
@@ -182,6 +183,7 @@ struct Stream {
   double agg_ms = 0;
   uint64_t agg_launches = 0;
   uint64_t agg_updates = 0;
+  unsigned long long* d_traffic = nullptr;  // [0] flushed buckets, [1] direct HBM updates
 };
 
 struct Store {

@@ -22,7 +22,8 @@ constexpr int kAggWaves = 4;                  // waves per aggregation workgroup
 constexpr uint32_t kChunk = 512;              // max destinations per work item
 constexpr int kPartThreads = 512;             // K1/K2 workgroup size
 constexpr int kPartEdges = 4096;              // undirected edges per K1/K2 workgroup
-constexpr int kSortThreads = 1024;            // K3 workgroup size (one workgroup per bucket)
+constexpr int kSortThreads = 1024;            // K4 workgroup size (one workgroup per bucket)
+constexpr int kSortBatch = 8;                 // entries per thread loaded together in K4
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 
 __host__ __device__ __forceinline__ uint32_t item_class(uint32_t len) {
@@ -35,6 +36,12 @@ __host__ __device__ __forceinline__ uint32_t item_class(uint32_t len) {
 __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_wave_barrier();
   asm volatile("" ::: "memory");
+}
+
+// set lanes of `mask` below this lane
+__device__ __forceinline__ uint32_t lane_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
 __device__ __forceinline__ void atomic_xor_u64(uint64_t* p, uint64_t v) {
@@ -62,16 +69,15 @@ __global__ void fill_u64_kernel(uint64_t* p, uint64_t n, uint64_t v) {
 }
 
 // ------------------------------------------------------------------------------------------
-// Vertex grouping for raw pairs, in two levels with few global atomics:
-//   K1 append_count  : one per submission: copies the pairs into the stream's fill buffer (in
-//                      place for host batches), drops out-of-range pairs (counted, reported at
-//                      the flush), self loops and pairs outside the subgraph filter (written as
-//                      the no-op pair (0, 0)), and adds the batch's directed entries per bucket of
-//                      hashed vertex ids to the fill buffer's bucket counts (LDS histogram, one
-//                      non-returning global add per workgroup and bucket);
-//   K2 bucket_scatter: once per fill buffer: entries written into their bucket's region (LDS
-//                      rank + one returning global add per workgroup and bucket);
-//   K3 bucket_sort   : one workgroup per bucket sorts its entries by vertex (LDS counting sort
+// Vertex grouping for raw pairs. Submissions are appended to the stream's fill buffer (a copy,
+// plus the subgraph filter when one is given); then, once per fill buffer:
+//   K1 bucket_count  : per partition block, the directed entries per bucket of hashed vertex ids
+//                      (LDS histogram, written out per block: no global atomics); out-of-range
+//                      pairs are counted as dropped (reported at the flush) and become the no-op
+//                      pair (0, 0), as self loops are;
+//   K2 bucket_offsets: per bucket, the exclusive prefix of the block counts and the total;
+//   K3 bucket_scatter: every block writes its entries at its own offsets (LDS ranks);
+//   K4 bucket_sort   : one workgroup per bucket sorts its entries by vertex (LDS counting sort
 //                      over the bucket's local ids) and emits the work items.
 // Vertex ids are spread over buckets by a bijective multiplicative hash on nbits bits, so the
 // skewed bit patterns of R-MAT ids do not pile up in one bucket.
@@ -86,53 +92,61 @@ __device__ __forceinline__ bool entry_ok(uint2 uv, uint32_t n) {
   return uv.x < n && uv.y < n && uv.x != uv.y;
 }
 
-// src and dst may be the same buffer (each pair is read and written by one thread)
-__global__ __launch_bounds__(kPartThreads) void append_count_kernel(
-    const uint2* src, uint2* dst, uint64_t nedges, uint32_t n, PartParams pp, uint64_t seed,
-    uint32_t min_depth, uint32_t depth_cap, uint32_t* __restrict__ counts_cur,
-    uint32_t* __restrict__ counts_next, uint32_t* __restrict__ cursors,
-    uint32_t* __restrict__ counters) {
-  extern __shared__ uint32_t hist[];
-  const uint32_t t = threadIdx.x;
-  if (blockIdx.x == 0) {
-    // the next fill buffer's counts and this one's scatter cursors start at zero; the class
-    // counters belong to the launch that will process this fill buffer
-    for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads) { counts_next[i] = 0; cursors[i] = 0; }
-    if (t < kClassBase + kClasses && t != 3) counters[t] = 0;  // [3] persists until a flush
+// copy of a submission into the fill buffer; with a subgraph filter (min_depth > 0) in-range
+// pairs outside it become (0, 0). src and dst may be the same buffer.
+__global__ void append_kernel(const uint2* src, uint2* dst, uint64_t nedges, uint32_t n,
+                              uint64_t seed, uint32_t min_depth, uint32_t depth_cap) {
+  for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < nedges;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    uint2 uv = src[e];
+    if (min_depth && uv.x < n && uv.y < n &&
+        !edge_selected(concat_pair(uv.x, uv.y), seed, min_depth, depth_cap))
+      uv = make_uint2(0, 0);
+    dst[e] = uv;
   }
-  for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads) hist[i] = 0;
+}
+
+__global__ __launch_bounds__(kPartThreads) void bucket_count_kernel(
+    uint2* pairs, uint64_t nedges, uint32_t n, PartParams pp, uint32_t nblk,
+    uint32_t* __restrict__ hist, uint32_t* __restrict__ counters) {
+  extern __shared__ uint32_t h[];
+  const uint32_t t = threadIdx.x;
+  if (blockIdx.x == 0 && t < kClassBase + kClasses && t != 3)
+    counters[t] = 0;  // class counts of this launch; [3] persists until a flush
+  for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads) h[i] = 0;
   __syncthreads();
   const uint64_t e0 = (uint64_t)blockIdx.x * kPartEdges;
 #pragma unroll
   for (int k = 0; k < kPartEdges / kPartThreads; k++) {
-    uint64_t e = e0 + t + (uint64_t)k * kPartThreads;
+    const uint64_t e = e0 + t + (uint64_t)k * kPartThreads;
     if (e >= nedges) continue;
-    uint2 uv = src[e];
-    bool ok = true;
+    const uint2 uv = pairs[e];
     if (uv.x >= n || uv.y >= n) {  // never index past the store: drop and report at flush
       atomicAdd(&counters[3], 1u);
-      ok = false;
-    } else if (uv.x == uv.y || !edge_selected(concat_pair(uv.x, uv.y), seed, min_depth, depth_cap)) {
-      ok = false;
+      pairs[e] = make_uint2(0, 0);
+      continue;
     }
-    dst[e] = ok ? uv : make_uint2(0, 0);
-    if (!ok) continue;
-    atomicAdd(&hist[bucket_of(pp, uv.x)], 1u);
-    atomicAdd(&hist[bucket_of(pp, uv.y)], 1u);
+    if (uv.x == uv.y) continue;
+    atomicAdd(&h[bucket_of(pp, uv.x)], 1u);
+    atomicAdd(&h[bucket_of(pp, uv.y)], 1u);
   }
   __syncthreads();
-  for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads)
-    if (hist[i]) __hip_atomic_fetch_add(&counts_cur[i], hist[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads) hist[(size_t)i * nblk + blockIdx.x] = h[i];
 }
 
-// exclusive prefix of counts[0..nb) into s_off (LDS), returns the total; blockDim threads
+// exclusive prefix of counts[0..nb) into s_off (LDS; may be counts itself), returns the total;
+// blockDim threads
 __device__ uint32_t block_exclusive_scan(const uint32_t* counts, uint32_t nb, uint32_t* s_off,
                                          uint32_t* s_part) {
   const uint32_t t = threadIdx.x, nt = blockDim.x;
   const uint32_t per = (nb + nt - 1) / nt;
   const uint32_t b0 = min(nb, t * per), b1 = min(nb, b0 + per);
   uint32_t sum = 0;
-  for (uint32_t b = b0; b < b1; b++) { s_off[b] = sum; sum += counts[b]; }
+  for (uint32_t b = b0; b < b1; b++) {
+    const uint32_t c = counts[b];
+    s_off[b] = sum;
+    sum += c;
+  }
   s_part[t] = sum;
   __syncthreads();
   for (uint32_t o = 1; o < nt; o <<= 1) {
@@ -148,17 +162,49 @@ __device__ uint32_t block_exclusive_scan(const uint32_t* counts, uint32_t nb, ui
   return total;
 }
 
+// one workgroup per bucket: the block counts of the bucket become exclusive offsets (in place)
+constexpr int kOffThreads = 256;
+__global__ __launch_bounds__(kOffThreads) void bucket_offsets_kernel(uint32_t* hist, uint32_t nblk,
+                                                                     uint32_t* __restrict__ totals) {
+  __shared__ uint32_t s_part[kOffThreads];
+  uint32_t* col = hist + (size_t)blockIdx.x * nblk;
+  const uint32_t t = threadIdx.x;
+  const uint32_t per = (nblk + kOffThreads - 1) / kOffThreads;
+  const uint32_t b0 = min(nblk, t * per), b1 = min(nblk, b0 + per);
+  uint32_t sum = 0;
+  for (uint32_t b = b0; b < b1; b++) sum += col[b];
+  s_part[t] = sum;
+  __syncthreads();
+  for (uint32_t o = 1; o < kOffThreads; o <<= 1) {
+    const uint32_t add = t >= o ? s_part[t - o] : 0;
+    __syncthreads();
+    s_part[t] += add;
+    __syncthreads();
+  }
+  uint32_t run = t ? s_part[t - 1] : 0;
+  for (uint32_t b = b0; b < b1; b++) {
+    const uint32_t c = col[b];
+    col[b] = run;
+    run += c;
+  }
+  if (t == kOffThreads - 1) totals[blockIdx.x] = s_part[t];
+}
+
+// The block's entries are first placed in LDS in bucket order, then written out so that
+// consecutive lanes write consecutive positions of one bucket's run (coalesced stores).
 __global__ __launch_bounds__(kPartThreads) void bucket_scatter_kernel(
-    const uint32_t* __restrict__ pairs, uint64_t nedges, uint32_t n, PartParams pp,
-    const uint32_t* __restrict__ counts, uint32_t* __restrict__ cursors,
+    const uint32_t* __restrict__ pairs, uint64_t nedges, uint32_t n, PartParams pp, uint32_t nblk,
+    const uint32_t* __restrict__ totals, const uint32_t* __restrict__ hist,
     uint2* __restrict__ entries) {
-  extern __shared__ uint32_t lds_u32[];
-  uint32_t* s_off = lds_u32;                   // nbuckets: region start of every bucket
-  uint32_t* s_hist = lds_u32 + pp.nbuckets;    // nbuckets: this block's count, then its base
+  extern __shared__ __align__(16) uint32_t lds_u32[];
+  uint2* s_stage = reinterpret_cast<uint2*>(lds_u32);  // 2 * kPartEdges entries
+  uint32_t* s_base = lds_u32 + 4 * kPartEdges;         // nbuckets: global start
+  uint32_t* s_cnt = s_base + pp.nbuckets;              // nbuckets: block counts, then offsets
+  uint32_t* s_loc = s_cnt;
   __shared__ uint32_t s_part[kPartThreads];
   const uint32_t t = threadIdx.x;
-  for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads) s_hist[i] = 0;
-  block_exclusive_scan(counts, pp.nbuckets, s_off, s_part);
+  for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads) s_cnt[i] = 0;
+  block_exclusive_scan(totals, pp.nbuckets, s_base, s_part);
 
   const uint64_t e0 = (uint64_t)blockIdx.x * kPartEdges;
   constexpr int K = kPartEdges / kPartThreads;
@@ -175,23 +221,31 @@ __global__ __launch_bounds__(kPartThreads) void bucket_scatter_kernel(
     uvs[k] = uv;
     bk[2 * k] = bucket_of(pp, uv.x);
     bk[2 * k + 1] = bucket_of(pp, uv.y);
-    rk[2 * k] = atomicAdd(&s_hist[bk[2 * k]], 1u);
-    rk[2 * k + 1] = atomicAdd(&s_hist[bk[2 * k + 1]], 1u);
+    rk[2 * k] = atomicAdd(&s_cnt[bk[2 * k]], 1u);
+    rk[2 * k + 1] = atomicAdd(&s_cnt[bk[2 * k + 1]], 1u);
   }
   __syncthreads();
-  for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads) {
-    uint32_t c = s_hist[i];
-    if (c) s_hist[i] = s_off[i] + __hip_atomic_fetch_add(&cursors[i], c, __ATOMIC_RELAXED,
-                                                          __HIP_MEMORY_SCOPE_AGENT);
-  }
-  __syncthreads();
+  const uint32_t total = block_exclusive_scan(s_cnt, pp.nbuckets, s_loc, s_part);
+  for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads)
+    s_base[i] += hist[(size_t)i * nblk + blockIdx.x];  // this block's start inside bucket i
 #pragma unroll
   for (int k = 0; k < K; k++) {
     if (bk[2 * k] == kEmpty) continue;
     const uint2 uv = uvs[k];
-    entries[s_hist[bk[2 * k]] + rk[2 * k]] = uv;
-    entries[s_hist[bk[2 * k + 1]] + rk[2 * k + 1]] = make_uint2(uv.y, uv.x);
+    s_stage[s_loc[bk[2 * k]] + rk[2 * k]] = uv;
+    s_stage[s_loc[bk[2 * k + 1]] + rk[2 * k + 1]] = make_uint2(uv.y, uv.x);
   }
+  __syncthreads();
+  for (uint32_t i = t; i < total; i += kPartThreads) {
+    const uint2 en = s_stage[i];
+    const uint32_t b = bucket_of(pp, en.x);
+    entries[s_base[b] + (i - s_loc[b])] = en;
+  }
+}
+
+// LDS bytes of bucket_scatter_kernel's dynamic part
+static size_t scatter_lds_bytes(uint32_t nbuckets) {
+  return sizeof(uint2) * 2 * kPartEdges + sizeof(uint32_t) * 2 * nbuckets;
 }
 
 // member of a group that owns global vertex v: the last base <= v (bases ascending, bases[0] = 0)
@@ -231,8 +285,28 @@ __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
   const uint32_t cnt = counts[b];
   __syncthreads();
   const uint32_t lmask = bins - 1;
-  for (uint32_t i = t; i < cnt; i += kSortThreads)
-    atomicAdd(&s_cnt[hashed_id(pp, entries[base + i].x) & lmask], 1u);
+  // LDS atomics aggregated per wave for the key of the wave's first lane: a hub vertex fills whole
+  // waves with one key, which would otherwise serialize on one LDS address
+  // the entries are loaded kSortBatch at a time (independent loads, one wait per batch)
+  for (uint32_t i0 = t; i0 < cnt; i0 += kSortBatch * kSortThreads) {
+    uint32_t vx[kSortBatch];
+#pragma unroll
+    for (int k = 0; k < kSortBatch; k++) {
+      const uint32_t i = i0 + k * kSortThreads;
+      vx[k] = entries[base + min(i, cnt - 1)].x;
+    }
+#pragma unroll
+    for (int k = 0; k < kSortBatch; k++) {
+      if (i0 + k * kSortThreads >= cnt) break;
+      const uint32_t key = hashed_id(pp, vx[k]) & lmask;
+      const uint32_t k0 = __builtin_amdgcn_readfirstlane(key);
+      const uint64_t same = __ballot(key == k0);
+      if (key != k0)
+        atomicAdd(&s_cnt[key], 1u);
+      else if (lane_below(same) == 0)
+        atomicAdd(&s_cnt[k0], (uint32_t)__popcll(same));
+    }
+  }
   __syncthreads();
   // entry offsets per local id (exclusive prefix over the bucket)
   const uint32_t per = (bins + kSortThreads - 1) / kSortThreads;
@@ -285,50 +359,86 @@ __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
     }
   }
   __syncthreads();
-  for (uint32_t i = t; i < cnt; i += kSortThreads) {
-    uint2 e = entries[base + i];
-    uint32_t pos = atomicAdd(&s_off[hashed_id(pp, e.x) & lmask], 1u);
-    sorted[base + pos] = e.y;
+  for (uint32_t i0 = t; i0 < cnt; i0 += kSortBatch * kSortThreads) {
+    uint2 ev[kSortBatch];
+#pragma unroll
+    for (int k = 0; k < kSortBatch; k++) {
+      const uint32_t i = i0 + k * kSortThreads;
+      ev[k] = entries[base + min(i, cnt - 1)];
+    }
+#pragma unroll
+    for (int k = 0; k < kSortBatch; k++) {
+      if (i0 + k * kSortThreads >= cnt) break;
+      const uint32_t key = hashed_id(pp, ev[k].x) & lmask;
+      const uint32_t k0 = __builtin_amdgcn_readfirstlane(key);
+      const uint64_t same = __ballot(key == k0);
+      const uint32_t below = lane_below(same);
+      uint32_t pos;
+      if (key != k0) {
+        pos = atomicAdd(&s_off[key], 1u);
+      } else {
+        uint32_t b0 = 0;
+        if (below == 0) b0 = atomicAdd(&s_off[k0], (uint32_t)__popcll(same));
+        // the leader is the lowest set lane of `same`
+        pos = (uint32_t)__builtin_amdgcn_readlane((int)b0, __builtin_ctzll(same)) + below;
+      }
+      sorted[base + pos] = ev[k].y;
+    }
   }
 }
 
 // ------------------------------------------------------------------------------------------
-// K4: aggregation, one wave64 per work item (see the file comment).
+// K5: aggregation, one wave64 per work item (see the file comment).
 //
 // Lane layout: lane = 8*e + s with e = entry slot of a group of 8 consecutive destinations and
 // s = column slot; the lane handles columns s, s+8, ..., s+8(T-1) (those < cols) of its entry.
-// Its column keys therefore stay in registers for the whole item, and an entry's staged values
-// are read from LDS once for T columns (8 lanes read one address: a broadcast).
+// Its column keys and LDS addresses therefore stay in registers for the whole item, and an
+// entry's staged values are read from LDS once for T columns (8 lanes read one address).
 //
-// Depth 0 receives half of all (update, column) pairs; those are XORed into per-lane registers
-// and merged into the LDS delta once per item. Depths 1..dl-1 are XORed into the LDS delta, so
-// the 8 lanes sharing a column rarely hit one LDS address in the same instruction. Depths >= dl
-// (probability 2^-dl per pair) go to HBM with 64-bit atomics.
-//
-// Per-wave LDS (64-bit words): the delta as structure of arrays (alpha[dslots], gamma[dslots],
-// slot = depth * cols + column) and the chunk's stage (edge index, checksum, mix(swap(idx)), 64
-// each).
+// Per-wave LDS delta: rows of `cols` 16-byte buckets {alpha, gamma}. Row 7 + d holds depth d
+// (d < dl); depth 0 receives half of all (update, column) pairs, so entry slots e = 1..7 XOR
+// their depth-0 pairs into private copies (rows e - 1) and slot 0 into row 7: no two lanes of an
+// instruction ever hit one depth-0 address. The copies are folded into row 7 once per item.
+// Every LDS address is one select and one 24-bit multiply-add; only depths >= the item's window
+// (and the exact-hash fallback for a 16-bit tie) take a branch, to 64-bit HBM atomics.
+// After the delta, the chunk's stage: edge index, checksum, mix(swap(idx)), 64 of each.
 // ------------------------------------------------------------------------------------------
+constexpr uint32_t kCopyRows = 7;  // private depth-0 rows of entry slots 1..7
 struct AggLayout {
-  uint32_t cols, dl, dslots;
-  __host__ __device__ AggLayout(uint32_t cols_, uint32_t dl_) : cols(cols_), dl(dl_), dslots(dl_ * cols_) {}
-  __host__ __device__ uint32_t wave_words() const { return 2 * dslots + 3 * 64; }
+  uint32_t cols, dl;
+  __host__ __device__ AggLayout(uint32_t cols_, uint32_t dl_) : cols(cols_), dl(dl_) {}
+  __host__ __device__ uint32_t delta_bytes() const { return (kCopyRows + dl) * cols * 16u; }
+  __host__ __device__ uint32_t wave_bytes() const { return delta_bytes() + 3 * 64 * 8; }
 };
 constexpr int kMaxColSteps = 8;  // T <= 8: up to 64 columns
 constexpr int kFlushRounds = 4;  // 64-slot rounds per flush step
+constexpr uint32_t kMaxLdsDepths = 16;
 
-// kGroup: items belong to the members of a group (descs[item.graph]); the LDS layout is sized
-// for the widest member (max_cols) and the per-member geometry is reloaded when a wave moves to
-// an item of another member. Otherwise every item belongs to the store `st`.
+// 64-bit XOR at an LDS byte address (kept as a plain 32-bit address so that no base is re-added
+// per access)
+typedef __attribute__((address_space(3))) unsigned long long lds_u64_t;
+__device__ __forceinline__ void lds_xor_u64(uint32_t addr, uint64_t v) {
+  __hip_atomic_fetch_xor(reinterpret_cast<lds_u64_t*>((uintptr_t)addr), (unsigned long long)v,
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// kGroup: items belong to the members of a group (descs[item.graph]); the LDS delta is sized for
+// the widest member (max_cols) and laid out with the current member's column count, which is
+// reloaded (with its geometry and keys) when a wave moves to an item of another member (the
+// delta is all zero between items). Otherwise every item belongs to the store `st`.
 // T = ceil(columns / 8) of the widest store the launch covers.
 template <bool kGroup, int T>
 __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
     StoreView st, const GraphDesc* __restrict__ descs, uint32_t max_cols,
     const WorkItem* __restrict__ items, ItemClasses cls,
     const uint32_t* __restrict__ class_counts, const uint32_t* __restrict__ dsts, uint32_t dl,
-    int exclusive_ok, uint32_t min_depth, uint32_t depth_cap) {
+    int exclusive_ok, uint32_t min_depth, uint32_t depth_cap,
+    unsigned long long* __restrict__ traffic) {
   extern __shared__ __align__(16) unsigned char lds[];
   const AggLayout L(kGroup ? max_cols : st.cols, dl);
+  // measurement (traffic != nullptr): device buckets read-modify-written by flushes (delta slots
+  // and deterministic buckets) and (vertex, bucket) updates XORed straight into HBM
+  uint32_t n_flush = 0, n_deep = 0;
   uint32_t cols = st.cols;
   uint32_t bpc = st.bpc;
   uint64_t seed = st.seed;
@@ -336,21 +446,30 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t wid = threadIdx.x / kWave;
   const uint32_t le = lane >> 3, ls = lane & 7;
-  uint64_t* da = reinterpret_cast<uint64_t*>(lds) + (size_t)wid * L.wave_words();
-  uint64_t* dg = da + L.dslots;
-  uint64_t* sidx = dg + L.dslots;
+  const uint32_t wbase = wid * L.wave_bytes();  // byte offset of this wave's delta
+  uint64_t* sidx = reinterpret_cast<uint64_t*>(lds + wbase + L.delta_bytes());
   uint64_t* schk = sidx + kWave;
   uint64_t* smix = schk + kWave;
+  const uint32_t row0 = le == 0 ? kCopyRows : le - 1;  // this lane's depth-0 row
 
   uint64_t colk[T];
-  uint64_t acc_a[T], acc_g[T];
+  // LDS addresses of this lane's buckets: depth 0, and depth d minus d rows
+  const uint32_t lbase = (uint32_t)(uintptr_t)lds + wbase;
+  uint32_t addr0[T], addrd[T];
+  auto set_columns = [&]() {
 #pragma unroll
-  for (int t = 0; t < T; t++) {
-    const uint32_t c = ls + 8 * t;
-    colk[t] = (!kGroup && c < cols) ? xxh3_seed_key(seed + 5ULL * c) : 0;
-    acc_a[t] = acc_g[t] = 0;
+    for (int t = 0; t < T; t++) {
+      const uint32_t c = ls + 8 * t;
+      colk[t] = c < cols ? xxh3_seed_key(seed + 5ULL * c) : 0;
+      addr0[t] = lbase + 16u * (row0 * cols + c);
+      addrd[t] = lbase + 16u * (kCopyRows * cols + c);
+    }
+  };
+  if (!kGroup) set_columns();
+  {
+    uint4* z = reinterpret_cast<uint4*>(lds + wbase);
+    for (uint32_t i = lane; i < L.delta_bytes() / 16; i += kWave) z[i] = make_uint4(0, 0, 0, 0);
   }
-  for (uint32_t e = lane; e < L.dslots; e += kWave) { da[e] = 0; dg[e] = 0; }
   wave_sync_lds();
   uint32_t cur_graph = 0xFFFFFFFFu;
 
@@ -385,9 +504,8 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
 
   // software pipeline: while item i is hashed and flushed, the descriptor of item i+2 and the
   // first chunk of destinations of item i+1 are in flight; inside an item the next chunk of
-  // destinations is loaded while the current one is processed
-  // the first chunk of an item's destinations; lanes past its length load its first entry (an
-  // unconditional load, so no branch joins force an early wait; those values are never used)
+  // destinations is loaded while the current one is processed. Lanes past an item's length load
+  // its first entry (unconditional loads: no branch joins force an early wait; never used).
   auto first_chunk = [&](uint32_t ln, uint32_t sp) -> uint32_t {
     return dsts[(uint64_t)sp + (lane < ln ? lane : 0u)];
   };
@@ -418,11 +536,7 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
         bpc = gd.bpc;
         seed = gd.seed;
         vbase = gd.base;
-#pragma unroll
-        for (int t = 0; t < T; t++) {
-          const uint32_t c = ls + 8 * t;
-          colk[t] = c < cols ? xxh3_seed_key(seed + 5ULL * c) : 0;
-        }
+        set_columns();
       }
     }
     const uint32_t v = v_g - vbase;
@@ -432,6 +546,7 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
     // about 1.5 per item, go to HBM atomics, and the flush scans only the window
     const uint32_t dli = min(dl, 35u - (uint32_t)__builtin_clz(len));
     const uint32_t islots = dli * cols;
+    const uint32_t row_bytes = 16u * cols;
     uint64_t det_a = 0, det_g = 0;
     Bucket16* vrows = st.rows + (size_t)v * st.vstride;
 
@@ -462,18 +577,21 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
           for (int t = 0; t < T; t++) {
             const uint32_t c = ls + 8 * t;
             if (c < cols) {
-              const uint32_t d = depth_fast(m ^ colk[t], bpc);
-              if (d == 0) {
-                acc_a[t] ^= a;
-                acc_g[t] ^= gm;
-              } else if (d < dli) {
-                const uint32_t slot = mul24_lo(d, cols) + c;
-                atomicXor(reinterpret_cast<unsigned long long*>(&da[slot]), (unsigned long long)a);
-                atomicXor(reinterpret_cast<unsigned long long*>(&dg[slot]), (unsigned long long)gm);
-              } else if (d < bpc) {  // rare: deep buckets straight to HBM
-                Bucket16* b = &vrows[(size_t)d * cols + c];
-                atomic_xor_u64(&b->alpha, a);
-                atomic_xor_u64(&b->gamma, gm);
+              const uint64_t x = m ^ colk[t];
+              uint32_t d = depth_low16(x, bpc);
+              const uint32_t offd = mul24_lo(d, row_bytes) + addrd[t];
+              if (d < dli) {
+                const uint32_t off = d ? offd : addr0[t];
+                lds_xor_u64(off, a);
+                lds_xor_u64(off + 8, gm);
+              } else {  // rare: deep buckets (and the 16-bit tie) straight to HBM
+                if (d == 16 && bpc > 16) d = depth_of(xxh3_finish(x), bpc);
+                if (d < bpc) {
+                  Bucket16* b = &vrows[(size_t)d * cols + c];
+                  atomic_xor_u64(&b->alpha, a);
+                  atomic_xor_u64(&b->gamma, gm);
+                  n_deep++;
+                }
               }
             }
           }
@@ -484,16 +602,26 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
     dcur = dfirst_next;
     wcur = wnext;
     wnext = wafter;
-    // depth-0 registers into the LDS delta (slot = column): the 8 lanes of a column slot XOR into
-    // one address, once per item
-#pragma unroll
-    for (int t = 0; t < T; t++) {
-      const uint32_t c = ls + 8 * t;
-      if (c < cols && (acc_a[t] | acc_g[t])) {
-        atomicXor(reinterpret_cast<unsigned long long*>(&da[c]), (unsigned long long)acc_a[t]);
-        atomicXor(reinterpret_cast<unsigned long long*>(&dg[c]), (unsigned long long)acc_g[t]);
+    // fold the private depth-0 rows of the entry slots that were used into row 7
+    const uint32_t ncopy = min(len, 8u) - 1;
+    if (lane < cols && ncopy) {
+      uint4 acc = make_uint4(0, 0, 0, 0);
+      for (uint32_t r = 0; r < ncopy; r++) {
+        uint4* p = reinterpret_cast<uint4*>(lds + wbase + r * row_bytes) + lane;
+        const uint4 x = *p;
+        acc.x ^= x.x;
+        acc.y ^= x.y;
+        acc.z ^= x.z;
+        acc.w ^= x.w;
+        *p = make_uint4(0, 0, 0, 0);
       }
-      acc_a[t] = acc_g[t] = 0;
+      uint4* q = reinterpret_cast<uint4*>(lds + wbase + kCopyRows * row_bytes) + lane;
+      uint4 y = *q;
+      y.x ^= acc.x;
+      y.y ^= acc.y;
+      y.z ^= acc.z;
+      y.w ^= acc.w;
+      *q = y;
     }
     det_a = wave_xor_reduce(det_a);
     det_g = wave_xor_reduce(det_g);
@@ -502,25 +630,30 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
       atomic_xor_u64(&b->alpha, det_a);
       atomic_xor_u64(&b->gamma, det_g);
     }
+    if (traffic && (det_a | det_g)) n_flush++;
     wave_sync_lds();
     // flush, kFlushRounds x 64 slots at a time: all LDS reads first, then the HBM reads of the
     // non-zero slots together, one wait, then the stores (exclusive item: plain read-xor-write;
-    // otherwise 64-bit atomics, which need no read)
+    // otherwise 64-bit atomics, which need no read). Slot s = depth * cols + column is LDS row
+    // 7 + depth and device bucket s of the vertex.
+    Bucket16* drow = reinterpret_cast<Bucket16*>(lds + wbase + kCopyRows * row_bytes);
     for (uint32_t s0 = 0; s0 < islots; s0 += kFlushRounds * kWave) {
       uint64_t xa[kFlushRounds], xg[kFlushRounds];
 #pragma unroll
       for (int r = 0; r < kFlushRounds; r++) {
         const uint32_t sl = s0 + r * kWave + lane;
-        xa[r] = sl < islots ? da[sl] : 0;
-        xg[r] = sl < islots ? dg[sl] : 0;
+        const Bucket16 b = sl < islots ? drow[sl] : Bucket16{0, 0};
+        xa[r] = b.alpha;
+        xg[r] = b.gamma;
       }
 #pragma unroll
       for (int r = 0; r < kFlushRounds; r++) {
         const uint32_t sl = s0 + r * kWave + lane;
-        if (sl < islots) {
-          da[sl] = 0;
-          dg[sl] = 0;
-        }
+        if (sl < islots) drow[sl] = Bucket16{0, 0};
+      }
+      if (traffic) {
+#pragma unroll
+        for (int r = 0; r < kFlushRounds; r++) n_flush += (uint32_t)__popcll(__ballot((xa[r] | xg[r]) != 0));
       }
       if (excl) {
         // unconditional loads (lanes with nothing to flush read the vertex's first bucket) so the
@@ -545,17 +678,24 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
     }
     wave_sync_lds();
   }
+  if (traffic) {
+    for (int o = 32; o > 0; o >>= 1) n_deep += __shfl_xor(n_deep, o);
+    if (lane == 0) {
+      atomicAdd(&traffic[0], (unsigned long long)n_flush);
+      atomicAdd(&traffic[1], (unsigned long long)n_deep);
+    }
+  }
 }
 
 // ------------------------------------------------------------------------------------------
 // launch helpers
 // ------------------------------------------------------------------------------------------
-// depths kept in the LDS delta; deeper updates (probability 2^-12 per column) go to HBM directly
-static uint32_t lds_window(const StoreView& v) { return v.bpc < 12 ? v.bpc : 12; }
+// depths kept in the LDS delta by default; deeper updates (probability 2^-9 per column) go to
+// HBM directly
+constexpr uint32_t kDefaultLdsDepths = 9;
 
 static size_t agg_lds_bytes(uint32_t cols, uint32_t dl) {
-  const AggLayout L(cols, dl);
-  return (size_t)kAggWaves * L.wave_words() * sizeof(uint64_t);
+  return (size_t)kAggWaves * AggLayout(cols, dl).wave_bytes();
 }
 
 // persistent grid: as many workgroups as can be resident at once (the occupancy calculator
@@ -607,6 +747,7 @@ struct AggArgs {
   uint32_t dl;
   int exclusive_ok;
   uint32_t min_depth, depth_cap;
+  unsigned long long* traffic;
 };
 struct AggKernel {
   const void* fn;
@@ -615,7 +756,8 @@ struct AggKernel {
 template <bool G, int T>
 static void agg_launch(dim3 grid, dim3 block, size_t lds, hipStream_t s, const AggArgs& a) {
   hipLaunchKernelGGL((aggregate_kernel<G, T>), grid, block, lds, s, a.v, a.descs, a.cols, a.items,
-                     a.cls, a.class_counts, a.dsts, a.dl, a.exclusive_ok, a.min_depth, a.depth_cap);
+                     a.cls, a.class_counts, a.dsts, a.dl, a.exclusive_ok, a.min_depth, a.depth_cap,
+                     a.traffic);
 }
 template <bool G, int T>
 static AggKernel agg_kernel_entry() {
@@ -643,7 +785,7 @@ static int launch_aggregate(Stream* s, const WorkItem* d_items, const ItemClasse
   int exclusive_ok = 1;
   if (grp) {
     cols = grp->max_cols;
-    dl = std::min(grp->min_bpc, s->tune_dl ? s->tune_dl : 12u);
+    dl = std::min(grp->min_bpc, s->tune_dl ? s->tune_dl : kDefaultLdsDepths);
     device = grp->members[0]->device;
     for (Store* m : grp->members) {  // plain stores only if the group is every member's only writer
       std::lock_guard<std::mutex> lk(m->mu);
@@ -652,11 +794,12 @@ static int launch_aggregate(Stream* s, const WorkItem* d_items, const ItemClasse
   } else {
     v = s->store->view;
     cols = v.cols;
-    dl = s->tune_dl ? std::min(s->tune_dl, v.bpc) : lds_window(v);
+    dl = std::min(v.bpc, s->tune_dl ? s->tune_dl : kDefaultLdsDepths);
     device = s->store->device;
     std::lock_guard<std::mutex> lk(s->store->mu);
     exclusive_ok = s->store->streams.size() == 1 ? 1 : 0;
   }
+  dl = std::min(dl, kMaxLdsDepths);  // the 16-bit depth needs no exact hash inside the window
   GS_CHECK(cols >= 1 && cols <= 8 * kMaxColSteps, GS_ERR_UNSUPPORTED,
            "sketches with %u columns (more than %d) are not supported by the update kernel", cols,
            8 * kMaxColSteps);
@@ -672,8 +815,13 @@ static int launch_aggregate(Stream* s, const WorkItem* d_items, const ItemClasse
     GS_HIP(hipEventRecord(e0, s->stream));
   }
   const dim3 grid(agg_grid(device, kern.fn, lds)), block(kAggWaves * kWave);
+  if (s->profile && !s->d_traffic) {
+    GS_HIP(hipMalloc(&s->d_traffic, 2 * sizeof(unsigned long long)));
+    GS_HIP(hipMemsetAsync(s->d_traffic, 0, 2 * sizeof(unsigned long long), s->stream));
+  }
   AggArgs a{v, grp ? grp->d_descs : nullptr, cols, d_items, cls, class_counts, d_dsts, dl,
-            exclusive_ok, grp ? 0u : min_depth, grp ? 0u : depth_cap};
+            exclusive_ok, grp ? 0u : min_depth, grp ? 0u : depth_cap,
+            s->profile ? s->d_traffic : nullptr};
   kern.launch(grid, block, lds, s->stream, a);
   GS_HIP(hipGetLastError());
   if (s->profile) {
@@ -696,8 +844,8 @@ void free_pair_scratch(PairScratch& ws) {
   (void)hipFree(ws.d_entries);
   (void)hipFree(ws.d_sorted);
   (void)hipFree(ws.d_items);
-  (void)hipFree(ws.d_counts);
-  (void)hipFree(ws.d_cursors);
+  (void)hipFree(ws.d_hist);
+  (void)hipFree(ws.d_totals);
   (void)hipFree(ws.d_counters);
   ws = PairScratch{};
 }
@@ -718,28 +866,29 @@ PartParams make_part_params(uint32_t n, int local_bits) {
   return p;
 }
 
+static uint32_t part_blocks(uint64_t nedges) {
+  return (uint32_t)((nedges + kPartEdges - 1) / kPartEdges);
+}
+
 int ensure_pair_scratch(Stream* s, uint64_t nedges) {
   PairScratch& ws = s->ws;
   const PartParams& pp = stream_part(s);
-  if (ws.cap_buckets < pp.nbuckets) {
-    GS_HIP(hipStreamSynchronize(s->stream));
-    (void)hipFree(ws.d_counts);
-    (void)hipFree(ws.d_cursors);
-    (void)hipFree(ws.d_counters);
-    ws.d_counts = ws.d_cursors = ws.d_counters = nullptr;
-    GS_HIP(hipMalloc(&ws.d_counts, sizeof(uint32_t) * 2 * pp.nbuckets));
-    GS_HIP(hipMalloc(&ws.d_cursors, sizeof(uint32_t) * pp.nbuckets));
+  if (!ws.d_counters) {
     GS_HIP(hipMalloc(&ws.d_counters, sizeof(uint32_t) * kCounterWords));
-    GS_HIP(hipMemsetAsync(ws.d_counts, 0, sizeof(uint32_t) * 2 * pp.nbuckets, s->stream));
     GS_HIP(hipMemsetAsync(ws.d_counters, 0, sizeof(uint32_t) * kCounterWords, s->stream));
-    ws.cap_buckets = pp.nbuckets;
-    ws.cur_buckets = pp.nbuckets;
-    ws.parity = 0;
   }
-  if (ws.cur_buckets != pp.nbuckets) {  // re-tuned: both count buffers must start at zero
-    GS_HIP(hipMemsetAsync(ws.d_counts, 0, sizeof(uint32_t) * 2 * ws.cap_buckets, s->stream));
-    ws.cur_buckets = pp.nbuckets;
-    ws.parity = 0;
+  const uint64_t hist_words = (uint64_t)pp.nbuckets * part_blocks(nedges);
+  if (ws.cap_buckets < pp.nbuckets || ws.cap_hist < hist_words) {
+    GS_HIP(hipStreamSynchronize(s->stream));
+    (void)hipFree(ws.d_hist);
+    (void)hipFree(ws.d_totals);
+    ws.d_hist = ws.d_totals = nullptr;
+    const uint32_t nb = std::max(ws.cap_buckets, pp.nbuckets);
+    const uint64_t hw = std::max(ws.cap_hist, hist_words);
+    GS_HIP(hipMalloc(&ws.d_hist, sizeof(uint32_t) * hw));
+    GS_HIP(hipMalloc(&ws.d_totals, sizeof(uint32_t) * nb));
+    ws.cap_buckets = nb;
+    ws.cap_hist = hw;
   }
   if (ws.cap_edges < nedges) {
     GS_HIP(hipStreamSynchronize(s->stream));
@@ -750,7 +899,6 @@ int ensure_pair_scratch(Stream* s, uint64_t nedges) {
     ws.d_sorted = nullptr;
     ws.d_items = nullptr;
     uint64_t cap = nedges;
-    // class k holds items of length >= 2^k, at most one shorter-than-chunk item per vertex
     const uint64_t verts = std::min<uint64_t>((uint64_t)stream_vertices(s), 2 * cap);
     uint64_t icap = 0;
     // classes that may hold full items (len >= 64, the smallest item size a stream can be tuned
@@ -795,30 +943,33 @@ static int ensure_pending(Stream* s) {
   return ensure_pair_scratch(s, want);
 }
 
-// K1 over `m` pairs at src, into the fill buffer behind what is pending
+// `m` pairs at src into the fill buffer behind what is pending (src may be that very place: a
+// host batch copied there; then only a subgraph filter needs a pass)
 static int append_pairs(Stream* s, const uint32_t* src, uint64_t m, uint32_t min_depth,
-                        uint32_t depth_cap) {
-  PairScratch& ws = s->ws;
-  const PartParams& pp = stream_part(s);
-  const uint32_t nverts = stream_vertices(s);
-  const uint64_t seed = s->group ? 0 : s->store->view.seed;
+                        uint32_t depth_cap, bool host_copy) {
   GS_CHECK(!s->group || min_depth == 0, GS_ERR_INVALID, "no subgraph filter on group streams");
-  uint32_t* counts_cur = ws.d_counts + (size_t)ws.parity * pp.nbuckets;
-  uint32_t* counts_next = ws.d_counts + (size_t)(ws.parity ^ 1) * pp.nbuckets;
-  int rc = ensure_dyn_lds((const void*)append_count_kernel, sizeof(uint32_t) * pp.nbuckets);
-  if (rc) return rc;
   uint2* dst = reinterpret_cast<uint2*>(s->d_pend[s->pend_buf]) + s->pend_n;
-  const uint32_t nblk = (uint32_t)((m + kPartEdges - 1) / kPartEdges);
-  hipLaunchKernelGGL(append_count_kernel, dim3(nblk), dim3(kPartThreads),
-                     sizeof(uint32_t) * pp.nbuckets, s->stream, reinterpret_cast<const uint2*>(src),
-                     dst, m, nverts, pp, seed, min_depth, depth_cap, counts_cur, counts_next,
-                     ws.d_cursors, ws.d_counters);
-  GS_HIP(hipGetLastError());
+  if (reinterpret_cast<const uint2*>(src) != dst || min_depth) {
+    const uint64_t seed = s->group ? 0 : s->store->view.seed;
+    const unsigned grid = (unsigned)std::min<uint64_t>((m + 255) / 256, 4096);
+    hipLaunchKernelGGL(append_kernel, dim3(grid), dim3(256), 0, host_copy ? s->copy_stream : s->stream,
+                       reinterpret_cast<const uint2*>(src), dst, m, stream_vertices(s), seed,
+                       min_depth, depth_cap);
+    GS_HIP(hipGetLastError());
+  }
+  if (host_copy) GS_HIP(hipEventRecord(s->copied, s->copy_stream));
   s->pend_n += m;
   return GS_OK;
 }
 
-// K2..K4 over the fill buffer; the other buffer becomes the fill buffer
+// host copies (copy stream) may write the fill buffer once the launch that last read it is done
+static int wait_fill_buffer(Stream* s) {
+  if (s->pend_used[s->pend_buf])
+    GS_HIP(hipStreamWaitEvent(s->copy_stream, s->pend_free[s->pend_buf], 0));
+  return GS_OK;
+}
+
+// K1..K5 over the fill buffer; the other buffer becomes the fill buffer
 static int process_pending(Stream* s) {
   if (s->pend_n == 0) return GS_OK;
   PairScratch& ws = s->ws;
@@ -826,26 +977,38 @@ static int process_pending(Stream* s) {
   const uint32_t nverts = stream_vertices(s);
   const uint64_t nedges = s->pend_n;
   const int b = s->pend_buf;
-  uint32_t* counts_cur = ws.d_counts + (size_t)ws.parity * pp.nbuckets;
-  const uint32_t nblk = (uint32_t)((nedges + kPartEdges - 1) / kPartEdges);
+  const uint32_t nblk = part_blocks(nedges);
   const size_t sort_lds = sizeof(uint32_t) * 3 * ((size_t)1 << pp.local_bits);
   int rc;
-  if ((rc = ensure_dyn_lds((const void*)bucket_scatter_kernel, sizeof(uint32_t) * 2 * pp.nbuckets)) ||
+  if ((rc = ensure_dyn_lds((const void*)bucket_count_kernel, sizeof(uint32_t) * pp.nbuckets)) ||
+      (rc = ensure_dyn_lds((const void*)bucket_scatter_kernel, scatter_lds_bytes(pp.nbuckets))) ||
       (rc = ensure_dyn_lds((const void*)bucket_sort_kernel, sort_lds)))
     return rc;
+  uint2* pend = reinterpret_cast<uint2*>(s->d_pend[b]);
+  if (s->copies_pending) {  // every host copy into the buffer has landed
+    GS_HIP(hipStreamWaitEvent(s->stream, s->copied, 0));
+    s->copies_pending = false;
+  }
+  hipLaunchKernelGGL(bucket_count_kernel, dim3(nblk), dim3(kPartThreads),
+                     sizeof(uint32_t) * pp.nbuckets, s->stream, pend, nedges, nverts, pp, nblk,
+                     ws.d_hist, ws.d_counters);
+  GS_HIP(hipGetLastError());
+  hipLaunchKernelGGL(bucket_offsets_kernel, dim3(pp.nbuckets), dim3(kOffThreads), 0, s->stream,
+                     ws.d_hist, nblk, ws.d_totals);
+  GS_HIP(hipGetLastError());
   hipLaunchKernelGGL(bucket_scatter_kernel, dim3(nblk), dim3(kPartThreads),
-                     sizeof(uint32_t) * 2 * pp.nbuckets, s->stream, s->d_pend[b], nedges, nverts,
-                     pp, counts_cur, ws.d_cursors, ws.d_entries);
+                     scatter_lds_bytes(pp.nbuckets), s->stream,
+                     reinterpret_cast<const uint32_t*>(pend), nedges, nverts, pp, nblk,
+                     ws.d_totals, ws.d_hist, ws.d_entries);
   GS_HIP(hipGetLastError());
   GS_HIP(hipEventRecord(s->pend_free[b], s->stream));  // the fill buffer has been read
   s->pend_used[b] = true;
   hipLaunchKernelGGL(bucket_sort_kernel, dim3(pp.nbuckets), dim3(kSortThreads), sort_lds,
-                     s->stream, ws.d_entries, pp, counts_cur, ws.d_sorted, ws.d_items, ws.cls,
+                     s->stream, ws.d_entries, pp, ws.d_totals, ws.d_sorted, ws.d_items, ws.cls,
                      s->tune_chunk ? s->tune_chunk : kChunk, ws.d_counters + kClassBase,
                      s->group ? (const uint32_t*)s->group->d_bases : (const uint32_t*)nullptr,
                      s->group ? (uint32_t)s->group->members.size() : 1u);
   GS_HIP(hipGetLastError());
-  ws.parity ^= 1;
   s->pend_n = 0;
   s->pend_buf ^= 1;
   return launch_aggregate(s, ws.d_items, ws.cls, ws.d_counters + kClassBase, ws.d_sorted,
@@ -858,10 +1021,10 @@ int submit_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uin
   std::lock_guard<std::mutex> lk(s->mu);
   int rc = ensure_pending(s);
   if (rc) return rc;
-  // (the launches that read the fill buffer earlier ran on this same stream)
+  // (the launch that last read the fill buffer ran on this same stream)
   for (uint64_t e0 = 0; e0 < nedges;) {
     const uint64_t m = std::min(nedges - e0, s->pend_cap - s->pend_n);
-    if ((rc = append_pairs(s, d_pairs + 2 * e0, m, min_depth, depth_cap))) return rc;
+    if ((rc = append_pairs(s, d_pairs + 2 * e0, m, min_depth, depth_cap, false))) return rc;
     e0 += m;
     if (s->pend_n == s->pend_cap && (rc = process_pending(s))) return rc;
   }
@@ -905,17 +1068,15 @@ int submit_pairs_host(Stream* s, const uint32_t* pairs, uint64_t nedges, uint32_
       src = s->h_stage;
     }
     // the fill buffer may still be read by the launch that processed it two buffers back
-    if (s->pend_used[s->pend_buf])
-      GS_HIP(hipStreamWaitEvent(s->copy_stream, s->pend_free[s->pend_buf], 0));
+    if ((rc = wait_fill_buffer(s))) return rc;
     uint32_t* dst = s->d_pend[s->pend_buf] + 2 * s->pend_n;
     GS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->copy_stream));
-    GS_HIP(hipEventRecord(s->copied, s->copy_stream));
-    GS_HIP(hipStreamWaitEvent(s->stream, s->copied, 0));
     if (!pinned) {
       GS_HIP(hipEventRecord(s->stage_free, s->copy_stream));
       s->stage_pending = true;
     }
-    if ((rc = append_pairs(s, dst, m, min_depth, depth_cap))) return rc;
+    if ((rc = append_pairs(s, dst, m, min_depth, depth_cap, true))) return rc;  // records `copied`
+    s->copies_pending = true;
     e0 += m;
     if (s->pend_n == s->pend_cap && (rc = process_pending(s))) return rc;
   }
@@ -948,11 +1109,11 @@ int autotune_stream(Stream* s, const uint32_t* d_pairs, uint64_t nedges, gs_tuni
   const uint32_t min_bpc = s->group ? s->group->min_bpc : s->store->view.bpc;
   const PartParams base = s->group ? s->group->part : s->store->part;
   std::vector<uint32_t> dls, chunks = {256, 512, 1024}, bits;
-  for (uint32_t d : {8u, 10u, 12u, 14u, 16u})
+  for (uint32_t d : {8u, 9u, 10u, 12u})
     if (d <= min_bpc) dls.push_back(d);
   if (dls.empty()) dls.push_back(min_bpc);
   for (int b = (int)base.local_bits - 1; b <= (int)base.local_bits + 1; b++)
-    if (b >= 0 && b <= 13 && b <= (int)base.nbits && (base.nbits - b) <= 14) bits.push_back((uint32_t)b);
+    if (b >= 0 && b <= 13 && b <= (int)base.nbits && (base.nbits - b) <= 13) bits.push_back((uint32_t)b);
   hipEvent_t e0, e1;
   GS_HIP(hipEventCreate(&e0));
   GS_HIP(hipEventCreate(&e1));

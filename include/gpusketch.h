@@ -154,10 +154,15 @@ int gs_min_cut(uint32_t n, const uint32_t* pairs, uint64_t m, uint64_t* value,
  * milliseconds, the launch count and the directed updates those launches processed. */
 int gs_stream_set_profiling(gs_stream* s, int enable);
 int gs_stream_timing(gs_stream* s, double* agg_ms, uint64_t* launches, uint64_t* updates);
+/* HBM work of the profiled aggregation launches, summed and reset: device buckets read-modify-
+ * written by flushes (non-zero delta slots and deterministic buckets) and (vertex, bucket)
+ * updates beyond the LDS window XORed straight into HBM. No counterpart in the reference (its
+ * benchmarks report throughput only); used for the bytes of the roofline. */
+int gs_stream_traffic(gs_stream* s, uint64_t* flushed_buckets, uint64_t* direct_updates);
 
 /* ---- Tuning ----------------------------------------------------------------------------------
  * Launch parameters of a stream's update path (0 = default): lds_depths = depths of every column
- * kept in the aggregation kernel's LDS delta (default 12), item_updates = destinations per work
+ * kept in the aggregation kernel's LDS delta (1..16, default 9), item_updates = destinations per work
  * item (64..1024, default 512), bucket_bits = vertex-id bits resolved inside one grouping
  * bucket (default log2(n) - 8, at most 13), launch_edges = submitted edge pairs coalesced into
  * one launch of the grouping and aggregation kernels (2^12..2^28, default 2^22; a flush or a

@@ -73,7 +73,8 @@ GS_HD uint32_t align_bits(uint32_t hi, uint32_t lo, uint32_t s) {
 // product x1 = x * M is needed in full (x2 = x1 ^ ((x1 >> 35) + 8) feeds x1's high word into x2's
 // low word; the added term is below 2^30, so x2's high word equals x1's). When those 16 bits are
 // all zero (probability 2^-16) and max_depth > 16 the full hash decides.
-GS_HD uint32_t depth_fast(uint64_t x, uint32_t max_depth) {
+// min(depth, 16) part: ctz of the low 16 bits of h, capped at min(max_depth, 16)
+GS_HD uint32_t depth_low16(uint64_t x, uint32_t max_depth) {
   constexpr uint32_t kMlo = (uint32_t)kRrmxmxMul, kMhi = (uint32_t)(kRrmxmxMul >> 32);
   const uint32_t xlo = (uint32_t)x, xhi = (uint32_t)(x >> 32);
   const uint64_t p = (uint64_t)xlo * kMlo;
@@ -83,7 +84,10 @@ GS_HD uint32_t depth_fast(uint64_t x, uint32_t max_depth) {
   const uint32_t x3hi = (uint32_t)(q >> 32) + mul24_lo(x2lo, kMhi) + mul24_lo(x1hi, kMlo);
   const uint32_t h16 = ((uint32_t)q ^ align_bits(x3hi, (uint32_t)q, 28)) & 0xFFFFu;
   const uint32_t cap = max_depth < 16u ? max_depth : 16u;
-  const uint32_t d = (uint32_t)__builtin_ctz(h16 | (1u << cap));
+  return (uint32_t)__builtin_ctz(h16 | (1u << cap));
+}
+GS_HD uint32_t depth_fast(uint64_t x, uint32_t max_depth) {
+  const uint32_t d = depth_low16(x, max_depth);
   if (d < 16u || max_depth <= 16u) return d;
   return depth_of(xxh3_finish(x), max_depth);
 }
