@@ -271,6 +271,7 @@ int gs_store_zero(gs_store* h) {
   rc = launch_zero_store(&h->impl, h->impl.qstream);
   if (rc) return rc;
   GS_HIP(hipStreamSynchronize(h->impl.qstream));
+  h->impl.symmetric = true;
   return GS_OK;
 }
 
@@ -328,6 +329,7 @@ static int submit_vertex_batches(gs_stream* sh, const uint32_t* src, const uint6
     nitems += (sizes[b] + chunk - 1) / chunk;
   }
   if (total == 0) return GS_OK;
+  s->store->symmetric = false;  // directed updates: the other directions may never come
   // staging: per-class item counts (all items in class 0), the items, the destinations
   const size_t head = 64;
   size_t item_bytes = nitems * sizeof(WorkItem);
@@ -458,6 +460,18 @@ int gs_import_buckets(gs_store* h, uint32_t v0, uint32_t count, const gs_bucket*
   const uint32_t nb = B * C + 1;
   const uint32_t per = 1024;
   std::vector<Bucket16> rows((size_t)per * vs), det(per);
+  // a whole store is symmetric (its vertex sketches XOR to zero) if the imported ones do; a part
+  // of a store cannot be judged from the part alone
+  bool zero_sum = v0 == 0 && count == st.view.n;
+  if (zero_sum) {
+    std::vector<uint64_t> sum(2 * (size_t)nb, 0);
+    for (uint64_t i = 0; i < (uint64_t)count * nb; i++) {
+      sum[2 * (i % nb)] ^= in[i].alpha;
+      sum[2 * (i % nb) + 1] ^= in[i].gamma;
+    }
+    for (uint64_t x : sum) zero_sum = zero_sum && x == 0;
+  }
+  st.symmetric = zero_sum;
   for (uint32_t a = 0; a < count; a += per) {
     uint32_t m = std::min(per, count - a);
     std::fill(rows.begin(), rows.end(), Bucket16{0, 0});

@@ -20,6 +20,15 @@
 namespace gs {
 
 constexpr int kQThreads = 256;
+// query flags: [0] modified this round, [1] out of samples, [2] forest edges, [3] giant
+// supernode of the round, [kFlagBig + k] big supernode k (see pick_big_kernel)
+constexpr uint32_t kNoGiant = 0xFFFFFFFFu;
+constexpr int kBig = 8;
+constexpr int kPickSamples = 256;
+constexpr uint32_t kMinBig = 4;
+constexpr int kFlagBig = 4;
+constexpr int kFlagWords = 16;
+constexpr int kAccBatch = 8;  // buckets loaded together per lane by the group accumulation
 
 __device__ __forceinline__ uint32_t load_parent(uint32_t* p, uint32_t x) {
   return __hip_atomic_load(&p[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -100,7 +109,7 @@ __global__ void clear_round_flags_kernel(uint32_t* flags) {
 }
 
 __global__ void clear_all_flags_kernel(uint32_t* flags) {
-  if (threadIdx.x < 4) flags[threadIdx.x] = 0;
+  if (threadIdx.x < kFlagWords) flags[threadIdx.x] = 0;
 }
 
 __global__ void zero_acc_kernel(Bucket16* acc, uint64_t count) {
@@ -109,81 +118,281 @@ __global__ void zero_acc_kernel(Bucket16* acc, uint64_t count) {
     acc[i] = Bucket16{0, 0};
 }
 
+// Sketch::sample (src/sketch.cpp:110-132) over buckets held one per lane: lanes 0..bpc-1 hold
+// depths 0..B-1 of the sampled column, lane bpc the deterministic bucket. ZERO if that bucket is
+// empty, GOOD on it if it is good, else GOOD on the first good depth, else FAIL. Every lane takes
+// part; the result and *idx are wave-uniform.
+__device__ __forceinline__ int wave_sample(const Bucket16 b, uint32_t lane, uint32_t bpc,
+                                           uint64_t seed, uint64_t* idx) {
+  const bool in = lane <= bpc;
+  const uint64_t good = __ballot(in && b.gamma == xxh3_len8(b.alpha, seed));
+  const uint64_t det_zero = __ballot(lane == bpc && b.alpha == 0 && b.gamma == 0);
+  if (det_zero) return 1;
+  int l = -1;
+  if ((good >> bpc) & 1) {
+    l = (int)bpc;
+  } else {
+    const uint64_t dm = good & ((1ULL << bpc) - 1);
+    if (dm) l = __builtin_ctzll(dm);
+  }
+  if (l < 0) return 2;
+  *idx = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(b.alpha >> 32), l) << 32) |
+         (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b.alpha, l);
+  return 0;
+}
+
+// Round 0: every vertex samples its own column col0 (run_round_zero, src/cc_sketch_alg.cpp:
+// 250-271). A wave takes 64 consecutive vertices, four at a time, one bucket per lane.
 __global__ __launch_bounds__(kQThreads) void round0_kernel(StoreView st, uint32_t col0,
                                                            uint32_t* parent, uint32_t* vflags,
                                                            uint32_t* flags, uint32_t* forest,
                                                            uint64_t forest_cap) {
-  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= st.n) return;
-  uint64_t idx = 0;
-  int res = sample_column(st.det[v], st.rows + (size_t)v * st.vstride + col0, st.cols, st.bpc,
-                          st.seed, &idx);
-  handle_sample(res, idx, v, st.n, parent, vflags, flags, forest, forest_cap);
-}
-
-// Rounds >= 1: XOR column `col` (depths 0..B-1) and the deterministic bucket of every vertex of
-// a live supernode into the supernode's accumulator. Lanes are vertices; the lanes of a wave
-// that share the supernode of the wave's first live lane (the common case once a large
-// component has formed) are XOR-reduced with DPP and issue one atomic per bucket, the others
-// issue their own. Every lane stays active through the reductions.
-constexpr int kAccBatch = 8;
-__global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st, uint32_t col,
-                                                        const uint32_t* __restrict__ comp,
-                                                        const uint32_t* __restrict__ vflags,
-                                                        Bucket16* __restrict__ acc) {
-  const uint32_t w = st.bpc + 1;
-  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63;
-  bool act = v < st.n;
-  uint32_t r = act ? comp[v] : 0;
-  if (act && (vflags[r] & kVDone)) act = false;
-  const uint64_t am = __ballot(act);
-  if (am == 0) return;  // wave-uniform
-  const uint32_t r0 = (uint32_t)__shfl((int)r, __builtin_ctzll(am));
-  const bool grp = act && r == r0;
-  const Bucket16* vrow = st.rows + (size_t)(act ? v : 0) * st.vstride + col;
-  Bucket16* a0 = acc + (size_t)r0 * w;
-  Bucket16* ar = acc + (size_t)r * w;
-  for (uint32_t j0 = 0; j0 < w; j0 += kAccBatch) {
-    Bucket16 b[kAccBatch];
+  const uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64 * 64;
+  if (v0 >= st.n) return;
+  const uint32_t cnt = min(64u, st.n - v0);
+  const uint32_t jj = lane <= st.bpc ? lane : 0;
+  int my_res = -1;  // the sample of vertex v0 + lane, handled by its own lane at the end
+  uint64_t my_idx = 0;
+  for (uint32_t i0 = 0; i0 < cnt; i0 += 4) {
+    Bucket16 bv[4];
 #pragma unroll
-    for (int k = 0; k < kAccBatch; k++) {
-      const uint32_t j = j0 + k;
-      b[k] = Bucket16{0, 0};
-      if (act && j < w) b[k] = j < st.bpc ? vrow[(size_t)j * st.cols] : st.det[v];
+    for (int q = 0; q < 4; q++) {
+      const uint32_t vi = v0 + min(i0 + q, cnt - 1);
+      bv[q] = jj < st.bpc ? st.rows[(size_t)vi * st.vstride + (size_t)jj * st.cols + col0] : st.det[vi];
     }
 #pragma unroll
-    for (int k = 0; k < kAccBatch; k++) {
-      const uint32_t j = j0 + k;
-      if (j >= w) break;
-      const uint64_t xa = wave_xor_reduce(grp ? b[k].alpha : 0);
-      const uint64_t xg = wave_xor_reduce(grp ? b[k].gamma : 0);
-      if (lane == 0 && (xa | xg)) {
-        __hip_atomic_fetch_xor(&a0[j].alpha, xa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_xor(&a0[j].gamma, xg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-      if (act && !grp && (b[k].alpha | b[k].gamma)) {
-        __hip_atomic_fetch_xor(&ar[j].alpha, b[k].alpha, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_xor(&ar[j].gamma, b[k].gamma, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int q = 0; q < 4; q++) {
+      if (i0 + q >= cnt) break;
+      uint64_t idx = 0;
+      const int res = wave_sample(bv[q], lane, st.bpc, st.seed, &idx);
+      if (lane == i0 + q) {
+        my_res = res;
+        my_idx = idx;
       }
     }
   }
+  if (my_res >= 0) handle_sample(my_res, my_idx, v0 + lane, st.n, parent, vflags, flags, forest, forest_cap);
 }
 
+// Big supernodes of a round, picked from 256 sampled vertices (pick_big_kernel): up to kBig live
+// supernodes holding at least kMinBig of the samples each (flags[4 + k], kNoGiant when unused),
+// largest first. Their column sums are formed from per-wave partial sums instead of atomics
+// into one accumulator row (which thousands of waves would contend for). When the store's
+// updates were all symmetric (edge pairs: every update XORs the same bucket of both endpoints)
+// the XOR of a column over all vertices is zero, and a supernode holding more than half of the
+// samples is the giant (flags[3]): its sum is the XOR over all vertices outside it, so its
+// members are not even read.
+
+__global__ __launch_bounds__(kPickSamples) void pick_big_kernel(uint32_t n,
+                                                                const uint32_t* __restrict__ comp,
+                                                                const uint32_t* __restrict__ vflags,
+                                                                int symmetric, uint32_t* flags) {
+  __shared__ uint32_t s_r[kPickSamples];
+  __shared__ uint32_t s_key[kPickSamples];
+  const uint32_t t = threadIdx.x;
+  const uint32_t v = (uint32_t)(((uint64_t)t * n) / kPickSamples + n / (2 * kPickSamples));
+  uint32_t r = comp[v];
+  if (vflags[r] & kVDone) r = kNoGiant;
+  s_r[t] = r;
+  __syncthreads();
+  // samples in this supernode, counted at its first sample only
+  uint32_t cnt = 0, first = t;
+  for (uint32_t i = 0; i < kPickSamples; i++) {
+    const bool same = s_r[i] == r;
+    cnt += same;
+    if (same && i < first) first = i;
+  }
+  uint32_t key = (r != kNoGiant && first == t && cnt >= kMinBig) ? (cnt << 16) | t : 0u;
+  for (int k = 0; k < kBig; k++) {
+    s_key[t] = key;
+    __syncthreads();
+    for (uint32_t o = kPickSamples / 2; o > 0; o >>= 1) {
+      if (t < o) s_key[t] = max(s_key[t], s_key[t + o]);
+      __syncthreads();
+    }
+    const uint32_t best = s_key[0];
+    __syncthreads();
+    if (k > 0 && best == 0) {  // no further candidates (block-uniform)
+      if (t == 0)
+        for (int q = k; q < kBig; q++) flags[kFlagBig + q] = kNoGiant;
+      break;
+    }
+    if (t == 0) {
+      const uint32_t root = best ? s_r[best & 0xFFFFu] : kNoGiant;
+      if (k == 0) {
+        const bool giant = best && symmetric && (best >> 16) > kPickSamples / 2;
+        flags[3] = giant ? root : kNoGiant;
+        flags[kFlagBig] = giant ? kNoGiant : root;  // the giant is formed from the rest
+      } else {
+        flags[kFlagBig + k] = root;
+      }
+    }
+    if (key == best) key = 0;
+  }
+}
+
+// Rounds >= 1: the column sums (column `col`, depths 0..B-1, plus the deterministic bucket) of
+// the live supernodes. A wave takes 64 consecutive vertices and walks the ones it must read, four
+// at a time; lane j holds bucket j (so one vertex is one load instruction, and its row of atomics
+// is one instruction over consecutive addresses). Big supernodes accumulate in registers and
+// leave per-wave partials bpart[wave][k]; with a giant, every vertex outside it (finished
+// supernodes included) also adds into cpart[wave]; the other supernodes take atomics.
+__global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st, uint32_t col,
+                                                        const uint32_t* __restrict__ comp,
+                                                        const uint32_t* __restrict__ vflags,
+                                                        const uint32_t* __restrict__ flags,
+                                                        Bucket16* __restrict__ acc,
+                                                        Bucket16* __restrict__ cpart,
+                                                        Bucket16* __restrict__ bpart) {
+  const uint32_t w = st.bpc + 1;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const uint32_t v0 = gwave * 64;
+  const uint32_t giant = flags[3];
+  uint32_t big[kBig];
+#pragma unroll
+  for (int k = 0; k < kBig; k++) big[k] = flags[kFlagBig + k];
+  const bool split = giant != kNoGiant;
+  const uint32_t v = v0 + lane;
+  const bool inr = v < st.n;
+  const uint32_t r = inr ? comp[v] : 0;
+  const bool done = inr && (vflags[r] & kVDone);
+  const bool rd = inr && (split ? r != giant : !done);  // this vertex's column is read
+  const bool act = rd && !done;                         // and belongs to a live supernode
+  int kb = -1;
+#pragma unroll
+  for (int k = kBig - 1; k >= 0; k--)
+    if (r == big[k]) kb = k;
+  const uint64_t rdm = __ballot(rd), actm = __ballot(act);
+  const bool lj = lane < w;
+  const uint32_t jj = lj ? lane : 0;
+  uint64_t ca = 0, cg = 0, ba[kBig], bg[kBig];
+#pragma unroll
+  for (int k = 0; k < kBig; k++) ba[k] = bg[k] = 0;
+  uint64_t m = rdm;
+  while (m) {
+    uint32_t ii[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      ii[q] = m ? (uint32_t)__builtin_ctzll(m) : ii[0];
+      m &= m ? m - 1 : 0;
+    }
+    Bucket16 bv[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint32_t vi = v0 + ii[q];
+      bv[q] = jj < st.bpc ? st.rows[(size_t)vi * st.vstride + (size_t)jj * st.cols + col] : st.det[vi];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      if (q > 0 && ii[q] == ii[0]) break;  // padding of the last group
+      const uint32_t i = ii[q];
+      const uint64_t xa = lj ? bv[q].alpha : 0, xg = lj ? bv[q].gamma : 0;
+      if (split) {
+        ca ^= xa;
+        cg ^= xg;
+      }
+      if (!((actm >> i) & 1)) continue;
+      const int kbi = __builtin_amdgcn_readlane(kb, i);
+      if (kbi >= 0) {
+#pragma unroll
+        for (int k = 0; k < kBig; k++)
+          if (kbi == k) {
+            ba[k] ^= xa;
+            bg[k] = bg[k] ^ xg;
+          }
+      } else if (lj && (xa | xg)) {
+        Bucket16* dst = acc + (size_t)__builtin_amdgcn_readlane((int)r, i) * w + jj;
+        __hip_atomic_fetch_xor(&dst->alpha, xa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_xor(&dst->gamma, xg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  }
+  if (lj) {
+    if (split) cpart[(size_t)gwave * w + lane] = Bucket16{ca, cg};
+#pragma unroll
+    for (int k = 0; k < kBig; k++)
+      if (big[k] != kNoGiant) bpart[((size_t)gwave * kBig + k) * w + lane] = Bucket16{ba[k], bg[k]};
+  }
+}
+
+// accumulators of the big supernodes (blockIdx.y = k < kBig) and of the giant (blockIdx.y =
+// kBig), bucket blockIdx.x: XOR of the wave partials (for the giant: the XOR over all vertices
+// being zero, the partials of everything outside it)
+__global__ __launch_bounds__(kQThreads) void big_acc_kernel(const uint32_t* __restrict__ flags,
+                                                            const Bucket16* __restrict__ cpart,
+                                                            const Bucket16* __restrict__ bpart,
+                                                            uint32_t nwaves, uint32_t w,
+                                                            Bucket16* __restrict__ acc) {
+  const uint32_t k = blockIdx.y;
+  const uint32_t root = k < (uint32_t)kBig ? flags[kFlagBig + k] : flags[3];
+  if (root == kNoGiant) return;
+  const uint32_t j = blockIdx.x;
+  uint64_t a = 0, g = 0;
+  for (uint32_t i = threadIdx.x; i < nwaves; i += kQThreads) {
+    const Bucket16 b = k < (uint32_t)kBig ? bpart[((size_t)i * kBig + k) * w + j] : cpart[(size_t)i * w + j];
+    a ^= b.alpha;
+    g ^= b.gamma;
+  }
+  a = wave_xor_reduce(a);
+  g = wave_xor_reduce(g);
+  __shared__ uint64_t sa[kQThreads / 64], sg[kQThreads / 64];
+  if ((threadIdx.x & 63) == 0) {
+    sa[threadIdx.x / 64] = a;
+    sg[threadIdx.x / 64] = g;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < kQThreads / 64; q++) {
+      a ^= sa[q];
+      g ^= sg[q];
+    }
+    acc[(size_t)root * w + j] = Bucket16{a, g};
+  }
+}
+
+// Every live root samples its accumulator (bucket bpc = the deterministic bucket) and clears it
+// for the next round. A wave takes the roots among 64 consecutive vertices, four at a time, one
+// bucket per lane.
 __global__ __launch_bounds__(kQThreads) void sample_kernel(StoreView st,
                                                            const uint32_t* __restrict__ comp,
                                                            Bucket16* __restrict__ acc,
                                                            uint32_t* parent, uint32_t* vflags,
                                                            uint32_t* flags, uint32_t* forest,
                                                            uint64_t forest_cap) {
-  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= st.n || comp[v] != v || (vflags[v] & kVDone)) return;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64 * 64;
+  const uint32_t v = v0 + lane;
   const uint32_t w = st.bpc + 1;
-  Bucket16* a = acc + (uint64_t)v * w;
-  uint64_t idx = 0;
-  int res = sample_column(a[st.bpc], a, 1, st.bpc, st.seed, &idx);
-  for (uint32_t j = 0; j < w; j++) a[j] = Bucket16{0, 0};
-  handle_sample(res, idx, v, st.n, parent, vflags, flags, forest, forest_cap);
+  uint64_t m = __ballot(v < st.n && comp[v] == v && !(vflags[v] & kVDone));
+  int my_res = -1;  // the sample of root v0 + lane, handled by its own lane at the end
+  uint64_t my_idx = 0;
+  while (m) {
+    uint32_t ii[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      ii[q] = m ? (uint32_t)__builtin_ctzll(m) : ii[0];
+      m &= m ? m - 1 : 0;
+    }
+    Bucket16 bv[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      bv[q] = lane < w ? acc[(size_t)(v0 + ii[q]) * w + lane] : Bucket16{0, 0};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      if (q > 0 && ii[q] == ii[0]) break;  // padding of the last group
+      const uint32_t root = v0 + ii[q];
+      uint64_t idx = 0;
+      const int res = wave_sample(bv[q], lane, st.bpc, st.seed, &idx);
+      if (lane < w) acc[(size_t)root * w + lane] = Bucket16{0, 0};
+      if (lane == ii[q]) {
+        my_res = res;
+        my_idx = idx;
+      }
+    }
+  }
+  if (my_res >= 0) handle_sample(my_res, my_idx, v, st.n, parent, vflags, flags, forest, forest_cap);
 }
 
 // Publishes the roots after a round's unions and settles the root states: a root that sampled
@@ -211,6 +420,9 @@ void free_query_scratch(Store* st) {
   (void)hipFree(st->d_comp);
   (void)hipFree(st->d_vflags);
   (void)hipFree(st->d_acc);
+  (void)hipFree(st->d_cpart);
+  (void)hipFree(st->d_bpart);
+  st->d_cpart = st->d_bpart = nullptr;
   (void)hipFree(st->d_flags);
   (void)hipFree(st->d_forest);
   if (st->h_flags) (void)hipHostFree(st->h_flags);
@@ -232,7 +444,10 @@ static int ensure_query_scratch(Store* st, uint64_t forest_pairs) {
     GS_HIP(hipMalloc(&st->d_comp, sizeof(uint32_t) * n));
     GS_HIP(hipMalloc(&st->d_vflags, sizeof(uint32_t) * n));
     GS_HIP(hipMalloc(&st->d_acc, sizeof(Bucket16) * (uint64_t)n * (st->view.bpc + 1)));
-    GS_HIP(hipMalloc(&st->d_flags, sizeof(uint32_t) * 4));
+    const uint64_t nwaves = (((uint64_t)n + kQThreads - 1) / kQThreads) * (kQThreads / 64);
+    GS_HIP(hipMalloc(&st->d_cpart, sizeof(Bucket16) * (st->view.bpc + 1) * nwaves));
+    GS_HIP(hipMalloc(&st->d_bpart, sizeof(Bucket16) * (st->view.bpc + 1) * nwaves * kBig));
+    GS_HIP(hipMalloc(&st->d_flags, sizeof(uint32_t) * kFlagWords));
     GS_HIP(hipHostMalloc((void**)&st->h_flags, sizeof(uint32_t) * 4, hipHostMallocDefault));
     uint64_t cnt = (uint64_t)n * (st->view.bpc + 1);
     hipLaunchKernelGGL(zero_acc_kernel, dim3(2048), dim3(256), 0, st->qstream, st->d_acc, cnt);
@@ -287,8 +502,13 @@ static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t
         return fail(GS_ERR_OUT_OF_SAMPLES,
                     "This sketch (seed=%llu, max samples=%u) cannot be sampled more times (cur idx=%u)!",
                     (unsigned long long)v.seed, st->num_samples, round);
+      const uint32_t w = v.bpc + 1, nwaves = grid.x * (kQThreads / 64);
+      hipLaunchKernelGGL(pick_big_kernel, dim3(1), dim3(kPickSamples), 0, q, n, st->d_comp,
+                         st->d_vflags, st->symmetric.load() ? 1 : 0, st->d_flags);
       hipLaunchKernelGGL(acc_kernel, grid, block, 0, q, v, round, st->d_comp, st->d_vflags,
-                         st->d_acc);
+                         st->d_flags, st->d_acc, st->d_cpart, st->d_bpart);
+      hipLaunchKernelGGL(big_acc_kernel, dim3(w, kBig + 1), dim3(kQThreads), 0, q, st->d_flags,
+                         st->d_cpart, st->d_bpart, nwaves, w, st->d_acc);
       hipLaunchKernelGGL(sample_kernel, grid, block, 0, q, v, st->d_comp, st->d_acc, st->d_parent,
                          st->d_vflags, st->d_flags, forest, cap);
     }

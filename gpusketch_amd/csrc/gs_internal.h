@@ -200,7 +200,13 @@ struct Store {
   uint32_t* d_comp = nullptr;        // supernode root of every vertex at round start
   uint32_t* d_vflags = nullptr;      // per-root query state (ZERO / merged / done)
   Bucket16* d_acc = nullptr;         // per-root XOR of one column + deterministic bucket
-  uint32_t* d_flags = nullptr;       // [0]=modified, [1]=out of samples, [2]=forest count
+  Bucket16* d_cpart = nullptr;       // per-wave partial column sums outside the giant supernode
+  Bucket16* d_bpart = nullptr;       // per-wave partial column sums of the big supernodes
+  // false once the store took updates that need not be symmetric (vertex batches, imports): the
+  // XOR of all its vertex sketches is then not known to be zero
+  std::atomic<bool> symmetric{true};
+  uint32_t* d_flags = nullptr;       // [0]=modified, [1]=out of samples, [2]=forest count,
+                                     // [3]=giant supernode of the round
   uint32_t* d_forest = nullptr;      // forest edges (min,max) pairs, capacity k * n
   uint64_t forest_cap = 0;           // pairs
   uint32_t* h_flags = nullptr;       // pinned mirror of d_flags

@@ -70,6 +70,36 @@ def test_cc_matches_oracle_boruvka_and_dsu(seed):
     check_forest(n, forest, net, labels)
 
 
+def test_cc_giant_split_and_direct_paths_agree():
+    # A giant component forms after a few rounds. Pairs-only stores (and whole-store imports
+    # whose vertex sketches XOR to zero) form the giant supernode's sum from everything outside
+    # it; a store filled by partial imports cannot know its sketches XOR to zero and
+    # accumulates every supernode directly. Both equal the oracle's Borůvka, round for round.
+    n = 6000
+    pairs = gs.gen_erdos_renyi(n, 3.0 / n, 0.05, 77)
+    key = 4242
+    g = orc.Geometry(n)
+    ref = g.empty_store()
+    orc.apply_edges(ref, g, key, pairs)
+    rc, olab, orounds, _ = orc.boruvka(ref, g, key)
+    assert rc == 0
+    with gs.SketchStore(n, key) as a, gs.SketchStore(n, key) as b, gs.SketchStore(n, key) as c:
+        s = a.stream()
+        s.submit_pairs(pairs)
+        s.flush()
+        lab_a, _, rounds_a = a.query_cc()
+        data = a.export_buckets()
+        b.import_buckets(data[: n // 2], 0)
+        b.import_buckets(data[n // 2:], n // 2)
+        lab_b, _, rounds_b = b.query_cc()
+        c.import_buckets(data, 0)
+        lab_c, _, rounds_c = c.query_cc()
+    for lab, rounds in ((lab_a, rounds_a), (lab_b, rounds_b), (lab_c, rounds_c)):
+        assert np.array_equal(lab, olab)
+        assert rounds == orounds
+    assert np.array_equal(lab_a, orc.dsu_components(n, net_edges(pairs)))
+
+
 def test_cc_config1_shape_reduced():
     # the ER(8192, 0.25) config at n=1024 (same density, seeded order, 10% re-deleted)
     n = 1024
