@@ -412,6 +412,10 @@ struct AggLayout {
 };
 constexpr int kMaxColSteps = 8;  // T <= 8: up to 64 columns
 constexpr int kFlushRounds = 4;  // 64-slot rounds per flush step
+// windows of at most this many 64-slot rounds flush deferred: groups of stores see mostly short
+// items and run at 4 waves per SIMD either way; a single store keeps 5 with 2
+template <bool kGroup>
+constexpr int defer_rounds() { return kGroup ? 4 : 2; }
 constexpr uint32_t kMaxLdsDepths = 16;
 
 // 64-bit XOR at an LDS byte address (kept as a plain 32-bit address so that no base is re-added
@@ -508,6 +512,19 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
   // its first entry (unconditional loads: no branch joins force an early wait; never used).
   auto first_chunk = [&](uint32_t ln, uint32_t sp) -> uint32_t {
     return dsts[(uint64_t)sp + (lane < ln ? lane : 0u)];
+  };
+  // deferred flush of the previous exclusive item (its window fits kDeferRounds x 64 slots)
+  constexpr int kDeferRounds = defer_rounds<kGroup>();
+  bool pending = false;
+  Bucket16* prows = nullptr;
+  uint64_t pa[kDeferRounds], pg[kDeferRounds];
+  Bucket16 pcur[kDeferRounds];
+  auto complete_pending = [&]() {
+    if (!pending) return;
+#pragma unroll
+    for (int r = 0; r < kDeferRounds; r++)
+      if (pa[r] | pg[r]) prows[r * kWave + lane] = Bucket16{pcur[r].alpha ^ pa[r], pcur[r].gamma ^ pg[r]};
+    pending = false;
   };
   const uint32_t last = nitems ? nitems - 1 : 0;
   uint32_t wcur = item_word(min(gw, last));
@@ -635,8 +652,13 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
     // flush, kFlushRounds x 64 slots at a time: all LDS reads first, then the HBM reads of the
     // non-zero slots together, one wait, then the stores (exclusive item: plain read-xor-write;
     // otherwise 64-bit atomics, which need no read). Slot s = depth * cols + column is LDS row
-    // 7 + depth and device bucket s of the vertex.
+    // 7 + depth and device bucket s of the vertex. An exclusive item whose window fits one step
+    // leaves its stores pending: they complete after the next item, so the reads' latency is
+    // hidden behind that item's work (the vertex is this item's alone; its deep buckets, written
+    // by atomics, lie outside the window).
     Bucket16* drow = reinterpret_cast<Bucket16*>(lds + wbase + kCopyRows * row_bytes);
+    const bool defer = excl && islots <= kDeferRounds * kWave;
+    if (!defer) complete_pending();
     for (uint32_t s0 = 0; s0 < islots; s0 += kFlushRounds * kWave) {
       uint64_t xa[kFlushRounds], xg[kFlushRounds];
 #pragma unroll
@@ -655,7 +677,17 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
 #pragma unroll
         for (int r = 0; r < kFlushRounds; r++) n_flush += (uint32_t)__popcll(__ballot((xa[r] | xg[r]) != 0));
       }
-      if (excl) {
+      if (defer) {
+        complete_pending();  // the previous item's stores, its reads long since arrived
+#pragma unroll
+        for (int r = 0; r < kDeferRounds; r++) {
+          pa[r] = xa[r];
+          pg[r] = xg[r];
+          pcur[r] = vrows[(xa[r] | xg[r]) ? r * kWave + lane : 0u];
+        }
+        prows = vrows;
+        pending = true;
+      } else if (excl) {
         // unconditional loads (lanes with nothing to flush read the vertex's first bucket) so the
         // reads issue together and one wait covers them
         Bucket16 cur[kFlushRounds];
@@ -678,6 +710,7 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
     }
     wave_sync_lds();
   }
+  complete_pending();
   if (traffic) {
     for (int o = 32; o > 0; o >>= 1) n_deep += __shfl_xor(n_deep, o);
     if (lane == 0) {
@@ -919,8 +952,13 @@ int ensure_pair_scratch(Stream* s, uint64_t nedges) {
   return GS_OK;
 }
 
+// Edges per launch: the tuned value, else 64 edges per vertex of the stream's vertex space
+// (more gains little: a launch of that size already gives a vertex ~128 updates to aggregate),
+// at least 2^16 and at most kLaunchEdges. The fill buffers and scratch are sized for it.
 uint64_t stream_launch_edges(const Stream* s) {
-  return s->tune_launch ? (uint64_t)s->tune_launch : kLaunchEdges;
+  if (s->tune_launch) return s->tune_launch;
+  const uint64_t per_vertex = 64ULL * stream_vertices(s);
+  return std::min(kLaunchEdges, std::max<uint64_t>(1ULL << 16, per_vertex));
 }
 
 // the fill buffers, sized for the stream's launch size (grown on demand, never while pending)
