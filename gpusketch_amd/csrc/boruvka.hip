@@ -21,12 +21,17 @@ namespace gs {
 
 constexpr int kQThreads = 256;
 // query flags: [0] modified this round, [1] out of samples, [2] forest edges, [3] giant
-// supernode of the round, [kFlagBig + k] big supernode k (see pick_big_kernel)
+// supernode of the round, [kFlagBig + k] big supernode k (see pick_big_kernel), [kFlagDone] the
+// partition is final (or out of samples): every later round's kernels return at once,
+// [kFlagRound] index of the round being run, [kFlagRounds] rounds the query took
 constexpr uint32_t kNoGiant = 0xFFFFFFFFu;
 constexpr int kBig = 8;
 constexpr int kPickSamples = 256;
 constexpr uint32_t kMinBig = 4;
 constexpr int kFlagBig = 4;
+constexpr int kFlagDone = 12;
+constexpr int kFlagRound = 13;
+constexpr int kFlagRounds = 14;
 constexpr int kFlagWords = 16;
 constexpr int kAccBatch = 8;  // buckets loaded together per lane by the group accumulation
 
@@ -104,8 +109,28 @@ __global__ void init_query_kernel(uint32_t n, uint32_t* parent, uint32_t* comp, 
   }
 }
 
-__global__ void clear_round_flags_kernel(uint32_t* flags) {
-  if (threadIdx.x == 0) { flags[0] = 0; flags[1] = 0; }
+// Rounds are enqueued several at a time without looking at the host; these two kernels
+// bracket a round on the device. begin: clear the round's modified flag. end (after compress):
+// a round that modified nothing leaves the partition final (the query took `round` rounds); a
+// modified round needs another, and past the last sample that is out of samples (the reference
+// throws when it samples past num_samples, src/sketch.cpp:110-114).
+__global__ void begin_round_kernel(uint32_t* flags) {
+  if (threadIdx.x == 0 && !flags[kFlagDone]) flags[0] = 0;
+}
+
+__global__ void end_round_kernel(uint32_t* flags, uint32_t num_samples) {
+  if (threadIdx.x != 0 || flags[kFlagDone]) return;
+  const uint32_t round = flags[kFlagRound];
+  if (!flags[0]) {
+    flags[kFlagDone] = 1;
+    flags[kFlagRounds] = round;
+  } else if (round + 1 >= num_samples) {
+    flags[1] = 1;
+    flags[kFlagDone] = 1;
+    flags[kFlagRounds] = round + 1;
+  } else {
+    flags[kFlagRound] = round + 1;
+  }
 }
 
 __global__ void clear_all_flags_kernel(uint32_t* flags) {
@@ -188,6 +213,7 @@ __global__ __launch_bounds__(kPickSamples) void pick_big_kernel(uint32_t n,
                                                                 const uint32_t* __restrict__ comp,
                                                                 const uint32_t* __restrict__ vflags,
                                                                 int symmetric, uint32_t* flags) {
+  if (flags[kFlagDone]) return;
   __shared__ uint32_t s_r[kPickSamples];
   __shared__ uint32_t s_key[kPickSamples];
   const uint32_t t = threadIdx.x;
@@ -238,13 +264,15 @@ __global__ __launch_bounds__(kPickSamples) void pick_big_kernel(uint32_t n,
 // is one instruction over consecutive addresses). Big supernodes accumulate in registers and
 // leave per-wave partials bpart[wave][k]; with a giant, every vertex outside it (finished
 // supernodes included) also adds into cpart[wave]; the other supernodes take atomics.
-__global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st, uint32_t col,
+__global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st,
                                                         const uint32_t* __restrict__ comp,
                                                         const uint32_t* __restrict__ vflags,
                                                         const uint32_t* __restrict__ flags,
                                                         Bucket16* __restrict__ acc,
                                                         Bucket16* __restrict__ cpart,
                                                         Bucket16* __restrict__ bpart) {
+  if (flags[kFlagDone]) return;
+  const uint32_t col = flags[kFlagRound];  // round r >= 1 samples column r
   const uint32_t w = st.bpc + 1;
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
@@ -327,7 +355,7 @@ __global__ __launch_bounds__(kQThreads) void big_acc_kernel(const uint32_t* __re
                                                             Bucket16* __restrict__ acc) {
   const uint32_t k = blockIdx.y;
   const uint32_t root = k < (uint32_t)kBig ? flags[kFlagBig + k] : flags[3];
-  if (root == kNoGiant) return;
+  if (flags[kFlagDone] || root == kNoGiant) return;
   const uint32_t j = blockIdx.x;
   uint64_t a = 0, g = 0;
   for (uint32_t i = threadIdx.x; i < nwaves; i += kQThreads) {
@@ -361,6 +389,7 @@ __global__ __launch_bounds__(kQThreads) void sample_kernel(StoreView st,
                                                            uint32_t* parent, uint32_t* vflags,
                                                            uint32_t* flags, uint32_t* forest,
                                                            uint64_t forest_cap) {
+  if (flags[kFlagDone]) return;
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64 * 64;
   const uint32_t v = v0 + lane;
@@ -397,10 +426,12 @@ __global__ __launch_bounds__(kQThreads) void sample_kernel(StoreView st,
 
 // Publishes the roots after a round's unions and settles the root states: a root that sampled
 // ZERO and absorbed nothing has no leaving edge and stays as it is in every later round.
+// `flags` (null in the group query) skips the work once the partition is final.
 __global__ __launch_bounds__(kQThreads) void compress_kernel(uint32_t n, uint32_t* parent,
-                                                             uint32_t* comp, uint32_t* vflags) {
+                                                             uint32_t* comp, uint32_t* vflags,
+                                                             const uint32_t* __restrict__ flags) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= n) return;
+  if (v >= n || (flags && flags[kFlagDone])) return;
   uint32_t r = find_root(parent, v);
   comp[v] = r;
   if (r == v) {
@@ -410,9 +441,10 @@ __global__ __launch_bounds__(kQThreads) void compress_kernel(uint32_t n, uint32_
 }
 
 __global__ __launch_bounds__(kQThreads) void finalize_parent_kernel(uint32_t n, uint32_t* parent,
-                                                                    const uint32_t* comp) {
+                                                                    const uint32_t* comp,
+                                                                    const uint32_t* __restrict__ flags) {
   uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v < n) parent[v] = comp[v];
+  if (v < n && !(flags && flags[kFlagDone])) parent[v] = comp[v];
 }
 
 void free_query_scratch(Store* st) {
@@ -448,7 +480,7 @@ static int ensure_query_scratch(Store* st, uint64_t forest_pairs) {
     GS_HIP(hipMalloc(&st->d_cpart, sizeof(Bucket16) * (st->view.bpc + 1) * nwaves));
     GS_HIP(hipMalloc(&st->d_bpart, sizeof(Bucket16) * (st->view.bpc + 1) * nwaves * kBig));
     GS_HIP(hipMalloc(&st->d_flags, sizeof(uint32_t) * kFlagWords));
-    GS_HIP(hipHostMalloc((void**)&st->h_flags, sizeof(uint32_t) * 4, hipHostMallocDefault));
+    GS_HIP(hipHostMalloc((void**)&st->h_flags, sizeof(uint32_t) * kFlagWords, hipHostMallocDefault));
     uint64_t cnt = (uint64_t)n * (st->view.bpc + 1);
     hipLaunchKernelGGL(zero_acc_kernel, dim3(2048), dim3(256), 0, st->qstream, st->d_acc, cnt);
     GS_HIP(hipGetLastError());
@@ -478,49 +510,62 @@ static int ensure_query_scratch(Store* st, uint64_t forest_pairs) {
 
 // One Borůvka pass. Forest edges are appended at d_forest[2*forest_base...]; returns the pass's
 // forest size in *forest_len.
+// Rounds are enqueued speculatively, kRoundsAhead at a time, and the host reads the flags once
+// per batch: a round after the final one only launches kernels that return at once.
+constexpr uint32_t kRoundsAhead = 4;
 static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t* rounds_out,
                         uint64_t* forest_len) {
   const StoreView& v = st->view;
   const uint32_t n = v.n;
   const dim3 grid((n + kQThreads - 1) / kQThreads), block(kQThreads);
+  const uint32_t w = v.bpc + 1, nwaves = grid.x * (kQThreads / 64);
   hipStream_t q = st->qstream;
   uint32_t* forest = st->d_forest + 2 * forest_base;
   uint64_t cap = st->forest_cap - forest_base;
+  if (col0 >= st->num_samples) return fail(GS_ERR_OUT_OF_SAMPLES, "sketch out of samples");
   hipLaunchKernelGGL(init_query_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
                      st->d_vflags);
   hipLaunchKernelGGL(clear_all_flags_kernel, dim3(1), dim3(64), 0, q, st->d_flags);
+  // round 0
+  hipLaunchKernelGGL(round0_kernel, grid, block, 0, q, v, col0, st->d_parent, st->d_vflags,
+                     st->d_flags, forest, cap);
+  hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
+                     st->d_vflags, st->d_flags);
+  hipLaunchKernelGGL(end_round_kernel, dim3(1), dim3(64), 0, q, st->d_flags, st->num_samples);
+  hipLaunchKernelGGL(finalize_parent_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
+                     st->d_flags);
   GS_HIP(hipGetLastError());
-  uint32_t round = 0;
-  for (;;) {
-    hipLaunchKernelGGL(clear_round_flags_kernel, dim3(1), dim3(64), 0, q, st->d_flags);
-    if (round == 0) {
-      if (col0 >= st->num_samples) return fail(GS_ERR_OUT_OF_SAMPLES, "sketch out of samples");
-      hipLaunchKernelGGL(round0_kernel, grid, block, 0, q, v, col0, st->d_parent, st->d_vflags,
-                         st->d_flags, forest, cap);
-    } else {
-      if (round >= st->num_samples)
-        return fail(GS_ERR_OUT_OF_SAMPLES,
-                    "This sketch (seed=%llu, max samples=%u) cannot be sampled more times (cur idx=%u)!",
-                    (unsigned long long)v.seed, st->num_samples, round);
-      const uint32_t w = v.bpc + 1, nwaves = grid.x * (kQThreads / 64);
+  const int symmetric = st->symmetric.load() ? 1 : 0;
+  for (uint32_t enqueued = 1;; enqueued += kRoundsAhead) {
+    for (uint32_t k = 0; k < kRoundsAhead; k++) {
+      hipLaunchKernelGGL(begin_round_kernel, dim3(1), dim3(64), 0, q, st->d_flags);
       hipLaunchKernelGGL(pick_big_kernel, dim3(1), dim3(kPickSamples), 0, q, n, st->d_comp,
-                         st->d_vflags, st->symmetric.load() ? 1 : 0, st->d_flags);
-      hipLaunchKernelGGL(acc_kernel, grid, block, 0, q, v, round, st->d_comp, st->d_vflags,
-                         st->d_flags, st->d_acc, st->d_cpart, st->d_bpart);
+                         st->d_vflags, symmetric, st->d_flags);
+      hipLaunchKernelGGL(acc_kernel, grid, block, 0, q, v, st->d_comp, st->d_vflags, st->d_flags,
+                         st->d_acc, st->d_cpart, st->d_bpart);
       hipLaunchKernelGGL(big_acc_kernel, dim3(w, kBig + 1), dim3(kQThreads), 0, q, st->d_flags,
                          st->d_cpart, st->d_bpart, nwaves, w, st->d_acc);
       hipLaunchKernelGGL(sample_kernel, grid, block, 0, q, v, st->d_comp, st->d_acc, st->d_parent,
                          st->d_vflags, st->d_flags, forest, cap);
+      hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
+                         st->d_vflags, st->d_flags);
+      hipLaunchKernelGGL(end_round_kernel, dim3(1), dim3(64), 0, q, st->d_flags, st->num_samples);
+      hipLaunchKernelGGL(finalize_parent_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
+                         st->d_flags);
     }
-    hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
-                       st->d_vflags);
     GS_HIP(hipGetLastError());
-    GS_HIP(hipMemcpyAsync(st->h_flags, st->d_flags, sizeof(uint32_t) * 4, hipMemcpyDeviceToHost, q));
+    GS_HIP(hipMemcpyAsync(st->h_flags, st->d_flags, sizeof(uint32_t) * kFlagWords,
+                          hipMemcpyDeviceToHost, q));
     GS_HIP(hipStreamSynchronize(q));
-    if (!st->h_flags[0]) break;
-    hipLaunchKernelGGL(finalize_parent_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp);
-    round++;
+    if (st->h_flags[kFlagDone]) break;
+    GS_CHECK(enqueued <= st->num_samples + kRoundsAhead, GS_ERR_INVALID,
+             "Boruvka rounds did not settle");
   }
+  const uint32_t round = st->h_flags[kFlagRounds];
+  if (st->h_flags[1])
+    return fail(GS_ERR_OUT_OF_SAMPLES,
+                "This sketch (seed=%llu, max samples=%u) cannot be sampled more times (cur idx=%u)!",
+                (unsigned long long)v.seed, st->num_samples, round);
   if (rounds_out) *rounds_out = round;
   uint64_t fl = st->h_flags[2];
   GS_CHECK(fl <= cap, GS_ERR_INVALID, "forest capacity exceeded");
@@ -757,9 +802,11 @@ static void launch_group_round(Group* g, hipStream_t q) {
                      g->max_bpc + 1);
   hipLaunchKernelGGL(g_sample_kernel, grid, block, 0, q, g->d_descs, g->d_vgraph, N, G, g->d_comp,
                      g->d_acc, (uint32_t)g->acc_stride, g->d_parent, g->d_vflags, g->d_gstate);
-  hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, N, g->d_parent, g->d_comp, g->d_vflags);
+  hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, N, g->d_parent, g->d_comp, g->d_vflags,
+                     nullptr);
   hipLaunchKernelGGL(g_settle_kernel, dim3(1), dim3(1024), 0, q, g->d_descs, G, g->d_gstate);
-  hipLaunchKernelGGL(finalize_parent_kernel, grid, block, 0, q, N, g->d_parent, g->d_comp);
+  hipLaunchKernelGGL(finalize_parent_kernel, grid, block, 0, q, N, g->d_parent, g->d_comp,
+                     nullptr);
 }
 
 int group_query_cc(Group* g, const uint8_t* mask, uint32_t* labels, uint32_t* num_cc,
@@ -799,9 +846,11 @@ int group_query_cc(Group* g, const uint8_t* mask, uint32_t* labels, uint32_t* nu
   hipLaunchKernelGGL(g_begin_kernel, dim3(1), dim3(64), 0, q, g->d_gstate, G);
   hipLaunchKernelGGL(g_round0_kernel, grid, block, 0, q, g->d_descs, g->d_vgraph, N, G, 0u,
                      g->d_parent, g->d_vflags, g->d_gstate);
-  hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, N, g->d_parent, g->d_comp, g->d_vflags);
+  hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, N, g->d_parent, g->d_comp, g->d_vflags,
+                     nullptr);
   hipLaunchKernelGGL(g_settle_kernel, dim3(1), dim3(1024), 0, q, g->d_descs, G, g->d_gstate);
-  hipLaunchKernelGGL(finalize_parent_kernel, grid, block, 0, q, N, g->d_parent, g->d_comp);
+  hipLaunchKernelGGL(finalize_parent_kernel, grid, block, 0, q, N, g->d_parent, g->d_comp,
+                     nullptr);
   GS_HIP(hipGetLastError());
   uint32_t max_samples = 0;
   for (Store* m : g->members) max_samples = std::max(max_samples, m->num_samples);
