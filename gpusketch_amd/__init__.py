@@ -38,7 +38,7 @@ class Geometry(ctypes.Structure):
                 ("num_samples", ctypes.c_uint32), ("cols_per_sample", ctypes.c_uint32),
                 ("num_columns", ctypes.c_uint32), ("num_buckets", ctypes.c_uint32),
                 ("seed", ctypes.c_uint64), ("vector_len", ctypes.c_uint64),
-                ("device_bytes", ctypes.c_uint64)]
+                ("device_bytes", ctypes.c_uint64), ("flags", ctypes.c_uint32)]
 
 
 class Tuning(ctypes.Structure):
@@ -61,6 +61,7 @@ _u32, _u64, _i32, _vp, _dbl, _sz = (ctypes.c_uint32, ctypes.c_uint64, ctypes.c_i
 _sigs = {
     "gs_last_error": ([], ctypes.c_char_p),
     "gs_geometry_for": ([_u32, _dbl, _u32, _vp], _i32),
+    "gs_geometry_for_sketch": ([_u32, _dbl, _u32, _u32, _vp], _i32),
     "gs_hash_known_answer": ([_vp, _vp, _u64, _vp], _i32),
     "gs_store_create": ([_u32, _u64, _u32, _u32, _u32, _vp], _i32),
     "gs_store_destroy": ([_vp], _i32),
@@ -128,9 +129,17 @@ def _pairs(pairs) -> np.ndarray:
     return np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
 
 
-def geometry_for(n: int, sketches_factor: float = 1.0, cols_per_sample: int = 1) -> Geometry:
+# gs_store_create flag: the CubeSketch (L0_SAMPLING) update rule
+STORE_CUBESKETCH = 1
+# columns per sample of the CubeSketch configuration (include/sketch.h:208-211)
+CUBESKETCH_COLS_PER_SAMPLE = 7
+
+
+def geometry_for(n: int, sketches_factor: float = 1.0, cols_per_sample: int = 1,
+                 cubesketch: bool = False) -> Geometry:
     g = Geometry()
-    _check(lib.gs_geometry_for(n, sketches_factor, cols_per_sample, ctypes.byref(g)))
+    _check(lib.gs_geometry_for_sketch(n, sketches_factor, cols_per_sample,
+                                      STORE_CUBESKETCH if cubesketch else 0, ctypes.byref(g)))
     return g
 
 
@@ -284,14 +293,21 @@ class Stream:
 
 
 class SketchStore:
-    """n vertex sketches resident in HBM (one seed, reference geometry)."""
+    """n vertex sketches resident in HBM (one seed, reference geometry).
+
+    cubesketch=True selects the L0_SAMPLING update rule (src/sketch.cpp:66-83) and its sizing;
+    cols_per_sample defaults to 7 then (include/sketch.h:208-211), to 1 otherwise."""
 
     def __init__(self, n: int, seed: int, num_samples: int | None = None,
-                 sketches_factor: float = 1.0):
+                 sketches_factor: float = 1.0, cols_per_sample: int | None = None,
+                 cubesketch: bool = False):
+        if cols_per_sample is None:
+            cols_per_sample = CUBESKETCH_COLS_PER_SAMPLE if cubesketch else 1
         if num_samples is None:
-            num_samples = geometry_for(n, sketches_factor).num_samples
+            num_samples = geometry_for(n, sketches_factor, cols_per_sample, cubesketch).num_samples
         self.h = ctypes.c_void_p()
-        _check(lib.gs_store_create(n, seed & (2**64 - 1), num_samples, 1, 0, ctypes.byref(self.h)))
+        _check(lib.gs_store_create(n, seed & (2**64 - 1), num_samples, cols_per_sample,
+                                   STORE_CUBESKETCH if cubesketch else 0, ctypes.byref(self.h)))
         self.geom = Geometry()
         _check(lib.gs_store_geometry(self.h, ctypes.byref(self.geom)))
         self.n = n

@@ -149,11 +149,18 @@ const char* gs_last_error(void) { return g_last_error.c_str(); }
 
 int gs_geometry_for(uint32_t n, double sketches_factor, uint32_t cols_per_sample,
                     gs_geometry* out) {
+  return gs_geometry_for_sketch(n, sketches_factor, cols_per_sample, 0, out);
+}
+
+int gs_geometry_for_sketch(uint32_t n, double sketches_factor, uint32_t cols_per_sample,
+                           uint32_t flags, gs_geometry* out) {
   GS_CHECK(out && n >= 2 && cols_per_sample >= 1 && sketches_factor > 0, GS_ERR_INVALID,
            "gs_geometry_for: bad argument");
+  GS_CHECK((flags & ~GS_STORE_CUBESKETCH) == 0, GS_ERR_INVALID, "gs_geometry_for: unknown flags");
   uint64_t g[5];
-  geometry(n, sketches_factor, cols_per_sample, 0, g);
+  geometry(n, sketches_factor, cols_per_sample, (flags & GS_STORE_CUBESKETCH) ? 1 : 0, g);
   std::memset(out, 0, sizeof(*out));
+  out->flags = flags;
   out->num_vertices = n;
   out->vector_len = g[0];
   out->bkt_per_col = (uint32_t)g[1];
@@ -178,16 +185,13 @@ int gs_store_create(uint32_t n, uint64_t seed, uint32_t num_samples, uint32_t co
   *out = nullptr;
   GS_CHECK(n >= 2, GS_ERR_INVALID, "gs_store_create: need at least 2 vertices");
   GS_CHECK(num_samples >= 1 && cols_per_sample >= 1, GS_ERR_INVALID, "gs_store_create: bad samples");
-  GS_CHECK(cols_per_sample == 1, GS_ERR_UNSUPPORTED,
-           "gs_store_create: only the support-finding update (1 column per sample) is built");
+  GS_CHECK((flags & ~GS_STORE_CUBESKETCH) == 0, GS_ERR_INVALID, "gs_store_create: unknown flags");
   uint64_t g[5];
   geometry(n, 1.0, 1, 0, g);
   const uint32_t bpc = (uint32_t)g[1];
   const uint64_t cols = (uint64_t)num_samples * cols_per_sample;
   GS_CHECK(bpc < 64 && cols <= 4096, GS_ERR_INVALID, "gs_store_create: geometry out of range");
   GS_CHECK(n <= (1u << 26), GS_ERR_UNSUPPORTED, "gs_store_create: at most 2^26 vertices per store");
-  GS_CHECK(cols <= 64, GS_ERR_UNSUPPORTED,
-           "gs_store_create: at most 64 columns (one wave64 lane per column) are supported");
   gs_store* h = new gs_store();
   Store& st = h->impl;
   st.num_samples = num_samples;
@@ -261,6 +265,7 @@ int gs_store_geometry(const gs_store* h, gs_geometry* out) {
   out->seed = st.view.seed;
   out->vector_len = g[0];
   out->device_bytes = st.bytes();
+  out->flags = st.flags;
   return GS_OK;
 }
 

@@ -166,6 +166,80 @@ __device__ __forceinline__ int wave_sample(const Bucket16 b, uint32_t lane, uint
   return 0;
 }
 
+// Sample slots of a sketch with cps columns per sample: slot j < sw = cps*B of sample r is column
+// r*cps + j / B at depth j % B (the scan order of Sketch::sample, src/sketch.cpp:121-128); slot
+// sw is the deterministic bucket. Offset of slot j inside a vertex's rows.
+__device__ __forceinline__ size_t slot_offset(const StoreView& st, uint32_t cps, uint32_t r,
+                                              uint32_t j) {
+  const uint32_t i = j / st.bpc;
+  return (size_t)(j - i * st.bpc) * st.cols + (size_t)r * cps + i;
+}
+
+// Sketch::sample over a row of sw + 1 slots (slot sw the deterministic bucket) that may be wider
+// than a wave: the deterministic bucket first, then the slots in scan order, 4 x 64 loads in
+// flight at a time, stopping at the first group holding a good bucket. load(j) reads slot j;
+// the result and *idx are wave-uniform.
+template <class Load>
+__device__ __forceinline__ int wave_sample_wide(Load load, uint32_t lane, uint32_t sw,
+                                                uint64_t seed, uint64_t* idx) {
+  const Bucket16 det = load(sw);
+  if (det.alpha == 0 && det.gamma == 0) return 1;
+  if (det.gamma == xxh3_len8(det.alpha, seed)) {
+    *idx = det.alpha;
+    return 0;
+  }
+  for (uint32_t j0 = 0; j0 < sw; j0 += 4 * 64) {
+    Bucket16 b[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint32_t j = j0 + q * 64 + lane;
+      b[q] = load(j < sw ? j : 0u);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint32_t j = j0 + q * 64 + lane;
+      const uint64_t good = __ballot(j < sw && b[q].gamma == xxh3_len8(b[q].alpha, seed));
+      if (good) {
+        const int l = __builtin_ctzll(good);
+        *idx = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(b[q].alpha >> 32), l) << 32) |
+               (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b[q].alpha, l);
+        return 0;
+      }
+    }
+  }
+  return 2;
+}
+
+// Round 0, sketches with several columns per sample: every vertex samples its own sample col0,
+// one vertex at a time per wave.
+__global__ __launch_bounds__(kQThreads) void round0_wide_kernel(StoreView st, uint32_t cps,
+                                                                uint32_t col0, uint32_t* parent,
+                                                                uint32_t* vflags, uint32_t* flags,
+                                                                uint32_t* forest,
+                                                                uint64_t forest_cap) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64 * 64;
+  if (v0 >= st.n) return;
+  const uint32_t cnt = min(64u, st.n - v0);
+  const uint32_t sw = cps * st.bpc;
+  int my_res = -1;
+  uint64_t my_idx = 0;
+  for (uint32_t i = 0; i < cnt; i++) {
+    const uint32_t vi = v0 + i;
+    const Bucket16* vrows = st.rows + (size_t)vi * st.vstride;
+    auto load = [&](uint32_t j) -> Bucket16 {
+      return j < sw ? vrows[slot_offset(st, cps, col0, j)] : st.det[vi];
+    };
+    uint64_t idx = 0;
+    const int res = wave_sample_wide(load, lane, sw, st.seed, &idx);
+    if (lane == i) {
+      my_res = res;
+      my_idx = idx;
+    }
+  }
+  if (my_res >= 0) handle_sample(my_res, my_idx, v0 + lane, st.n, parent, vflags, flags, forest, forest_cap);
+}
+
 // Round 0: every vertex samples its own column col0 (run_round_zero, src/cc_sketch_alg.cpp:
 // 250-271). A wave takes 64 consecutive vertices, four at a time, one bucket per lane.
 __global__ __launch_bounds__(kQThreads) void round0_kernel(StoreView st, uint32_t col0,
@@ -258,13 +332,14 @@ __global__ __launch_bounds__(kPickSamples) void pick_big_kernel(uint32_t n,
   }
 }
 
-// Rounds >= 1: the column sums (column `col`, depths 0..B-1, plus the deterministic bucket) of
-// the live supernodes. A wave takes 64 consecutive vertices and walks the ones it must read, four
-// at a time; lane j holds bucket j (so one vertex is one load instruction, and its row of atomics
-// is one instruction over consecutive addresses). Big supernodes accumulate in registers and
-// leave per-wave partials bpart[wave][k]; with a giant, every vertex outside it (finished
-// supernodes included) also adds into cpart[wave]; the other supernodes take atomics.
-__global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st,
+// Rounds >= 1: the sample sums (sample r: its cps columns, depths 0..B-1, plus the deterministic
+// bucket; w = cps*B + 1 slots) of the live supernodes. A wave takes 64 consecutive vertices and
+// walks the ones it must read, four at a time; lane l holds slot 64*blockIdx.y + l (so one
+// vertex is one load instruction, and its row of atomics is one instruction over consecutive
+// addresses). Big supernodes accumulate in registers and leave per-wave partials
+// bpart[wave][k]; with a giant, every vertex outside it (finished supernodes included) also adds
+// into cpart[wave]; the other supernodes take atomics.
+__global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st, uint32_t cps,
                                                         const uint32_t* __restrict__ comp,
                                                         const uint32_t* __restrict__ vflags,
                                                         const uint32_t* __restrict__ flags,
@@ -272,9 +347,10 @@ __global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st,
                                                         Bucket16* __restrict__ cpart,
                                                         Bucket16* __restrict__ bpart) {
   if (flags[kFlagDone]) return;
-  const uint32_t col = flags[kFlagRound];  // round r >= 1 samples column r
-  const uint32_t w = st.bpc + 1;
+  const uint32_t col = flags[kFlagRound];  // round r >= 1 samples sample r
+  const uint32_t sw = cps * st.bpc, w = sw + 1;
   const uint32_t lane = threadIdx.x & 63;
+  const uint32_t slot = blockIdx.y * 64 + lane;
   const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
   const uint32_t v0 = gwave * 64;
   const uint32_t giant = flags[3];
@@ -293,8 +369,10 @@ __global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st,
   for (int k = kBig - 1; k >= 0; k--)
     if (r == big[k]) kb = k;
   const uint64_t rdm = __ballot(rd), actm = __ballot(act);
-  const bool lj = lane < w;
-  const uint32_t jj = lj ? lane : 0;
+  const bool lj = slot < w;
+  const uint32_t jj = lj ? slot : 0;
+  const bool isdet = jj == sw;
+  const size_t soff = isdet ? 0 : slot_offset(st, cps, col, jj);
   uint64_t ca = 0, cg = 0, ba[kBig], bg[kBig];
 #pragma unroll
   for (int k = 0; k < kBig; k++) ba[k] = bg[k] = 0;
@@ -310,7 +388,7 @@ __global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st,
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const uint32_t vi = v0 + ii[q];
-      bv[q] = jj < st.bpc ? st.rows[(size_t)vi * st.vstride + (size_t)jj * st.cols + col] : st.det[vi];
+      bv[q] = isdet ? st.det[vi] : st.rows[(size_t)vi * st.vstride + soff];
     }
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -338,10 +416,10 @@ __global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st,
     }
   }
   if (lj) {
-    if (split) cpart[(size_t)gwave * w + lane] = Bucket16{ca, cg};
+    if (split) cpart[(size_t)gwave * w + slot] = Bucket16{ca, cg};
 #pragma unroll
     for (int k = 0; k < kBig; k++)
-      if (big[k] != kNoGiant) bpart[((size_t)gwave * kBig + k) * w + lane] = Bucket16{ba[k], bg[k]};
+      if (big[k] != kNoGiant) bpart[((size_t)gwave * kBig + k) * w + slot] = Bucket16{ba[k], bg[k]};
   }
 }
 
@@ -424,6 +502,37 @@ __global__ __launch_bounds__(kQThreads) void sample_kernel(StoreView st,
   if (my_res >= 0) handle_sample(my_res, my_idx, v, st.n, parent, vflags, flags, forest, forest_cap);
 }
 
+// sample_kernel for rows of w = cps*B + 1 slots: one root at a time per wave
+__global__ __launch_bounds__(kQThreads) void sample_wide_kernel(StoreView st, uint32_t cps,
+                                                                const uint32_t* __restrict__ comp,
+                                                                Bucket16* __restrict__ acc,
+                                                                uint32_t* parent, uint32_t* vflags,
+                                                                uint32_t* flags, uint32_t* forest,
+                                                                uint64_t forest_cap) {
+  if (flags[kFlagDone]) return;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64 * 64;
+  const uint32_t v = v0 + lane;
+  const uint32_t sw = cps * st.bpc, w = sw + 1;
+  uint64_t m = __ballot(v < st.n && comp[v] == v && !(vflags[v] & kVDone));
+  int my_res = -1;
+  uint64_t my_idx = 0;
+  while (m) {
+    const uint32_t i = (uint32_t)__builtin_ctzll(m);
+    m &= m - 1;
+    Bucket16* row = acc + (size_t)(v0 + i) * w;
+    auto load = [&](uint32_t j) -> Bucket16 { return row[j]; };
+    uint64_t idx = 0;
+    const int res = wave_sample_wide(load, lane, sw, st.seed, &idx);
+    for (uint32_t j = lane; j < w; j += 64) row[j] = Bucket16{0, 0};
+    if (lane == i) {
+      my_res = res;
+      my_idx = idx;
+    }
+  }
+  if (my_res >= 0) handle_sample(my_res, my_idx, v, st.n, parent, vflags, flags, forest, forest_cap);
+}
+
 // Publishes the roots after a round's unions and settles the root states: a root that sampled
 // ZERO and absorbed nothing has no leaving edge and stays as it is in every later round.
 // `flags` (null in the group query) skips the work once the partition is final.
@@ -475,13 +584,14 @@ static int ensure_query_scratch(Store* st, uint64_t forest_pairs) {
     GS_HIP(hipMalloc(&st->d_parent, sizeof(uint32_t) * n));
     GS_HIP(hipMalloc(&st->d_comp, sizeof(uint32_t) * n));
     GS_HIP(hipMalloc(&st->d_vflags, sizeof(uint32_t) * n));
-    GS_HIP(hipMalloc(&st->d_acc, sizeof(Bucket16) * (uint64_t)n * (st->view.bpc + 1)));
+    const uint64_t w = (uint64_t)st->cols_per_sample * st->view.bpc + 1;  // slots per sample
+    GS_HIP(hipMalloc(&st->d_acc, sizeof(Bucket16) * (uint64_t)n * w));
     const uint64_t nwaves = (((uint64_t)n + kQThreads - 1) / kQThreads) * (kQThreads / 64);
-    GS_HIP(hipMalloc(&st->d_cpart, sizeof(Bucket16) * (st->view.bpc + 1) * nwaves));
-    GS_HIP(hipMalloc(&st->d_bpart, sizeof(Bucket16) * (st->view.bpc + 1) * nwaves * kBig));
+    GS_HIP(hipMalloc(&st->d_cpart, sizeof(Bucket16) * w * nwaves));
+    GS_HIP(hipMalloc(&st->d_bpart, sizeof(Bucket16) * w * nwaves * kBig));
     GS_HIP(hipMalloc(&st->d_flags, sizeof(uint32_t) * kFlagWords));
     GS_HIP(hipHostMalloc((void**)&st->h_flags, sizeof(uint32_t) * kFlagWords, hipHostMallocDefault));
-    uint64_t cnt = (uint64_t)n * (st->view.bpc + 1);
+    uint64_t cnt = (uint64_t)n * w;
     hipLaunchKernelGGL(zero_acc_kernel, dim3(2048), dim3(256), 0, st->qstream, st->d_acc, cnt);
     GS_HIP(hipGetLastError());
   }
@@ -518,7 +628,9 @@ static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t
   const StoreView& v = st->view;
   const uint32_t n = v.n;
   const dim3 grid((n + kQThreads - 1) / kQThreads), block(kQThreads);
-  const uint32_t w = v.bpc + 1, nwaves = grid.x * (kQThreads / 64);
+  const uint32_t cps = st->cols_per_sample;
+  const uint32_t w = cps * v.bpc + 1, nwaves = grid.x * (kQThreads / 64);
+  const bool wide = cps > 1;  // several columns per sample: the one-vertex-per-wave samplers
   hipStream_t q = st->qstream;
   uint32_t* forest = st->d_forest + 2 * forest_base;
   uint64_t cap = st->forest_cap - forest_base;
@@ -527,8 +639,12 @@ static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t
                      st->d_vflags);
   hipLaunchKernelGGL(clear_all_flags_kernel, dim3(1), dim3(64), 0, q, st->d_flags);
   // round 0
-  hipLaunchKernelGGL(round0_kernel, grid, block, 0, q, v, col0, st->d_parent, st->d_vflags,
-                     st->d_flags, forest, cap);
+  if (wide)
+    hipLaunchKernelGGL(round0_wide_kernel, grid, block, 0, q, v, cps, col0, st->d_parent,
+                       st->d_vflags, st->d_flags, forest, cap);
+  else
+    hipLaunchKernelGGL(round0_kernel, grid, block, 0, q, v, col0 * cps, st->d_parent,
+                       st->d_vflags, st->d_flags, forest, cap);
   hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
                      st->d_vflags, st->d_flags);
   hipLaunchKernelGGL(end_round_kernel, dim3(1), dim3(64), 0, q, st->d_flags, st->num_samples);
@@ -541,12 +657,17 @@ static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t
       hipLaunchKernelGGL(begin_round_kernel, dim3(1), dim3(64), 0, q, st->d_flags);
       hipLaunchKernelGGL(pick_big_kernel, dim3(1), dim3(kPickSamples), 0, q, n, st->d_comp,
                          st->d_vflags, symmetric, st->d_flags);
-      hipLaunchKernelGGL(acc_kernel, grid, block, 0, q, v, st->d_comp, st->d_vflags, st->d_flags,
-                         st->d_acc, st->d_cpart, st->d_bpart);
+      hipLaunchKernelGGL(acc_kernel, dim3(grid.x, (w + 63) / 64), block, 0, q, v, cps,
+                         st->d_comp, st->d_vflags, st->d_flags, st->d_acc, st->d_cpart,
+                         st->d_bpart);
       hipLaunchKernelGGL(big_acc_kernel, dim3(w, kBig + 1), dim3(kQThreads), 0, q, st->d_flags,
                          st->d_cpart, st->d_bpart, nwaves, w, st->d_acc);
-      hipLaunchKernelGGL(sample_kernel, grid, block, 0, q, v, st->d_comp, st->d_acc, st->d_parent,
-                         st->d_vflags, st->d_flags, forest, cap);
+      if (wide)
+        hipLaunchKernelGGL(sample_wide_kernel, grid, block, 0, q, v, cps, st->d_comp, st->d_acc,
+                           st->d_parent, st->d_vflags, st->d_flags, forest, cap);
+      else
+        hipLaunchKernelGGL(sample_kernel, grid, block, 0, q, v, st->d_comp, st->d_acc,
+                           st->d_parent, st->d_vflags, st->d_flags, forest, cap);
       hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
                          st->d_vflags, st->d_flags);
       hipLaunchKernelGGL(end_round_kernel, dim3(1), dim3(64), 0, q, st->d_flags, st->num_samples);

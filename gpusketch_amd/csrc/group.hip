@@ -56,6 +56,14 @@ int gs_group_create(gs_store* const* stores, uint32_t count, gs_group** out) {
         delete h;
         return fail(GS_ERR_INVALID, "gs_group_create: store given twice (members %u, %u)", j, i);
       }
+    if (st->cols_per_sample != 1 || (st->flags & GS_STORE_CUBESKETCH) || st->view.cols > 64) {
+      delete h;
+      return fail(GS_ERR_UNSUPPORTED,
+                  "gs_group_create: member %u is a wide sketch (%u columns, %u per sample%s); "
+                  "groups take one-column-per-sample sketches of at most 64 columns",
+                  i, st->view.cols, st->cols_per_sample,
+                  (st->flags & GS_STORE_CUBESKETCH) ? ", CubeSketch rule" : "");
+    }
     g.members.push_back(st);
     g.bases.push_back((uint32_t)total);
     const StoreView& v = st->view;

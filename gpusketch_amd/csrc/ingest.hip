@@ -721,6 +721,165 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
 }
 
 // ------------------------------------------------------------------------------------------
+// K5 for wide sketches: more than 64 columns (several columns per sample) or the CubeSketch
+// rule (src/sketch.cpp:66-83: an update of depth d toggles depths 0..d of the column).
+//
+// One wave per (work item, group of up to 64 columns), lane = column. The item's destinations
+// are staged 64 at a time (edge index, checksum, mix(swap(idx)) in LDS, read back as broadcasts)
+// and each lane hashes every entry for its own column. The LDS delta is private per lane:
+// depth-major planes alpha[d][64] and gamma[d][64] of u64 (8-byte lane stride: no bank
+// conflicts, no contention), so a pair is one 24-bit multiply-add and two ds_xor_b64.
+// CubeSketch: a depth-d entry XORs only depth d (the window's last depth when d lies beyond it)
+// and the flush takes the suffix XOR over depths, which is the rule's column exactly; depths
+// past the window (rare) go to HBM with atomics, for the CubeSketch rule every one of them
+// up to d. The flush writes a lane's non-zero depths (consecutive columns: one coalesced row of
+// atomics per depth). The column group 0 wave also owns the deterministic bucket.
+// ------------------------------------------------------------------------------------------
+constexpr int kWideWaves = 4;
+__host__ __device__ __forceinline__ uint32_t wide_wave_bytes(uint32_t dl) {
+  return dl * kWave * 16u + 3u * kWave * 8u;
+}
+
+template <bool kL0>
+__global__ __launch_bounds__(kWideWaves* kWave) void wide_aggregate_kernel(
+    StoreView st, const WorkItem* __restrict__ items, ItemClasses cls,
+    const uint32_t* __restrict__ class_counts, const uint32_t* __restrict__ dsts, uint32_t dl,
+    uint32_t gcols, uint32_t ngroups, uint32_t min_depth, uint32_t depth_cap,
+    unsigned long long* __restrict__ traffic) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint32_t wid = threadIdx.x / kWave;
+  const uint32_t wbase = wid * wide_wave_bytes(dl);
+  uint64_t* pa = reinterpret_cast<uint64_t*>(lds + wbase);  // alpha plane [dl][64]
+  uint64_t* pg = pa + dl * kWave;                            // gamma plane [dl][64]
+  uint64_t* sidx = pg + dl * kWave;
+  uint64_t* schk = sidx + kWave;
+  uint64_t* smix = schk + kWave;
+  const uint32_t abase = (uint32_t)(uintptr_t)lds + wbase + 8u * lane;
+  const uint32_t gplane = dl * kWave * 8u;
+  for (uint32_t i = lane; i < dl * kWave; i += kWave) pa[i] = pg[i] = 0;
+  wave_sync_lds();
+  uint32_t n_flush = 0, n_deep = 0;
+
+  uint32_t ccount[kClasses];
+  uint32_t nitems = 0;
+#pragma unroll
+  for (int k = 0; k < kClasses; k++) {
+    ccount[k] = __builtin_amdgcn_readfirstlane(class_counts[k]);
+    nitems += ccount[k];
+  }
+  const uint32_t units = nitems * ngroups;
+  const uint32_t gw = __builtin_amdgcn_readfirstlane(blockIdx.x * kWideWaves + wid);
+  const uint32_t nw = gridDim.x * kWideWaves;
+  for (uint32_t u = gw; u < units; u += nw) {
+    // item u / ngroups in largest-class-first order, column group u % ngroups
+    uint32_t vi = u / ngroups;
+    const uint32_t grp = u - vi * ngroups;
+    uint64_t off = 0;
+    bool found = false;
+#pragma unroll
+    for (int k = kClasses - 1; k >= 0; k--) {
+      const bool here = !found && (k == 0 || vi < ccount[k]);
+      off = here ? cls.off[k] + vi : off;
+      found = found || here;
+      vi = found ? vi : vi - ccount[k];
+    }
+    const WorkItem wi = items[off];
+    const uint32_t v = __builtin_amdgcn_readfirstlane(wi.vertex);
+    const uint32_t len = __builtin_amdgcn_readfirstlane(wi.len) & ~kItemExclusive;
+    const uint32_t start = __builtin_amdgcn_readfirstlane(wi.start);
+    const uint32_t c = grp * gcols + lane;
+    const bool active = lane < gcols && c < st.cols;
+    const uint64_t colk = active ? xxh3_seed_key(st.seed + 5ULL * c) : 0;
+    const uint32_t dli = min(dl, 35u - (uint32_t)__builtin_clz(len));
+    Bucket16* vrows = st.rows + (size_t)v * st.vstride;
+    uint64_t det_a = 0, det_g = 0;
+    for (uint32_t j0 = 0; j0 < len; j0 += kWave) {
+      const uint32_t cnt = min((uint32_t)kWave, len - j0);
+      if (lane < cnt) {
+        const uint64_t idx0 = concat_pair(v, dsts[(uint64_t)start + j0 + lane]);
+        uint64_t idx = idx0, chk = xxh3_len8(idx0, st.seed);
+        if (min_depth && depth_of(chk, depth_cap) < min_depth) idx = chk = 0;
+        sidx[lane] = idx;
+        schk[lane] = chk;
+        smix[lane] = xxh3_mix(xxh3_swap(idx));
+        det_a ^= idx;
+        det_g ^= chk;
+      }
+      wave_sync_lds();
+      if (active) {
+        for (uint32_t j = 0; j < cnt; j++) {
+          const uint64_t a = sidx[j], gm = schk[j], x = smix[j] ^ colk;
+          uint32_t d = depth_low16(x, st.bpc);
+          if (d < dli) {
+            const uint32_t ad = abase + d * (kWave * 8u);
+            lds_xor_u64(ad, a);
+            lds_xor_u64(ad + gplane, gm);
+          } else {  // rare: past the window (and the 16-bit tie)
+            if (d == 16 && st.bpc > 16) d = depth_of(xxh3_finish(x), st.bpc);
+            if (d < st.bpc) {
+              if (kL0) {
+                const uint32_t ad = abase + (dli - 1) * (kWave * 8u);
+                lds_xor_u64(ad, a);
+                lds_xor_u64(ad + gplane, gm);
+              }
+              for (uint32_t k = kL0 ? dli : d; k <= d; k++) {
+                Bucket16* b = &vrows[(size_t)k * st.cols + c];
+                atomic_xor_u64(&b->alpha, a);
+                atomic_xor_u64(&b->gamma, gm);
+                n_deep++;
+              }
+            }
+          }
+        }
+      }
+      wave_sync_lds();
+    }
+    if (grp == 0) {
+      det_a = wave_xor_reduce(det_a);
+      det_g = wave_xor_reduce(det_g);
+      if (lane == 0 && (det_a | det_g)) {
+        Bucket16* b = &st.det[v];
+        atomic_xor_u64(&b->alpha, det_a);
+        atomic_xor_u64(&b->gamma, det_g);
+      }
+      if (traffic && lane == 0 && (det_a | det_g)) n_flush++;
+    }
+    // flush, deepest depth first (the CubeSketch suffix runs that way)
+    uint64_t ra = 0, rg = 0;
+    for (uint32_t k = dli; k-- > 0;) {
+      const uint32_t s = k * kWave + lane;
+      uint64_t xa = pa[s], xg = pg[s];
+      pa[s] = 0;
+      pg[s] = 0;
+      if (kL0) {
+        ra ^= xa;
+        rg ^= xg;
+        xa = ra;
+        xg = rg;
+      }
+      if (active && (xa | xg)) {
+        Bucket16* b = &vrows[(size_t)k * st.cols + c];
+        atomic_xor_u64(&b->alpha, xa);
+        atomic_xor_u64(&b->gamma, xg);
+        n_flush++;
+      }
+    }
+    wave_sync_lds();
+  }
+  if (traffic) {
+    for (int o = 32; o > 0; o >>= 1) {
+      n_deep += __shfl_xor(n_deep, o);
+      n_flush += __shfl_xor(n_flush, o);
+    }
+    if (lane == 0) {
+      atomicAdd(&traffic[0], (unsigned long long)n_flush);
+      atomicAdd(&traffic[1], (unsigned long long)n_deep);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
 // launch helpers
 // ------------------------------------------------------------------------------------------
 // depths kept in the LDS delta by default; deeper updates (probability 2^-9 per column) go to
@@ -734,14 +893,14 @@ static size_t agg_lds_bytes(uint32_t cols, uint32_t dl) {
 // persistent grid: as many workgroups as can be resident at once (the occupancy calculator
 // accounts for LDS and registers), so the static round-robin over work items never waits on a
 // late-starting workgroup
-static int agg_grid(int device, const void* kernel, size_t lds_bytes) {
+static int agg_grid(int device, const void* kernel, size_t lds_bytes, int waves = kAggWaves) {
   static int cu = 0;
   if (cu == 0) {
     if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cu <= 0)
       cu = 256;
   }
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kAggWaves * kWave, lds_bytes) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, waves * kWave, lds_bytes) !=
           hipSuccess ||
       per_cu <= 0) {
     (void)hipGetLastError();
@@ -808,10 +967,56 @@ static const AggKernel& agg_kernel_for(bool group, uint32_t cols) {
   return table[group ? 1 : 0][t - 1];
 }
 
+// wide sketches (more than 64 columns, or the CubeSketch rule): wide_aggregate_kernel
+static int launch_wide_aggregate(Stream* s, const WorkItem* d_items, const ItemClasses& cls,
+                                 const uint32_t* class_counts, const uint32_t* d_dsts,
+                                 uint64_t updates, uint32_t min_depth, uint32_t depth_cap) {
+  Store* st = s->store;
+  const StoreView& v = st->view;
+  const bool l0 = (st->flags & GS_STORE_CUBESKETCH) != 0;
+  uint32_t dl = std::min(v.bpc, s->tune_dl ? s->tune_dl : kDefaultLdsDepths);
+  dl = std::min(dl, kMaxLdsDepths);
+  const uint32_t ngroups = (v.cols + kWave - 1) / kWave;
+  const uint32_t gcols = (v.cols + ngroups - 1) / ngroups;
+  const size_t lds = (size_t)kWideWaves * wide_wave_bytes(dl);
+  const void* fn = l0 ? reinterpret_cast<const void*>(&wide_aggregate_kernel<true>)
+                      : reinterpret_cast<const void*>(&wide_aggregate_kernel<false>);
+  int rc_lds = ensure_dyn_lds(fn, lds);
+  if (rc_lds) return rc_lds;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (s->profile) {
+    GS_HIP(hipEventCreate(&e0));
+    GS_HIP(hipEventCreate(&e1));
+    GS_HIP(hipEventRecord(e0, s->stream));
+    if (!s->d_traffic) {
+      GS_HIP(hipMalloc(&s->d_traffic, 2 * sizeof(unsigned long long)));
+      GS_HIP(hipMemsetAsync(s->d_traffic, 0, 2 * sizeof(unsigned long long), s->stream));
+    }
+  }
+  const dim3 grid(agg_grid(st->device, fn, lds, kWideWaves)), block(kWideWaves * kWave);
+  unsigned long long* tr = s->profile ? s->d_traffic : nullptr;
+  if (l0)
+    hipLaunchKernelGGL(wide_aggregate_kernel<true>, grid, block, lds, s->stream, v, d_items, cls,
+                       class_counts, d_dsts, dl, gcols, ngroups, min_depth, depth_cap, tr);
+  else
+    hipLaunchKernelGGL(wide_aggregate_kernel<false>, grid, block, lds, s->stream, v, d_items, cls,
+                       class_counts, d_dsts, dl, gcols, ngroups, min_depth, depth_cap, tr);
+  GS_HIP(hipGetLastError());
+  if (s->profile) {
+    GS_HIP(hipEventRecord(e1, s->stream));
+    s->ev_pairs.emplace_back(e0, e1);
+    s->agg_updates += updates;
+  }
+  return GS_OK;
+}
+
 static int launch_aggregate(Stream* s, const WorkItem* d_items, const ItemClasses& cls,
                             const uint32_t* class_counts, const uint32_t* d_dsts,
                             uint64_t updates, uint32_t min_depth = 0, uint32_t depth_cap = 0) {
   Group* grp = s->group;
+  if (!grp && ((s->store->flags & GS_STORE_CUBESKETCH) || s->store->view.cols > 8 * kMaxColSteps))
+    return launch_wide_aggregate(s, d_items, cls, class_counts, d_dsts, updates, min_depth,
+                                 depth_cap);
   StoreView v{};
   uint32_t cols, dl;
   int device = 0;

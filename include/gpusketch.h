@@ -45,13 +45,19 @@ typedef struct gs_geometry {
   uint32_t num_vertices;
   uint32_t bkt_per_col;     /* depths per column, B */
   uint32_t num_samples;     /* S */
-  uint32_t cols_per_sample; /* 1 (CameoSketch) */
+  uint32_t cols_per_sample; /* columns scanned per sample: 1 by default, 7 under L0_SAMPLING */
   uint32_t num_columns;     /* C = S * cols_per_sample */
   uint32_t num_buckets;     /* C * B + 1, the reference per-vertex bucket count */
   uint64_t seed;
   uint64_t vector_len;      /* n(n-1)/2 */
   uint64_t device_bytes;    /* HBM held by the store */
+  uint32_t flags;           /* GS_STORE_* flags of the store (or of the sizing request) */
 } gs_geometry;
+
+/* gs_store_create flags.
+ * GS_STORE_CUBESKETCH: the L0_SAMPLING update rule (src/sketch.cpp:66-83): an update of depth d in
+ * a column toggles every depth 0..d of it (default: depth d only, src/sketch.cpp:85-99). */
+#define GS_STORE_CUBESKETCH 1u
 
 /* Thread-local message of the last failing call ("" if none). */
 const char* gs_last_error(void);
@@ -60,6 +66,10 @@ const char* gs_last_error(void);
  * (include/sketch.h:63-77,198-206) for n vertices and a CCAlgConfiguration sketches_factor. */
 int gs_geometry_for(uint32_t n, double sketches_factor, uint32_t cols_per_sample,
                     gs_geometry* out);
+/* The same for a sketch variant: flags GS_STORE_CUBESKETCH takes the L0_SAMPLING num_samples_div
+ * (include/sketch.h:208-211, log2(3) - 1) instead of the default (:213-215). */
+int gs_geometry_for_sketch(uint32_t n, double sketches_factor, uint32_t cols_per_sample,
+                           uint32_t flags, gs_geometry* out);
 
 /* Device XXH3 known-answer check: out[i] = XXH3_64bits_withSeed(&keys[i], 8, seeds[i]) computed
  * by a device kernel (replaces CUDA_XXH64, include/cuda_xxhash64.cuh:587-591). Host arrays. */
@@ -69,7 +79,8 @@ int gs_hash_known_answer(const uint64_t* keys, const uint64_t* seeds, uint64_t c
 /* Create a zeroed sketch store of n vertex sketches resident in HBM, all with the same seed
  * (the reference gives every vertex the same seed, src/cc_sketch_alg.cpp:11-22). Replaces the
  * cudaMallocManaged sketch buffer + SketchParams set-up (tools/cuda_process_stream.cu:97-116,
- * include/cc_gpu_sketch_alg.h:36-97). flags: reserved, pass 0. */
+ * include/cc_gpu_sketch_alg.h:36-97). flags: GS_STORE_* (0 = the default update rule).
+ * num_samples * cols_per_sample columns, at most 4096. */
 int gs_store_create(uint32_t n, uint64_t seed, uint32_t num_samples, uint32_t cols_per_sample,
                     uint32_t flags, gs_store** out);
 int gs_store_destroy(gs_store* st);

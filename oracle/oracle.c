@@ -125,9 +125,9 @@ void orc_apply_vertex_batches(uint64_t *store, uint32_t bpc, uint32_t cols, uint
 /* One vertex's sketch (any storage) after toggling concat(src, dsts[i]) for i < count -- the
  * per-vertex batch of src/cc_sketch_alg.cpp:139-141 without the store around it. */
 void orc_update_vertex(uint64_t *bk, uint32_t bpc, uint32_t cols, uint64_t seed, uint32_t src,
-                       const uint32_t *dsts, uint64_t count) {
+                       const uint32_t *dsts, uint64_t count, int l0) {
   for (uint64_t i = 0; i < count; i++)
-    orc_sketch_update(bk, bpc, cols, seed, orc_concat_pair(src, dsts[i]), 0);
+    orc_sketch_update(bk, bpc, cols, seed, orc_concat_pair(src, dsts[i]), l0);
 }
 
 /* Both endpoints of each undirected update, as the driver inserts {u,v} and {v,u}

@@ -22,7 +22,10 @@ How each function is pinned (checked against something other than this restateme
 * ``sketch_update`` / ``apply_edges`` / ``apply_vertex_batches`` -- an independent pure-Python
   restatement on small cases (tests/test_oracle.py), plus the invariants the reference tests:
   updating an index twice is the identity, order independence, merge linearity
-  (``test/sketch_test.cpp:423-458``), index 0 updated 3 times samples GOOD 0 (``:66-80``).
+  (``test/sketch_test.cpp:423-458``), index 0 updated 3 times samples GOOD 0 (``:66-80``). The
+  CubeSketch rule (``l0=True``, ``src/sketch.cpp:66-83``) is pinned by the same pure-Python
+  restatement and by a closed form: each of its columns is the suffix XOR over depths of the
+  support-finding column; its geometry by the quoted values (n=2^17: B=34, S=30, C=210).
 * ``sample`` -- brute force on tiny vectors: GOOD implies the index is a live coordinate, ZERO
   iff the vector is empty, the crafted FAIL vector of ``test/sketch_test.cpp:14-64``.
 * ``boruvka`` / ``dsu_components`` -- the partition equals a union-find over the net graph on
@@ -75,7 +78,7 @@ def _load():
     lib.orc_sketch_update.argtypes = [vp, u32, u32, u64, u64, i32]
     lib.orc_apply_vertex_batches.argtypes = [vp, u32, u32, u64, vp, vp, vp, u32, vp, i32]
     lib.orc_apply_edges.argtypes = [vp, u32, u32, u64, vp, u64, i32]
-    lib.orc_update_vertex.argtypes = [vp, u32, u32, u64, u32, vp, u64]
+    lib.orc_update_vertex.argtypes = [vp, u32, u32, u64, u32, vp, u64, i32]
     lib.orc_sample.argtypes = [vp, u32, u32, u32, u64, u32, vp]
     lib.orc_sample.restype = i32
     lib.orc_dsu_components.argtypes = [u32, vp, u64, vp]
@@ -153,11 +156,12 @@ def apply_edges(store: np.ndarray, g: Geometry, seed: int, pairs: np.ndarray,
                             pairs.shape[0], int(l0))
 
 
-def vertex_sketch(g: Geometry, seed: int, src: int, dsts: np.ndarray) -> np.ndarray:
+def vertex_sketch(g: Geometry, seed: int, src: int, dsts: np.ndarray, l0: bool = False) -> np.ndarray:
     """The sketch of vertex ``src`` alone after toggling concat(src, d) for every d in dsts."""
     dsts = np.ascontiguousarray(dsts, dtype=np.uint32)
     bk = np.zeros((g.num_buckets, 2), dtype=np.uint64)
-    _load().orc_update_vertex(_p(bk), g.bkt_per_col, g.num_columns, seed, src, _p(dsts), dsts.size)
+    _load().orc_update_vertex(_p(bk), g.bkt_per_col, g.num_columns, seed, src, _p(dsts), dsts.size,
+                              int(l0))
     return bk
 
 

@@ -2,7 +2,8 @@
 pin the C oracle on small cases (it shares no code with oracle/oracle.c).
 
 xxh3_len8 follows the published XXH3 short-input path (len 4..8 + rrmxmx); the sketch update
-follows src/sketch.cpp:85-99 and include/bucket.h:54-73; sample follows src/sketch.cpp:110-132.
+follows src/sketch.cpp:85-99 (CubeSketch rule :66-83) and include/bucket.h:54-73; sample follows
+src/sketch.cpp:110-132.
 """
 M64 = (1 << 64) - 1
 SECRET8 = 0x1CAD21F72C81017C
@@ -36,16 +37,18 @@ def concat(i, j):
     return (i << 32) | j
 
 
-def update(bk, bpc, cols, seed, idx):
-    """bk: list of [alpha, gamma] of length cols*bpc+1 (reference order)."""
+def update(bk, bpc, cols, seed, idx, l0=False):
+    """bk: list of [alpha, gamma] of length cols*bpc+1 (reference order). l0: the CubeSketch
+    rule of src/sketch.cpp:66-83 (every depth 0..d of the column), else :85-99 (depth d only)."""
     chk = xxh3_len8(idx, seed)
     bk[-1][0] ^= idx
     bk[-1][1] ^= chk
     for c in range(cols):
         d = depth(idx, (seed + 5 * c) & M64, bpc)
         if d < bpc:
-            bk[c * bpc + d][0] ^= idx
-            bk[c * bpc + d][1] ^= chk
+            for j in (range(d + 1) if l0 else (d,)):
+                bk[c * bpc + j][0] ^= idx
+                bk[c * bpc + j][1] ^= chk
 
 
 def sample(bk, bpc, cps, seed, sample_idx):

@@ -34,6 +34,12 @@ def test_geometry_matches_oracle():
             o.bkt_per_col, o.num_samples, o.num_columns, o.num_buckets, o.vector_len)
     g = gs.geometry_for(1 << 15, 2.7)
     assert g.num_columns == 55
+    # CubeSketch sizing: 7 columns per sample, num_samples_div = log2(3) - 1
+    for n in (300, 8192, 1 << 17):
+        g = gs.geometry_for(n, 1.0, 7, cubesketch=True)
+        o = orc.Geometry(n, 1.0, 7, True)
+        assert (g.bkt_per_col, g.num_samples, g.num_columns, g.num_buckets, g.flags) == (
+            o.bkt_per_col, o.num_samples, o.num_columns, o.num_buckets, gs.STORE_CUBESKETCH)
 
 
 def test_invalid_arguments_report_errors():

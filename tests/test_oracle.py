@@ -257,6 +257,91 @@ def test_sample_brute_force_small_vectors():
 
 
 # ---------------------------------------------------------------------------------------------
+# CubeSketch (L0_SAMPLING) rule, src/sketch.cpp:66-83, and several columns per sample
+# ---------------------------------------------------------------------------------------------
+def test_l0_geometry_closed_form():
+    # include/sketch.h:208-211: 7 columns per sample, num_samples_div = log2(3) - 1
+    g = orc.Geometry(1 << 17, 1.0, 7, True)
+    assert (g.bkt_per_col, g.num_samples, g.num_columns) == (34, 30, 210)
+    g = orc.Geometry(8192, 1.0, 7, True)
+    assert (g.bkt_per_col, g.num_samples, g.num_columns) == (26, 23, 161)
+    g = orc.Geometry(300, 1.0, 7, True)  # ceil(log2(300) / 0.585) = 15 < 18
+    assert (g.num_samples, g.num_columns) == (18, 126)
+
+
+def test_l0_update_matches_pure_python():
+    rng = random.Random(15)
+    g = orc.Geometry(300, 1.0, 7, True)
+    seed = rng.getrandbits(64)
+    bk = np.zeros((g.num_buckets, 2), dtype=np.uint64)
+    ref = [[0, 0] for _ in range(g.num_buckets)]
+    for _ in range(150):
+        u, v = rng.sample(range(300), 2)
+        idx = refpy.concat(u, v)
+        orc.sketch_update(bk, g, seed, idx, l0=True)
+        refpy.update(ref, g.bkt_per_col, g.num_columns, seed, idx, l0=True)
+    assert bk.tolist() == ref
+
+
+def test_l0_is_the_suffix_xor_of_the_point_rule():
+    # depth j of a CubeSketch column holds every update of depth >= j (< B): the XOR of depths
+    # j..B-1 of the same column under the support-finding rule; the deterministic bucket agrees
+    rng = np.random.default_rng(16)
+    n = 400
+    g = orc.Geometry(n, 1.0, 7, True)
+    pairs = rng.integers(0, n, size=(4000, 2), dtype=np.uint32)
+    pairs = pairs[pairs[:, 0] != pairs[:, 1]]
+    a, b = g.empty_store(), g.empty_store()
+    orc.apply_edges(a, g, 21, pairs, l0=True)
+    orc.apply_edges(b, g, 21, pairs, l0=False)
+    B, C = g.bkt_per_col, g.num_columns
+    pa = a[:, :C * B].reshape(n, C, B, 2)
+    pb = b[:, :C * B].reshape(n, C, B, 2)
+    suffix = np.bitwise_xor.accumulate(pb[:, :, ::-1], axis=2)[:, :, ::-1]
+    assert np.array_equal(pa, suffix)
+    assert np.array_equal(a[:, -1], b[:, -1])
+    assert pa.any()
+
+
+def test_l0_sample_brute_force_small_vectors():
+    rng = random.Random(17)
+    g = orc.Geometry.explicit(1, 6, 3, 7)
+    for _ in range(200):
+        seed = rng.getrandbits(64)
+        live = set(rng.sample(range(1, 60), rng.randrange(0, 8)))
+        sk = np.zeros((g.num_buckets, 2), dtype=np.uint64)
+        ref = [[0, 0] for _ in range(g.num_buckets)]
+        for x in live:
+            orc.sketch_update(sk, g, seed, x, l0=True)
+            refpy.update(ref, g.bkt_per_col, g.num_columns, seed, x, l0=True)
+        assert sk.tolist() == ref
+        for s in range(g.num_samples):
+            idx, res = orc.sample(sk, g, seed, s)
+            ridx, rres = refpy.sample(ref, g.bkt_per_col, 7, seed, s)
+            assert (idx, {0: "GOOD", 1: "ZERO", 2: "FAIL"}[res]) == (ridx, rres)
+            if res == orc.GOOD:
+                assert idx in live
+            assert (res == orc.ZERO) == (len(live) == 0)
+
+
+@pytest.mark.parametrize("cps,l0", [(7, True), (3, False)])
+def test_boruvka_several_columns_per_sample_equals_dsu(cps, l0):
+    rng = np.random.default_rng(18 + cps)
+    n = 1500
+    pairs = rng.integers(0, n, size=(1300, 2), dtype=np.uint32)
+    pairs = pairs[pairs[:, 0] != pairs[:, 1]]
+    dels = pairs[rng.choice(len(pairs), size=len(pairs) // 10, replace=False)]
+    stream = np.concatenate([pairs, dels])
+    g = orc.Geometry(n, 1.0, cps, l0)
+    st = g.empty_store()
+    orc.apply_edges(st, g, 99, stream, l0=l0)
+    rc, lab, rounds, forest = orc.boruvka(st, g, 99)
+    assert rc == 0
+    assert np.array_equal(lab, orc.dsu_components(n, _net_edges(stream)))
+    assert len(forest) == n - len(np.unique(lab))
+
+
+# ---------------------------------------------------------------------------------------------
 # a19, a20: Borůvka and DSU
 # ---------------------------------------------------------------------------------------------
 def _net_edges(pairs):
