@@ -40,16 +40,16 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(pairs: np.ndarray, threads: int) -> dict:
+def cpu_baseline(pairs: np.ndarray, threads: int, cubesketch: bool = False) -> dict:
     """Reference CPU update path (per-thread delta sketch + mutex merge) on pre-grouped vertex
     batches of 4 * num_buckets destinations, the reference's batch size."""
     from oracle import oracle as orc
-    g = orc.Geometry(N)
+    g = orc.Geometry(N, 1.0, 7, True) if cubesketch else orc.Geometry(N)
     t0 = time.perf_counter()
     src, sizes, offs, dsts = orc.group_by_vertex(pairs, N, 4 * g.num_buckets)
     group_s = time.perf_counter() - t0
     store = g.empty_store()
-    secs = orc.cpu_baseline(store, g, SEED_SKETCH, src, sizes, offs, dsts, threads)
+    secs = orc.cpu_baseline(store, g, SEED_SKETCH, src, sizes, offs, dsts, threads, cubesketch)
     del store
     return {"value": len(pairs) / secs, "unit": "edges/s", "cores": threads, "kind": "port",
             "sample": f"first {len(pairs)} updates of the same stream ({len(pairs) // BATCH} "
@@ -273,7 +273,9 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pcie-batches", type=int, default=32,
                     help="batches submitted from pinned host memory for the PCIe-inclusive rate")
-    ap.add_argument("--mode", choices=["single", "batched", "serving", "mincut"], default="single")
+    ap.add_argument("--mode", choices=["single", "cubesketch", "batched", "serving", "mincut"],
+                    default="single",
+                    help="cubesketch: the single-stream workload under the L0_SAMPLING rule")
     ap.add_argument("--no-serving", action="store_true",
                     help="single mode: skip the served-vs-plain trace")
     ap.add_argument("--graphs", type=int, default=256, help="batched mode: independent graphs")
@@ -310,12 +312,17 @@ def main() -> None:
 
     import gpusketch_amd as gs
 
+    cube = args.mode == "cubesketch"
+    if cube:  # 9x the hashing per update: no serving trace or PCIe leg, a short CPU sample
+        args.no_serving = True
+        args.pcie_batches = 0
+        args.cpu_batches = min(args.cpu_batches, 2)
     total_batches = args.warmup + args.steps
     t0 = time.perf_counter()
     pairs = gs.gen_rmat(SCALE, total_batches * BATCH, SEED_STREAM, *RMAT, del_frac=DEL_FRAC)
     log(f"generated {len(pairs)} R-MAT updates in {time.perf_counter() - t0:.1f}s")
 
-    store = gs.SketchStore(N, SEED_SKETCH)
+    store = gs.SketchStore(N, SEED_SKETCH, cubesketch=cube)
     geo = store.geom
     stream = store.stream()
     dbuf = gs.DeviceBuffer(pairs)
@@ -339,8 +346,9 @@ def main() -> None:
     agg_ms, launches, agg_updates = stream.timing()
     flushed, direct = stream.traffic()
     stream.set_profiling(False)
+    kname = "wide_aggregate_kernel" if cube else "aggregate_kernel"
     log(f"timed {args.steps} batches in {elapsed:.4f}s ({launches} launches of the grouping and "
-        f"aggregation kernels); aggregate_kernel {agg_ms / launches:.4f} ms avg")
+        f"aggregation kernels); {kname} {agg_ms / launches:.4f} ms avg")
 
     # PCIe-inclusive rate: the same batches handed over from pinned host memory (copied by DMA
     # on the stream's copy stream while the kernels run), into a second store
@@ -385,12 +393,14 @@ def main() -> None:
     agg_s = agg_ms / launches / 1e3
     achieved = alg_bytes / agg_s / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_aggregate_rmat17.json")
+    tpath = os.path.join(ROOT, "profiles", "traffic_wide_aggregate_rmat17_cubesketch.json" if cube
+                         else "traffic_aggregate_rmat17.json")
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
 
     result = {
-        "metric": "edge updates/s at n=2^17 + p50/p99 CC query latency, served vs plain",
+        "metric": ("edge updates/s at n=2^17, CubeSketch rule (7 columns per sample)" if cube else
+                   "edge updates/s at n=2^17 + p50/p99 CC query latency, served vs plain"),
         "value": args.steps * BATCH / elapsed,
         "unit": "edges/s",
         "n_gpus": 1,
@@ -402,13 +412,14 @@ def main() -> None:
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic",
-        "config": {"workload": "rmat17_single_stream_cc", "n": N, "batch_edges": BATCH,
+        "config": {"workload": "rmat17_cubesketch_cc" if cube else "rmat17_single_stream_cc",
+                   "n": N, "batch_edges": BATCH,
                    "rmat_abcd": [*RMAT, round(1 - sum(RMAT), 2)], "del_frac": DEL_FRAC,
                    "bkt_per_col": geo.bkt_per_col, "num_columns": geo.num_columns,
                    "buckets_per_vertex": geo.num_buckets, "stream_updates": len(pairs)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "aggregate_kernel", "kernel_ms": agg_ms / launches,
+                     "kernel": kname, "kernel_ms": agg_ms / launches,
                      "edges_per_launch": edges_per_launch,
                      "alg_bytes_per_launch": alg_bytes,
                      "alg_bytes_per_edge": alg_bytes / edges_per_launch,
@@ -433,7 +444,7 @@ def main() -> None:
         threads = len(os.sched_getaffinity(0))
         threads = max(1, min(threads, 16))
         nb = min(args.cpu_batches, total_batches)
-        cb = cpu_baseline(pairs[: nb * BATCH], threads)
+        cb = cpu_baseline(pairs[: nb * BATCH], threads, cube)
         cb.pop("seconds")
         result["cpu_baseline"] = cb
         log(f"cpu baseline {cb['value']:.3e} edges/s on {threads} threads")

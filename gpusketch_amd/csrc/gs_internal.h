@@ -249,6 +249,11 @@ inline const PartParams& stream_part(const Stream* s) {
 inline uint32_t stream_vertices(const Stream* s) {
   return s->group ? s->group->n_total : s->store->view.n;
 }
+// a stream of a wide store (more than 64 columns, or the CubeSketch rule): the lane-per-column
+// update kernel, which hashes every column of every update and so gains most from long items
+inline bool stream_wide(const Stream* s) {
+  return !s->group && ((s->store->flags & GS_STORE_CUBESKETCH) || s->store->view.cols > 64);
+}
 
 // pinned (+ device, unless `device` is false) staging of a stream (api.hip): ensure_stage waits
 // until the previous staged submission has been consumed; release_stage marks the staging busy

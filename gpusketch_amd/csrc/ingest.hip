@@ -20,6 +20,7 @@ namespace gs {
 constexpr int kWave = 64;
 constexpr int kAggWaves = 4;                  // waves per aggregation workgroup
 constexpr uint32_t kChunk = 512;              // max destinations per work item
+constexpr uint32_t kWideChunk = 1024;         // the same for wide stores
 constexpr int kPartThreads = 512;             // K1/K2 workgroup size
 constexpr int kPartEdges = 4096;              // undirected edges per K1/K2 workgroup
 constexpr int kSortThreads = 1024;            // K4 workgroup size (one workgroup per bucket)
@@ -808,27 +809,39 @@ __global__ __launch_bounds__(kWideWaves* kWave) void wide_aggregate_kernel(
       }
       wave_sync_lds();
       if (active) {
+        // the next entry's staged values are read while this one is hashed (broadcast reads)
+        uint64_t na = sidx[0], ng = schk[0], nm = smix[0];
         for (uint32_t j = 0; j < cnt; j++) {
-          const uint64_t a = sidx[j], gm = schk[j], x = smix[j] ^ colk;
+          const uint64_t a = na, gm = ng, x = nm ^ colk;
+          const uint32_t jn = min(j + 1, cnt - 1);
+          na = sidx[jn];
+          ng = schk[jn];
+          nm = smix[jn];
           uint32_t d = depth_low16(x, st.bpc);
-          if (d < dli) {
+          if (kL0) {
+            // every entry of depth >= the window's last depth lands there (the suffix XOR of
+            // the flush spreads it over depths 0..dli-1); the rare branch below settles the rest
+            const uint32_t ad = abase + min(d, dli - 1) * (kWave * 8u);
+            lds_xor_u64(ad, a);
+            lds_xor_u64(ad + gplane, gm);
+          } else if (d < dli) {
             const uint32_t ad = abase + d * (kWave * 8u);
             lds_xor_u64(ad, a);
             lds_xor_u64(ad + gplane, gm);
-          } else {  // rare: past the window (and the 16-bit tie)
+          }
+          if (d >= dli) {  // rare: past the window (and the 16-bit tie)
             if (d == 16 && st.bpc > 16) d = depth_of(xxh3_finish(x), st.bpc);
             if (d < st.bpc) {
-              if (kL0) {
-                const uint32_t ad = abase + (dli - 1) * (kWave * 8u);
-                lds_xor_u64(ad, a);
-                lds_xor_u64(ad + gplane, gm);
-              }
               for (uint32_t k = kL0 ? dli : d; k <= d; k++) {
                 Bucket16* b = &vrows[(size_t)k * st.cols + c];
                 atomic_xor_u64(&b->alpha, a);
                 atomic_xor_u64(&b->gamma, gm);
                 n_deep++;
               }
+            } else if (kL0) {  // depth B updates nothing: take the window entry back out
+              const uint32_t ad = abase + (dli - 1) * (kWave * 8u);
+              lds_xor_u64(ad, a);
+              lds_xor_u64(ad + gplane, gm);
             }
           }
         }
@@ -1014,7 +1027,7 @@ static int launch_aggregate(Stream* s, const WorkItem* d_items, const ItemClasse
                             const uint32_t* class_counts, const uint32_t* d_dsts,
                             uint64_t updates, uint32_t min_depth = 0, uint32_t depth_cap = 0) {
   Group* grp = s->group;
-  if (!grp && ((s->store->flags & GS_STORE_CUBESKETCH) || s->store->view.cols > 8 * kMaxColSteps))
+  if (stream_wide(s))
     return launch_wide_aggregate(s, d_items, cls, class_counts, d_dsts, updates, min_depth,
                                  depth_cap);
   StoreView v{};
@@ -1162,7 +1175,9 @@ int ensure_pair_scratch(Stream* s, uint64_t nedges) {
 // at least 2^16 and at most kLaunchEdges. The fill buffers and scratch are sized for it.
 uint64_t stream_launch_edges(const Stream* s) {
   if (s->tune_launch) return s->tune_launch;
-  const uint64_t per_vertex = 64ULL * stream_vertices(s);
+  // wide stores: 256 edges per vertex (2^25 at n = 2^17), which halves the buckets each update
+  // changes against 64 per vertex
+  const uint64_t per_vertex = (stream_wide(s) ? 256ULL : 64ULL) * stream_vertices(s);
   return std::min(kLaunchEdges, std::max<uint64_t>(1ULL << 16, per_vertex));
 }
 
@@ -1248,7 +1263,8 @@ static int process_pending(Stream* s) {
   s->pend_used[b] = true;
   hipLaunchKernelGGL(bucket_sort_kernel, dim3(pp.nbuckets), dim3(kSortThreads), sort_lds,
                      s->stream, ws.d_entries, pp, ws.d_totals, ws.d_sorted, ws.d_items, ws.cls,
-                     s->tune_chunk ? s->tune_chunk : kChunk, ws.d_counters + kClassBase,
+                     s->tune_chunk ? s->tune_chunk : (stream_wide(s) ? kWideChunk : kChunk),
+                     ws.d_counters + kClassBase,
                      s->group ? (const uint32_t*)s->group->d_bases : (const uint32_t*)nullptr,
                      s->group ? (uint32_t)s->group->members.size() : 1u);
   GS_HIP(hipGetLastError());

@@ -427,7 +427,8 @@ uint64_t orc_distinct_slots(uint32_t bpc, uint32_t cols, uint64_t seed, const ui
  * src/cc_sketch_alg.cpp:133-145): each worker thread zeroes its delta sketch, applies
  * Sketch::update for each dst of a pre-grouped vertex batch, then merges the delta into the
  * vertex sketch under that sketch's mutex. Batches are taken round-robin by threads. The store
- * is in reference layout. Returns elapsed seconds (monotonic clock).
+ * is in reference layout; l0 selects the CubeSketch rule. Returns elapsed seconds (monotonic
+ * clock).
  * ------------------------------------------------------------------------------------------- */
 typedef struct {
   uint64_t *store;
@@ -440,6 +441,7 @@ typedef struct {
   uint32_t tid, nthreads;
   pthread_mutex_t *locks;
   uint32_t nlocks;
+  int l0;
 } cpu_job_t;
 
 static void *cpu_worker(void *arg) {
@@ -451,7 +453,7 @@ static void *cpu_worker(void *arg) {
     uint32_t s = j->src[b];
     for (uint64_t i = 0; i < j->sizes[b]; i++)
       orc_sketch_update(delta, j->bpc, j->cols, j->seed,
-                        orc_concat_pair(s, j->dsts[j->offsets[b] + i]), 0);
+                        orc_concat_pair(s, j->dsts[j->offsets[b] + i]), j->l0);
     pthread_mutex_t *lk = &j->locks[s % j->nlocks];
     pthread_mutex_lock(lk);
     uint64_t *sk = j->store + 2 * nb * s;
@@ -464,7 +466,7 @@ static void *cpu_worker(void *arg) {
 
 double orc_cpu_baseline(uint64_t *store, uint32_t bpc, uint32_t cols, uint64_t seed,
                         const uint32_t *src, const uint64_t *sizes, const uint64_t *offsets,
-                        uint32_t nbatches, const uint32_t *dsts, uint32_t nthreads) {
+                        uint32_t nbatches, const uint32_t *dsts, uint32_t nthreads, int l0) {
   if (nthreads < 1) nthreads = 1;
   uint32_t nlocks = 4096;
   pthread_mutex_t *locks = (pthread_mutex_t *)malloc(sizeof(pthread_mutex_t) * nlocks);
@@ -474,7 +476,7 @@ double orc_cpu_baseline(uint64_t *store, uint32_t bpc, uint32_t cols, uint64_t s
   struct timespec t0, t1;
   clock_gettime(CLOCK_MONOTONIC, &t0);
   for (uint32_t t = 0; t < nthreads; t++) {
-    cpu_job_t j = {store, bpc, cols, seed, src, sizes, offsets, dsts, nbatches, t, nthreads, locks, nlocks};
+    cpu_job_t j = {store, bpc, cols, seed, src, sizes, offsets, dsts, nbatches, t, nthreads, locks, nlocks, l0};
     jobs[t] = j;
     pthread_create(&th[t], NULL, cpu_worker, &jobs[t]);
   }

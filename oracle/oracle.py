@@ -90,7 +90,7 @@ def _load():
     lib.orc_stoer_wagner.restype = u64
     lib.orc_distinct_slots.argtypes = [u32, u32, u64, vp, u64]
     lib.orc_distinct_slots.restype = u64
-    lib.orc_cpu_baseline.argtypes = [vp, u32, u32, u64, vp, vp, vp, u32, vp, u32]
+    lib.orc_cpu_baseline.argtypes = [vp, u32, u32, u64, vp, vp, vp, u32, vp, u32, i32]
     lib.orc_cpu_baseline.restype = dbl
     _lib = lib
     return lib
@@ -232,14 +232,16 @@ def distinct_slots(g: Geometry, seed: int, pairs: np.ndarray) -> int:
                                           pairs.shape[0]))
 
 
-def cpu_baseline(store, g: Geometry, seed, src, sizes, offsets, dsts, nthreads: int) -> float:
+def cpu_baseline(store, g: Geometry, seed, src, sizes, offsets, dsts, nthreads: int,
+                 l0: bool = False) -> float:
     """Times the reference CPU update path on pre-grouped vertex batches; returns seconds."""
     src = np.ascontiguousarray(src, dtype=np.uint32)
     sizes = np.ascontiguousarray(sizes, dtype=np.uint64)
     offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
     dsts = np.ascontiguousarray(dsts, dtype=np.uint32)
     return float(_load().orc_cpu_baseline(_p(store), g.bkt_per_col, g.num_columns, seed, _p(src),
-                                          _p(sizes), _p(offsets), len(src), _p(dsts), nthreads))
+                                          _p(sizes), _p(offsets), len(src), _p(dsts), nthreads,
+                                          int(l0)))
 
 
 def group_by_vertex(pairs: np.ndarray, n: int, batch_size: int):
