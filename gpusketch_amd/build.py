@@ -59,20 +59,27 @@ def build_library(jobs: int = 8) -> str:
 
 
 HOST_LIB = os.path.join(HERE, "libgpusketch_host.so")
+# the same C++ API compiled with L0_SAMPLING (the reference's CubeSketch build option)
+HOST_LIB_L0 = os.path.join(HERE, "libgpusketch_host_l0.so")
 HOST_FLAGS = ["-O2", "-std=c++17", "-pthread", "-Wall", "-Wextra", "-Wno-unused-parameter",
               "-I", os.path.join(ROOT, "include")]
 
 
 def build_host_programs() -> list[str]:
     """The C++ API (include/gpusketch/*.h, gpusketch_amd/host/*.cpp) as libgpusketch_host.so over
-    libgpusketch.so, and the C++ test programs host/prog_*.cpp linked against both."""
+    libgpusketch.so, and the C++ test programs host/prog_*.cpp linked against both; the API and
+    the two facade programs once more with L0_SAMPLING defined (libgpusketch_host_l0.so,
+    bin/facade_cpu_l0, bin/facade_gpu_l0)."""
     os.makedirs(BIN, exist_ok=True)
     lib_srcs = sorted(p for p in glob.glob(os.path.join(HOST, "*.cpp"))
                       if not os.path.basename(p).startswith("prog_"))
-    tmp = HOST_LIB + ".tmp"
-    _run(["g++", *HOST_FLAGS, "-fPIC", "-shared", *lib_srcs, "-o", tmp, f"-L{HERE}",
-          "-lgpusketch", "-Wl,-rpath,$ORIGIN"])
-    os.replace(tmp, HOST_LIB)
+    libs = [(HOST_LIB, []), (HOST_LIB_L0, ["-DL0_SAMPLING"])]
+    cmds = [["g++", *HOST_FLAGS, *defs, "-fPIC", "-shared", *lib_srcs, "-o", lib + ".tmp",
+             f"-L{HERE}", "-lgpusketch", "-Wl,-rpath,$ORIGIN"] for lib, defs in libs]
+    with ThreadPoolExecutor(max_workers=2) as ex:
+        list(ex.map(_run, cmds))
+    for lib, _ in libs:
+        os.replace(lib + ".tmp", lib)
     outs = []
     cmds = []
     for p in sorted(glob.glob(os.path.join(HOST, "prog_*.cpp"))):
@@ -80,6 +87,11 @@ def build_host_programs() -> list[str]:
         outs.append(out)
         cmds.append(["g++", *HOST_FLAGS, p, "-o", out, f"-L{HERE}", "-lgpusketch_host",
                      "-lgpusketch", "-Wl,-rpath,$ORIGIN/.."])
+    for prog in ("facade_cpu", "facade_gpu"):
+        p = os.path.join(HOST, f"prog_{prog}.cpp")
+        outs.append(os.path.join(BIN, f"{prog}_l0"))
+        cmds.append(["g++", *HOST_FLAGS, "-DL0_SAMPLING", p, "-o", outs[-1], f"-L{HERE}",
+                     "-lgpusketch_host_l0", "-lgpusketch", "-Wl,-rpath,$ORIGIN/.."])
     with ThreadPoolExecutor(max_workers=8) as ex:
         list(ex.map(_run, cmds))
     return outs

@@ -12,10 +12,20 @@ void gs_throw_if(int status) {
   throw std::runtime_error("gpusketch: " + msg);
 }
 
+// Built with L0_SAMPLING, the sketches follow the CubeSketch rule with its columns per sample,
+// as the reference's Sketch does under that definition (src/sketch.cpp:66-83,
+// include/sketch.h:208-211).
+#ifdef L0_SAMPLING
+static constexpr uint32_t kStoreFlags = GS_STORE_CUBESKETCH;
+#else
+static constexpr uint32_t kStoreFlags = 0;
+#endif
+
 DeviceSketchStore::DeviceSketchStore(node_id_t n, uint64_t seed, size_t num_samples,
                                      size_t nstreams, size_t flush)
     : flush_updates(flush ? flush : 1) {
-  gs_throw_if(gs_store_create(n, seed, (uint32_t)num_samples, 1, 0, &store));
+  gs_throw_if(gs_store_create(n, seed, (uint32_t)num_samples,
+                              (uint32_t)Sketch::default_cols_per_sample, kStoreFlags, &store));
   gs_throw_if(gs_store_geometry(store, &geo));
   ensure_streams(std::max<size_t>(1, nstreams));
 }

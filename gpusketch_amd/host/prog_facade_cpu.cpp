@@ -110,10 +110,11 @@ static void test_sketch_range_merge_and_serialize() {
   for (node_id_t v = 1; v < 200; v++) b.update(concat_pairing_fn(0, v));
   a.range_merge(b, 2, 3);
   const size_t B = a.get_bkt_per_col();
+  const size_t cps = a.get_columns() / a.get_num_samples();  // columns per sample
   const Bucket* pa = a.get_readonly_bucket_ptr();
   const Bucket* pb = b.get_readonly_bucket_ptr();
   for (size_t i = 0; i < a.get_buckets() - 1; i++) {
-    const bool inside = i >= 2 * B && i < 5 * B;
+    const bool inside = i >= 2 * cps * B && i < 5 * cps * B;
     CHECK((pa[i].alpha == (inside ? pb[i].alpha : 0)) &&
           (pa[i].gamma == (inside ? pb[i].gamma : 0)));
   }

@@ -14,10 +14,14 @@ PROG = os.path.join(ROOT, "gpusketch_amd", "bin", "facade_gpu")
 FIXTURE = os.path.join(ROOT, "tests", "golden", "multiples_graph_1024_stream.data")
 
 
-def test_facade_gpu_program(tmp_path):
-    if not os.path.exists(PROG):
-        pytest.fail("facade_gpu is not built; run `python gpusketch_amd/build.py`")
-    r = subprocess.run([PROG, FIXTURE, str(tmp_path)], capture_output=True, text=True,
+@pytest.mark.parametrize("prog", ["facade_gpu", "facade_gpu_l0"])
+def test_facade_gpu_program(tmp_path, prog):
+    """facade_gpu_l0: the same program and C++ API compiled with L0_SAMPLING (CubeSketch rule,
+    7 columns per sample), against host Sketch::update compiled the same way."""
+    path = os.path.join(os.path.dirname(PROG), prog)
+    if not os.path.exists(path):
+        pytest.fail(f"{prog} is not built; run `python gpusketch_amd/build.py`")
+    r = subprocess.run([path, FIXTURE, str(tmp_path)], capture_output=True, text=True,
                        timeout=600)
     sys.stdout.write(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
