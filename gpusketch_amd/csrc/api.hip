@@ -115,7 +115,8 @@ void destroy_stream_resources(Stream& s) {
   s.copied = nullptr;
   s.copy_stream = nullptr;
   s.stream = nullptr;
-  s.pend_cap = s.pend_n = 0;
+  s.pend_cap = s.pend_n = s.fill_n = 0;
+  s.zsegs.clear();
 }
 
 int sync_store(Store* st) {

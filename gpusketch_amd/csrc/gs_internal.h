@@ -158,14 +158,17 @@ struct Stream {
   hipEvent_t stage_free = nullptr;  // recorded when the kernels that read the staging are done
   bool stage_pending = false;
   // Coalescing: submitted edge pairs are appended (range-checked and filtered) to the fill
-  // buffer d_pend[pend_buf]; the grouping and aggregation kernels run once the buffer holds
-  // pend_cap edges, or at a flush. Host batches are copied on copy_stream into the fill buffer
-  // while the kernels read the other one; pend_free[b] is recorded after the kernel that read
-  // buffer b. `mu` serializes submissions and drains of one stream.
+  // buffer d_pend[pend_buf], or, for long unfiltered device batches, kept as segments read in
+  // place; the grouping and aggregation kernels run once pend_cap edges are pending, or at a
+  // flush. Host batches are copied on copy_stream into the fill buffer while the kernels read
+  // the other one; pend_free[b] is recorded after the kernel that read buffer b. `mu`
+  // serializes submissions and drains of one stream.
   std::mutex mu;
   uint32_t* d_pend[2] = {nullptr, nullptr};
-  uint64_t pend_cap = 0;  // edges per buffer
-  uint64_t pend_n = 0;    // edges waiting in the fill buffer
+  uint64_t pend_cap = 0;  // edges per launch (and per buffer)
+  uint64_t pend_n = 0;    // edges pending: the fill buffer's plus the segments'
+  uint64_t fill_n = 0;    // edges in the fill buffer
+  std::vector<std::pair<const uint32_t*, uint64_t>> zsegs;  // device batches read in place
   int pend_buf = 0;
   hipEvent_t pend_free[2] = {nullptr, nullptr};
   bool pend_used[2] = {false, false};

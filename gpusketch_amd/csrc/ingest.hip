@@ -93,6 +93,27 @@ __device__ __forceinline__ bool entry_ok(uint2 uv, uint32_t n) {
   return uv.x < n && uv.y < n && uv.x != uv.y;
 }
 
+// The pairs of one launch: up to kMaxSegs segments (the fill buffer, then device batches read in
+// place), each covered by whole K1/K2 workgroups of kPartEdges edges; blk0[i] is segment i's first
+// workgroup, blk0[nseg] the total.
+constexpr int kMaxSegs = 32;
+constexpr uint64_t kZeroCopyEdges = 1ULL << 18;  // device batches at least this long: in place
+struct PairSegs {
+  const uint2* ptr[kMaxSegs];
+  uint64_t n[kMaxSegs];
+  uint32_t blk0[kMaxSegs + 1];
+  uint32_t nseg;
+};
+// this workgroup's pairs: [e0, e1) of segment `p` (uniform per workgroup)
+__device__ __forceinline__ void block_pairs(const PairSegs& sg, uint32_t b, const uint2*& p,
+                                            uint64_t& e0, uint64_t& e1) {
+  uint32_t i = 0;
+  while (i + 1 < sg.nseg && sg.blk0[i + 1] <= b) i++;
+  p = sg.ptr[i];
+  e0 = (uint64_t)(b - sg.blk0[i]) * kPartEdges;
+  e1 = min(sg.n[i], e0 + kPartEdges);
+}
+
 // copy of a submission into the fill buffer; with a subgraph filter (min_depth > 0) in-range
 // pairs outside it become (0, 0). src and dst may be the same buffer.
 __global__ void append_kernel(const uint2* src, uint2* dst, uint64_t nedges, uint32_t n,
@@ -107,24 +128,27 @@ __global__ void append_kernel(const uint2* src, uint2* dst, uint64_t nedges, uin
   }
 }
 
+// out-of-range pairs are counted (reported at the next flush) and skipped here and in K2
+// (entry_ok), never written: the pairs may be the caller's own device memory
 __global__ __launch_bounds__(kPartThreads) void bucket_count_kernel(
-    uint2* pairs, uint64_t nedges, uint32_t n, PartParams pp, uint32_t nblk,
-    uint32_t* __restrict__ hist, uint32_t* __restrict__ counters) {
+    PairSegs sg, uint32_t n, PartParams pp, uint32_t nblk, uint32_t* __restrict__ hist,
+    uint32_t* __restrict__ counters) {
   extern __shared__ uint32_t h[];
   const uint32_t t = threadIdx.x;
   if (blockIdx.x == 0 && t < kClassBase + kClasses && t != 3)
     counters[t] = 0;  // class counts of this launch; [3] persists until a flush
   for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads) h[i] = 0;
   __syncthreads();
-  const uint64_t e0 = (uint64_t)blockIdx.x * kPartEdges;
+  const uint2* pairs;
+  uint64_t e0, e1;
+  block_pairs(sg, blockIdx.x, pairs, e0, e1);
 #pragma unroll
   for (int k = 0; k < kPartEdges / kPartThreads; k++) {
     const uint64_t e = e0 + t + (uint64_t)k * kPartThreads;
-    if (e >= nedges) continue;
+    if (e >= e1) continue;
     const uint2 uv = pairs[e];
     if (uv.x >= n || uv.y >= n) {  // never index past the store: drop and report at flush
       atomicAdd(&counters[3], 1u);
-      pairs[e] = make_uint2(0, 0);
       continue;
     }
     if (uv.x == uv.y) continue;
@@ -194,9 +218,8 @@ __global__ __launch_bounds__(kOffThreads) void bucket_offsets_kernel(uint32_t* h
 // The block's entries are first placed in LDS in bucket order, then written out so that
 // consecutive lanes write consecutive positions of one bucket's run (coalesced stores).
 __global__ __launch_bounds__(kPartThreads) void bucket_scatter_kernel(
-    const uint32_t* __restrict__ pairs, uint64_t nedges, uint32_t n, PartParams pp, uint32_t nblk,
-    const uint32_t* __restrict__ totals, const uint32_t* __restrict__ hist,
-    uint2* __restrict__ entries) {
+    PairSegs sg, uint32_t n, PartParams pp, uint32_t nblk, const uint32_t* __restrict__ totals,
+    const uint32_t* __restrict__ hist, uint2* __restrict__ entries) {
   extern __shared__ __align__(16) uint32_t lds_u32[];
   uint2* s_stage = reinterpret_cast<uint2*>(lds_u32);  // 2 * kPartEdges entries
   uint32_t* s_base = lds_u32 + 4 * kPartEdges;         // nbuckets: global start
@@ -207,7 +230,9 @@ __global__ __launch_bounds__(kPartThreads) void bucket_scatter_kernel(
   for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads) s_cnt[i] = 0;
   block_exclusive_scan(totals, pp.nbuckets, s_base, s_part);
 
-  const uint64_t e0 = (uint64_t)blockIdx.x * kPartEdges;
+  const uint2* pairs;
+  uint64_t e0, e1;
+  block_pairs(sg, blockIdx.x, pairs, e0, e1);
   constexpr int K = kPartEdges / kPartThreads;
   uint32_t bk[2 * K], rk[2 * K];
   uint2 uvs[K];
@@ -216,8 +241,8 @@ __global__ __launch_bounds__(kPartThreads) void bucket_scatter_kernel(
     uint64_t e = e0 + t + (uint64_t)k * kPartThreads;
     bk[2 * k] = bk[2 * k + 1] = kEmpty;
     uvs[k] = make_uint2(0, 0);
-    if (e >= nedges) continue;
-    uint2 uv = reinterpret_cast<const uint2*>(pairs)[e];
+    if (e >= e1) continue;
+    uint2 uv = pairs[e];
     if (!entry_ok(uv, n)) continue;
     uvs[k] = uv;
     bk[2 * k] = bucket_of(pp, uv.x);
@@ -1128,7 +1153,7 @@ int ensure_pair_scratch(Stream* s, uint64_t nedges) {
     GS_HIP(hipMalloc(&ws.d_counters, sizeof(uint32_t) * kCounterWords));
     GS_HIP(hipMemsetAsync(ws.d_counters, 0, sizeof(uint32_t) * kCounterWords, s->stream));
   }
-  const uint64_t hist_words = (uint64_t)pp.nbuckets * part_blocks(nedges);
+  const uint64_t hist_words = (uint64_t)pp.nbuckets * (part_blocks(nedges) + kMaxSegs);
   if (ws.cap_buckets < pp.nbuckets || ws.cap_hist < hist_words) {
     GS_HIP(hipStreamSynchronize(s->stream));
     (void)hipFree(ws.d_hist);
@@ -1206,7 +1231,7 @@ static int ensure_pending(Stream* s) {
 static int append_pairs(Stream* s, const uint32_t* src, uint64_t m, uint32_t min_depth,
                         uint32_t depth_cap, bool host_copy) {
   GS_CHECK(!s->group || min_depth == 0, GS_ERR_INVALID, "no subgraph filter on group streams");
-  uint2* dst = reinterpret_cast<uint2*>(s->d_pend[s->pend_buf]) + s->pend_n;
+  uint2* dst = reinterpret_cast<uint2*>(s->d_pend[s->pend_buf]) + s->fill_n;
   if (reinterpret_cast<const uint2*>(src) != dst || min_depth) {
     const uint64_t seed = s->group ? 0 : s->store->view.seed;
     const unsigned grid = (unsigned)std::min<uint64_t>((m + 255) / 256, 4096);
@@ -1216,6 +1241,7 @@ static int append_pairs(Stream* s, const uint32_t* src, uint64_t m, uint32_t min
     GS_HIP(hipGetLastError());
   }
   if (host_copy) GS_HIP(hipEventRecord(s->copied, s->copy_stream));
+  s->fill_n += m;
   s->pend_n += m;
   return GS_OK;
 }
@@ -1235,28 +1261,37 @@ static int process_pending(Stream* s) {
   const uint32_t nverts = stream_vertices(s);
   const uint64_t nedges = s->pend_n;
   const int b = s->pend_buf;
-  const uint32_t nblk = part_blocks(nedges);
+  PairSegs sg{};
+  uint32_t nblk = 0;
+  auto add_seg = [&](const uint32_t* p, uint64_t m) {
+    sg.ptr[sg.nseg] = reinterpret_cast<const uint2*>(p);
+    sg.n[sg.nseg] = m;
+    sg.blk0[sg.nseg] = nblk;
+    nblk += part_blocks(m);
+    sg.nseg++;
+  };
+  if (s->fill_n) add_seg(s->d_pend[b], s->fill_n);
+  for (const auto& z : s->zsegs) add_seg(z.first, z.second);
+  sg.blk0[sg.nseg] = nblk;
   const size_t sort_lds = sizeof(uint32_t) * 3 * ((size_t)1 << pp.local_bits);
   int rc;
   if ((rc = ensure_dyn_lds((const void*)bucket_count_kernel, sizeof(uint32_t) * pp.nbuckets)) ||
       (rc = ensure_dyn_lds((const void*)bucket_scatter_kernel, scatter_lds_bytes(pp.nbuckets))) ||
       (rc = ensure_dyn_lds((const void*)bucket_sort_kernel, sort_lds)))
     return rc;
-  uint2* pend = reinterpret_cast<uint2*>(s->d_pend[b]);
   if (s->copies_pending) {  // every host copy into the buffer has landed
     GS_HIP(hipStreamWaitEvent(s->stream, s->copied, 0));
     s->copies_pending = false;
   }
   hipLaunchKernelGGL(bucket_count_kernel, dim3(nblk), dim3(kPartThreads),
-                     sizeof(uint32_t) * pp.nbuckets, s->stream, pend, nedges, nverts, pp, nblk,
-                     ws.d_hist, ws.d_counters);
+                     sizeof(uint32_t) * pp.nbuckets, s->stream, sg, nverts, pp, nblk, ws.d_hist,
+                     ws.d_counters);
   GS_HIP(hipGetLastError());
   hipLaunchKernelGGL(bucket_offsets_kernel, dim3(pp.nbuckets), dim3(kOffThreads), 0, s->stream,
                      ws.d_hist, nblk, ws.d_totals);
   GS_HIP(hipGetLastError());
   hipLaunchKernelGGL(bucket_scatter_kernel, dim3(nblk), dim3(kPartThreads),
-                     scatter_lds_bytes(pp.nbuckets), s->stream,
-                     reinterpret_cast<const uint32_t*>(pend), nedges, nverts, pp, nblk,
+                     scatter_lds_bytes(pp.nbuckets), s->stream, sg, nverts, pp, nblk,
                      ws.d_totals, ws.d_hist, ws.d_entries);
   GS_HIP(hipGetLastError());
   GS_HIP(hipEventRecord(s->pend_free[b], s->stream));  // the fill buffer has been read
@@ -1269,6 +1304,8 @@ static int process_pending(Stream* s) {
                      s->group ? (uint32_t)s->group->members.size() : 1u);
   GS_HIP(hipGetLastError());
   s->pend_n = 0;
+  s->fill_n = 0;
+  s->zsegs.clear();
   s->pend_buf ^= 1;
   return launch_aggregate(s, ws.d_items, ws.cls, ws.d_counters + kClassBase, ws.d_sorted,
                           2 * nedges);
@@ -1280,10 +1317,17 @@ int submit_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uin
   std::lock_guard<std::mutex> lk(s->mu);
   int rc = ensure_pending(s);
   if (rc) return rc;
-  // (the launch that last read the fill buffer ran on this same stream)
+  // (the launch that last read the fill buffer ran on this same stream). Long unfiltered
+  // batches are read in place by the grouping kernels (the caller keeps them valid until a
+  // flush or sync); the rest is copied into the fill buffer.
   for (uint64_t e0 = 0; e0 < nedges;) {
     const uint64_t m = std::min(nedges - e0, s->pend_cap - s->pend_n);
-    if ((rc = append_pairs(s, d_pairs + 2 * e0, m, min_depth, depth_cap, false))) return rc;
+    if (min_depth == 0 && m >= kZeroCopyEdges && s->zsegs.size() + 1 < (size_t)kMaxSegs) {
+      s->zsegs.emplace_back(d_pairs + 2 * e0, m);
+      s->pend_n += m;
+    } else if ((rc = append_pairs(s, d_pairs + 2 * e0, m, min_depth, depth_cap, false))) {
+      return rc;
+    }
     e0 += m;
     if (s->pend_n == s->pend_cap && (rc = process_pending(s))) return rc;
   }
@@ -1328,7 +1372,7 @@ int submit_pairs_host(Stream* s, const uint32_t* pairs, uint64_t nedges, uint32_
     }
     // the fill buffer may still be read by the launch that processed it two buffers back
     if ((rc = wait_fill_buffer(s))) return rc;
-    uint32_t* dst = s->d_pend[s->pend_buf] + 2 * s->pend_n;
+    uint32_t* dst = s->d_pend[s->pend_buf] + 2 * s->fill_n;
     GS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->copy_stream));
     if (!pinned) {
       GS_HIP(hipEventRecord(s->stage_free, s->copy_stream));

@@ -1,5 +1,6 @@
 """GPU parity of the sketch update path: every bucket the HIP kernels write must equal the CPU
 oracle bit for bit (integer XOR arithmetic, no tolerance)."""
+import ctypes
 import json
 import os
 
@@ -142,6 +143,39 @@ def test_out_of_range_vertex_is_rejected():
         s = st.stream()
         with pytest.raises(gs.GsError):
             s.submit_pairs(np.array([[1, 100]], dtype=np.uint32))
+
+
+def test_mixed_submissions_in_one_launch_and_caller_memory_untouched():
+    # one launch holds a host batch (copied into the fill buffer), a short device batch (copied)
+    # and long device batches (read in place, one of them holding out-of-range pairs); the
+    # out-of-range pairs are dropped and reported at the flush, the caller's memory is not written
+    rng = np.random.default_rng(13)
+    n = 4000
+    host = rng.integers(0, n, size=(70000, 2), dtype=np.uint32)
+    short = rng.integers(0, n, size=(5000, 2), dtype=np.uint32)
+    long1 = rng.integers(0, n, size=(300000, 2), dtype=np.uint32)
+    long2 = rng.integers(0, n, size=(280000, 2), dtype=np.uint32)
+    bad_rows = rng.choice(len(long2), size=50, replace=False)
+    long2_bad = long2.copy()
+    long2_bad[bad_rows, 1] = n + 7
+    with gs.SketchStore(n, 2024) as st:
+        s = st.stream()
+        s.set_tuning(launch_edges=1 << 21)  # every submission below lands in one launch
+        bufs = [gs.DeviceBuffer(a) for a in (short, long1, long2_bad)]
+        s.submit_pairs(host)
+        s.submit_pairs_device(bufs[0].offset(0), len(short))
+        s.submit_pairs_device(bufs[1].offset(0), len(long1))
+        s.submit_pairs_device(bufs[2].offset(0), len(long2_bad))
+        with pytest.raises(gs.GsError, match="dropped"):
+            s.flush()
+        back = np.zeros_like(long2_bad)
+        gs.lib.gs_memcpy_d2h(back.ctypes.data_as(ctypes.c_void_p), bufs[2].ptr, back.nbytes)
+        got = st.export_buckets()
+        for b in bufs:
+            b.free()
+    assert np.array_equal(back, long2_bad)
+    good = np.delete(long2, bad_rows, axis=0)
+    assert np.array_equal(got, oracle_store(n, 2024, np.concatenate([host, short, long1, good])))
 
 
 def test_multiple_streams_share_a_store():
