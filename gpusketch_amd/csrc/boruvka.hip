@@ -109,15 +109,11 @@ __global__ void init_query_kernel(uint32_t n, uint32_t* parent, uint32_t* comp, 
   }
 }
 
-// Rounds are enqueued several at a time without looking at the host; these two kernels
-// bracket a round on the device. begin: clear the round's modified flag. end (after compress):
-// a round that modified nothing leaves the partition final (the query took `round` rounds); a
-// modified round needs another, and past the last sample that is out of samples (the reference
-// throws when it samples past num_samples, src/sketch.cpp:110-114).
-__global__ void begin_round_kernel(uint32_t* flags) {
-  if (threadIdx.x == 0 && !flags[kFlagDone]) flags[0] = 0;
-}
-
+// Rounds are enqueued several at a time without looking at the host; this kernel ends a round
+// on the device (after compress): a round that modified nothing leaves the partition final (the
+// query took `round` rounds); a modified round needs another, and past the last sample that is
+// out of samples (the reference throws when it samples past num_samples,
+// src/sketch.cpp:110-114); otherwise the next round starts with its modified flag clear.
 __global__ void end_round_kernel(uint32_t* flags, uint32_t num_samples) {
   if (threadIdx.x != 0 || flags[kFlagDone]) return;
   const uint32_t round = flags[kFlagRound];
@@ -130,6 +126,7 @@ __global__ void end_round_kernel(uint32_t* flags, uint32_t num_samples) {
     flags[kFlagRounds] = round + 1;
   } else {
     flags[kFlagRound] = round + 1;
+    flags[0] = 0;
   }
 }
 
@@ -144,7 +141,8 @@ __global__ void zero_acc_kernel(Bucket16* acc, uint64_t count) {
 }
 
 // Sketch::sample (src/sketch.cpp:110-132) over buckets held one per lane: lanes 0..bpc-1 hold
-// depths 0..B-1 of the sampled column, lane bpc the deterministic bucket. ZERO if that bucket is
+// depths 0..bpc-1 of the sampled column (bpc = B, or fewer for a first pass over the shallow
+// depths, which then answers FAIL for "none of these"), lane bpc the deterministic bucket. ZERO if that bucket is
 // empty, GOOD on it if it is good, else GOOD on the first good depth, else FAIL. Every lane takes
 // part; the result and *idx are wave-uniform.
 __device__ __forceinline__ int wave_sample(const Bucket16 b, uint32_t lane, uint32_t bpc,
@@ -241,7 +239,11 @@ __global__ __launch_bounds__(kQThreads) void round0_wide_kernel(StoreView st, ui
 }
 
 // Round 0: every vertex samples its own column col0 (run_round_zero, src/cc_sketch_alg.cpp:
-// 250-271). A wave takes 64 consecutive vertices, four at a time, one bucket per lane.
+// 250-271). A wave takes 64 consecutive vertices, four at a time, one bucket per lane. Each
+// bucket of the column is a 128-byte line of its own (depth-major rows), so the first pass reads
+// the deterministic bucket and depths 0..15 only; a vertex with no good bucket among them reads
+// its deeper depths in a second pass (the scan order stays the reference's).
+constexpr uint32_t kRound0Depths = 16;
 __global__ __launch_bounds__(kQThreads) void round0_kernel(StoreView st, uint32_t col0,
                                                            uint32_t* parent, uint32_t* vflags,
                                                            uint32_t* flags, uint32_t* forest,
@@ -250,7 +252,7 @@ __global__ __launch_bounds__(kQThreads) void round0_kernel(StoreView st, uint32_
   const uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64 * 64;
   if (v0 >= st.n) return;
   const uint32_t cnt = min(64u, st.n - v0);
-  const uint32_t jj = lane <= st.bpc ? lane : 0;
+  const uint32_t nd = min(st.bpc, kRound0Depths);  // depths of the first pass, det in lane nd
   int my_res = -1;  // the sample of vertex v0 + lane, handled by its own lane at the end
   uint64_t my_idx = 0;
   for (uint32_t i0 = 0; i0 < cnt; i0 += 4) {
@@ -258,13 +260,28 @@ __global__ __launch_bounds__(kQThreads) void round0_kernel(StoreView st, uint32_
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const uint32_t vi = v0 + min(i0 + q, cnt - 1);
-      bv[q] = jj < st.bpc ? st.rows[(size_t)vi * st.vstride + (size_t)jj * st.cols + col0] : st.det[vi];
+      bv[q] = lane > nd    ? Bucket16{0, 0}
+              : lane < nd ? st.rows[(size_t)vi * st.vstride + (size_t)lane * st.cols + col0]
+                          : st.det[vi];
     }
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       if (i0 + q >= cnt) break;
       uint64_t idx = 0;
-      const int res = wave_sample(bv[q], lane, st.bpc, st.seed, &idx);
+      int res = wave_sample(bv[q], lane, nd, st.seed, &idx);
+      if (res == 2 && nd < st.bpc) {  // wave-uniform: depths nd..B-1, one per lane
+        const uint32_t dd = nd + lane;
+        const Bucket16 b2 = dd < st.bpc
+            ? st.rows[(size_t)(v0 + i0 + q) * st.vstride + (size_t)dd * st.cols + col0]
+            : Bucket16{0, 0};
+        const uint64_t good = __ballot(dd < st.bpc && b2.gamma == xxh3_len8(b2.alpha, st.seed));
+        if (good) {
+          const int l = __builtin_ctzll(good);
+          idx = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(b2.alpha >> 32), l) << 32) |
+                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b2.alpha, l);
+          res = 0;
+        }
+      }
       if (lane == i0 + q) {
         my_res = res;
         my_idx = idx;
@@ -289,46 +306,43 @@ __global__ __launch_bounds__(kPickSamples) void pick_big_kernel(uint32_t n,
                                                                 int symmetric, uint32_t* flags) {
   if (flags[kFlagDone]) return;
   __shared__ uint32_t s_r[kPickSamples];
-  __shared__ uint32_t s_key[kPickSamples];
+  __shared__ uint32_t s_cand[kPickSamples];
+  __shared__ uint32_t s_ncand;
   const uint32_t t = threadIdx.x;
   const uint32_t v = (uint32_t)(((uint64_t)t * n) / kPickSamples + n / (2 * kPickSamples));
   uint32_t r = comp[v];
   if (vflags[r] & kVDone) r = kNoGiant;
   s_r[t] = r;
+  if (t == 0) s_ncand = 0;
   __syncthreads();
-  // samples in this supernode, counted at its first sample only
+  // samples in this supernode, counted at its first sample only; a candidate key orders by
+  // count, then by sample position (unique)
   uint32_t cnt = 0, first = t;
   for (uint32_t i = 0; i < kPickSamples; i++) {
     const bool same = s_r[i] == r;
     cnt += same;
     if (same && i < first) first = i;
   }
-  uint32_t key = (r != kNoGiant && first == t && cnt >= kMinBig) ? (cnt << 16) | t : 0u;
-  for (int k = 0; k < kBig; k++) {
-    s_key[t] = key;
-    __syncthreads();
-    for (uint32_t o = kPickSamples / 2; o > 0; o >>= 1) {
-      if (t < o) s_key[t] = max(s_key[t], s_key[t + o]);
-      __syncthreads();
+  const uint32_t key = (r != kNoGiant && first == t && cnt >= kMinBig) ? (cnt << 16) | t : 0u;
+  if (key) s_cand[atomicAdd(&s_ncand, 1u)] = key;  // at most kPickSamples / kMinBig = 64
+  __syncthreads();
+  if (t >= 64) return;
+  // wave 0: bitonic sort of the candidates over 64 lanes, largest first
+  const uint32_t nc = s_ncand;
+  uint32_t x = t < nc ? s_cand[t] : 0u;
+  for (uint32_t k = 2; k <= 64; k <<= 1)
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      const uint32_t y = (uint32_t)__shfl_xor((int)x, (int)j);
+      x = (((t & j) == 0) == ((t & k) == 0)) ? max(x, y) : min(x, y);
     }
-    const uint32_t best = s_key[0];
-    __syncthreads();
-    if (k > 0 && best == 0) {  // no further candidates (block-uniform)
-      if (t == 0)
-        for (int q = k; q < kBig; q++) flags[kFlagBig + q] = kNoGiant;
-      break;
-    }
-    if (t == 0) {
-      const uint32_t root = best ? s_r[best & 0xFFFFu] : kNoGiant;
-      if (k == 0) {
-        const bool giant = best && symmetric && (best >> 16) > kPickSamples / 2;
-        flags[3] = giant ? root : kNoGiant;
-        flags[kFlagBig] = giant ? kNoGiant : root;  // the giant is formed from the rest
-      } else {
-        flags[kFlagBig + k] = root;
-      }
-    }
-    if (key == best) key = 0;
+  if (t >= (uint32_t)kBig) return;
+  const uint32_t root = x ? s_r[x & 0xFFFFu] : kNoGiant;
+  if (t == 0) {
+    const bool giant = x && symmetric && (x >> 16) > kPickSamples / 2;
+    flags[3] = giant ? root : kNoGiant;
+    flags[kFlagBig] = giant ? kNoGiant : root;  // the giant is formed from the rest
+  } else {
+    flags[kFlagBig + t] = root;
   }
 }
 
@@ -340,6 +354,7 @@ __global__ __launch_bounds__(kPickSamples) void pick_big_kernel(uint32_t n,
 // bpart[wave][k]; with a giant, every vertex outside it (finished supernodes included) also adds
 // into cpart[wave]; the other supernodes take atomics.
 __global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st, uint32_t cps,
+                                                        uint32_t* __restrict__ parent,
                                                         const uint32_t* __restrict__ comp,
                                                         const uint32_t* __restrict__ vflags,
                                                         const uint32_t* __restrict__ flags,
@@ -351,6 +366,12 @@ __global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st, uint32_t c
   const uint32_t sw = cps * st.bpc, w = sw + 1;
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t slot = blockIdx.y * 64 + lane;
+  // the previous round's unions, compressed: every vertex links straight to its root again
+  // (nothing reads `parent` before this round's sample_kernel)
+  if (blockIdx.y == 0) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < st.n) parent[u] = comp[u];
+  }
   const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
   const uint32_t v0 = gwave * 64;
   const uint32_t giant = flags[3];
@@ -648,18 +669,15 @@ static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t
   hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
                      st->d_vflags, st->d_flags);
   hipLaunchKernelGGL(end_round_kernel, dim3(1), dim3(64), 0, q, st->d_flags, st->num_samples);
-  hipLaunchKernelGGL(finalize_parent_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
-                     st->d_flags);
   GS_HIP(hipGetLastError());
   const int symmetric = st->symmetric.load() ? 1 : 0;
   for (uint32_t enqueued = 1;; enqueued += kRoundsAhead) {
     for (uint32_t k = 0; k < kRoundsAhead; k++) {
-      hipLaunchKernelGGL(begin_round_kernel, dim3(1), dim3(64), 0, q, st->d_flags);
       hipLaunchKernelGGL(pick_big_kernel, dim3(1), dim3(kPickSamples), 0, q, n, st->d_comp,
                          st->d_vflags, symmetric, st->d_flags);
       hipLaunchKernelGGL(acc_kernel, dim3(grid.x, (w + 63) / 64), block, 0, q, v, cps,
-                         st->d_comp, st->d_vflags, st->d_flags, st->d_acc, st->d_cpart,
-                         st->d_bpart);
+                         st->d_parent, st->d_comp, st->d_vflags, st->d_flags, st->d_acc,
+                         st->d_cpart, st->d_bpart);
       hipLaunchKernelGGL(big_acc_kernel, dim3(w, kBig + 1), dim3(kQThreads), 0, q, st->d_flags,
                          st->d_cpart, st->d_bpart, nwaves, w, st->d_acc);
       if (wide)
@@ -671,8 +689,6 @@ static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t
       hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
                          st->d_vflags, st->d_flags);
       hipLaunchKernelGGL(end_round_kernel, dim3(1), dim3(64), 0, q, st->d_flags, st->num_samples);
-      hipLaunchKernelGGL(finalize_parent_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
-                         st->d_flags);
     }
     GS_HIP(hipGetLastError());
     GS_HIP(hipMemcpyAsync(st->h_flags, st->d_flags, sizeof(uint32_t) * kFlagWords,

@@ -140,6 +140,31 @@ def test_cc_config1_full_size():
     assert np.array_equal(labels, orc.dsu_components(n, net_edges(pairs)))
 
 
+def test_cc_hubs_sampled_past_the_shallow_depths():
+    # two hubs of 2^17 leaves each, joined by one edge, leaves also in short paths: a hub's first
+    # good bucket in round 0 lies at depth ~17, past the 16 shallow depths of the first read
+    # pass, so its sample comes from the deep pass; every forest edge must be a live edge
+    half = 1 << 17
+    n = 2 * half + 2
+    leaves_a = np.arange(2, 2 + half, dtype=np.uint32)
+    leaves_b = np.arange(2 + half, n, dtype=np.uint32)
+    pairs = np.concatenate([
+        np.stack([np.zeros(half, np.uint32), leaves_a], 1),
+        np.stack([np.ones(half, np.uint32), leaves_b], 1),
+        np.array([[0, 1]], dtype=np.uint32),
+        np.stack([leaves_a[:-1:2], leaves_a[1::2]], 1)[:1000],
+    ]).astype(np.uint32)
+    with gs.SketchStore(n, 0xB16) as st:
+        assert st.geom.bkt_per_col > 16
+        s = st.stream()
+        s.submit_pairs(pairs)
+        s.flush()
+        labels, ncc, rounds, forest = st.query_cc(want_forest=True)
+    assert ncc == 1
+    net = net_edges(pairs)
+    check_forest(n, forest, net, labels)
+
+
 def test_out_of_samples_reported():
     n = 64
     path = np.array([[i, i + 1] for i in range(n - 1)], dtype=np.uint32)
