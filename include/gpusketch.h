@@ -116,7 +116,10 @@ int gs_submit_vertex_batches_filtered(gs_stream* s, const uint32_t* src, const u
  * and CCSketchAlg::update applies them (src/cc_sketch_alg.cpp:154-160). Grouping by vertex
  * happens on the device. Host memory; asynchronous after staging. */
 int gs_submit_edge_pairs(gs_stream* s, const uint32_t* pairs, uint64_t nedges);
-/* Same, input already resident in device memory (must stay valid until gs_flush). */
+/* Same, input already resident in device memory. It must stay valid and unchanged until the
+ * stream's next gs_flush (or gs_sync): long batches (2^18 edges and more) are read in place by
+ * the launch that applies them, shorter ones are copied on the stream. The library never writes
+ * it; pairs naming a vertex >= n are skipped and reported by that flush. */
 int gs_submit_edge_pairs_device(gs_stream* s, const uint32_t* d_pairs, uint64_t nedges);
 /* Subgraph-filtered variant for the min-cut path: only edges whose subgraph depth
  * get_index_depth(idx, seed, depth_cap) (src/mc_gpu_sketch_alg.cpp:160) is >= min_depth are
