@@ -164,28 +164,33 @@ __device__ __forceinline__ int wave_sample(const Bucket16 b, uint32_t lane, uint
   return 0;
 }
 
-// Sample slots of a sketch with cps columns per sample: slot j < sw = cps*B of sample r is column
-// r*cps + j / B at depth j % B (the scan order of Sketch::sample, src/sketch.cpp:121-128); slot
-// sw is the deterministic bucket. Offset of slot j inside a vertex's rows.
+// Sample slots of a sketch with cps columns per sample, in storage order: slot j < sw = cps*B of
+// sample r is column r*cps + j % cps at depth j / cps (depth-major like the rows, so the cps
+// columns of one depth are adjacent in memory), slot sw is the deterministic bucket. Offset of
+// slot j inside a vertex's rows, and its position in the scan order of Sketch::sample
+// (src/sketch.cpp:121-128: column by column, depths ascending).
 __device__ __forceinline__ size_t slot_offset(const StoreView& st, uint32_t cps, uint32_t r,
                                               uint32_t j) {
-  const uint32_t i = j / st.bpc;
-  return (size_t)(j - i * st.bpc) * st.cols + (size_t)r * cps + i;
+  const uint32_t d = j / cps;
+  return (size_t)d * st.cols + (size_t)r * cps + (j - d * cps);
+}
+__device__ __forceinline__ uint32_t slot_scan_pos(uint32_t cps, uint32_t bpc, uint32_t j) {
+  const uint32_t d = j / cps;
+  return (j - d * cps) * bpc + d;
 }
 
-// Sketch::sample over a row of sw + 1 slots (slot sw the deterministic bucket) that may be wider
-// than a wave: the deterministic bucket first, then the slots in scan order, 4 x 64 loads in
-// flight at a time, stopping at the first group holding a good bucket. load(j) reads slot j;
-// the result and *idx are wave-uniform.
+// Sketch::sample over a row of sw + 1 slots in storage order (slot sw the deterministic bucket)
+// that may be wider than a wave: ZERO or GOOD on the deterministic bucket first, else GOOD on the
+// good slot earliest in scan order (a wave minimum of scan positions), else FAIL. The
+// deterministic bucket and 4 x 64 slots are loaded together. load(j) reads slot j; the result
+// and *idx are wave-uniform.
 template <class Load>
 __device__ __forceinline__ int wave_sample_wide(Load load, uint32_t lane, uint32_t sw,
-                                                uint64_t seed, uint64_t* idx) {
+                                                uint32_t cps, uint32_t bpc, uint64_t seed,
+                                                uint64_t* idx) {
   const Bucket16 det = load(sw);
-  if (det.alpha == 0 && det.gamma == 0) return 1;
-  if (det.gamma == xxh3_len8(det.alpha, seed)) {
-    *idx = det.alpha;
-    return 0;
-  }
+  uint32_t best = 0xFFFFFFFFu;  // this lane's lowest scan position of a good slot
+  uint64_t best_idx = 0;
   for (uint32_t j0 = 0; j0 < sw; j0 += 4 * 64) {
     Bucket16 b[4];
 #pragma unroll
@@ -193,19 +198,32 @@ __device__ __forceinline__ int wave_sample_wide(Load load, uint32_t lane, uint32
       const uint32_t j = j0 + q * 64 + lane;
       b[q] = load(j < sw ? j : 0u);
     }
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const uint32_t j = j0 + q * 64 + lane;
-      const uint64_t good = __ballot(j < sw && b[q].gamma == xxh3_len8(b[q].alpha, seed));
-      if (good) {
-        const int l = __builtin_ctzll(good);
-        *idx = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(b[q].alpha >> 32), l) << 32) |
-               (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b[q].alpha, l);
+    if (j0 == 0) {
+      if (det.alpha == 0 && det.gamma == 0) return 1;
+      if (det.gamma == xxh3_len8(det.alpha, seed)) {
+        *idx = det.alpha;
         return 0;
       }
     }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint32_t j = j0 + q * 64 + lane;
+      if (j < sw && b[q].gamma == xxh3_len8(b[q].alpha, seed)) {
+        const uint32_t p = slot_scan_pos(cps, bpc, j);
+        if (p < best) {
+          best = p;
+          best_idx = b[q].alpha;
+        }
+      }
+    }
   }
-  return 2;
+  uint32_t m = best;
+  for (int o = 32; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, o));
+  if (m == 0xFFFFFFFFu) return 2;
+  const int l = __builtin_ctzll(__ballot(best == m));
+  *idx = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(best_idx >> 32), l) << 32) |
+         (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)best_idx, l);
+  return 0;
 }
 
 // Round 0, sketches with several columns per sample: every vertex samples its own sample col0,
@@ -229,7 +247,7 @@ __global__ __launch_bounds__(kQThreads) void round0_wide_kernel(StoreView st, ui
       return j < sw ? vrows[slot_offset(st, cps, col0, j)] : st.det[vi];
     };
     uint64_t idx = 0;
-    const int res = wave_sample_wide(load, lane, sw, st.seed, &idx);
+    const int res = wave_sample_wide(load, lane, sw, cps, st.bpc, st.seed, &idx);
     if (lane == i) {
       my_res = res;
       my_idx = idx;
@@ -544,7 +562,7 @@ __global__ __launch_bounds__(kQThreads) void sample_wide_kernel(StoreView st, ui
     Bucket16* row = acc + (size_t)(v0 + i) * w;
     auto load = [&](uint32_t j) -> Bucket16 { return row[j]; };
     uint64_t idx = 0;
-    const int res = wave_sample_wide(load, lane, sw, st.seed, &idx);
+    const int res = wave_sample_wide(load, lane, sw, cps, st.bpc, st.seed, &idx);
     for (uint32_t j = lane; j < w; j += 64) row[j] = Bucket16{0, 0};
     if (lane == i) {
       my_res = res;
