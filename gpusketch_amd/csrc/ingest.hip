@@ -286,6 +286,31 @@ __device__ __forceinline__ uint32_t graph_of(const uint32_t* __restrict__ bases,
   return lo;
 }
 
+// LDS atomicAdd of `inc` at cnt[key] for every active lane, returning each lane's old value as
+// if the lanes had added one after another; the lanes holding the first active lane's key share
+// one atomic (a hub vertex fills whole waves with one key, which would otherwise serialize on
+// one LDS address)
+__device__ __forceinline__ uint32_t wave_aggregated_add(uint32_t* cnt, uint32_t key, uint32_t inc) {
+  const uint32_t k0 = __builtin_amdgcn_readfirstlane(key);
+  const uint64_t same = __ballot(key == k0);
+  const uint32_t below = lane_below(same);
+  if (key != k0) return atomicAdd(&cnt[key], inc);
+  uint32_t b0 = 0;
+  if (below == 0) b0 = atomicAdd(&cnt[k0], inc * (uint32_t)__popcll(same));
+  // the leader is the lowest set lane of `same`
+  return (uint32_t)__builtin_amdgcn_readlane((int)b0, __builtin_ctzll(same)) + inc * below;
+}
+
+constexpr uint32_t kSortTile = kSortBatch * kSortThreads;  // entries per output tile
+constexpr uint32_t kTileBins = kSortThreads;               // local ids the tiled output handles
+
+// dynamic LDS of bucket_sort_kernel for 2^local_bits local ids
+static size_t sort_lds_bytes(uint32_t local_bits) {
+  const size_t bins = (size_t)1 << local_bits;
+  if (bins > kTileBins) return sizeof(uint32_t) * 3 * bins;
+  return sizeof(uint32_t) * (6 * bins + 2 * kSortTile);
+}
+
 __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
     const uint2* __restrict__ entries, PartParams pp, const uint32_t* __restrict__ counts,
     uint32_t* __restrict__ sorted, WorkItem* __restrict__ items, ItemClasses cls, uint32_t chunk,
@@ -385,31 +410,83 @@ __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
     }
   }
   __syncthreads();
-  for (uint32_t i0 = t; i0 < cnt; i0 += kSortBatch * kSortThreads) {
+  if (bins > kTileBins) {
+    // wide local ids: every destination goes straight to its place (scattered 4-byte stores)
+    for (uint32_t i0 = t; i0 < cnt; i0 += kSortTile) {
+      uint2 ev[kSortBatch];
+#pragma unroll
+      for (int k = 0; k < kSortBatch; k++) {
+        const uint32_t i = i0 + k * kSortThreads;
+        ev[k] = entries[base + min(i, cnt - 1)];
+      }
+#pragma unroll
+      for (int k = 0; k < kSortBatch; k++) {
+        if (i0 + k * kSortThreads >= cnt) break;
+        const uint32_t key = hashed_id(pp, ev[k].x) & lmask;
+        sorted[base + wave_aggregated_add(s_off, key, 1u)] = ev[k].y;
+      }
+    }
+    return;
+  }
+  // Tiles of kSortTile entries: each tile is counting-sorted by local id in LDS, every id
+  // reserves its run of the bucket's output (the id's cursor s_off advances by the tile's count),
+  // and the tile is written out in LDS order, so consecutive lanes store consecutive positions of
+  // one id's run instead of scattering 4-byte stores over the whole bucket.
+  uint32_t* t_cnt = s_off + bins;          // per id: entries in this tile
+  uint32_t* t_off = t_cnt + bins;          // per id: start of its run in the tile
+  uint32_t* t_cur = t_off + bins;          // per id: cursor inside the tile
+  uint32_t* t_gb = t_cur + bins;           // per id: start of its run in the bucket
+  uint32_t* st_val = t_gb + bins;          // the tile in id order: destination
+  uint32_t* st_pos = st_val + kSortTile;   // and its position in the bucket
+  __shared__ uint32_t s_wsum[kSortThreads / 64];
+  const uint32_t lane = t & 63, wid = t >> 6;
+  for (uint32_t tile0 = 0; tile0 < cnt; tile0 += kSortTile) {
+    const uint32_t i0 = tile0 + t;
     uint2 ev[kSortBatch];
+    uint32_t key[kSortBatch];
 #pragma unroll
     for (int k = 0; k < kSortBatch; k++) {
       const uint32_t i = i0 + k * kSortThreads;
       ev[k] = entries[base + min(i, cnt - 1)];
     }
+    for (uint32_t i = t; i < bins; i += kSortThreads) t_cnt[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kSortBatch; k++) {
+      key[k] = hashed_id(pp, ev[k].x) & lmask;
+      if (i0 + k * kSortThreads >= cnt) break;
+      (void)wave_aggregated_add(t_cnt, key[k], 1u);
+    }
+    __syncthreads();
+    // exclusive scan of the tile counts (bins <= kSortThreads: one id per thread)
+    const uint32_t c = t < bins ? t_cnt[t] : 0u;
+    uint32_t x = c;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)x, o);
+      if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63) s_wsum[wid] = x;
+    __syncthreads();
+    if (t < bins) {
+      uint32_t excl = x - c;
+      for (uint32_t w = 0; w < wid; w++) excl += s_wsum[w];
+      t_off[t] = excl;
+      t_cur[t] = excl;
+      t_gb[t] = s_off[t];
+      s_off[t] += c;
+    }
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < kSortBatch; k++) {
       if (i0 + k * kSortThreads >= cnt) break;
-      const uint32_t key = hashed_id(pp, ev[k].x) & lmask;
-      const uint32_t k0 = __builtin_amdgcn_readfirstlane(key);
-      const uint64_t same = __ballot(key == k0);
-      const uint32_t below = lane_below(same);
-      uint32_t pos;
-      if (key != k0) {
-        pos = atomicAdd(&s_off[key], 1u);
-      } else {
-        uint32_t b0 = 0;
-        if (below == 0) b0 = atomicAdd(&s_off[k0], (uint32_t)__popcll(same));
-        // the leader is the lowest set lane of `same`
-        pos = (uint32_t)__builtin_amdgcn_readlane((int)b0, __builtin_ctzll(same)) + below;
-      }
-      sorted[base + pos] = ev[k].y;
+      const uint32_t lpos = wave_aggregated_add(t_cur, key[k], 1u);
+      st_val[lpos] = ev[k].y;
+      st_pos[lpos] = t_gb[key[k]] + (lpos - t_off[key[k]]);
     }
+    __syncthreads();
+    const uint32_t ntile = min(kSortTile, cnt - tile0);
+    for (uint32_t j = t; j < ntile; j += kSortThreads) sorted[base + st_pos[j]] = st_val[j];
+    __syncthreads();
   }
 }
 
@@ -1273,7 +1350,7 @@ static int process_pending(Stream* s) {
   if (s->fill_n) add_seg(s->d_pend[b], s->fill_n);
   for (const auto& z : s->zsegs) add_seg(z.first, z.second);
   sg.blk0[sg.nseg] = nblk;
-  const size_t sort_lds = sizeof(uint32_t) * 3 * ((size_t)1 << pp.local_bits);
+  const size_t sort_lds = sort_lds_bytes(pp.local_bits);
   int rc;
   if ((rc = ensure_dyn_lds((const void*)bucket_count_kernel, sizeof(uint32_t) * pp.nbuckets)) ||
       (rc = ensure_dyn_lds((const void*)bucket_scatter_kernel, scatter_lds_bytes(pp.nbuckets))) ||
