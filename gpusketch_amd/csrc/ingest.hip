@@ -160,10 +160,11 @@ __global__ __launch_bounds__(kPartThreads) void bucket_count_kernel(
 }
 
 // exclusive prefix of counts[0..nb) into s_off (LDS; may be counts itself), returns the total;
-// blockDim threads
+// blockDim threads (a multiple of 64). Each thread sums a run of counts, the runs are scanned
+// within each wave by shuffles, and the wave totals (s_part, one per wave) once more.
 __device__ uint32_t block_exclusive_scan(const uint32_t* counts, uint32_t nb, uint32_t* s_off,
                                          uint32_t* s_part) {
-  const uint32_t t = threadIdx.x, nt = blockDim.x;
+  const uint32_t t = threadIdx.x, nt = blockDim.x, lane = t & 63, wid = t >> 6;
   const uint32_t per = (nb + nt - 1) / nt;
   const uint32_t b0 = min(nb, t * per), b1 = min(nb, b0 + per);
   uint32_t sum = 0;
@@ -172,17 +173,20 @@ __device__ uint32_t block_exclusive_scan(const uint32_t* counts, uint32_t nb, ui
     s_off[b] = sum;
     sum += c;
   }
-  s_part[t] = sum;
-  __syncthreads();
-  for (uint32_t o = 1; o < nt; o <<= 1) {
-    uint32_t add = t >= o ? s_part[t - o] : 0;
-    __syncthreads();
-    s_part[t] += add;
-    __syncthreads();
+  uint32_t x = sum;  // inclusive scan over the wave
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, o);
+    if (lane >= (uint32_t)o) x += y;
   }
-  uint32_t base = t ? s_part[t - 1] : 0;
+  if (lane == 63) s_part[wid] = x;
+  __syncthreads();
+  uint32_t base = x - sum, total = 0;
+  for (uint32_t w = 0; w < nt / 64; w++) {
+    const uint32_t p = s_part[w];
+    base += w < wid ? p : 0u;
+    total += p;
+  }
   for (uint32_t b = b0; b < b1; b++) s_off[b] += base;
-  uint32_t total = s_part[nt - 1];
   __syncthreads();
   return total;
 }
