@@ -268,7 +268,8 @@ def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64)
-    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=32,
+                    help="untimed batches first (32: one full launch of the kernels)")
     ap.add_argument("--cpu-batches", type=int, default=32, help="stream prefix for the CPU leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pcie-batches", type=int, default=104,
