@@ -125,7 +125,7 @@ struct ItemClasses {
 };
 
 // edges coalesced per launch of the grouping and aggregation kernels (default)
-constexpr uint64_t kLaunchEdges = 1ULL << 25;
+constexpr uint64_t kLaunchEdges = 1ULL << 27;
 
 struct PairScratch {
   uint64_t cap_edges = 0;        // capacity in undirected edges

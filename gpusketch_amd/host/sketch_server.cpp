@@ -175,8 +175,17 @@ void SketchServer::ensure_group() {
   if (group) gs_throw_if(gs_group_destroy(group));
   group = nullptr;
   group_members = members;
-  if (!members.empty())
+  if (!members.empty()) {
     gs_throw_if(gs_group_create(stores.data(), (uint32_t)stores.size(), &group));
+    // a step's host batches are copied while the kernels of the launches before them run; the
+    // default launch grows with the group's vertex count (to 2^27 edges), which would leave a
+    // step's copies and kernels one after the other, so the loop keeps launches at 2^25
+    gs_stream* gsm = nullptr;
+    gs_throw_if(gs_group_stream(group, &gsm));
+    gs_tuning tun{};
+    tun.launch_edges = 1u << 25;
+    gs_throw_if(gs_stream_set_tuning(gsm, &tun));
+  }
 }
 
 void SketchServer::finish(Request& r, RequestResult res) {

@@ -177,10 +177,12 @@ int gs_stream_traffic(gs_stream* s, uint64_t* flushed_buckets, uint64_t* direct_
 /* ---- Tuning ----------------------------------------------------------------------------------
  * Launch parameters of a stream's update path (0 = default): lds_depths = depths of every column
  * kept in the aggregation kernel's LDS delta (1..16, default 9), item_updates = destinations per work
- * item (64..1024, default 512), bucket_bits = vertex-id bits resolved inside one grouping
- * bucket (default log2(n) - 8, at most 13), launch_edges = submitted edge pairs coalesced into
- * one launch of the grouping and aggregation kernels (2^12..2^28; default 64 per vertex of
- * the stream's vertex space, within [2^16, 2^25]; a flush or a query applies a partial launch). The reference fixes its launch shape at compile time
+ * item (64..1024, default 512; 1024 for wide sketches), bucket_bits = vertex-id bits resolved
+ * inside one grouping bucket (default log2(n) - 8, at most 13), launch_edges = submitted edge
+ * pairs coalesced into one launch of the grouping and aggregation kernels (2^12..2^28; default
+ * 256 per vertex of the stream's vertex space, within [2^16, 2^27]; scratch ≈ 40 B per edge of
+ * it; a flush or a query applies a partial launch). The reference fixes its launch shape at
+ * compile time
  * (num_device_threads = 1024, num_batch_per_buffer = 540, include/cc_gpu_sketch_alg.h:16-23). */
 typedef struct gs_tuning {
   uint32_t lds_depths;
