@@ -124,7 +124,8 @@ struct ItemClasses {
   uint64_t off[kClasses];  // start of each class's region in the item array
 };
 
-// edges coalesced per launch of the grouping and aggregation kernels (default)
+// most edges coalesced per launch of the grouping and aggregation kernels by default (the
+// default is 256 per vertex of the stream's vertex space, within [2^16, kLaunchEdges])
 constexpr uint64_t kLaunchEdges = 1ULL << 27;
 
 struct PairScratch {

@@ -1,10 +1,12 @@
 // Sketch update kernels for gfx950.
 //
-// Two entry paths share one aggregation kernel:
+// Two entry paths share the aggregation kernels:
 //   * pre-grouped vertex batches (the contract of the reference's sketchUpdate kernels,
 //     src/cuda_kernel.cu:152-257): the host turns each batch into work items;
 //   * raw undirected pairs (edge batches): the device groups directed entries by vertex in three
 //     kernels (bucket_count, bucket_scatter, bucket_sort) and emits the work items itself.
+// Sketches of up to 64 columns under the default rule use aggregate_kernel; wider ones and the
+// CubeSketch rule (src/sketch.cpp:66-83) use wide_aggregate_kernel (lane = column).
 //
 // aggregate_kernel: one wave64 per work item (<= kChunk destinations of one vertex). For each
 // destination the wave hashes the edge index once for the checksum (XXH3 with the sketch seed)
