@@ -221,8 +221,16 @@ __global__ __launch_bounds__(kOffThreads) void bucket_offsets_kernel(uint32_t* h
   if (t == kOffThreads - 1) totals[blockIdx.x] = s_part[t];
 }
 
+// A directed entry of a bucket: the vertex (from which the sort takes its local id, the low
+// local_bits of the hashed id) and the other endpoint. kPacked (local_bits + nbits <= 32): one
+// u32, local id << nbits | other, half the bytes through K3 and K4; else a uint2 (vertex, other).
+__host__ __device__ __forceinline__ bool packed_entries_fit(const PartParams& pp) {
+  return pp.local_bits + pp.nbits <= 32;
+}
+
 // The block's entries are first placed in LDS in bucket order, then written out so that
 // consecutive lanes write consecutive positions of one bucket's run (coalesced stores).
+template <bool kPacked>
 __global__ __launch_bounds__(kPartThreads) void bucket_scatter_kernel(
     PairSegs sg, uint32_t n, PartParams pp, uint32_t nblk, const uint32_t* __restrict__ totals,
     const uint32_t* __restrict__ hist, uint2* __restrict__ entries) {
@@ -270,8 +278,14 @@ __global__ __launch_bounds__(kPartThreads) void bucket_scatter_kernel(
   __syncthreads();
   for (uint32_t i = t; i < total; i += kPartThreads) {
     const uint2 en = s_stage[i];
-    const uint32_t b = bucket_of(pp, en.x);
-    entries[s_base[b] + (i - s_loc[b])] = en;
+    const uint32_t h = hashed_id(pp, en.x);
+    const uint32_t b = h >> pp.local_bits;
+    const uint32_t pos = s_base[b] + (i - s_loc[b]);
+    if (kPacked)
+      reinterpret_cast<uint32_t*>(entries)[pos] =
+          ((h & ((1u << pp.local_bits) - 1u)) << pp.nbits) | en.y;
+    else
+      entries[pos] = en;
   }
 }
 
@@ -317,6 +331,7 @@ static size_t sort_lds_bytes(uint32_t local_bits) {
   return sizeof(uint32_t) * (6 * bins + 2 * kSortTile);
 }
 
+template <bool kPacked>
 __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
     const uint2* __restrict__ entries, PartParams pp, const uint32_t* __restrict__ counts,
     uint32_t* __restrict__ sorted, WorkItem* __restrict__ items, ItemClasses cls, uint32_t chunk,
@@ -342,20 +357,31 @@ __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
   const uint32_t cnt = counts[b];
   __syncthreads();
   const uint32_t lmask = bins - 1;
+  const uint32_t omask = kPacked ? (1u << pp.nbits) - 1u : 0u;
+  const uint32_t* pe = reinterpret_cast<const uint32_t*>(entries);
+  // entry i of the bucket: its local id and the other endpoint
+  auto entry = [&](uint32_t i, uint32_t& key, uint32_t& val) {
+    if (kPacked) {
+      const uint32_t e = pe[base + i];
+      key = e >> pp.nbits;
+      val = e & omask;
+    } else {
+      const uint2 e = entries[base + i];
+      key = hashed_id(pp, e.x) & lmask;
+      val = e.y;
+    }
+  };
   // LDS atomics aggregated per wave for the key of the wave's first lane: a hub vertex fills whole
   // waves with one key, which would otherwise serialize on one LDS address
   // the entries are loaded kSortBatch at a time (independent loads, one wait per batch)
   for (uint32_t i0 = t; i0 < cnt; i0 += kSortBatch * kSortThreads) {
-    uint32_t vx[kSortBatch];
+    uint32_t ka[kSortBatch], va[kSortBatch];
 #pragma unroll
-    for (int k = 0; k < kSortBatch; k++) {
-      const uint32_t i = i0 + k * kSortThreads;
-      vx[k] = entries[base + min(i, cnt - 1)].x;
-    }
+    for (int k = 0; k < kSortBatch; k++) entry(min(i0 + k * kSortThreads, cnt - 1), ka[k], va[k]);
 #pragma unroll
     for (int k = 0; k < kSortBatch; k++) {
       if (i0 + k * kSortThreads >= cnt) break;
-      const uint32_t key = hashed_id(pp, vx[k]) & lmask;
+      const uint32_t key = ka[k];
       const uint32_t k0 = __builtin_amdgcn_readfirstlane(key);
       const uint64_t same = __ballot(key == k0);
       if (key != k0)
@@ -419,17 +445,13 @@ __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
   if (bins > kTileBins) {
     // wide local ids: every destination goes straight to its place (scattered 4-byte stores)
     for (uint32_t i0 = t; i0 < cnt; i0 += kSortTile) {
-      uint2 ev[kSortBatch];
+      uint32_t ka[kSortBatch], va[kSortBatch];
 #pragma unroll
-      for (int k = 0; k < kSortBatch; k++) {
-        const uint32_t i = i0 + k * kSortThreads;
-        ev[k] = entries[base + min(i, cnt - 1)];
-      }
+      for (int k = 0; k < kSortBatch; k++) entry(min(i0 + k * kSortThreads, cnt - 1), ka[k], va[k]);
 #pragma unroll
       for (int k = 0; k < kSortBatch; k++) {
         if (i0 + k * kSortThreads >= cnt) break;
-        const uint32_t key = hashed_id(pp, ev[k].x) & lmask;
-        sorted[base + wave_aggregated_add(s_off, key, 1u)] = ev[k].y;
+        sorted[base + wave_aggregated_add(s_off, ka[k], 1u)] = va[k];
       }
     }
     return;
@@ -448,18 +470,13 @@ __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
   const uint32_t lane = t & 63, wid = t >> 6;
   for (uint32_t tile0 = 0; tile0 < cnt; tile0 += kSortTile) {
     const uint32_t i0 = tile0 + t;
-    uint2 ev[kSortBatch];
-    uint32_t key[kSortBatch];
+    uint32_t key[kSortBatch], val[kSortBatch];
 #pragma unroll
-    for (int k = 0; k < kSortBatch; k++) {
-      const uint32_t i = i0 + k * kSortThreads;
-      ev[k] = entries[base + min(i, cnt - 1)];
-    }
+    for (int k = 0; k < kSortBatch; k++) entry(min(i0 + k * kSortThreads, cnt - 1), key[k], val[k]);
     for (uint32_t i = t; i < bins; i += kSortThreads) t_cnt[i] = 0;
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kSortBatch; k++) {
-      key[k] = hashed_id(pp, ev[k].x) & lmask;
       if (i0 + k * kSortThreads >= cnt) break;
       (void)wave_aggregated_add(t_cnt, key[k], 1u);
     }
@@ -486,7 +503,7 @@ __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
     for (int k = 0; k < kSortBatch; k++) {
       if (i0 + k * kSortThreads >= cnt) break;
       const uint32_t lpos = wave_aggregated_add(t_cur, key[k], 1u);
-      st_val[lpos] = ev[k].y;
+      st_val[lpos] = val[k];
       st_pos[lpos] = t_gb[key[k]] + (lpos - t_off[key[k]]);
     }
     __syncthreads();
@@ -1359,9 +1376,14 @@ static int process_pending(Stream* s) {
   sg.blk0[sg.nseg] = nblk;
   const size_t sort_lds = sort_lds_bytes(pp.local_bits);
   int rc;
+  const bool packed = packed_entries_fit(pp);
+  const void* scatter_fn = packed ? (const void*)bucket_scatter_kernel<true>
+                                  : (const void*)bucket_scatter_kernel<false>;
+  const void* sort_fn = packed ? (const void*)bucket_sort_kernel<true>
+                               : (const void*)bucket_sort_kernel<false>;
   if ((rc = ensure_dyn_lds((const void*)bucket_count_kernel, sizeof(uint32_t) * pp.nbuckets)) ||
-      (rc = ensure_dyn_lds((const void*)bucket_scatter_kernel, scatter_lds_bytes(pp.nbuckets))) ||
-      (rc = ensure_dyn_lds((const void*)bucket_sort_kernel, sort_lds)))
+      (rc = ensure_dyn_lds(scatter_fn, scatter_lds_bytes(pp.nbuckets))) ||
+      (rc = ensure_dyn_lds(sort_fn, sort_lds)))
     return rc;
   if (s->copies_pending) {  // every host copy into the buffer has landed
     GS_HIP(hipStreamWaitEvent(s->stream, s->copied, 0));
@@ -1374,18 +1396,28 @@ static int process_pending(Stream* s) {
   hipLaunchKernelGGL(bucket_offsets_kernel, dim3(pp.nbuckets), dim3(kOffThreads), 0, s->stream,
                      ws.d_hist, nblk, ws.d_totals);
   GS_HIP(hipGetLastError());
-  hipLaunchKernelGGL(bucket_scatter_kernel, dim3(nblk), dim3(kPartThreads),
-                     scatter_lds_bytes(pp.nbuckets), s->stream, sg, nverts, pp, nblk,
-                     ws.d_totals, ws.d_hist, ws.d_entries);
+  if (packed)
+    hipLaunchKernelGGL(bucket_scatter_kernel<true>, dim3(nblk), dim3(kPartThreads),
+                       scatter_lds_bytes(pp.nbuckets), s->stream, sg, nverts, pp, nblk,
+                       ws.d_totals, ws.d_hist, ws.d_entries);
+  else
+    hipLaunchKernelGGL(bucket_scatter_kernel<false>, dim3(nblk), dim3(kPartThreads),
+                       scatter_lds_bytes(pp.nbuckets), s->stream, sg, nverts, pp, nblk,
+                       ws.d_totals, ws.d_hist, ws.d_entries);
   GS_HIP(hipGetLastError());
   GS_HIP(hipEventRecord(s->pend_free[b], s->stream));  // the fill buffer has been read
   s->pend_used[b] = true;
-  hipLaunchKernelGGL(bucket_sort_kernel, dim3(pp.nbuckets), dim3(kSortThreads), sort_lds,
-                     s->stream, ws.d_entries, pp, ws.d_totals, ws.d_sorted, ws.d_items, ws.cls,
-                     s->tune_chunk ? s->tune_chunk : (stream_wide(s) ? kWideChunk : kChunk),
-                     ws.d_counters + kClassBase,
-                     s->group ? (const uint32_t*)s->group->d_bases : (const uint32_t*)nullptr,
-                     s->group ? (uint32_t)s->group->members.size() : 1u);
+  const uint32_t chunk = s->tune_chunk ? s->tune_chunk : (stream_wide(s) ? kWideChunk : kChunk);
+  const uint32_t* bases = s->group ? (const uint32_t*)s->group->d_bases : nullptr;
+  const uint32_t ngraphs = s->group ? (uint32_t)s->group->members.size() : 1u;
+  if (packed)
+    hipLaunchKernelGGL(bucket_sort_kernel<true>, dim3(pp.nbuckets), dim3(kSortThreads), sort_lds,
+                       s->stream, ws.d_entries, pp, ws.d_totals, ws.d_sorted, ws.d_items, ws.cls,
+                       chunk, ws.d_counters + kClassBase, bases, ngraphs);
+  else
+    hipLaunchKernelGGL(bucket_sort_kernel<false>, dim3(pp.nbuckets), dim3(kSortThreads), sort_lds,
+                       s->stream, ws.d_entries, pp, ws.d_totals, ws.d_sorted, ws.d_items, ws.cls,
+                       chunk, ws.d_counters + kClassBase, bases, ngraphs);
   GS_HIP(hipGetLastError());
   s->pend_n = 0;
   s->fill_n = 0;

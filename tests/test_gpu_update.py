@@ -236,6 +236,33 @@ def test_full_size_rmat_sampled_vertices_bit_exact():
         assert np.array_equal(got[i], orc.vertex_sketch(g, 0xABCDEF, v, others)), v
 
 
+@pytest.mark.parametrize("bucket_bits", [0, 13])
+def test_large_vertex_ids_both_entry_forms(bucket_bits):
+    """n = 2^19 + 7 (20-bit ids): by default 12 local-id bits + 20 id bits fill a packed 32-bit
+    grouping entry exactly; 13 local-id bits need the 8-byte entry. Sampled vertices bit-exact
+    against the oracle, then the toggle identity."""
+    n = (1 << 19) + 7
+    rng = np.random.default_rng(70 + bucket_bits)
+    pairs = rng.integers(0, n, size=(1 << 20, 2), dtype=np.uint32)
+    pairs[:4096, 0] = n - 1  # a hub at the largest id
+    sample = np.unique(np.concatenate([[0, 1, n - 2, n - 1], rng.integers(0, n, size=12)]))
+    with gs.SketchStore(n, 0x1D5) as st:
+        s = st.stream()
+        s.set_tuning(bucket_bits=bucket_bits)
+        s.submit_pairs(pairs)
+        s.flush()
+        got = np.concatenate([st.export_buckets(int(v), 1) for v in sample])
+        s.submit_pairs(pairs)
+        s.flush()
+        again = st.export_buckets(n - 64, 64)
+    assert not again.any()
+    g = orc.Geometry(n)
+    for i, v in enumerate(sample.tolist()):
+        others = np.concatenate([pairs[pairs[:, 0] == v, 1], pairs[pairs[:, 1] == v, 0]])
+        others = others[others != v]
+        assert np.array_equal(got[i], orc.vertex_sketch(g, 0x1D5, v, others)), v
+
+
 @pytest.mark.parametrize("lds_depths,item_updates,bucket_bits", [(8, 128, 4), (16, 512, 6), (12, 64, 0)])
 def test_tuned_launch_parameters_match_oracle(lds_depths, item_updates, bucket_bits):
     pairs = gs.gen_rmat(12, 120000, 21, del_frac=0.05)
