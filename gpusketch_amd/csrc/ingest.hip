@@ -15,6 +15,8 @@
 // read-xor-write when the item owns its vertex in this launch, 64-bit atomic XOR otherwise. The
 // deterministic bucket is a wave XOR-reduction. Depths >= the LDS window go straight to HBM with
 // atomics (probability 2^-16 per column and update).
+#include <chrono>
+
 #include "gs_internal.h"
 
 namespace gs {
@@ -1499,7 +1501,17 @@ int submit_pairs_host(Stream* s, const uint32_t* pairs, uint64_t nedges, uint32_
     e0 += m;
     if (s->pend_n == s->pend_cap && (rc = process_pending(s))) return rc;
   }
-  if (pinned) GS_HIP(hipEventSynchronize(s->copied));  // the caller may reuse its buffer
+  if (pinned) {  // the caller may reuse its buffer once the DMA has read it
+    // poll first: the copy of a batch takes ~0.15 ms, and a blocking wait's wake-up delay would
+    // otherwise idle the link between one submission and the next; block if it runs long
+    const auto t0 = std::chrono::steady_clock::now();
+    hipError_t q;
+    while ((q = hipEventQuery(s->copied)) == hipErrorNotReady &&
+           std::chrono::steady_clock::now() - t0 < std::chrono::milliseconds(2)) {
+    }
+    if (q == hipErrorNotReady) q = hipEventSynchronize(s->copied);
+    GS_HIP(q);
+  }
   return GS_OK;
 }
 
