@@ -839,36 +839,80 @@ __global__ __launch_bounds__(kQThreads) void g_round0_kernel(
   handle_group_sample(res, idx, V, d, parent, vflags, &gstate[g]);
 }
 
-// the group form of acc_kernel: lanes are global vertices of possibly different members
+// Per member and round: the supernode whose sum is formed from the rest of the member (the
+// group form of the giant, see pick_big_kernel): the root holding more than half of 8 evenly
+// spaced sampled vertices, on a member whose vertex sketches XOR to zero (pair-fed), else none.
+// Any supernode's sum equals the XOR of every vertex outside it then, so the choice only decides
+// which vertices are read.
+constexpr int kGiantSamples = 8;
+__global__ void g_pick_giant_kernel(const GraphDesc* __restrict__ descs, uint32_t G,
+                                    const uint32_t* __restrict__ comp,
+                                    const uint32_t* __restrict__ vflags,
+                                    const uint32_t* __restrict__ gsym,
+                                    const uint32_t* __restrict__ gstate,
+                                    uint32_t* __restrict__ giant) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  uint32_t pick = kNoGiant;
+  if (gstate[G + g] == kGActive && gsym[g]) {
+    const GraphDesc d = descs[g];
+    uint32_t r[kGiantSamples];
+    for (int i = 0; i < kGiantSamples; i++)
+      r[i] = comp[d.base + (uint32_t)(((uint64_t)i * d.n) / kGiantSamples + d.n / (2 * kGiantSamples))];
+    for (int i = 0; i < kGiantSamples; i++) {
+      int c = 0;
+      for (int j = 0; j < kGiantSamples; j++) c += r[j] == r[i];
+      if (2 * c > kGiantSamples) pick = r[i];
+    }
+    if (pick != kNoGiant && (vflags[pick] & kVDone)) pick = kNoGiant;
+  }
+  giant[g] = pick;
+}
+
+// the group form of acc_kernel: lanes are global vertices of possibly different members. A
+// vertex of a live supernode adds its column into that supernode's sum; with a giant picked for
+// its member (g_pick_giant_kernel), a vertex inside it is not read at all and every other vertex
+// of the member (finished supernodes included) also adds into the giant's sum. Lanes sharing the
+// wave's first root (or first giant) are XOR-reduced into one atomic per bucket.
 __global__ __launch_bounds__(kQThreads) void g_acc_kernel(
     const GraphDesc* __restrict__ descs, const uint32_t* __restrict__ vgraph, uint32_t N,
     uint32_t G, const uint32_t* __restrict__ comp, const uint32_t* __restrict__ vflags,
-    const uint32_t* __restrict__ gstate, Bucket16* __restrict__ acc, uint32_t acc_stride,
-    uint32_t max_w) {
+    const uint32_t* __restrict__ gstate, const uint32_t* __restrict__ giant,
+    Bucket16* __restrict__ acc, uint32_t acc_stride, uint32_t max_w) {
   const uint32_t col = gstate[3 * G];
   const uint32_t V = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63;
-  bool act = V < N;
-  uint32_t g = 0, r = 0;
-  if (act) {
+  bool inm = V < N;
+  uint32_t g = 0, r = 0, gg = kNoGiant;
+  bool done = false;
+  if (inm) {
     g = vgraph[V];
-    act = gstate[G + g] == kGActive;
+    inm = gstate[G + g] == kGActive;
   }
-  if (act) {
+  if (inm) {
     r = comp[V];
-    act = !(vflags[r] & kVDone);
+    done = (vflags[r] & kVDone) != 0;
+    gg = giant[g];
   }
+  const bool ingiant = inm && gg != kNoGiant && r == gg;
+  const bool act = inm && !done && !ingiant;         // into its own supernode's sum
+  const bool cmp = inm && gg != kNoGiant && !ingiant;  // into its member's giant
+  const bool rd = act || cmp;
   GraphDesc d{};
-  if (act) d = descs[g];
-  const uint64_t am = __ballot(act);
-  if (am == 0) return;  // wave-uniform
-  const uint32_t r0 = (uint32_t)__shfl((int)r, __builtin_ctzll(am));
+  if (rd) d = descs[g];
+  const uint64_t am = __ballot(act), cm = __ballot(cmp);
+  if ((am | cm) == 0) return;  // wave-uniform
+  const uint32_t r0 = am ? (uint32_t)__shfl((int)r, __builtin_ctzll(am)) : 0u;
+  const uint32_t gg0 = cm ? (uint32_t)__shfl((int)gg, __builtin_ctzll(cm)) : 0u;
   const bool grp = act && r == r0;
-  const uint32_t w = act ? d.bpc + 1 : 0;
-  const uint32_t v = act ? V - d.base : 0;
-  const Bucket16* vrow = act ? d.rows + (size_t)v * d.vstride + col : nullptr;
+  const bool cgrp = cmp && gg == gg0;
+  const uint32_t w = rd ? d.bpc + 1 : 0;
+  const uint32_t v = rd ? V - d.base : 0;
+  const Bucket16* vrow = rd ? d.rows + (size_t)v * d.vstride + col : nullptr;
   Bucket16* a0 = acc + (size_t)r0 * acc_stride;
   Bucket16* ar = acc + (size_t)r * acc_stride;
+  Bucket16* c0 = acc + (size_t)gg0 * acc_stride;
+  Bucket16* cr = acc + (size_t)(cmp ? gg : 0u) * acc_stride;
   for (uint32_t j0 = 0; j0 < max_w; j0 += kAccBatch) {
     Bucket16 b[kAccBatch];
 #pragma unroll
@@ -881,15 +925,30 @@ __global__ __launch_bounds__(kQThreads) void g_acc_kernel(
     for (int k = 0; k < kAccBatch; k++) {
       const uint32_t j = j0 + k;
       if (j >= max_w) break;
-      const uint64_t xa = wave_xor_reduce(grp ? b[k].alpha : 0);
-      const uint64_t xg = wave_xor_reduce(grp ? b[k].gamma : 0);
-      if (lane == 0 && (xa | xg)) {
-        __hip_atomic_fetch_xor(&a0[j].alpha, xa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_xor(&a0[j].gamma, xg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const bool nz = (b[k].alpha | b[k].gamma) != 0;
+      if (am) {
+        const uint64_t xa = wave_xor_reduce(grp ? b[k].alpha : 0);
+        const uint64_t xg = wave_xor_reduce(grp ? b[k].gamma : 0);
+        if (lane == 0 && (xa | xg)) {
+          __hip_atomic_fetch_xor(&a0[j].alpha, xa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_xor(&a0[j].gamma, xg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (act && !grp && nz) {
+          __hip_atomic_fetch_xor(&ar[j].alpha, b[k].alpha, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_xor(&ar[j].gamma, b[k].gamma, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
       }
-      if (act && !grp && (b[k].alpha | b[k].gamma)) {
-        __hip_atomic_fetch_xor(&ar[j].alpha, b[k].alpha, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_xor(&ar[j].gamma, b[k].gamma, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (cm) {
+        const uint64_t xa = wave_xor_reduce(cgrp ? b[k].alpha : 0);
+        const uint64_t xg = wave_xor_reduce(cgrp ? b[k].gamma : 0);
+        if (lane == 0 && (xa | xg)) {
+          __hip_atomic_fetch_xor(&c0[j].alpha, xa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_xor(&c0[j].gamma, xg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (cmp && !cgrp && nz) {
+          __hip_atomic_fetch_xor(&cr[j].alpha, b[k].alpha, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_xor(&cr[j].gamma, b[k].gamma, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
       }
     }
   }
@@ -940,6 +999,9 @@ void free_group_query(Group* g) {
   (void)hipFree(g->d_vflags);
   (void)hipFree(g->d_gstate);
   (void)hipFree(g->d_acc);
+  (void)hipFree(g->d_giant);
+  (void)hipFree(g->d_gsym);
+  g->d_giant = g->d_gsym = nullptr;
   if (g->h_gstate) (void)hipHostFree(g->h_gstate);
   if (g->round_exec) (void)hipGraphExecDestroy(g->round_exec);
   g->d_parent = g->d_comp = g->d_vflags = g->d_gstate = g->h_gstate = nullptr;
@@ -952,8 +1014,10 @@ static void launch_group_round(Group* g, hipStream_t q) {
   const uint32_t N = g->n_total, G = (uint32_t)g->members.size();
   const dim3 grid((N + kQThreads - 1) / kQThreads), block(kQThreads);
   hipLaunchKernelGGL(g_begin_kernel, dim3(1), dim3(64), 0, q, g->d_gstate, G);
+  hipLaunchKernelGGL(g_pick_giant_kernel, dim3((G + 255) / 256), dim3(256), 0, q, g->d_descs, G,
+                     g->d_comp, g->d_vflags, g->d_gsym, g->d_gstate, g->d_giant);
   hipLaunchKernelGGL(g_acc_kernel, grid, block, 0, q, g->d_descs, g->d_vgraph, N, G, g->d_comp,
-                     g->d_vflags, g->d_gstate, g->d_acc, (uint32_t)g->acc_stride,
+                     g->d_vflags, g->d_gstate, g->d_giant, g->d_acc, (uint32_t)g->acc_stride,
                      g->max_bpc + 1);
   hipLaunchKernelGGL(g_sample_kernel, grid, block, 0, q, g->d_descs, g->d_vgraph, N, G, g->d_comp,
                      g->d_acc, (uint32_t)g->acc_stride, g->d_parent, g->d_vflags, g->d_gstate);
@@ -977,6 +1041,8 @@ int group_query_cc(Group* g, const uint8_t* mask, uint32_t* labels, uint32_t* nu
     GS_HIP(hipHostMalloc((void**)&g->h_gstate, sizeof(uint32_t) * (3 * (size_t)G + 2),
                          hipHostMallocDefault));
     GS_HIP(hipMalloc(&g->d_acc, sizeof(Bucket16) * (size_t)N * g->acc_stride));
+    GS_HIP(hipMalloc(&g->d_giant, sizeof(uint32_t) * G));
+    GS_HIP(hipMalloc(&g->d_gsym, sizeof(uint32_t) * G));
     hipLaunchKernelGGL(zero_acc_kernel, dim3(2048), dim3(256), 0, q, g->d_acc,
                        (uint64_t)N * g->acc_stride);
     GS_HIP(hipGetLastError());
@@ -990,6 +1056,10 @@ int group_query_cc(Group* g, const uint8_t* mask, uint32_t* labels, uint32_t* nu
     (void)hipGraphDestroy(graph);
     GS_HIP(e);
   }
+  // which members' sketches XOR to zero (the giant of such a member is summed from the rest)
+  std::vector<uint32_t> sym(G);
+  for (uint32_t i = 0; i < G; i++) sym[i] = g->members[i]->symmetric.load() ? 1u : 0u;
+  GS_HIP(hipMemcpy(g->d_gsym, sym.data(), sizeof(uint32_t) * G, hipMemcpyHostToDevice));
   // host-initialised state: nothing modified, members outside the mask already done, round 0
   std::vector<uint32_t> init(3 * (size_t)G + 2, 0);
   for (uint32_t i = 0; i < G; i++) init[G + i] = (mask && !mask[i]) ? kGDone : kGActive;

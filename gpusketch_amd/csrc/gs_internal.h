@@ -242,6 +242,8 @@ struct Group {
   uint32_t* h_gstate = nullptr;  // pinned mirror
   Bucket16* d_acc = nullptr;
   uint64_t acc_stride = 0;       // buckets per vertex in d_acc (max_bpc + 1)
+  uint32_t* d_giant = nullptr;   // per member, each round: the supernode summed from the rest
+  uint32_t* d_gsym = nullptr;    // per member, each query: its vertex sketches XOR to zero
   hipGraphExec_t round_exec = nullptr;  // one captured Boruvka round over every member
 };
 

@@ -130,3 +130,45 @@ def test_group_member_out_of_samples():
     finally:
         ok.close()
         tiny.close()
+
+
+def test_group_query_giants_with_pair_and_batch_fed_members():
+    # dense random members form a giant component: pair-fed members (sketches XOR to zero) sum
+    # it from the rest of the member; a member fed by vertex batches (one direction only, so its
+    # sketches do not XOR to zero) must sum it directly; both agree with single-store queries
+    # and with a union-find over the net graph
+    rng = np.random.default_rng(77)
+    sizes = [5000, 3000, 8000]
+    seeds = [31, 32, 33]
+    pairs = [member_pairs(rng, n, 3 * n) for n in sizes]
+    stores = [gs.SketchStore(n, s) for n, s in zip(sizes, seeds)]
+    singles = [gs.SketchStore(n, s) for n, s in zip(sizes, seeds)]
+    try:
+        # member 1: vertex batches holding both directions of every edge (the same sketches as
+        # pairs, but the store no longer knows that they XOR to zero)
+        clean = pairs[1][pairs[1][:, 0] != pairs[1][:, 1]]
+        src, sizes_b, offs, dsts = orc.group_by_vertex(clean, sizes[1], 512)
+        for st in (stores[1], singles[1]):
+            s = st.stream()
+            s.submit_vertex_batches(src, sizes_b, offs, dsts)
+            s.flush()
+            s.close()
+        with gs.StoreGroup(stores) as grp:
+            grp.submit_pairs([pairs[0], None, pairs[2]])
+            grp.flush()
+            for i in (0, 2):
+                s = singles[i].stream()
+                s.submit_pairs(pairs[i])
+                s.flush()
+                s.close()
+            labels, ncc, rounds, status = grp.query_cc()
+            for i, n in enumerate(sizes):
+                lab1, ncc1, r1 = singles[i].query_cc()
+                net = net_edges(clean if i == 1 else pairs[i])
+                expect = orc.dsu_components(n, net)
+                assert np.bincount(expect).max() > n // 2, f"member {i} has no giant"
+                assert np.array_equal(labels[i], expect), f"member {i}"
+                assert np.array_equal(labels[i], lab1) and rounds[i] == r1 and status[i] == 0
+    finally:
+        for s in stores + singles:
+            s.close()
