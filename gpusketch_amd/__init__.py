@@ -94,6 +94,7 @@ _sigs = {
     "gs_group_submit_pairs": ([_vp, _vp, _vp], _i32),
     "gs_group_submit_pairs_device": ([_vp, _vp, _u64], _i32),
     "gs_group_query_cc": ([_vp, _vp, _vp, _vp, _vp, _vp], _i32),
+    "gs_group_prepare": ([_vp], _i32),
     "gs_host_alloc": ([_sz, _vp], _i32),
     "gs_host_free": ([_vp], _i32),
     "gs_device_alloc": ([_sz, _vp], _i32),
@@ -405,6 +406,10 @@ class StoreGroup:
 
     def submit_pairs_device(self, dptr, nedges: int) -> None:
         _check(lib.gs_group_submit_pairs_device(self.h, dptr, nedges))
+
+    def prepare(self) -> None:
+        """Allocates the group's ingest buffers and query scratch now rather than on first use."""
+        _check(lib.gs_group_prepare(self.h))
 
     def flush(self) -> None:
         self.stream.flush()

@@ -1028,8 +1028,8 @@ static void launch_group_round(Group* g, hipStream_t q) {
                      nullptr);
 }
 
-int group_query_cc(Group* g, const uint8_t* mask, uint32_t* labels, uint32_t* num_cc,
-                   uint32_t* rounds, int32_t* status) {
+// the batched query's scratch and its captured round, made once per group
+int prepare_group_query(Group* g) {
   const uint32_t N = g->n_total, G = (uint32_t)g->members.size();
   hipStream_t q = g->s->stream;
   if (!g->d_parent) {
@@ -1056,6 +1056,15 @@ int group_query_cc(Group* g, const uint8_t* mask, uint32_t* labels, uint32_t* nu
     (void)hipGraphDestroy(graph);
     GS_HIP(e);
   }
+  return GS_OK;
+}
+
+int group_query_cc(Group* g, const uint8_t* mask, uint32_t* labels, uint32_t* num_cc,
+                   uint32_t* rounds, int32_t* status) {
+  const uint32_t N = g->n_total, G = (uint32_t)g->members.size();
+  hipStream_t q = g->s->stream;
+  int prc = prepare_group_query(g);
+  if (prc) return prc;
   // which members' sketches XOR to zero (the giant of such a member is summed from the rest)
   std::vector<uint32_t> sym(G);
   for (uint32_t i = 0; i < G; i++) sym[i] = g->members[i]->symmetric.load() ? 1u : 0u;

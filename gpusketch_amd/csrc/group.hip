@@ -10,6 +10,10 @@
 
 namespace gs {
 
+// host pairs of a group submission are offset into the group's vertex space through a pinned
+// staging of this many edges
+constexpr uint64_t kGroupStageEdges = 1ULL << 22;
+
 static void unregister_from_members(Group& g) {
   for (Store* m : g.members) {
     std::lock_guard<std::mutex> lk(m->mu);
@@ -141,7 +145,7 @@ int gs_group_submit_pairs(gs_group* h, const uint32_t* const* pairs, const uint6
   const uint32_t count = (uint32_t)g.members.size();
   for (uint32_t i = 0; i < count; i++)
     GS_CHECK(pairs[i] || nedges[i] == 0, GS_ERR_INVALID, "member %u: null pairs", i);
-  const uint64_t chunk = 1ULL << 22;  // edges per staged copy
+  const uint64_t chunk = kGroupStageEdges;
   uint32_t mi = 0;
   uint64_t ei = 0;  // next edge of member mi
   for (;;) {
@@ -185,6 +189,17 @@ int gs_group_submit_pairs_device(gs_group* h, const uint32_t* d_pairs, uint64_t 
   GS_CHECK(h, GS_ERR_INVALID, "null group");
   GS_CHECK(d_pairs || nedges == 0, GS_ERR_INVALID, "null pairs");
   return submit_pairs_device(h->impl.s, d_pairs, nedges, 0, 0);
+}
+
+int gs_group_prepare(gs_group* h) {
+  GS_CHECK(h, GS_ERR_INVALID, "null group");
+  Group& g = h->impl;
+  int rc = prepare_stream(g.s);
+  if (rc) return rc;
+  if ((rc = ensure_stage(g.s, kGroupStageEdges * 2 * sizeof(uint32_t)))) return rc;
+  if ((rc = prepare_group_query(&g))) return rc;
+  GS_HIP(hipStreamSynchronize(g.s->stream));
+  return GS_OK;
 }
 
 int gs_group_query_cc(gs_group* h, const uint8_t* member_mask, uint32_t* labels,

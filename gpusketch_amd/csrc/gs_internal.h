@@ -304,6 +304,9 @@ void free_query_scratch(Store* st);
 int group_query_cc(Group* g, const uint8_t* member_mask, uint32_t* labels, uint32_t* num_cc,
                    uint32_t* rounds, int32_t* status);
 void free_group_query(Group* g);
+int prepare_group_query(Group* g);
+// the stream's fill buffers and grouping scratch for its launch size, allocated now
+int prepare_stream(Stream* s);
 int k_forests(Store* st, uint32_t k, uint32_t* edges_out, uint64_t edge_cap, uint64_t* counts);
 
 // geometry helper shared with the host façade (include/sketch.h:63-77,198-216)

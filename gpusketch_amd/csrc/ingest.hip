@@ -1329,6 +1329,11 @@ static int ensure_pending(Stream* s) {
   return ensure_pair_scratch(s, want);
 }
 
+int prepare_stream(Stream* s) {
+  std::lock_guard<std::mutex> lk(s->mu);
+  return ensure_pending(s);
+}
+
 // `m` pairs at src into the fill buffer behind what is pending (src may be that very place: a
 // host batch copied there; then only a subgraph filter needs a pass)
 static int append_pairs(Stream* s, const uint32_t* src, uint64_t m, uint32_t min_depth,

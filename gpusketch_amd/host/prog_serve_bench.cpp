@@ -109,11 +109,14 @@ static double pct(std::vector<double> v, double p) {
   return v[lo] + (v[hi] - v[lo]) * (idx - lo);
 }
 
+static uint64_t g_max_step_updates = 0;  // 0: the server's default
+
 static RunStats run(const Trace& tr, bool batching, uint64_t budget, uint32_t k,
                     const std::vector<uint32_t>& only_tenants, bool keep_exports) {
   ServerConfig cfg;
   cfg.batching = batching;
   cfg.memory_budget_bytes = budget;
+  if (g_max_step_updates) cfg.max_step_updates = g_max_step_updates;
   SketchServer srv(cfg);
   std::vector<int> tid(tr.tenants.size(), -1);
   for (size_t i = 0; i < tr.tenants.size(); i++)
@@ -206,6 +209,7 @@ int main(int argc, char** argv) {
     else if (a == "--max-n") max_n = (uint32_t)std::atoi(v);
     else if (a == "--max-edges") max_edges = std::stoull(v);
     else if (a == "--alone-check") alone = std::atoi(v);
+    else if (a == "--max-step-updates") g_max_step_updates = std::stoull(v);
     else {
       std::fprintf(stderr, "unknown option %s\n", a.c_str());
       return 2;

@@ -1,6 +1,7 @@
 // SketchServer (see sketch_server.h): admission, arrival-ordered scheduling, continuous batching
 // of many tenants' updates into one store-group submission and of their connectivity queries
-// into one batched Boruvka. Every device call is made from the server's worker thread.
+// into one batched Boruvka. Every device call is made from the server's worker thread; the host
+// part of a min-cut query (the exact cut of k forests) runs on a second thread meanwhile.
 #include "gpusketch/sketch_server.h"
 
 #include <algorithm>
@@ -24,6 +25,7 @@ uint64_t fnv1a(const uint32_t* p, size_t n) {
 
 SketchServer::SketchServer(ServerConfig c) : cfg(c), t0(std::chrono::steady_clock::now()) {
   worker = std::thread([this] { loop(); });
+  cutter = std::thread([this] { cut_loop(); });
 }
 
 SketchServer::~SketchServer() {
@@ -33,6 +35,12 @@ SketchServer::~SketchServer() {
   }
   cv.notify_all();
   worker.join();
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    cut_stop = true;
+  }
+  cut_cv.notify_all();
+  cutter.join();
   if (group) gs_group_destroy(group);
   for (auto& t : tenants) {
     if (t->stream) gs_stream_destroy(t->stream);
@@ -185,6 +193,8 @@ void SketchServer::ensure_group() {
     gs_tuning tun{};
     tun.launch_edges = 1u << 25;
     gs_throw_if(gs_stream_set_tuning(gsm, &tun));
+    // buffers and query scratch now, not in the first step that updates or queries
+    gs_throw_if(gs_group_prepare(group));
   }
 }
 
@@ -291,7 +301,8 @@ void SketchServer::run_step(std::vector<Request>& step) {
       }
     }
   }
-  // 3. min-cut queries: k spanning forests on the device, exact cut of their union
+  // 3. min-cut queries: k spanning forests on the device; the exact cut of their union is a
+  //    host computation of up to tens of milliseconds, done by the cutter thread meanwhile
   for (Request* r : mc) {
     Tenant& t = *tenants[r->tenant];
     const uint32_t n = t.spec.num_vertices, k = std::max<uint32_t>(1, r->k);
@@ -302,17 +313,36 @@ void SketchServer::run_step(std::vector<Request>& step) {
     if (res.status == GS_OK) {
       uint64_t m = 0;
       for (uint64_t c : counts) m += c;
-      uint64_t value = 0;
-      std::vector<uint8_t> side(n);
-      gs_throw_if(gs_min_cut(n, edges.data(), m, &value, side.data()));
-      res.mincut = std::min<uint64_t>(value, k);
-    } else if (res.status != GS_ERR_OUT_OF_SAMPLES) {
-      gs_throw_if(res.status);
+      std::lock_guard<std::mutex> lk(mu);
+      step_handed.push_back(r->id);
+      cut_jobs.push_back(CutJob{*r, n, k, std::move(edges), m});
+      cut_cv.notify_one();
+      continue;
     }
+    if (res.status != GS_ERR_OUT_OF_SAMPLES) gs_throw_if(res.status);
     finish(*r, std::move(res));
   }
   // 4. every other tenant's updates
   apply_updates(upd_o);
+}
+
+void SketchServer::cut_loop() {
+  for (;;) {
+    CutJob job;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cut_cv.wait(lk, [&] { return cut_stop || !cut_jobs.empty(); });
+      if (cut_jobs.empty()) return;
+      job = std::move(cut_jobs.front());
+      cut_jobs.pop_front();
+    }
+    RequestResult res;
+    uint64_t value = 0;
+    res.status = gs_min_cut(job.n, job.edges.data(), job.m, &value, nullptr);
+    if (res.status == GS_OK) res.mincut = std::min<uint64_t>(value, job.k);
+    finish(job.req, std::move(res));
+    idle_cv.notify_all();
+  }
 }
 
 void SketchServer::loop() {
@@ -364,6 +394,7 @@ void SketchServer::loop() {
         }
       }
       busy = true;
+      step_handed.clear();
     }
     if (!release.empty()) {
       // drop the group before the stores it refers to
@@ -389,7 +420,9 @@ void SketchServer::loop() {
         std::lock_guard<std::mutex> lk(mu);
         last_error = e.what();
         for (auto& r : step) {
-          bool answered_already = false;
+          // answered already, or answered by the cutter
+          bool answered_already =
+              std::find(step_handed.begin(), step_handed.end(), r.id) != step_handed.end();
           for (auto& d : done)
             if (d.id == r.id) answered_already = true;
           if (answered_already) continue;

@@ -217,6 +217,11 @@ int gs_group_stream(gs_group* g, gs_stream** out);
 int gs_group_submit_pairs(gs_group* g, const uint32_t* const* pairs, const uint64_t* nedges);
 /* Device-resident pairs already in global ids (base_i added); must stay valid until gs_flush. */
 int gs_group_submit_pairs_device(gs_group* g, const uint32_t* d_pairs, uint64_t nedges);
+/* Allocates now what the group's first submission and first query would allocate (the stream's
+ * fill buffers and grouping scratch at its current launch size, the pinned staging, the batched
+ * query's scratch and captured round), so that a serving loop allocates nothing after it starts.
+ * Call after gs_stream_set_tuning on the group's stream. No counterpart in the reference. */
+int gs_group_prepare(gs_group* g);
 /* Connected components of the members selected by member_mask (NULL: all) in one batched
  * Borůvka whose rounds >= 1 replay a captured HIP graph. labels: total entries, member-local
  * smallest vertex of each component; num_cc / rounds / status: one entry per member, status

@@ -111,6 +111,15 @@ class SketchServer {
   void apply_updates(std::vector<Request*>& upd);
   void ensure_group();
   void finish(Request& r, RequestResult res);
+  void cut_loop();
+
+  // a min-cut query whose forests are on the host: its exact cut is computed off the worker
+  struct CutJob {
+    Request req;
+    uint32_t n, k;
+    std::vector<uint32_t> edges;
+    uint64_t m;
+  };
 
   ServerConfig cfg;
   std::chrono::steady_clock::time_point t0;
@@ -130,4 +139,10 @@ class SketchServer {
   gs_group* group = nullptr;
   std::vector<uint32_t> group_members;
   std::thread worker;
+  // host min-cut jobs, run by `cutter` while the worker goes on with the device
+  std::deque<CutJob> cut_jobs;
+  std::vector<uint64_t> step_handed;  // ids of the running step's requests given to the cutter
+  bool cut_stop = false;
+  std::condition_variable cut_cv;
+  std::thread cutter;
 };

@@ -100,6 +100,8 @@ def test_group_device_resident_global_pairs():
     pairs = [member_pairs(rng, n, 5 * n) for n in sizes]
     try:
         with gs.StoreGroup(stores) as grp:
+            grp.prepare()  # buffers and query scratch allocated up front; twice is harmless
+            grp.prepare()
             glob = np.concatenate([p + np.uint32(b) for p, b in zip(pairs, grp.bases)])
             glob = glob[rng.permutation(len(glob))]  # members interleaved in one stream
             buf = gs.DeviceBuffer(glob)
@@ -108,6 +110,10 @@ def test_group_device_resident_global_pairs():
             buf.free()
             for i, (n, s) in enumerate(zip(sizes, seeds)):
                 assert np.array_equal(stores[i].export_buckets(), oracle_store(n, s, pairs[i]))
+            labels, ncc, _, status = grp.query_cc()
+            for i, n in enumerate(sizes):
+                assert status[i] == 0
+                assert np.array_equal(labels[i], orc.dsu_components(n, net_edges(pairs[i])))
     finally:
         for s in stores:
             s.close()
