@@ -426,33 +426,82 @@ int gs_sync(gs_store* h) {
   return sync_store(&h->impl);
 }
 
+// ---- reference-order transfers ----------------------------------------------------------------
+// A vertex's buckets in reference order are [c*B + d] for column c, depth d, then the deterministic
+// bucket; the device keeps them depth-major ([d*C + c], row stride vstride) with the deterministic
+// buckets in their own array. One thread per reference-order bucket of a chunk of vertices: the
+// reference-order side is read or written coalesced, the device side gathered within a row.
+__global__ void to_reference_order_kernel(const Bucket16* __restrict__ rows,
+                                          const Bucket16* __restrict__ det, uint32_t vstride,
+                                          uint32_t B, uint32_t C, uint32_t v0, uint64_t total,
+                                          Bucket16* __restrict__ out) {
+  const uint32_t nb = B * C + 1;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = (uint32_t)(i / nb), j = (uint32_t)(i % nb);
+    const uint32_t c = j / B, d = j % B;
+    out[i] = j == nb - 1 ? det[v0 + v] : rows[(uint64_t)(v0 + v) * vstride + (uint64_t)d * C + c];
+  }
+}
+
+__global__ void from_reference_order_kernel(const Bucket16* __restrict__ in, uint32_t vstride,
+                                            uint32_t B, uint32_t C, uint32_t v0, uint64_t total,
+                                            Bucket16* __restrict__ rows, Bucket16* __restrict__ det) {
+  const uint32_t nb = B * C + 1;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = (uint32_t)(i / nb), j = (uint32_t)(i % nb);
+    const uint32_t c = j / B, d = j % B;
+    if (j == nb - 1) det[v0 + v] = in[i];
+    else rows[(uint64_t)(v0 + v) * vstride + (uint64_t)d * C + c] = in[i];
+  }
+}
+
+// XOR of bucket slot j over the chunk's vertices into acc[j] (slots across threads: coalesced)
+__global__ void slot_xor_kernel(const Bucket16* __restrict__ in, uint32_t nb, uint32_t count,
+                                uint32_t per_block, unsigned long long* acc) {
+  const uint32_t v_lo = blockIdx.y * per_block;
+  const uint32_t v_hi = min(count, v_lo + per_block);
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nb; j += gridDim.x * blockDim.x) {
+    uint64_t a = 0, g = 0;
+    for (uint32_t v = v_lo; v < v_hi; v++) {
+      a ^= in[(uint64_t)v * nb + j].alpha;
+      g ^= in[(uint64_t)v * nb + j].gamma;
+    }
+    if (a) atomicXor(&acc[2 * j], (unsigned long long)a);
+    if (g) atomicXor(&acc[2 * j + 1], (unsigned long long)g);
+  }
+}
+
+// vertices per transfer chunk: about 256 MB of reference-order buckets
+static uint32_t transfer_chunk(uint32_t nb) {
+  return std::max<uint32_t>(1, (uint32_t)((256ull << 20) / ((uint64_t)nb * sizeof(Bucket16))));
+}
+
 int gs_export_buckets(gs_store* h, uint32_t v0, uint32_t count, gs_bucket* out) {
   GS_CHECK(h && (out || count == 0), GS_ERR_INVALID, "null argument");
   Store& st = h->impl;
   GS_CHECK((uint64_t)v0 + count <= st.view.n, GS_ERR_INVALID, "vertex range out of bounds");
   int rc = gs_sync(h);
-  if (rc) return rc;
+  if (rc || count == 0) return rc;
   const uint32_t B = st.view.bpc, C = st.view.cols, vs = st.view.vstride;
   const uint32_t nb = B * C + 1;
-  const uint32_t per = 1024;
-  std::vector<Bucket16> rows((size_t)per * vs), det(per);
-  for (uint32_t a = 0; a < count; a += per) {
-    uint32_t m = std::min(per, count - a);
-    GS_HIP(hipMemcpy(rows.data(), st.view.rows + (size_t)(v0 + a) * vs,
-                     (size_t)m * vs * sizeof(Bucket16), hipMemcpyDeviceToHost));
-    GS_HIP(hipMemcpy(det.data(), st.view.det + (v0 + a), (size_t)m * sizeof(Bucket16),
-                     hipMemcpyDeviceToHost));
-    for (uint32_t i = 0; i < m; i++) {
-      gs_bucket* o = out + (size_t)(a + i) * nb;
-      const Bucket16* r = rows.data() + (size_t)i * vs;
-      for (uint32_t c = 0; c < C; c++)
-        for (uint32_t d = 0; d < B; d++) {
-          const Bucket16& b = r[(size_t)d * C + c];
-          o[(size_t)c * B + d] = gs_bucket{b.alpha, b.gamma};
-        }
-      o[nb - 1] = gs_bucket{det[i].alpha, det[i].gamma};
-    }
+  const uint32_t per = std::min(count, transfer_chunk(nb));
+  Bucket16* tmp = nullptr;
+  if (hipMalloc((void**)&tmp, (size_t)per * nb * sizeof(Bucket16)) != hipSuccess)
+    return fail(GS_ERR_OOM, "export staging of %u vertices unavailable", per);
+  hipError_t e = hipSuccess;
+  for (uint32_t a = 0; a < count && e == hipSuccess; a += per) {
+    const uint32_t m = std::min(per, count - a);
+    const uint64_t total = (uint64_t)m * nb;
+    hipLaunchKernelGGL(to_reference_order_kernel, dim3((uint32_t)std::min<uint64_t>((total + 255) / 256, 8192)),
+                       dim3(256), 0, nullptr, st.view.rows, st.view.det, vs, B, C, v0 + a, total, tmp);
+    e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipMemcpy(out + (size_t)a * nb, tmp, total * sizeof(Bucket16), hipMemcpyDeviceToHost);
   }
+  (void)hipFree(tmp);
+  GS_HIP(e);
   return GS_OK;
 }
 
@@ -462,40 +511,46 @@ int gs_import_buckets(gs_store* h, uint32_t v0, uint32_t count, const gs_bucket*
   GS_CHECK((uint64_t)v0 + count <= st.view.n, GS_ERR_INVALID, "vertex range out of bounds");
   int rc = gs_sync(h);
   if (rc) return rc;
-  const uint32_t B = st.view.bpc, C = st.view.cols, vs = st.view.vstride;
-  const uint32_t nb = B * C + 1;
-  const uint32_t per = 1024;
-  std::vector<Bucket16> rows((size_t)per * vs), det(per);
   // a whole store is symmetric (its vertex sketches XOR to zero) if the imported ones do; a part
   // of a store cannot be judged from the part alone
-  bool zero_sum = v0 == 0 && count == st.view.n;
-  if (zero_sum) {
-    std::vector<uint64_t> sum(2 * (size_t)nb, 0);
-    for (uint64_t i = 0; i < (uint64_t)count * nb; i++) {
-      sum[2 * (i % nb)] ^= in[i].alpha;
-      sum[2 * (i % nb) + 1] ^= in[i].gamma;
-    }
-    for (uint64_t x : sum) zero_sum = zero_sum && x == 0;
+  const bool whole = v0 == 0 && count == st.view.n;
+  st.symmetric = false;
+  if (count == 0) return GS_OK;
+  const uint32_t B = st.view.bpc, C = st.view.cols, vs = st.view.vstride;
+  const uint32_t nb = B * C + 1;
+  const uint32_t per = std::min(count, transfer_chunk(nb));
+  Bucket16* tmp = nullptr;
+  unsigned long long* acc = nullptr;
+  if (hipMalloc((void**)&tmp, (size_t)per * nb * sizeof(Bucket16)) != hipSuccess ||
+      hipMalloc((void**)&acc, 2 * (size_t)nb * sizeof(unsigned long long)) != hipSuccess) {
+    (void)hipFree(tmp);
+    return fail(GS_ERR_OOM, "import staging of %u vertices unavailable", per);
   }
+  hipError_t e = hipMemset(acc, 0, 2 * (size_t)nb * sizeof(unsigned long long));
+  for (uint32_t a = 0; a < count && e == hipSuccess; a += per) {
+    const uint32_t m = std::min(per, count - a);
+    const uint64_t total = (uint64_t)m * nb;
+    e = hipMemcpy(tmp, in + (size_t)a * nb, total * sizeof(Bucket16), hipMemcpyHostToDevice);
+    if (e != hipSuccess) break;
+    hipLaunchKernelGGL(from_reference_order_kernel, dim3((uint32_t)std::min<uint64_t>((total + 255) / 256, 8192)),
+                       dim3(256), 0, nullptr, tmp, vs, B, C, v0 + a, total, st.view.rows, st.view.det);
+    if (whole) {
+      const uint32_t pb = 256;  // vertices per block row
+      hipLaunchKernelGGL(slot_xor_kernel, dim3((nb + 255) / 256, (m + pb - 1) / pb), dim3(256), 0,
+                         nullptr, tmp, nb, m, pb, acc);
+    }
+    e = hipGetLastError();  // the next chunk's copy into tmp waits for these (same stream)
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+  std::vector<unsigned long long> sum(2 * (size_t)nb, 0);
+  if (e == hipSuccess && whole)
+    e = hipMemcpy(sum.data(), acc, sum.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+  (void)hipFree(tmp);
+  (void)hipFree(acc);
+  GS_HIP(e);
+  bool zero_sum = whole;
+  for (unsigned long long x : sum) zero_sum = zero_sum && x == 0;
   st.symmetric = zero_sum;
-  for (uint32_t a = 0; a < count; a += per) {
-    uint32_t m = std::min(per, count - a);
-    std::fill(rows.begin(), rows.end(), Bucket16{0, 0});
-    for (uint32_t i = 0; i < m; i++) {
-      const gs_bucket* src = in + (size_t)(a + i) * nb;
-      Bucket16* r = rows.data() + (size_t)i * vs;
-      for (uint32_t c = 0; c < C; c++)
-        for (uint32_t d = 0; d < B; d++) {
-          const gs_bucket& b = src[(size_t)c * B + d];
-          r[(size_t)d * C + c] = Bucket16{b.alpha, b.gamma};
-        }
-      det[i] = Bucket16{src[nb - 1].alpha, src[nb - 1].gamma};
-    }
-    GS_HIP(hipMemcpy(st.view.rows + (size_t)(v0 + a) * vs, rows.data(),
-                     (size_t)m * vs * sizeof(Bucket16), hipMemcpyHostToDevice));
-    GS_HIP(hipMemcpy(st.view.det + (v0 + a), det.data(), (size_t)m * sizeof(Bucket16),
-                     hipMemcpyHostToDevice));
-  }
   return GS_OK;
 }
 

@@ -1,14 +1,17 @@
 // Serving benchmark and mix-versus-alone check for SketchServer.
 //   serve_bench [--tenants 64] [--requests 1000] [--rate 2000] [--budget-gb 200] [--seed 7]
 //               [--mc-tenants 8] [--k 4] [--min-n 4096] [--max-n 262144]
-//               [--max-edges 1000000] [--alone-check 0|1]
+//               [--max-edges 1000000] [--alone-check 0|1] [--evict 0|1]
+//               [--max-step-updates N] [--pinned-spill 0|1]
 // Tenants: n log-uniform in [min-n, max-n]. Requests arrive as a Poisson process of the given
 // rate (requests/s): 1% connectivity queries on random tenants, 0.5% min-cut queries (k) on the
 // first mc-tenants tenants, the rest update batches of log-uniform 1k..max-edges edges, 5% of
 // which delete an earlier inserted edge of the tenant. The same trace runs twice: plain (one
 // request at a time on per-tenant streams) and served (continuous batching). Prints one JSON
 // line: updates/s and p50/p99 latencies per mode, and whether every answer and every final
-// sketch is identical in both modes (and, with --alone-check, for a tenant served alone).
+// sketch is identical in both modes (and, with --alone-check, for a tenant served alone). With a
+// budget below the tenants' total, --evict 1 (the default) spills idle tenants to host memory to
+// serve the others; --evict 0 serves only the tenants admitted before the first request.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -95,7 +98,8 @@ struct RunStats {
   double makespan = 0;
   double upd_per_s = 0;
   std::map<std::string, std::pair<double, double>> p50p99;  // kind -> (p50 ms, p99 ms)
-  uint64_t steps = 0;
+  uint64_t steps = 0, evictions = 0, restores = 0;
+  SketchServer::MoveStats moves;
   std::map<uint64_t, RequestResult> by_index;  // trace index -> result
   std::vector<std::vector<uint64_t>> exports;
   std::string error;
@@ -110,6 +114,7 @@ static double pct(std::vector<double> v, double p) {
 }
 
 static uint64_t g_max_step_updates = 0;  // 0: the server's default
+static bool g_evict = true, g_pinned_spill = true;
 
 static RunStats run(const Trace& tr, bool batching, uint64_t budget, uint32_t k,
                     const std::vector<uint32_t>& only_tenants, bool keep_exports) {
@@ -117,6 +122,8 @@ static RunStats run(const Trace& tr, bool batching, uint64_t budget, uint32_t k,
   cfg.batching = batching;
   cfg.memory_budget_bytes = budget;
   if (g_max_step_updates) cfg.max_step_updates = g_max_step_updates;
+  cfg.evict_idle = g_evict;
+  cfg.pinned_spill = g_pinned_spill;
   SketchServer srv(cfg);
   std::vector<int> tid(tr.tenants.size(), -1);
   for (size_t i = 0; i < tr.tenants.size(); i++)
@@ -128,7 +135,7 @@ static RunStats run(const Trace& tr, bool batching, uint64_t budget, uint32_t k,
   const double start = srv.now_s();
   for (size_t q = 0; q < tr.reqs.size(); q++) {
     const Req& r = tr.reqs[q];
-    if (tid[r.tenant] < 0 || !srv.is_admitted((uint32_t)tid[r.tenant])) continue;
+    if (tid[r.tenant] < 0 || (!g_evict && !srv.is_admitted((uint32_t)tid[r.tenant]))) continue;
     const double due = start + r.t;
     double now = srv.now_s();
     if (due > now) std::this_thread::sleep_for(std::chrono::duration<double>(due - now));
@@ -159,9 +166,12 @@ static RunStats run(const Trace& tr, bool batching, uint64_t budget, uint32_t k,
   st.upd_per_s = upd / st.makespan;
   for (auto& kv : lat) st.p50p99[kv.first] = {pct(kv.second, 50), pct(kv.second, 99)};
   st.steps = srv.steps();
+  st.evictions = srv.evictions();
+  st.restores = srv.restores();
+  st.moves = srv.move_stats();
   if (keep_exports)
     for (size_t i = 0; i < tr.tenants.size(); i++)
-      if (tid[i] >= 0 && srv.is_admitted((uint32_t)tid[i]))
+      if (tid[i] >= 0 && srv.has_sketch((uint32_t)tid[i]))
         st.exports.push_back(srv.export_tenant((uint32_t)tid[i]));
   return st;
 }
@@ -183,7 +193,13 @@ static bool same_answers(const RunStats& a, const RunStats& b, std::string* why)
 static std::string stats_json(const RunStats& s) {
   std::string j = "{\"updates_per_s\": " + std::to_string(s.upd_per_s) +
                   ", \"makespan_s\": " + std::to_string(s.makespan) +
-                  ", \"steps\": " + std::to_string(s.steps);
+                  ", \"steps\": " + std::to_string(s.steps) +
+                  ", \"evictions\": " + std::to_string(s.evictions) +
+                  ", \"restores\": " + std::to_string(s.restores) +
+                  ", \"spill_gb\": " + std::to_string(s.moves.spill_bytes / 1e9) +
+                  ", \"spill_s\": " + std::to_string(s.moves.spill_s) +
+                  ", \"restore_gb\": " + std::to_string(s.moves.restore_bytes / 1e9) +
+                  ", \"restore_s\": " + std::to_string(s.moves.restore_s);
   for (auto& kv : s.p50p99)
     j += ", \"" + kv.first + "_ms\": {\"p50\": " + std::to_string(kv.second.first) +
          ", \"p99\": " + std::to_string(kv.second.second) + "}";
@@ -210,6 +226,8 @@ int main(int argc, char** argv) {
     else if (a == "--max-edges") max_edges = std::stoull(v);
     else if (a == "--alone-check") alone = std::atoi(v);
     else if (a == "--max-step-updates") g_max_step_updates = std::stoull(v);
+    else if (a == "--evict") g_evict = std::atoi(v) != 0;
+    else if (a == "--pinned-spill") g_pinned_spill = std::atoi(v) != 0;
     else {
       std::fprintf(stderr, "unknown option %s\n", a.c_str());
       return 2;

@@ -45,7 +45,9 @@ SketchServer::~SketchServer() {
   for (auto& t : tenants) {
     if (t->stream) gs_stream_destroy(t->stream);
     if (t->store) gs_store_destroy(t->store);
+    give_back(t->spill);
   }
+  for (auto& kv : pinned_free) gs_host_free(kv.second);
 }
 
 double SketchServer::now_s() const {
@@ -78,6 +80,11 @@ void SketchServer::remove_tenant(uint32_t t) {
 bool SketchServer::is_admitted(uint32_t t) {
   std::lock_guard<std::mutex> lk(mu);
   return tenants.at(t)->admitted;
+}
+
+bool SketchServer::has_sketch(uint32_t t) {
+  std::lock_guard<std::mutex> lk(mu);
+  return tenants.at(t)->admitted || tenants.at(t)->spilled;
 }
 
 void SketchServer::wait_admissions() {
@@ -145,6 +152,7 @@ std::vector<uint64_t> SketchServer::export_tenant(uint32_t t) {
   drain();
   std::lock_guard<std::mutex> lk(mu);
   Tenant& tn = *tenants.at(t);
+  if (tn.spilled) return std::vector<uint64_t>(tn.spill.p, tn.spill.p + tn.spill.words);
   if (!tn.store) throw std::invalid_argument("tenant has no store");
   gs_geometry g{};
   gs_throw_if(gs_store_geometry(tn.store, &g));
@@ -166,6 +174,130 @@ void SketchServer::admit_locked() {
     admitted += t.bytes;
     waiting.erase(waiting.begin());
   }
+}
+
+// The first tenant in queue order that has a pending request but no store: if its memory fits
+// now, or once idle resident tenants (no unanswered request) are spilled, least recently served
+// first, sets *need and the tenants to spill and returns true. Only the first such tenant is
+// considered, so tenants become resident in the order their requests arrived. (mu held)
+bool SketchServer::plan_room_locked(std::vector<uint32_t>* evict, uint32_t* need) {
+  if (!cfg.evict_idle) return false;
+  for (auto& r : queue) {
+    const Tenant& t = *tenants[r.tenant];
+    if (t.admitted || t.removed) continue;
+    uint64_t free = cfg.memory_budget_bytes - admitted;
+    std::vector<uint32_t> idle;
+    if (t.bytes > free) {
+      for (uint32_t i = 0; i < tenants.size(); i++)
+        if (tenants[i]->admitted && !tenants[i]->removed && tenants[i]->queued == 0)
+          idle.push_back(i);
+      std::sort(idle.begin(), idle.end(), [&](uint32_t a, uint32_t b) {
+        return tenants[a]->last_step != tenants[b]->last_step
+                   ? tenants[a]->last_step < tenants[b]->last_step
+                   : a < b;
+      });
+      size_t k = 0;
+      while (k < idle.size() && t.bytes > free) free += tenants[idle[k++]]->bytes;
+      if (t.bytes > free) return false;
+      idle.resize(k);
+    }
+    if (evict) *evict = std::move(idle);
+    if (need) *need = r.tenant;
+    return true;
+  }
+  return false;
+}
+
+// a host buffer of at least `words` words: a free pinned one of at most twice that, else a new
+// one (worker thread)
+SketchServer::SpillBuf SketchServer::take_spill_buffer(size_t words) {
+  SpillBuf b;
+  b.words = words;
+  if (!cfg.pinned_spill) {
+    b.p = new uint64_t[words];  // uninitialised: the export writes every word
+    b.cap = words;
+    return b;
+  }
+  b.pinned = true;
+  auto it = pinned_free.lower_bound(words);
+  if (it != pinned_free.end() && it->first <= 2 * words) {
+    b.cap = it->first;
+    b.p = it->second;
+    pinned_free.erase(it);
+    return b;
+  }
+  void* p = nullptr;
+  gs_throw_if(gs_host_alloc(words * sizeof(uint64_t), &p));
+  b.p = static_cast<uint64_t*>(p);
+  b.cap = words;
+  return b;
+}
+
+void SketchServer::give_back(SpillBuf& b) {
+  if (b.p) {
+    if (b.pinned) pinned_free.emplace(b.cap, b.p);
+    else delete[] b.p;
+  }
+  b = SpillBuf();
+}
+
+// spills `evict` to host memory, then gives `need` a store (restored from its spill, or new)
+void SketchServer::make_resident(uint32_t need, const std::vector<uint32_t>& evict) {
+  if (!evict.empty() && group) {
+    // drop the group before the stores it refers to
+    gs_throw_if(gs_group_destroy(group));
+    group = nullptr;
+    group_members.clear();
+  }
+  for (uint32_t i : evict) {
+    const double t_start = now_s();
+    Tenant& t = *tenants[i];
+    const uint64_t n = t.spec.num_vertices;
+    gs_geometry g{};
+    gs_throw_if(gs_store_geometry(t.store, &g));
+    const size_t words = n * g.num_buckets * 2;
+    SpillBuf buf = take_spill_buffer(words);
+    int rc = gs_export_buckets(t.store, 0, (uint32_t)n, reinterpret_cast<gs_bucket*>(buf.p));
+    if (rc != GS_OK) give_back(buf);
+    gs_throw_if(rc);
+    if (t.stream) gs_throw_if(gs_stream_destroy(t.stream));
+    gs_throw_if(gs_store_destroy(t.store));
+    std::lock_guard<std::mutex> lk(mu);
+    t.stream = nullptr;
+    t.store = nullptr;
+    t.spill = buf;
+    t.spilled = true;
+    t.admitted = false;
+    admitted -= t.bytes;
+    num_evictions++;
+    moves.spill_s += now_s() - t_start;
+    moves.spill_bytes += words * sizeof(uint64_t);
+  }
+  const double t_start = now_s();
+  Tenant& t = *tenants[need];
+  const uint32_t samples =
+      (uint32_t)Sketch::calc_cc_samples(t.spec.num_vertices, t.spec.sketches_factor);
+  gs_store* st = nullptr;
+  gs_stream* stream = nullptr;
+  gs_throw_if(gs_store_create(t.spec.num_vertices, t.spec.seed, samples, 1, 0, &st));
+  if (t.spilled)
+    gs_throw_if(gs_import_buckets(st, 0, t.spec.num_vertices,
+                                  reinterpret_cast<const gs_bucket*>(t.spill.p)));
+  if (!cfg.batching) gs_throw_if(gs_stream_create(st, &stream));
+  std::lock_guard<std::mutex> lk(mu);
+  if (t.spilled) {
+    moves.restore_s += now_s() - t_start;
+    moves.restore_bytes += t.spill.words * sizeof(uint64_t);
+    give_back(t.spill);
+    t.spilled = false;
+    num_restores++;
+  } else {
+    waiting.erase(std::find(waiting.begin(), waiting.end(), need));
+  }
+  t.store = st;
+  t.stream = stream;
+  t.admitted = true;
+  admitted += t.bytes;
 }
 
 void SketchServer::ensure_group() {
@@ -247,7 +379,8 @@ void SketchServer::apply_updates(std::vector<Request*>& upd) {
 }
 
 void SketchServer::run_step(std::vector<Request>& step) {
-  num_steps++;
+  const uint64_t step_no = ++num_steps;
+  for (auto& r : step) tenants[r.tenant]->last_step = step_no;
   if (cfg.batching) ensure_group();
   std::vector<Request*> upd_q, upd_o, cc, mc;
   std::vector<char> queried(tenants.size(), 0);
@@ -348,7 +481,8 @@ void SketchServer::cut_loop() {
 void SketchServer::loop() {
   for (;;) {
     std::vector<Request> step;
-    std::vector<uint32_t> release;
+    std::vector<uint32_t> release, evict;
+    uint32_t need = UINT32_MAX;
     {
       std::unique_lock<std::mutex> lk(mu);
       auto runnable = [&] {
@@ -363,7 +497,8 @@ void SketchServer::loop() {
       };
       cv.wait(lk, [&] {
         return stop || runnable() || removable() ||
-               (!waiting.empty() && admitted + tenants[waiting.front()]->bytes <= cfg.memory_budget_bytes);
+               (!waiting.empty() && admitted + tenants[waiting.front()]->bytes <= cfg.memory_budget_bytes) ||
+               plan_room_locked(nullptr, nullptr);
       });
       if (stop && queue.empty()) return;
       for (uint32_t i = 0; i < tenants.size(); i++)
@@ -371,8 +506,8 @@ void SketchServer::loop() {
           tenants[i]->removed = true;
           release.push_back(i);
         }
-      if (release.empty()) {
-        admit_locked();
+      if (release.empty()) admit_locked();
+      if (release.empty() && !plan_room_locked(&evict, &need)) {
         std::vector<char> blocked(tenants.size(), 0);
         uint64_t nupd = 0;
         for (auto it = queue.begin(); it != queue.end();) {
@@ -410,8 +545,37 @@ void SketchServer::loop() {
         t.store = nullptr;
         if (t.admitted) admitted -= t.bytes;
         t.admitted = false;
+        give_back(t.spill);
+        t.spilled = false;
+        auto w = std::find(waiting.begin(), waiting.end(), i);
+        if (w != waiting.end()) waiting.erase(w);
       }
       admit_locked();
+    } else if (need != UINT32_MAX) {
+      try {
+        make_resident(need, evict);
+      } catch (const std::exception& e) {
+        // a tenant that cannot be made resident fails its pending requests
+        std::lock_guard<std::mutex> lk(mu);
+        last_error = e.what();
+        for (auto it = queue.begin(); it != queue.end();) {
+          if (it->tenant != need) {
+            ++it;
+            continue;
+          }
+          RequestResult res;
+          res.id = it->id;
+          res.tenant = it->tenant;
+          res.kind = it->kind;
+          res.arrival_s = it->arrival_s;
+          res.done_s = now_s();
+          res.status = GS_ERR_HIP;
+          tenants[need]->queued--;
+          answered++;
+          done.push_back(std::move(res));
+          it = queue.erase(it);
+        }
+      }
     } else {
       try {
         run_step(step);

@@ -15,6 +15,7 @@
 #include <condition_variable>
 #include <cstdint>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -28,6 +29,13 @@ struct ServerConfig {
   uint64_t memory_budget_bytes = 200ull * 1000 * 1000 * 1000;  // sketch memory of admitted tenants
   uint64_t max_step_updates = 1ull << 25;  // updates per device submission when batching
   bool batching = true;
+  // a tenant with pending requests that does not fit the budget makes room by spilling idle
+  // resident tenants (least recently served first) to host memory; a spilled tenant is restored
+  // bit-exact when its next request comes
+  bool evict_idle = true;
+  // spills go to pinned host buffers, kept for reuse by later spills once their tenant is restored
+  // (else to pageable memory, allocated per spill)
+  bool pinned_spill = true;
   bool keep_labels = false;  // keep the labels of every CC answer (tests)
 };
 
@@ -65,6 +73,8 @@ class SketchServer {
   // release a tenant after its queued requests; its memory admits waiting tenants
   void remove_tenant(uint32_t tenant);
   bool is_admitted(uint32_t tenant);
+  // resident, or spilled to host memory
+  bool has_sketch(uint32_t tenant);
   // wait until every waiting tenant that fits the budget has been admitted
   void wait_admissions();
   uint64_t admitted_bytes();
@@ -81,6 +91,17 @@ class SketchServer {
   std::vector<uint64_t> export_tenant(uint32_t tenant);
   double now_s() const;
   uint64_t steps() const { return num_steps.load(); }
+  uint64_t evictions() const { return num_evictions.load(); }
+  uint64_t restores() const { return num_restores.load(); }
+  // seconds spent spilling / restoring tenants and the bytes moved (after drain)
+  struct MoveStats {
+    double spill_s = 0, restore_s = 0;
+    uint64_t spill_bytes = 0, restore_bytes = 0;
+  };
+  MoveStats move_stats() {
+    std::lock_guard<std::mutex> lk(mu);
+    return moves;
+  }
   // message of the last device error that failed a step ("" if none)
   std::string error() {
     std::lock_guard<std::mutex> lk(mu);
@@ -96,6 +117,12 @@ class SketchServer {
     std::vector<uint32_t> pairs;
     double arrival_s;
   };
+  struct SpillBuf {
+    uint64_t* p = nullptr;
+    size_t words = 0;  // in use
+    size_t cap = 0;    // allocated
+    bool pinned = false;
+  };
   struct Tenant {
     TenantSpec spec;
     uint64_t bytes = 0;
@@ -103,10 +130,17 @@ class SketchServer {
     gs_stream* stream = nullptr;  // plain mode
     bool admitted = false, removing = false, removed = false;
     uint64_t queued = 0;          // requests not yet answered
+    bool spilled = false;         // buckets held in `spill` (reference order), no store
+    SpillBuf spill;
+    uint64_t last_step = 0;       // last step that served the tenant
   };
 
   void loop();
   void admit_locked();
+  bool plan_room_locked(std::vector<uint32_t>* evict, uint32_t* need);
+  void make_resident(uint32_t need, const std::vector<uint32_t>& evict);
+  SpillBuf take_spill_buffer(size_t words);
+  void give_back(SpillBuf& b);
   void run_step(std::vector<Request>& step);
   void apply_updates(std::vector<Request*>& upd);
   void ensure_group();
@@ -133,7 +167,9 @@ class SketchServer {
   bool stop = false;
   bool busy = false;
   std::vector<RequestResult> done;
-  std::atomic<uint64_t> num_steps{0};
+  std::atomic<uint64_t> num_steps{0}, num_evictions{0}, num_restores{0};
+  MoveStats moves;
+  std::multimap<size_t, uint64_t*> pinned_free;  // pinned spill buffers by capacity (words)
   std::string last_error;
   // store group over the admitted tenants (batching mode), rebuilt when membership changes
   gs_group* group = nullptr;
