@@ -25,15 +25,33 @@ constexpr int kQThreads = 256;
 // partition is final (or out of samples): every later round's kernels return at once,
 // [kFlagRound] index of the round being run, [kFlagRounds] rounds the query took
 constexpr uint32_t kNoGiant = 0xFFFFFFFFu;
-constexpr int kBig = 8;
+// big supernodes per round (pick_big_kernel) and the share of kPickSamples sampled vertices one
+// must hold: 2 of 256 at n = 2^17 is about 1000 members (R-MAT, plain CC query: 1.10 ms against
+// 1.22 ms with 8 of at least 4 samples; 1024 samples cost more to pick than they save)
+constexpr int kBig = 16;
 constexpr int kPickSamples = 256;
-constexpr uint32_t kMinBig = 4;
+constexpr uint32_t kMinBig = 2;
 constexpr int kFlagBig = 4;
-constexpr int kFlagDone = 12;
-constexpr int kFlagRound = 13;
-constexpr int kFlagRounds = 14;
-constexpr int kFlagWords = 16;
+constexpr int kFlagDone = kFlagBig + kBig;
+constexpr int kFlagRound = kFlagDone + 1;
+constexpr int kFlagRounds = kFlagDone + 2;
+constexpr int kFlagWords = kFlagDone + 4;
 constexpr int kAccBatch = 8;  // buckets loaded together per lane by the group accumulation
+// Vertices per wave of the per-vertex kernels of the single-store query (one lane per vertex for
+// the vertex attributes, the wave's vertices then taken a few at a time, one bucket per lane).
+// Fewer vertices per wave put more waves in flight but more of them on the same hot rows and
+// union-find words at once: at n = 2^17 (R-MAT) 16, 8 and 4 per wave were 1.3x, 1.8x and 2.7x
+// slower than 64.
+constexpr uint32_t kR0Verts = 64;
+constexpr uint32_t kAccVerts = 64;
+constexpr uint32_t kSampleVerts = 64;
+static_assert(kR0Verts <= 64 && kAccVerts <= 64 && kSampleVerts <= 64, "a lane per vertex");
+
+// blocks of kQThreads threads for waves of `per_wave` vertices over n vertices
+static inline dim3 wave_grid(uint32_t n, uint32_t per_wave) {
+  const uint64_t waves = ((uint64_t)n + per_wave - 1) / per_wave;
+  return dim3((uint32_t)((waves * 64 + kQThreads - 1) / kQThreads));
+}
 
 __device__ __forceinline__ uint32_t load_parent(uint32_t* p, uint32_t x) {
   return __hip_atomic_load(&p[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -78,25 +96,41 @@ __device__ __forceinline__ int sample_column(const Bucket16 det, const Bucket16*
   return 2;
 }
 
+// One sample per active lane of the wave (self: the lane's root). The words every lane of the
+// grid would otherwise write -- the round's modified flag, the forest counter, the merged flag of
+// a root that absorbs many others (a hub) -- are written once per wave, or only when not yet set.
 __device__ __forceinline__ void handle_sample(int res, uint64_t idx, uint32_t self, uint32_t n,
                                               uint32_t* parent, uint32_t* vflags,
                                               uint32_t* flags, uint32_t* forest,
                                               uint64_t forest_cap) {
-  if (res == 2) {
-    flags[0] = 1;
-  } else if (res == 1) {
+  const uint32_t a = (uint32_t)(idx >> 32), b = (uint32_t)idx;
+  bool merged = false;
+  if (res == 1) {
     atomicOr(&vflags[self], kVZero);
-  } else {
-    uint32_t a = (uint32_t)(idx >> 32), b = (uint32_t)idx;
-    if (a < n && b < n && a != b && unite(parent, a, b)) {
-      // the new root is the smaller of the two roots; find it again after the link
-      atomicOr(&vflags[find_root(parent, a)], kVMerged);
-      flags[0] = 1;
-      uint32_t pos = atomicAdd(&flags[2], 1u);
-      if (pos < forest_cap) {
-        forest[2 * (uint64_t)pos] = min(a, b);
-        forest[2 * (uint64_t)pos + 1] = max(a, b);
-      }
+  } else if (res == 0 && a < n && b < n && a != b && unite(parent, a, b)) {
+    merged = true;
+    // the new root is the smaller of the two roots; find it again after the link
+    const uint32_t r = find_root(parent, a);
+    if (!(__hip_atomic_load(&vflags[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kVMerged))
+      atomicOr(&vflags[r], kVMerged);
+  }
+  const uint32_t lane = __lane_id();
+  const uint64_t active = __ballot(1);
+  const uint64_t mm = __ballot(merged);
+  const uint32_t leader = (uint32_t)__builtin_ctzll(active);
+  const bool modified = (__ballot(res == 2) | mm) != 0;
+  uint32_t base = 0;
+  if (lane == leader) {
+    if (modified && !__hip_atomic_load(&flags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      __hip_atomic_store(&flags[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (mm) base = atomicAdd(&flags[2], (uint32_t)__builtin_popcountll(mm));
+  }
+  base = __shfl(base, (int)leader);
+  if (merged) {
+    const uint32_t pos = base + (uint32_t)__builtin_popcountll(mm & ((1ULL << lane) - 1));
+    if (pos < forest_cap) {
+      forest[2 * (uint64_t)pos] = min(a, b);
+      forest[2 * (uint64_t)pos + 1] = max(a, b);
     }
   }
 }
@@ -234,9 +268,9 @@ __global__ __launch_bounds__(kQThreads) void round0_wide_kernel(StoreView st, ui
                                                                 uint32_t* forest,
                                                                 uint64_t forest_cap) {
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64 * 64;
+  const uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64 * kR0Verts;
   if (v0 >= st.n) return;
-  const uint32_t cnt = min(64u, st.n - v0);
+  const uint32_t cnt = min(kR0Verts, st.n - v0);
   const uint32_t sw = cps * st.bpc;
   int my_res = -1;
   uint64_t my_idx = 0;
@@ -257,7 +291,7 @@ __global__ __launch_bounds__(kQThreads) void round0_wide_kernel(StoreView st, ui
 }
 
 // Round 0: every vertex samples its own column col0 (run_round_zero, src/cc_sketch_alg.cpp:
-// 250-271). A wave takes 64 consecutive vertices, four at a time, one bucket per lane. Each
+// 250-271). A wave takes kR0Verts consecutive vertices, four at a time, one bucket per lane. Each
 // bucket of the column is a 128-byte line of its own (depth-major rows), so the first pass reads
 // the deterministic bucket and depths 0..15 only; a vertex with no good bucket among them reads
 // its deeper depths in a second pass (the scan order stays the reference's).
@@ -267,9 +301,9 @@ __global__ __launch_bounds__(kQThreads) void round0_kernel(StoreView st, uint32_
                                                            uint32_t* flags, uint32_t* forest,
                                                            uint64_t forest_cap) {
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64 * 64;
+  const uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64 * kR0Verts;
   if (v0 >= st.n) return;
-  const uint32_t cnt = min(64u, st.n - v0);
+  const uint32_t cnt = min(kR0Verts, st.n - v0);
   const uint32_t nd = min(st.bpc, kRound0Depths);  // depths of the first pass, det in lane nd
   int my_res = -1;  // the sample of vertex v0 + lane, handled by its own lane at the end
   uint64_t my_idx = 0;
@@ -309,7 +343,7 @@ __global__ __launch_bounds__(kQThreads) void round0_kernel(StoreView st, uint32_
   if (my_res >= 0) handle_sample(my_res, my_idx, v0 + lane, st.n, parent, vflags, flags, forest, forest_cap);
 }
 
-// Big supernodes of a round, picked from 256 sampled vertices (pick_big_kernel): up to kBig live
+// Big supernodes of a round, picked from kPickSamples sampled vertices (pick_big_kernel): up to kBig live
 // supernodes holding at least kMinBig of the samples each (flags[4 + k], kNoGiant when unused),
 // largest first. Their column sums are formed from per-wave partial sums instead of atomics
 // into one accumulator row (which thousands of waves would contend for). When the store's
@@ -342,10 +376,12 @@ __global__ __launch_bounds__(kPickSamples) void pick_big_kernel(uint32_t n,
     if (same && i < first) first = i;
   }
   const uint32_t key = (r != kNoGiant && first == t && cnt >= kMinBig) ? (cnt << 16) | t : 0u;
-  if (key) s_cand[atomicAdd(&s_ncand, 1u)] = key;  // at most kPickSamples / kMinBig = 64
+  if (key) s_cand[atomicAdd(&s_ncand, 1u)] = key;  // at most kPickSamples / kMinBig
   __syncthreads();
   if (t >= 64) return;
-  // wave 0: bitonic sort of the candidates over 64 lanes, largest first
+  // wave 0: bitonic sort of (the first 64 of) the candidates over 64 lanes, largest first. More
+  // than 64 candidates leave no giant (one holds over half of the samples, leaving at most
+  // kPickSamples / 2 / kMinBig - 1 = 63 others), so a giant is always among the sorted ones.
   const uint32_t nc = s_ncand;
   uint32_t x = t < nc ? s_cand[t] : 0u;
   for (uint32_t k = 2; k <= 64; k <<= 1)
@@ -368,9 +404,9 @@ __global__ __launch_bounds__(kPickSamples) void pick_big_kernel(uint32_t n,
 // bucket; w = cps*B + 1 slots) of the live supernodes. A wave takes 64 consecutive vertices and
 // walks the ones it must read, four at a time; lane l holds slot 64*blockIdx.y + l (so one
 // vertex is one load instruction, and its row of atomics is one instruction over consecutive
-// addresses). Big supernodes accumulate in registers and leave per-wave partials
-// bpart[wave][k]; with a giant, every vertex outside it (finished supernodes included) also adds
-// into cpart[wave]; the other supernodes take atomics.
+// addresses). Big supernodes accumulate in registers and leave per-block partials
+// bpart[block][k] (the block's waves summed through LDS); with a giant, every vertex outside it
+// (finished supernodes included) also adds into cpart[block]; the other supernodes take atomics.
 __global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st, uint32_t cps,
                                                         uint32_t* __restrict__ parent,
                                                         const uint32_t* __restrict__ comp,
@@ -391,14 +427,14 @@ __global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st, uint32_t c
     if (u < st.n) parent[u] = comp[u];
   }
   const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
-  const uint32_t v0 = gwave * 64;
+  const uint32_t v0 = gwave * kAccVerts;
   const uint32_t giant = flags[3];
   uint32_t big[kBig];
 #pragma unroll
   for (int k = 0; k < kBig; k++) big[k] = flags[kFlagBig + k];
   const bool split = giant != kNoGiant;
   const uint32_t v = v0 + lane;
-  const bool inr = v < st.n;
+  const bool inr = lane < kAccVerts && v < st.n;
   const uint32_t r = inr ? comp[v] : 0;
   const bool done = inr && (vflags[r] & kVDone);
   const bool rd = inr && (split ? r != giant : !done);  // this vertex's column is read
@@ -454,28 +490,45 @@ __global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st, uint32_t c
       }
     }
   }
-  if (lj) {
-    if (split) cpart[(size_t)gwave * w + slot] = Bucket16{ca, cg};
-#pragma unroll
-    for (int k = 0; k < kBig; k++)
-      if (big[k] != kNoGiant) bpart[((size_t)gwave * kBig + k) * w + slot] = Bucket16{ba[k], bg[k]};
+  // the block's partial sums (one per slot): waves 1.. hand theirs to wave 0 through LDS
+  __shared__ Bucket16 s_red[kQThreads / 64 - 1][64];
+  const uint32_t wv = threadIdx.x / 64;
+  auto block_sum = [&](uint64_t& a, uint64_t& g) {
+    if (wv > 0) s_red[wv - 1][lane] = Bucket16{a, g};
+    __syncthreads();
+    if (wv == 0)
+      for (int q = 0; q < kQThreads / 64 - 1; q++) {
+        a ^= s_red[q][lane].alpha;
+        g ^= s_red[q][lane].gamma;
+      }
+    __syncthreads();
+  };
+  if (split) {
+    block_sum(ca, cg);
+    if (wv == 0 && lj) cpart[(size_t)blockIdx.x * w + slot] = Bucket16{ca, cg};
   }
+#pragma unroll
+  for (int k = 0; k < kBig; k++)
+    if (big[k] != kNoGiant) {
+      block_sum(ba[k], bg[k]);
+      if (wv == 0 && lj) bpart[((size_t)blockIdx.x * kBig + k) * w + slot] = Bucket16{ba[k], bg[k]};
+    }
 }
 
 // accumulators of the big supernodes (blockIdx.y = k < kBig) and of the giant (blockIdx.y =
-// kBig), bucket blockIdx.x: XOR of the wave partials (for the giant: the XOR over all vertices
-// being zero, the partials of everything outside it)
+// kBig), bucket blockIdx.x: XOR of the block partials of acc_kernel (for the giant: the XOR over
+// all vertices being zero, the partials of everything outside it)
 __global__ __launch_bounds__(kQThreads) void big_acc_kernel(const uint32_t* __restrict__ flags,
                                                             const Bucket16* __restrict__ cpart,
                                                             const Bucket16* __restrict__ bpart,
-                                                            uint32_t nwaves, uint32_t w,
+                                                            uint32_t nparts, uint32_t w,
                                                             Bucket16* __restrict__ acc) {
   const uint32_t k = blockIdx.y;
   const uint32_t root = k < (uint32_t)kBig ? flags[kFlagBig + k] : flags[3];
   if (flags[kFlagDone] || root == kNoGiant) return;
   const uint32_t j = blockIdx.x;
   uint64_t a = 0, g = 0;
-  for (uint32_t i = threadIdx.x; i < nwaves; i += kQThreads) {
+  for (uint32_t i = threadIdx.x; i < nparts; i += kQThreads) {
     const Bucket16 b = k < (uint32_t)kBig ? bpart[((size_t)i * kBig + k) * w + j] : cpart[(size_t)i * w + j];
     a ^= b.alpha;
     g ^= b.gamma;
@@ -498,8 +551,8 @@ __global__ __launch_bounds__(kQThreads) void big_acc_kernel(const uint32_t* __re
 }
 
 // Every live root samples its accumulator (bucket bpc = the deterministic bucket) and clears it
-// for the next round. A wave takes the roots among 64 consecutive vertices, four at a time, one
-// bucket per lane.
+// for the next round. A wave takes the roots among kSampleVerts consecutive vertices, four at a
+// time, one bucket per lane.
 __global__ __launch_bounds__(kQThreads) void sample_kernel(StoreView st,
                                                            const uint32_t* __restrict__ comp,
                                                            Bucket16* __restrict__ acc,
@@ -508,10 +561,10 @@ __global__ __launch_bounds__(kQThreads) void sample_kernel(StoreView st,
                                                            uint64_t forest_cap) {
   if (flags[kFlagDone]) return;
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64 * 64;
+  const uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64 * kSampleVerts;
   const uint32_t v = v0 + lane;
   const uint32_t w = st.bpc + 1;
-  uint64_t m = __ballot(v < st.n && comp[v] == v && !(vflags[v] & kVDone));
+  uint64_t m = __ballot(lane < kSampleVerts && v < st.n && comp[v] == v && !(vflags[v] & kVDone));
   int my_res = -1;  // the sample of root v0 + lane, handled by its own lane at the end
   uint64_t my_idx = 0;
   while (m) {
@@ -550,10 +603,10 @@ __global__ __launch_bounds__(kQThreads) void sample_wide_kernel(StoreView st, ui
                                                                 uint64_t forest_cap) {
   if (flags[kFlagDone]) return;
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64 * 64;
+  const uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64 * kSampleVerts;
   const uint32_t v = v0 + lane;
   const uint32_t sw = cps * st.bpc, w = sw + 1;
-  uint64_t m = __ballot(v < st.n && comp[v] == v && !(vflags[v] & kVDone));
+  uint64_t m = __ballot(lane < kSampleVerts && v < st.n && comp[v] == v && !(vflags[v] & kVDone));
   int my_res = -1;
   uint64_t my_idx = 0;
   while (m) {
@@ -625,9 +678,9 @@ static int ensure_query_scratch(Store* st, uint64_t forest_pairs) {
     GS_HIP(hipMalloc(&st->d_vflags, sizeof(uint32_t) * n));
     const uint64_t w = (uint64_t)st->cols_per_sample * st->view.bpc + 1;  // slots per sample
     GS_HIP(hipMalloc(&st->d_acc, sizeof(Bucket16) * (uint64_t)n * w));
-    const uint64_t nwaves = (((uint64_t)n + kQThreads - 1) / kQThreads) * (kQThreads / 64);
-    GS_HIP(hipMalloc(&st->d_cpart, sizeof(Bucket16) * w * nwaves));
-    GS_HIP(hipMalloc(&st->d_bpart, sizeof(Bucket16) * w * nwaves * kBig));
+    const uint64_t nparts = wave_grid(n, kAccVerts).x;  // acc_kernel blocks
+    GS_HIP(hipMalloc(&st->d_cpart, sizeof(Bucket16) * w * nparts));
+    GS_HIP(hipMalloc(&st->d_bpart, sizeof(Bucket16) * w * nparts * kBig));
     GS_HIP(hipMalloc(&st->d_flags, sizeof(uint32_t) * kFlagWords));
     GS_HIP(hipHostMalloc((void**)&st->h_flags, sizeof(uint32_t) * kFlagWords, hipHostMallocDefault));
     uint64_t cnt = (uint64_t)n * w;
@@ -668,7 +721,10 @@ static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t
   const uint32_t n = v.n;
   const dim3 grid((n + kQThreads - 1) / kQThreads), block(kQThreads);
   const uint32_t cps = st->cols_per_sample;
-  const uint32_t w = cps * v.bpc + 1, nwaves = grid.x * (kQThreads / 64);
+  const uint32_t w = cps * v.bpc + 1;
+  const dim3 g_r0 = wave_grid(n, kR0Verts), g_acc = wave_grid(n, kAccVerts),
+             g_smp = wave_grid(n, kSampleVerts);
+  const uint32_t nparts = g_acc.x;
   const bool wide = cps > 1;  // several columns per sample: the one-vertex-per-wave samplers
   hipStream_t q = st->qstream;
   uint32_t* forest = st->d_forest + 2 * forest_base;
@@ -679,10 +735,10 @@ static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t
   hipLaunchKernelGGL(clear_all_flags_kernel, dim3(1), dim3(64), 0, q, st->d_flags);
   // round 0
   if (wide)
-    hipLaunchKernelGGL(round0_wide_kernel, grid, block, 0, q, v, cps, col0, st->d_parent,
+    hipLaunchKernelGGL(round0_wide_kernel, g_r0, block, 0, q, v, cps, col0, st->d_parent,
                        st->d_vflags, st->d_flags, forest, cap);
   else
-    hipLaunchKernelGGL(round0_kernel, grid, block, 0, q, v, col0 * cps, st->d_parent,
+    hipLaunchKernelGGL(round0_kernel, g_r0, block, 0, q, v, col0 * cps, st->d_parent,
                        st->d_vflags, st->d_flags, forest, cap);
   hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
                      st->d_vflags, st->d_flags);
@@ -693,16 +749,16 @@ static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t
     for (uint32_t k = 0; k < kRoundsAhead; k++) {
       hipLaunchKernelGGL(pick_big_kernel, dim3(1), dim3(kPickSamples), 0, q, n, st->d_comp,
                          st->d_vflags, symmetric, st->d_flags);
-      hipLaunchKernelGGL(acc_kernel, dim3(grid.x, (w + 63) / 64), block, 0, q, v, cps,
+      hipLaunchKernelGGL(acc_kernel, dim3(g_acc.x, (w + 63) / 64), block, 0, q, v, cps,
                          st->d_parent, st->d_comp, st->d_vflags, st->d_flags, st->d_acc,
                          st->d_cpart, st->d_bpart);
       hipLaunchKernelGGL(big_acc_kernel, dim3(w, kBig + 1), dim3(kQThreads), 0, q, st->d_flags,
-                         st->d_cpart, st->d_bpart, nwaves, w, st->d_acc);
+                         st->d_cpart, st->d_bpart, nparts, w, st->d_acc);
       if (wide)
-        hipLaunchKernelGGL(sample_wide_kernel, grid, block, 0, q, v, cps, st->d_comp, st->d_acc,
+        hipLaunchKernelGGL(sample_wide_kernel, g_smp, block, 0, q, v, cps, st->d_comp, st->d_acc,
                            st->d_parent, st->d_vflags, st->d_flags, forest, cap);
       else
-        hipLaunchKernelGGL(sample_kernel, grid, block, 0, q, v, st->d_comp, st->d_acc,
+        hipLaunchKernelGGL(sample_kernel, g_smp, block, 0, q, v, st->d_comp, st->d_acc,
                            st->d_parent, st->d_vflags, st->d_flags, forest, cap);
       hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
                          st->d_vflags, st->d_flags);
@@ -814,8 +870,11 @@ __device__ __forceinline__ void handle_group_sample(int res, uint64_t idx, uint3
   } else {
     const uint32_t a = (uint32_t)(idx >> 32), b = (uint32_t)idx;
     if (a < d.n && b < d.n && a != b && unite(parent, a + d.base, b + d.base)) {
-      atomicOr(&vflags[find_root(parent, a + d.base)], kVMerged);
-      *gmod = 1;
+      const uint32_t r = find_root(parent, a + d.base);
+      if (!(__hip_atomic_load(&vflags[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kVMerged))
+        atomicOr(&vflags[r], kVMerged);
+      if (!__hip_atomic_load(gmod, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        __hip_atomic_store(gmod, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }
