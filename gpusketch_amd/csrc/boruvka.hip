@@ -45,7 +45,11 @@ constexpr int kAccBatch = 8;  // buckets loaded together per lane by the group a
 constexpr uint32_t kR0Verts = 64;
 constexpr uint32_t kAccVerts = 64;
 constexpr uint32_t kSampleVerts = 64;
-static_assert(kR0Verts <= 64 && kAccVerts <= 64 && kSampleVerts <= 64, "a lane per vertex");
+// the wide sampler takes one root at a time over 4 x 64 slots: 16 per wave (CubeSketch query at
+// n = 2^17: 1.87 ms against 1.98 with 64)
+constexpr uint32_t kSampleWideVerts = 16;
+static_assert(kR0Verts <= 64 && kAccVerts <= 64 && kSampleVerts <= 64 && kSampleWideVerts <= 64,
+              "a lane per vertex");
 
 // blocks of kQThreads threads for waves of `per_wave` vertices over n vertices
 static inline dim3 wave_grid(uint32_t n, uint32_t per_wave) {
@@ -603,10 +607,10 @@ __global__ __launch_bounds__(kQThreads) void sample_wide_kernel(StoreView st, ui
                                                                 uint64_t forest_cap) {
   if (flags[kFlagDone]) return;
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64 * kSampleVerts;
+  const uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64 * kSampleWideVerts;
   const uint32_t v = v0 + lane;
   const uint32_t sw = cps * st.bpc, w = sw + 1;
-  uint64_t m = __ballot(lane < kSampleVerts && v < st.n && comp[v] == v && !(vflags[v] & kVDone));
+  uint64_t m = __ballot(lane < kSampleWideVerts && v < st.n && comp[v] == v && !(vflags[v] & kVDone));
   int my_res = -1;
   uint64_t my_idx = 0;
   while (m) {
@@ -722,10 +726,10 @@ static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t
   const dim3 grid((n + kQThreads - 1) / kQThreads), block(kQThreads);
   const uint32_t cps = st->cols_per_sample;
   const uint32_t w = cps * v.bpc + 1;
-  const dim3 g_r0 = wave_grid(n, kR0Verts), g_acc = wave_grid(n, kAccVerts),
-             g_smp = wave_grid(n, kSampleVerts);
-  const uint32_t nparts = g_acc.x;
   const bool wide = cps > 1;  // several columns per sample: the one-vertex-per-wave samplers
+  const dim3 g_r0 = wave_grid(n, kR0Verts), g_acc = wave_grid(n, kAccVerts),
+             g_smp = wave_grid(n, wide ? kSampleWideVerts : kSampleVerts);
+  const uint32_t nparts = g_acc.x;
   hipStream_t q = st->qstream;
   uint32_t* forest = st->d_forest + 2 * forest_base;
   uint64_t cap = st->forest_cap - forest_base;
