@@ -280,10 +280,14 @@ void SketchServer::make_resident(uint32_t need, const std::vector<uint32_t>& evi
   gs_store* st = nullptr;
   gs_stream* stream = nullptr;
   gs_throw_if(gs_store_create(t.spec.num_vertices, t.spec.seed, samples, 1, 0, &st));
+  int rc = GS_OK;
   if (t.spilled)
-    gs_throw_if(gs_import_buckets(st, 0, t.spec.num_vertices,
-                                  reinterpret_cast<const gs_bucket*>(t.spill.p)));
-  if (!cfg.batching) gs_throw_if(gs_stream_create(st, &stream));
+    rc = gs_import_buckets(st, 0, t.spec.num_vertices, reinterpret_cast<const gs_bucket*>(t.spill.p));
+  if (rc == GS_OK && !cfg.batching) rc = gs_stream_create(st, &stream);
+  if (rc != GS_OK) {
+    gs_store_destroy(st);  // the spill stays, for a later attempt
+    gs_throw_if(rc);
+  }
   std::lock_guard<std::mutex> lk(mu);
   if (t.spilled) {
     moves.restore_s += now_s() - t_start;
