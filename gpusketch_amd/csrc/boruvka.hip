@@ -127,7 +127,8 @@ __device__ __forceinline__ void handle_sample(int res, uint64_t idx, uint32_t se
   if (lane == leader) {
     if (modified && !__hip_atomic_load(&flags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
       __hip_atomic_store(&flags[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (mm) base = atomicAdd(&flags[2], (uint32_t)__builtin_popcountll(mm));
+    // (a query that returns no forest records none: no counter, the grid's one shared atomic)
+    if (mm && forest_cap) base = atomicAdd(&flags[2], (uint32_t)__builtin_popcountll(mm));
   }
   base = __shfl(base, (int)leader);
   if (merged) {
@@ -720,7 +721,7 @@ static int ensure_query_scratch(Store* st, uint64_t forest_pairs) {
 // per batch: a round after the final one only launches kernels that return at once.
 constexpr uint32_t kRoundsAhead = 4;
 static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t* rounds_out,
-                        uint64_t* forest_len) {
+                        uint64_t* forest_len, bool record_forest = true) {
   const StoreView& v = st->view;
   const uint32_t n = v.n;
   const dim3 grid((n + kQThreads - 1) / kQThreads), block(kQThreads);
@@ -731,8 +732,8 @@ static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t
              g_smp = wave_grid(n, wide ? kSampleWideVerts : kSampleVerts);
   const uint32_t nparts = g_acc.x;
   hipStream_t q = st->qstream;
-  uint32_t* forest = st->d_forest + 2 * forest_base;
-  uint64_t cap = st->forest_cap - forest_base;
+  uint32_t* forest = record_forest ? st->d_forest + 2 * forest_base : nullptr;
+  uint64_t cap = record_forest ? st->forest_cap - forest_base : 0;
   if (col0 >= st->num_samples) return fail(GS_ERR_OUT_OF_SAMPLES, "sketch out of samples");
   hipLaunchKernelGGL(init_query_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
                      st->d_vflags);
@@ -798,11 +799,12 @@ int boruvka_query(Store* st, const uint32_t*, uint32_t* label_out, uint32_t* num
   rc = ensure_query_scratch(st, st->view.n);
   if (rc) return rc;
   uint64_t fl = 0;
-  rc = boruvka_pass(st, 0, 0, rounds, &fl);
+  const bool record = forest_out && forest_cap;
+  rc = boruvka_pass(st, 0, 0, rounds, &fl, record);
   if (rc) return rc;
   const uint32_t n = st->view.n;
   if (label_out) GS_HIP(hipMemcpy(label_out, st->d_comp, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
-  if (num_cc) {
+  if (num_cc || (forest_len && !record)) {
     std::vector<uint32_t> lab;
     const uint32_t* l = label_out;
     if (!l) {
@@ -812,7 +814,8 @@ int boruvka_query(Store* st, const uint32_t*, uint32_t* label_out, uint32_t* num
     }
     uint32_t c = 0;
     for (uint32_t i = 0; i < n; i++) c += l[i] == i;
-    *num_cc = c;
+    if (num_cc) *num_cc = c;
+    if (!record) fl = n - c;  // a spanning forest has one edge per union: n - components
   }
   if (forest_len) *forest_len = fl;
   if (forest_out && forest_cap) {
