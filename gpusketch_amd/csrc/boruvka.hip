@@ -39,17 +39,14 @@ constexpr int kFlagWords = kFlagDone + 4;
 constexpr int kAccBatch = 8;  // buckets loaded together per lane by the group accumulation
 // Vertices per wave of the per-vertex kernels of the single-store query (one lane per vertex for
 // the vertex attributes, the wave's vertices then taken a few at a time, one bucket per lane).
-// Fewer vertices per wave put more waves in flight but more of them on the same hot rows and
-// union-find words at once: at n = 2^17 (R-MAT) 16, 8 and 4 per wave were 1.3x, 1.8x and 2.7x
-// slower than 64.
-constexpr uint32_t kR0Verts = 64;
+// Fewer vertices per wave put more waves in flight (64 per wave is 2 waves per SIMD at n = 2^17)
+// but more of them on the same hot rows and words at once. CC query at n = 2^17 (R-MAT), per
+// (round 0, accumulate, sample): 64/64/64 0.93 ms, 16/64/16 0.89, 16/32/16 0.92, 4/64/64 0.92;
+// while every wave still added to one shared forest counter, 16 and fewer were up to 2.7x slower.
+constexpr uint32_t kR0Verts = 16;
 constexpr uint32_t kAccVerts = 64;
-constexpr uint32_t kSampleVerts = 64;
-// the wide sampler takes one root at a time over 4 x 64 slots: 16 per wave (CubeSketch query at
-// n = 2^17: 1.87 ms against 1.98 with 64)
-constexpr uint32_t kSampleWideVerts = 16;
-static_assert(kR0Verts <= 64 && kAccVerts <= 64 && kSampleVerts <= 64 && kSampleWideVerts <= 64,
-              "a lane per vertex");
+constexpr uint32_t kSampleVerts = 16;
+static_assert(kR0Verts <= 64 && kAccVerts <= 64 && kSampleVerts <= 64, "a lane per vertex");
 
 // blocks of kQThreads threads for waves of `per_wave` vertices over n vertices
 static inline dim3 wave_grid(uint32_t n, uint32_t per_wave) {
@@ -608,10 +605,10 @@ __global__ __launch_bounds__(kQThreads) void sample_wide_kernel(StoreView st, ui
                                                                 uint64_t forest_cap) {
   if (flags[kFlagDone]) return;
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64 * kSampleWideVerts;
+  const uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64 * kSampleVerts;
   const uint32_t v = v0 + lane;
   const uint32_t sw = cps * st.bpc, w = sw + 1;
-  uint64_t m = __ballot(lane < kSampleWideVerts && v < st.n && comp[v] == v && !(vflags[v] & kVDone));
+  uint64_t m = __ballot(lane < kSampleVerts && v < st.n && comp[v] == v && !(vflags[v] & kVDone));
   int my_res = -1;
   uint64_t my_idx = 0;
   while (m) {
@@ -729,7 +726,7 @@ static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t
   const uint32_t w = cps * v.bpc + 1;
   const bool wide = cps > 1;  // several columns per sample: the one-vertex-per-wave samplers
   const dim3 g_r0 = wave_grid(n, kR0Verts), g_acc = wave_grid(n, kAccVerts),
-             g_smp = wave_grid(n, wide ? kSampleWideVerts : kSampleVerts);
+             g_smp = wave_grid(n, kSampleVerts);
   const uint32_t nparts = g_acc.x;
   hipStream_t q = st->qstream;
   uint32_t* forest = record_forest ? st->d_forest + 2 * forest_base : nullptr;
