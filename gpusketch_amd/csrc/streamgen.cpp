@@ -3,10 +3,16 @@
 // this library's own. Every stream has toggle semantics: an update of a live edge deletes it, so
 // each generator guarantees that deletions only hit live edges and inserts never duplicate a
 // live edge, and it never emits self loops.
+#include <sched.h>
+
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
+#include <memory>
+#include <thread>
 #include <unordered_set>
 #include <vector>
 
@@ -95,33 +101,6 @@ void emit(const std::vector<uint64_t>& keys, uint32_t* pairs) {
   }
 }
 
-// live-edge membership for R-MAT: dense bitset over the key space when small enough
-struct LiveSet {
-  std::vector<uint64_t> bits;
-  std::unordered_set<uint64_t> hs;
-  uint32_t scale;
-  bool dense;
-  explicit LiveSet(uint32_t sc) : scale(sc), dense(2 * sc <= 34) {
-    if (dense) bits.assign(((uint64_t)1 << (2 * sc)) / 64 + 1, 0);
-  }
-  uint64_t slot(uint64_t k) const { return ((k >> 32) << scale) | (k & 0xFFFFFFFFu); }
-  bool test(uint64_t k) const {
-    if (!dense) return hs.count(k) != 0;
-    uint64_t s = slot(k);
-    return (bits[s >> 6] >> (s & 63)) & 1;
-  }
-  void set(uint64_t k) {
-    if (!dense) { hs.insert(k); return; }
-    uint64_t s = slot(k);
-    bits[s >> 6] |= 1ULL << (s & 63);
-  }
-  void clear(uint64_t k) {
-    if (!dense) { hs.erase(k); return; }
-    uint64_t s = slot(k);
-    bits[s >> 6] &= ~(1ULL << (s & 63));
-  }
-};
-
 }  // namespace
 
 extern "C" int gs_gen_erdos_renyi(uint32_t n, double p, double del_frac, uint64_t seed,
@@ -156,57 +135,320 @@ extern "C" int gs_gen_erdos_renyi(uint32_t n, double p, double del_frac, uint64_
   return GS_OK;
 }
 
+// ---- R-MAT, generated in parallel ------------------------------------------------------------
+// The stream is a pure function of the seed, whatever the thread count:
+//   * candidate i (i = 0, 1, ...) is an R-MAT edge drawn from its own counter-seeded generator
+//     (RmatGen; a self loop redraws from the same generator, at most 64 times, after which the
+//     candidate is skipped);
+//   * a candidate is an insertion when its edge never appeared among the candidates before it
+//     (first occurrence wins: duplicates are dropped, no edge is inserted twice). The check is
+//     sharded over threads by the 64-bit word of the edge's bit in a dense bitset (or by a hash
+//     of the edge when the bitset would be too large), each shard walking its candidates in
+//     index order, so the outcome is the sequential one;
+//   * each insertion j (j-th kept edge) is deleted with probability del_frac / (1 - del_frac),
+//     at a position drawn uniformly from the insertions after it (as the Erdős–Rényi stream does);
+//   * the stream is the insertions in order with the deletions placed behind them, cut to
+//     `count` updates.
+namespace {
+
+constexpr int kGenShardBits = 10;
+constexpr uint64_t kGenChunk = 1ULL << 24;  // candidates per dedupe round
+constexpr uint64_t kNoKey = ~0ULL;
+
+inline uint64_t mix_index(uint64_t seed, uint64_t i, uint64_t salt) {
+  uint64_t s = seed ^ (i * 0xD1B54A32D192ED03ULL) ^ salt;
+  return splitmix64(s);
+}
+
+unsigned gen_threads() {
+  if (const char* e = std::getenv("GPUSKETCH_GEN_THREADS")) {
+    const int t = std::atoi(e);
+    if (t >= 1) return (unsigned)std::min(t, 256);
+  }
+  cpu_set_t set;
+  unsigned n = 0;
+  if (sched_getaffinity(0, sizeof(set), &set) == 0) n = (unsigned)CPU_COUNT(&set);
+  if (n == 0) n = std::max(1u, std::thread::hardware_concurrency());
+  return std::min(n, 64u);
+}
+
+// run f(t) for t in [0, nt) on nt threads
+template <class F>
+void parallel_for_threads(unsigned nt, F&& f) {
+  std::vector<std::thread> th;
+  th.reserve(nt);
+  for (unsigned t = 1; t < nt; t++) th.emplace_back([&f, t] { f(t); });
+  f(0);
+  for (auto& x : th) x.join();
+}
+
+// R-MAT edge sampler: two levels of the recursion per table lookup. The 16 outcomes of two levels
+// (quadrants a = (0,0), b = (0,1), c = (1,0), d = (1,1) of u's and v's bit at each level) get
+// round(p * 2^16) of the 65536 entries of a table indexed by 16 random bits (largest remainders
+// settle the sum), so a pair of levels costs one lookup; an odd last level compares 32 random
+// bits with the thresholds directly.
+struct RmatGen {
+  uint32_t scale;
+  uint64_t seed, ta, tab, tabc;
+  std::vector<uint8_t> pair_table;  // 16 random bits -> (u bit pair << 2) | v bit pair
+  RmatGen(uint32_t sc, uint64_t sd, double a, double b, double c)
+      : scale(sc), seed(sd), ta((uint64_t)(a * 4294967296.0)), tab((uint64_t)((a + b) * 4294967296.0)),
+        tabc((uint64_t)((a + b + c) * 4294967296.0)), pair_table(1u << 16) {
+    const double pq[4] = {a, b, c, std::max(0.0, 1 - a - b - c)};
+    const uint32_t qu[4] = {0, 0, 1, 1}, qv[4] = {0, 1, 0, 1};
+    uint32_t cnt[16];
+    double rem[16];
+    uint32_t used = 0;
+    for (int o = 0; o < 16; o++) {
+      const double x = pq[o >> 2] * pq[o & 3] * 65536.0;
+      cnt[o] = (uint32_t)std::floor(x);
+      rem[o] = x - cnt[o];
+      used += cnt[o];
+    }
+    while (used < 65536) {  // largest remainder first (ties: lowest outcome)
+      int best = 0;
+      for (int o = 1; o < 16; o++)
+        if (rem[o] > rem[best]) best = o;
+      cnt[best]++;
+      rem[best] = -1;
+      used++;
+    }
+    uint32_t pos = 0;
+    for (int o = 0; o < 16; o++) {
+      const uint32_t q1 = o >> 2, q2 = o & 3;
+      const uint8_t code = (uint8_t)((((qu[q1] << 1) | qu[q2]) << 2) | ((qv[q1] << 1) | qv[q2]));
+      for (uint32_t k = 0; k < cnt[o]; k++) pair_table[pos++] = code;
+    }
+  }
+  // candidate i's edge key, or kNoKey when it only drew self loops
+  uint64_t key(uint64_t i) const {
+    uint64_t s = mix_index(seed, i, 0x52A7C0DEULL);
+    const uint32_t npairs = scale >> 1;
+    for (int tries = 0; tries < 64; tries++) {
+      uint32_t u = 0, v = 0;
+      uint64_t r = 0;
+      for (uint32_t k = 0; k < npairs; k++) {
+        if ((k & 3) == 0) r = splitmix64(s);
+        const uint8_t code = pair_table[(r >> (16 * (k & 3))) & 0xFFFFu];
+        u = (u << 2) | (code >> 2);
+        v = (v << 2) | (code & 3u);
+      }
+      if (scale & 1) {
+        const uint64_t x = splitmix64(s) & 0xFFFFFFFFu;
+        u = (u << 1) | (uint32_t)(x >= tab);
+        v = (v << 1) | (uint32_t)((x >= ta && x < tab) || x >= tabc);
+      }
+      if (u != v) return key_of(u, v);
+    }
+    return kNoKey;
+  }
+};
+
+}  // namespace
+
 extern "C" int gs_gen_rmat(uint32_t scale, double a, double b, double c, uint64_t count,
                            double del_frac, uint64_t seed, uint32_t* pairs) {
   if (!pairs || scale < 1 || scale > 30 || a < 0 || b < 0 || c < 0 || a + b + c > 1 ||
       del_frac < 0 || del_frac >= 1)
     return GS_ERR_INVALID;
-  Rng rng(seed);
-  LiveSet live(scale);
-  std::vector<uint64_t> list;
-  list.reserve(count);
-  const uint64_t ta = (uint64_t)(a * 4294967296.0), tab = (uint64_t)((a + b) * 4294967296.0),
-                 tabc = (uint64_t)((a + b + c) * 4294967296.0);
-  const uint64_t tdel = (uint64_t)(del_frac * 18446744073709551616.0);
-  for (uint64_t i = 0; i < count; i++) {
-    bool del = !list.empty() && rng.next() < tdel;
-    if (!del) {
-      uint64_t key = 0;
-      int tries = 0;
-      for (; tries < 4096; tries++) {
-        uint32_t u = 0, v = 0;
-        uint64_t r = 0;
-        for (uint32_t l = 0; l < scale; l++) {
-          if ((l & 1) == 0) r = rng.next();
-          uint64_t x = (l & 1) ? (r >> 32) : (r & 0xFFFFFFFFu);
-          uint32_t bu = x >= tab, bv = (x >= ta && x < tab) || x >= tabc;
-          u = (u << 1) | bu;
-          v = (v << 1) | bv;
+  if (count == 0) return GS_OK;
+  const RmatGen gen(scale, seed, a, b, c);
+  const unsigned nt = gen_threads();
+  const uint32_t nshards = 1u << kGenShardBits;
+  const bool dense = 2 * scale <= 34;
+  // every 64-bit word of the bitset belongs to one shard, so shard threads never share one;
+  // calloc: the zero pages are mapped lazily, by the shard threads that touch them
+  const size_t nwords = dense ? (((uint64_t)1 << (2 * scale)) / 64 + 1) : 1;
+  std::unique_ptr<uint64_t, decltype(&std::free)> bits_mem(
+      static_cast<uint64_t*>(std::calloc(nwords, sizeof(uint64_t))), &std::free);
+  if (!bits_mem) return GS_ERR_OOM;
+  uint64_t* const bits = bits_mem.get();
+  std::vector<std::unordered_set<uint64_t>> seen(dense ? 0 : nshards);
+  auto slot_of = [scale](uint64_t k) -> uint64_t { return ((k >> 32) << scale) | (k & 0xFFFFFFFFu); };
+  auto shard_of = [&](uint64_t k) -> uint32_t {
+    const uint64_t w = dense ? (slot_of(k) >> 6) : k;
+    return (uint32_t)((w * 0x9E3779B97F4A7C15ULL) >> (64 - kGenShardBits));
+  };
+  // kept insertions: enough that, with deletions added, the stream reaches `count` (with a
+  // margin of several standard deviations; retried with more if not)
+  const double q = del_frac / (1 - del_frac);
+  uint64_t target = (uint64_t)std::ceil((double)count * (1 - del_frac) +
+                                        6 * std::sqrt((double)count) + 64);
+  target = std::min(target, count);
+  std::vector<uint64_t> ins;
+  ins.reserve(target);
+  const uint64_t chunk = std::min<uint64_t>(kGenChunk, std::max<uint64_t>(1ULL << 16, 2 * target));
+  std::vector<uint64_t> keys(chunk), order(chunk);
+  std::vector<uint8_t> kept(chunk);
+  std::vector<uint64_t> cnt((size_t)nt * nshards), tkept(nt);
+  uint64_t next = 0, stalled = 0;
+  for (;;) {
+    // insertions: dedupe rounds over candidate chunks until `target` are kept
+    while (ins.size() < target) {
+      const uint64_t base = next, m = chunk;
+      next += m;
+      auto range = [&](unsigned t, uint64_t& lo, uint64_t& hi) {
+        lo = m * t / nt;
+        hi = m * (t + 1) / nt;
+      };
+      parallel_for_threads(nt, [&](unsigned t) {
+        uint64_t lo, hi;
+        range(t, lo, hi);
+        uint64_t* ct = &cnt[(size_t)t * nshards];
+        std::fill(ct, ct + nshards, 0);
+        for (uint64_t i = lo; i < hi; i++) {
+          keys[i] = gen.key(base + i);
+          if (keys[i] != kNoKey) ct[shard_of(keys[i])]++;
         }
-        if (u == v) continue;
-        key = key_of(u, v);
-        if (!live.test(key)) break;
+      });
+      // stable partition of the chunk's candidates by shard
+      uint64_t run = 0;
+      std::vector<uint64_t> shard_start(nshards + 1);
+      for (uint32_t sh = 0; sh < nshards; sh++) {
+        shard_start[sh] = run;
+        for (unsigned t = 0; t < nt; t++) {
+          const uint64_t x = cnt[(size_t)t * nshards + sh];
+          cnt[(size_t)t * nshards + sh] = run;
+          run += x;
+        }
       }
-      if (tries == 4096) {
-        if (list.empty()) return GS_ERR_INVALID;
-        del = true;  // saturated: turn this update into a deletion
-      } else {
-        live.set(key);
-        list.push_back(key);
-        pairs[2 * i] = (uint32_t)(key >> 32);
-        pairs[2 * i + 1] = (uint32_t)key;
-        continue;
+      shard_start[nshards] = run;
+      parallel_for_threads(nt, [&](unsigned t) {
+        uint64_t lo, hi;
+        range(t, lo, hi);
+        uint64_t* ct = &cnt[(size_t)t * nshards];
+        for (uint64_t i = lo; i < hi; i++) {
+          kept[i] = 0;
+          if (keys[i] != kNoKey) order[ct[shard_of(keys[i])]++] = i;
+        }
+      });
+      // shards in candidate order: the first occurrence of an edge is kept
+      std::atomic<uint32_t> next_shard{0};
+      parallel_for_threads(nt, [&](unsigned) {
+        for (uint32_t sh; (sh = next_shard.fetch_add(1)) < nshards;) {
+          const uint64_t p0 = shard_start[sh], p1 = shard_start[sh + 1];
+          constexpr uint64_t kAhead = 16;  // bitset words requested ahead of their test
+          if (dense)
+            for (uint64_t p = p0; p < std::min(p1, p0 + kAhead); p++)
+              __builtin_prefetch(&bits[slot_of(keys[order[p]]) >> 6], 1);
+          for (uint64_t p = p0; p < p1; p++) {
+            const uint64_t i = order[p], k = keys[i];
+            if (dense) {
+              if (p + kAhead < p1) __builtin_prefetch(&bits[slot_of(keys[order[p + kAhead]]) >> 6], 1);
+              const uint64_t slot = slot_of(k);
+              uint64_t& w = bits[slot >> 6];
+              const uint64_t bit = 1ULL << (slot & 63);
+              if (!(w & bit)) {
+                w |= bit;
+                kept[i] = 1;
+              }
+            } else if (seen[sh].insert(k).second) {
+              kept[i] = 1;
+            }
+          }
+        }
+      });
+      // append the kept candidates in index order (up to target)
+      parallel_for_threads(nt, [&](unsigned t) {
+        uint64_t lo, hi, c0 = 0;
+        range(t, lo, hi);
+        for (uint64_t i = lo; i < hi; i++) c0 += kept[i];
+        tkept[t] = c0;
+      });
+      const uint64_t have = ins.size();
+      uint64_t add = 0;
+      for (unsigned t = 0; t < nt; t++) add += tkept[t];
+      stalled = add ? 0 : stalled + 1;
+      if (stalled > 64) return GS_ERR_INVALID;  // the R-MAT distribution is exhausted
+      const uint64_t take = std::min(add, target - have);
+      ins.resize(have + take);
+      uint64_t off = have;
+      std::vector<uint64_t> toff(nt);
+      for (unsigned t = 0; t < nt; t++) {
+        toff[t] = off;
+        off += tkept[t];
+      }
+      parallel_for_threads(nt, [&](unsigned t) {
+        uint64_t lo, hi, o = toff[t];
+        range(t, lo, hi);
+        for (uint64_t i = lo; i < hi && o < have + take; i++)
+          if (kept[i]) ins[o++] = keys[i];
+      });
+    }
+    // deletions: insertion j is deleted with probability q, behind insertion pos in (j, I]
+    // (pos = I: at the very end); grouped by pos, in insertion order within one pos
+    const uint64_t I = ins.size();
+    const uint64_t tdel = q >= 1 ? ~0ULL : (uint64_t)(q * 18446744073709551616.0);
+    std::vector<std::vector<std::pair<uint64_t, uint64_t>>> tdels(nt);
+    parallel_for_threads(nt, [&](unsigned t) {
+      const uint64_t lo = I * t / nt, hi = I * (t + 1) / nt;
+      auto& out = tdels[t];
+      for (uint64_t j = lo; j < hi; j++) {
+        uint64_t s = mix_index(seed, j, 0xDE1E7EULL);
+        if (splitmix64(s) >= tdel) continue;
+        const uint64_t pos = j + 1 + splitmix64(s) % (I - j);  // in [j+1, I]
+        out.push_back({pos, j});
+      }
+    });
+    uint64_t D = 0;
+    for (auto& v : tdels) D += v.size();
+    if (I + D < count) {  // short of the requested length: more insertions and redo
+      if (target >= count) return GS_ERR_INVALID;
+      target = std::min(count, target + (count - I - D) * 2 + 64);
+      continue;
+    }
+    // all deletions sorted by (pos, j): bucketed by pos range, then sorted per bucket
+    std::vector<std::vector<std::pair<uint64_t, uint64_t>>> buck(nt);
+    {
+      std::vector<std::vector<std::vector<std::pair<uint64_t, uint64_t>>>> part(
+          nt, std::vector<std::vector<std::pair<uint64_t, uint64_t>>>(nt));
+      std::vector<uint64_t> lo_of(nt);  // first insertion of each merge segment
+      for (unsigned bkt = 0; bkt < nt; bkt++) lo_of[bkt] = I * bkt / nt;
+      parallel_for_threads(nt, [&](unsigned t) {
+        for (auto& e : tdels[t]) {  // the segment holding insertion pos - 1
+          const unsigned bkt =
+              (unsigned)(std::upper_bound(lo_of.begin(), lo_of.end(), e.first - 1) - lo_of.begin()) - 1;
+          part[t][bkt].push_back(e);
+        }
+        tdels[t].clear();
+        tdels[t].shrink_to_fit();
+      });
+      parallel_for_threads(nt, [&](unsigned bkt) {
+        auto& out = buck[bkt];
+        for (unsigned t = 0; t < nt; t++) out.insert(out.end(), part[t][bkt].begin(), part[t][bkt].end());
+        std::sort(out.begin(), out.end());
+      });
+    }
+    // merge: segment b covers the insertions [I*b/nt, I*(b+1)/nt) and the deletions whose pos
+    // lies in (I*b/nt, I*(b+1)/nt] -- exactly bucket b. Deletions at pos follow insertion pos-1.
+    std::vector<uint64_t> out0(nt);
+    {
+      uint64_t run = 0;
+      for (unsigned bkt = 0; bkt < nt; bkt++) {
+        out0[bkt] = run;
+        run += (I * (bkt + 1) / nt - I * bkt / nt) + buck[bkt].size();
       }
     }
-    uint64_t j = rng.below(list.size());
-    uint64_t key = list[j];
-    list[j] = list.back();
-    list.pop_back();
-    live.clear(key);
-    pairs[2 * i] = (uint32_t)(key >> 32);
-    pairs[2 * i + 1] = (uint32_t)key;
+    parallel_for_threads(nt, [&](unsigned bkt) {
+      const uint64_t lo = I * bkt / nt, hi = I * (bkt + 1) / nt;
+      uint64_t o = out0[bkt];
+      size_t d = 0;
+      const auto& dl = buck[bkt];
+      auto put = [&](uint64_t k) {
+        if (o < count) {
+          pairs[2 * o] = (uint32_t)(k >> 32);
+          pairs[2 * o + 1] = (uint32_t)k;
+        }
+        o++;
+      };
+      for (uint64_t j = lo; j < hi; j++) {
+        put(ins[j]);
+        while (d < dl.size() && dl[d].first == j + 1) put(ins[dl[d++].second]);
+      }
+      while (d < dl.size()) put(ins[dl[d++].second]);  // pos == I at the end of the last bucket
+    });
+    return GS_OK;
   }
-  return GS_OK;
 }
 
 extern "C" int gs_gen_planted_cut(uint32_t n, double p, uint32_t cut, double churn_frac,

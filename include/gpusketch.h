@@ -252,8 +252,12 @@ int gs_device_copy(void* dst, const void* src, size_t bytes);
  * random later positions. */
 int gs_gen_erdos_renyi(uint32_t n, double p, double del_frac, uint64_t seed, uint32_t* pairs,
                        uint64_t* count);
-/* R-MAT(a, b, c, 1-a-b-c) with n = 2^scale: `count` updates, `del_frac` of which delete a
- * random earlier live edge. */
+/* R-MAT(a, b, c, 1-a-b-c) with n = 2^scale: `count` updates, about `del_frac` of which delete an
+ * earlier inserted edge. R-MAT samples are drawn in order; self loops and edges that were drawn
+ * before are dropped (every edge is inserted at most once), and each insertion is deleted with
+ * probability del_frac / (1 - del_frac) behind a uniformly chosen later insertion. Generated on
+ * all host cores (GPUSKETCH_GEN_THREADS overrides the count); the stream depends only on the
+ * arguments. */
 int gs_gen_rmat(uint32_t scale, double a, double b, double c, uint64_t count, double del_frac,
                 uint64_t seed, uint32_t* pairs);
 /* Two ER(p) halves of n/2 vertices joined by exactly `cut` crossing edges, shuffled, plus

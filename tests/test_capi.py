@@ -106,6 +106,26 @@ def test_rmat_stream_deletes_only_live_edges():
     assert deg.argmax() == 0
 
 
+def test_rmat_stream_same_for_any_thread_count():
+    """The generator shards its duplicate check over threads; the stream must not depend on how
+    many there are (GPUSKETCH_GEN_THREADS), and a longer request only extends the kept insertions
+    of a shorter one (same candidate sequence)."""
+    import subprocess
+    import sys
+    code = ("import sys, hashlib; sys.path.insert(0, %r); import gpusketch_amd as gs; "
+            "p = gs.gen_rmat(13, 400000, 77); print(hashlib.sha1(p.tobytes()).hexdigest())" % ROOT)
+    digests = set()
+    for t in ("1", "3", "8"):
+        r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                           env={**os.environ, "GPUSKETCH_GEN_THREADS": t}, timeout=120)
+        assert r.returncode == 0, r.stderr
+        digests.add(r.stdout.strip())
+    assert len(digests) == 1
+    pairs = gs.gen_rmat(13, 400000, 77)
+    live = _check_toggle_semantics(pairs, 1 << 13)
+    assert len(live) > 0
+
+
 def test_planted_cut_stream_net_graph():
     pairs = gs.gen_planted_cut(200, 0.2, 5, 0.1, 7)
     live = _check_toggle_semantics(pairs, 200)
