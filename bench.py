@@ -1,17 +1,20 @@
 """Benchmark: sketch edge updates/s at n = 2^17 on one MI355X.
 
-Workload (single-stream GPU CC config): R-MAT n=2^17, (a,b,c,d) = (0.57,0.19,0.19,0.05), 5% of
-updates delete a previously inserted live edge, no self loops or duplicate live edges, batches of
-2^20 uint32 pairs. One step = one batch submitted to the device update path with the batch
-already resident in HBM; the stream coalesces submissions into launches of the grouping and
-aggregation kernels (2^23 edges each by default) and the timed region ends with a flush, so every
-timed update is applied. Every step uses a distinct batch of one continuous seeded stream
-(warmup batches first, then the timed ones).
+Workload (single-stream GPU CC config): R-MAT n=2^17, (a,b,c,d) = (0.57,0.19,0.19,0.05), about
+5% of updates delete an earlier inserted edge, no self loops or duplicate edges, batches of 2^20
+uint32 pairs. One step = one launch of the grouping and aggregation kernels over 2^25 edges,
+submitted as 32 batches that are already resident in HBM (config.input); the timed region ends
+with a flush, so every timed update is applied. Every step takes the next 32 batches of one
+continuous seeded stream (warmup steps first); the defaults (5 + 27 steps) cover a 2^30-update
+stream. The rate from pinned host memory, PCIe copies included, is reported beside it
+(pcie_inclusive), never as the value.
 
 Prints one JSON line (the contract of the driver) with a `roofline` object for the dominant
 kernel (aggregate_kernel, timed with HIP events on its own stream; its bytes counted by the
-kernel itself) and a `cpu_baseline` object (the CPU restatement of the reference update path,
-timed on this host's cores over a bounded prefix of the same stream).
+kernel itself; its VALU issue rate from a committed counter profile of the same launch size) and a
+`cpu_baseline` object (the CPU restatement of the reference update path, timed on every usable
+host core over a bounded prefix of the same stream), plus the CPU Boruvka query on the final
+store beside the GPU query latency.
 """
 from __future__ import annotations
 
@@ -53,8 +56,9 @@ def cpu_baseline(pairs: np.ndarray, threads: int, cubesketch: bool = False) -> d
     del store
     return {"value": len(pairs) / secs, "unit": "edges/s", "cores": threads, "kind": "port",
             "sample": f"first {len(pairs)} updates of the same stream ({len(pairs) // BATCH} "
-                      f"batches), grouped into {len(src)} vertex batches of <= "
-                      f"{4 * g.num_buckets} (grouping {group_s:.2f}s, not timed)",
+                      f"batches of 2^20), grouped into {len(src)} vertex batches of <= "
+                      f"{4 * g.num_buckets} (grouping {group_s:.2f}s, not timed), "
+                      f"{secs:.1f}s timed",
             "seconds": secs}
 
 
@@ -265,17 +269,254 @@ def run_serving(rate: float, requests: int = 1000, tenants: int = 64,
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
+LAUNCH_EDGES = 1 << 25            # edges per launch of the grouping + aggregation kernels = 1 step
+STEP_BATCHES = LAUNCH_EDGES // BATCH
+CU_COUNT, CLOCK_GHZ = 256, 2.4
+# VALU issue ceiling: each CU's 4 SIMDs are 16 lanes wide and take 4 clocks per wave64
+# instruction, so a CU issues one wave64 VALU instruction per clock
+VALU_PEAK_GINSTR = CU_COUNT * CLOCK_GHZ
+
+
+def host_cpu() -> dict:
+    """CPU model, the host's logical CPUs, those this process may run on, and the cgroup CPU
+    quota; `usable` = the smaller of the last two (one worker thread per usable core)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = max(1, int(round(int(q) / int(period))))
+    except (OSError, ValueError):
+        pass
+    usable = min(affinity, quota) if quota else affinity
+    return {"model": model, "nproc": os.cpu_count(), "affinity": affinity,
+            "cgroup_quota_cores": quota, "usable": usable}
+
+
+def load_profile(name: str, edges_per_launch: float):
+    """A committed counter summary (profiles/<name>), used only when it was measured at the launch
+    size this run times."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    prof = json.load(open(path))
+    if abs(prof.get("edges_per_launch", 0) - edges_per_launch) > 1e-3 * edges_per_launch:
+        return None
+    return prof
+
+
+def bench_single(args) -> dict:
+    """Single-stream GPU CC (R-MAT n=2^17). One step = one launch of the grouping and aggregation
+    kernels over 2^25 edges, handed to the stream as 32 device-resident batches of 2^20 pairs;
+    every step takes the next 32 batches of one continuous seeded stream (warmup steps first)."""
+    import gpusketch_amd as gs
+    cube = args.mode == "cubesketch"
+    if cube:  # 9x the hashing per update: no serving trace or PCIe leg, a short CPU sample
+        args.no_serving = True
+        args.pcie_batches = 0
+        args.cpu_batches = min(args.cpu_batches, 2)
+    total_steps = args.warmup + args.steps
+    total = total_steps * LAUNCH_EDGES
+    t0 = time.perf_counter()
+    pairs = gs.gen_rmat(SCALE, total, SEED_STREAM, *RMAT, del_frac=DEL_FRAC)
+    gen_s = time.perf_counter() - t0
+    log(f"generated {len(pairs)} R-MAT updates in {gen_s:.1f}s")
+
+    store = gs.SketchStore(N, SEED_SKETCH, cubesketch=cube)
+    geo = store.geom
+    stream = store.stream()
+    stream.set_tuning(launch_edges=LAUNCH_EDGES)
+    dbuf = gs.DeviceBuffer(pairs)
+    log(f"store {geo.device_bytes / 1e9:.3f} GB, B={geo.bkt_per_col} C={geo.num_columns}; "
+        f"input resident in HBM ({dbuf.nbytes / 1e9:.2f} GB)")
+
+    def run_steps(first: int, count: int) -> None:
+        for st in range(first, first + count):
+            for k in range(STEP_BATCHES):
+                b = st * STEP_BATCHES + k
+                stream.submit_pairs_device(dbuf.offset(b * BATCH * 8), BATCH)
+
+    run_steps(0, args.warmup)
+    stream.flush()
+    log(f"warmup {args.warmup} steps done")
+
+    stream.set_profiling(True)
+    stream.kernel_times()
+    stream.traffic()  # reset the counters
+    gs.lib.gs_device_synchronize()
+    t0 = time.perf_counter()
+    run_steps(args.warmup, args.steps)
+    stream.flush()
+    gs.lib.gs_device_synchronize()
+    elapsed = time.perf_counter() - t0
+    grp_ms, agg_ms, launches, agg_updates = stream.kernel_times()
+    flushed, direct = stream.traffic()
+    stream.set_profiling(False)
+    kname = "wide_aggregate_kernel" if cube else "aggregate_kernel"
+    log(f"timed {args.steps} steps in {elapsed:.4f}s ({launches} launches); per launch: grouping "
+        f"{grp_ms / launches:.4f} ms, {kname} {agg_ms / launches:.4f} ms")
+
+    # PCIe-inclusive rate: batches of the same stream handed over from pinned host memory (the
+    # DMA runs on the stream's copy stream while the kernels work), into a second store
+    pcie = None
+    if args.pcie_batches > 0:
+        nb = max(2, args.pcie_batches)
+        warm = min(8, nb - 1)
+        pin = gs.PinnedBuffer((nb * BATCH, 2))
+        for b in range(nb):
+            src = b % (total_steps * STEP_BATCHES)
+            pin.array[b * BATCH:(b + 1) * BATCH] = pairs[src * BATCH:(src + 1) * BATCH]
+        with gs.SketchStore(N, SEED_SKETCH) as st2:
+            s2 = st2.stream()
+            s2.set_tuning(launch_edges=LAUNCH_EDGES)
+            for b in range(warm):
+                s2.submit_pairs(pin.array[b * BATCH:(b + 1) * BATCH])
+            s2.flush()
+            gs.lib.gs_device_synchronize()
+            p0 = time.perf_counter()
+            for b in range(warm, nb):
+                s2.submit_pairs(pin.array[b * BATCH:(b + 1) * BATCH])
+            s2.flush()
+            gs.lib.gs_device_synchronize()
+            pe = time.perf_counter() - p0
+            s2.close()
+        pin.free()
+        timed = nb - warm
+        pcie = {"value": timed * BATCH / pe, "unit": "edges/s", "batches": timed,
+                "h2d_gb_per_s": timed * BATCH * 8 / pe / 1e9,
+                "input": "pinned-host", "batch": "2^20 uint32 pairs per submission"}
+        log(f"PCIe-inclusive {pcie['value']:.3e} edges/s ({pcie['h2d_gb_per_s']:.1f} GB/s of "
+            f"pairs) over {timed} pinned host batches")
+
+    # CC query latency on the final store (the plain single-stream path): the first query of a
+    # store allocates its query scratch (reported apart), then args.query_reps timed queries
+    q0 = time.perf_counter()
+    labels, _, _ = store.query_cc()
+    first_ms = (time.perf_counter() - q0) * 1e3
+    lat = []
+    for _ in range(max(1, args.query_reps)):
+        q0 = time.perf_counter()
+        _, ncc, rounds = store.query_cc()
+        lat.append((time.perf_counter() - q0) * 1e3)
+    log(f"CC query: {ncc} components, {rounds} rounds, p50 {np.percentile(lat, 50):.2f} ms, "
+        f"p99 {np.percentile(lat, 99):.2f} ms over {len(lat)} (first query {first_ms:.2f} ms)")
+
+    cpu = host_cpu()
+    cpu_query = None
+    if not args.no_cpu and not cube:
+        # the CPU Boruvka (the oracle's serial restatement of CCSketchAlg::boruvka_emulation) on
+        # this very store, exported in reference order; its partition must equal the device's
+        from oracle import oracle as orc
+        host_store = store.export_buckets()
+        g = orc.Geometry(N)
+        c0 = time.perf_counter()
+        rc, olab, orounds, _ = orc.boruvka(host_store, g, SEED_SKETCH)
+        cq = time.perf_counter() - c0
+        del host_store
+        cpu_query = {"ms": cq * 1e3, "threads": 1, "kind": "port",
+                     "rounds": orounds, "labels_equal_device": bool(rc == 0 and np.array_equal(olab, labels))}
+        log(f"CPU Boruvka query {cq * 1e3:.0f} ms (1 thread), labels equal: "
+            f"{cpu_query['labels_equal_device']}")
+
+    # algorithmic bytes of the aggregation kernel: 8 B per undirected edge (its pair, read once
+    # by the grouping) + 16 B read + 16 B written per device bucket the launch changed (counted
+    # by the kernel: flushed delta slots, deterministic buckets, updates XORed straight to HBM)
+    edges_per_launch = agg_updates / 2 / launches
+    slots_per_launch = (flushed + direct) / launches
+    alg_bytes = 8.0 * edges_per_launch + 32.0 * slots_per_launch
+    agg_s = agg_ms / launches / 1e3
+    hbm_gbs = alg_bytes / agg_s / 1e9
+    prof = load_profile("counters_wide_aggregate_rmat17.json" if cube
+                        else "counters_aggregate_rmat17.json", edges_per_launch)
+    hbm = {"achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm_gbs / HBM_PEAK_GBS}
+    roof = {"bound": "hbm", **hbm, "traffic": None}
+    if prof:
+        roof["traffic"] = prof["hbm_bytes_per_dispatch"]
+        valu_g = prof["sq_insts_valu_per_dispatch"] / agg_s / 1e9
+        valu = {"achieved": valu_g, "peak": VALU_PEAK_GINSTR, "unit": "G wave64 VALU instr/s",
+                "frac": valu_g / VALU_PEAK_GINSTR, "valu_busy_pct": prof.get("valu_busy_pct"),
+                "source": "SQ_INSTS_VALU per dispatch (profiles, same launch size) / this run's "
+                          "kernel time"}
+        if valu["frac"] > hbm["frac"]:  # the binding ceiling first
+            roof = {"bound": "valu", **valu, "traffic": roof["traffic"], "hbm": hbm}
+        else:
+            roof["valu"] = valu
+    roof.update({"kernel": kname, "kernel_ms": agg_ms / launches,
+                 "grouping_ms": grp_ms / launches, "edges_per_launch": edges_per_launch,
+                 "alg_bytes_per_launch": alg_bytes, "alg_bytes_per_edge": alg_bytes / edges_per_launch,
+                 "buckets_changed_per_launch": slots_per_launch})
+
+    result = {
+        "metric": ("edge updates/s at n=2^17, CubeSketch rule (7 columns per sample)" if cube else
+                   "edge updates/s at n=2^17 + p50/p99 CC query latency, served vs plain"),
+        "value": args.steps * LAUNCH_EDGES / elapsed,
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "rmat17_cubesketch_cc" if cube else "rmat17_single_stream_cc",
+                   "n": N, "batch_edges": BATCH, "batches_per_step": STEP_BATCHES,
+                   "input": "device-resident",
+                   "rmat_abcd": [*RMAT, round(1 - sum(RMAT), 2)], "del_frac": DEL_FRAC,
+                   "bkt_per_col": geo.bkt_per_col, "num_columns": geo.num_columns,
+                   "buckets_per_vertex": geo.num_buckets, "stream_updates": len(pairs),
+                   "stream_generation_s": round(gen_s, 1)},
+        "roofline": roof,
+        "cc_query_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                        "queries": len(lat), "first_query_with_scratch_setup": first_ms,
+                        "components": ncc, "rounds": rounds, "cpu_boruvka": cpu_query},
+        "pcie_inclusive": pcie,
+    }
+    stream.close()
+    store.close()
+    dbuf.free()
+    if not args.no_serving:
+        sv = run_serving(2000)
+        result["serving"] = {
+            "workload": f"{sv['tenants']} tenants, {sv['requests']} Poisson requests at "
+                        f"{sv['rate_req_per_s']:g}/s, {sv['updates']} updates, {sv['queries']} queries",
+            "plain": sv["plain"], "served": sv["served"],
+            "answers_identical": sv["answers_identical"],
+            "sketches_identical": sv["sketches_identical"],
+            "alone_equals_mix": sv["alone_equals_mix"]}
+    if not args.no_cpu:
+        nb = min(args.cpu_batches, total_steps * STEP_BATCHES)
+        cb = cpu_baseline(pairs[: nb * BATCH], cpu["usable"], cube)
+        cb.pop("seconds")
+        cb["host"] = cpu
+        result["cpu_baseline"] = cb
+        log(f"cpu baseline {cb['value']:.3e} edges/s on {cpu['usable']} threads ({cpu['model']})")
+    return result
+
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=64)
-    ap.add_argument("--warmup", type=int, default=32,
-                    help="untimed batches first (32: one full launch of the kernels)")
-    ap.add_argument("--cpu-batches", type=int, default=32, help="stream prefix for the CPU leg")
+    ap.add_argument("--steps", type=int, default=27,
+                    help="timed steps; a step is one 2^25-edge launch (32 batches of 2^20 pairs)")
+    ap.add_argument("--warmup", type=int, default=5,
+                    help="untimed steps first (defaults: 5 + 27 steps = the 2^30-update stream)")
+    ap.add_argument("--cpu-batches", type=int, default=64,
+                    help="stream prefix (2^20-pair batches) timed on the CPU leg")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--pcie-batches", type=int, default=104,
+    ap.add_argument("--pcie-batches", type=int, default=264,
                     help="batches submitted from pinned host memory for the PCIe-inclusive rate "
-                         "(8 warmup, then timed; several 2^25-edge launches so that copies overlap "
+                         "(8 warmup, then 256 timed: eight 2^25-edge launches whose copies overlap "
                          "the kernels as in a long stream)")
     ap.add_argument("--mode", choices=["single", "cubesketch", "batched", "serving", "mincut"],
                     default="single",
@@ -284,7 +525,7 @@ def main() -> None:
                     help="single mode: skip the served-vs-plain trace")
     ap.add_argument("--graphs", type=int, default=256, help="batched mode: independent graphs")
     ap.add_argument("--query-reps", type=int, default=0,
-                    help="query repetitions (default: 3 in batched mode, 50 in single mode)")
+                    help="query repetitions (default: 3 in batched mode, 100 in single mode)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the known-size copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -297,7 +538,7 @@ def main() -> None:
         sys.exit(2)
 
     if not args.query_reps:
-        args.query_reps = 3 if args.mode == "batched" else 50
+        args.query_reps = 3 if args.mode == "batched" else 100
     if args.mode == "batched":
         print(json.dumps(bench_batched(args)), flush=True)
         return
@@ -320,155 +561,7 @@ def main() -> None:
             "eviction_30gb_budget": evict}), flush=True)
         return
 
-    import gpusketch_amd as gs
-
-    cube = args.mode == "cubesketch"
-    if cube:  # 9x the hashing per update: no serving trace or PCIe leg, a short CPU sample
-        args.no_serving = True
-        args.pcie_batches = 0
-        args.cpu_batches = min(args.cpu_batches, 2)
-    total_batches = args.warmup + args.steps
-    t0 = time.perf_counter()
-    pairs = gs.gen_rmat(SCALE, total_batches * BATCH, SEED_STREAM, *RMAT, del_frac=DEL_FRAC)
-    log(f"generated {len(pairs)} R-MAT updates in {time.perf_counter() - t0:.1f}s")
-
-    store = gs.SketchStore(N, SEED_SKETCH, cubesketch=cube)
-    geo = store.geom
-    stream = store.stream()
-    dbuf = gs.DeviceBuffer(pairs)
-    log(f"store {geo.device_bytes / 1e9:.3f} GB, B={geo.bkt_per_col} C={geo.num_columns}; "
-        f"input resident in HBM ({dbuf.nbytes / 1e9:.2f} GB)")
-
-    for b in range(args.warmup):
-        stream.submit_pairs_device(dbuf.offset(b * BATCH * 8), BATCH)
-    stream.flush()
-    log(f"warmup {args.warmup} batches done")
-
-    stream.set_profiling(True)
-    stream.traffic()  # reset the counters
-    gs.lib.gs_device_synchronize()
-    t0 = time.perf_counter()
-    for b in range(args.warmup, total_batches):
-        stream.submit_pairs_device(dbuf.offset(b * BATCH * 8), BATCH)
-    stream.flush()
-    gs.lib.gs_device_synchronize()
-    elapsed = time.perf_counter() - t0
-    agg_ms, launches, agg_updates = stream.timing()
-    flushed, direct = stream.traffic()
-    stream.set_profiling(False)
-    kname = "wide_aggregate_kernel" if cube else "aggregate_kernel"
-    log(f"timed {args.steps} batches in {elapsed:.4f}s ({launches} launches of the grouping and "
-        f"aggregation kernels); {kname} {agg_ms / launches:.4f} ms avg")
-
-    # PCIe-inclusive rate: the stream's batches (cycled when more are asked for than it holds)
-    # handed over from pinned host memory (copied by DMA on the stream's copy stream while the
-    # kernels run), into a second store
-    pcie = None
-    if args.pcie_batches > 0:
-        nb = max(2, args.pcie_batches)
-        pin = gs.PinnedBuffer((nb * BATCH, 2))
-        for b in range(nb):
-            src = b % total_batches
-            pin.array[b * BATCH:(b + 1) * BATCH] = pairs[src * BATCH:(src + 1) * BATCH]
-        with gs.SketchStore(N, SEED_SKETCH) as st2:
-            s2 = st2.stream()
-            warm = min(8, nb - 1)
-            for b in range(warm):
-                s2.submit_pairs(pin.array[b * BATCH:(b + 1) * BATCH])
-            s2.flush()
-            gs.lib.gs_device_synchronize()
-            p0 = time.perf_counter()
-            for b in range(warm, nb):
-                s2.submit_pairs(pin.array[b * BATCH:(b + 1) * BATCH])
-            s2.flush()
-            gs.lib.gs_device_synchronize()
-            pe = time.perf_counter() - p0
-            s2.close()
-        pin.free()
-        pcie = {"value": (nb - warm) * BATCH / pe, "unit": "edges/s", "batches": nb - warm,
-                "input": "pinned host uint32 pairs, 2^20 per submission"}
-        log(f"PCIe-inclusive {pcie['value']:.3e} edges/s over {nb - warm} pinned host batches")
-
-    # CC query latency on the final store (plain single-stream path). The first query of a store
-    # allocates its query scratch (a once-per-store cost, reported apart); then args.query_reps
-    # timed queries
-    q0 = time.perf_counter()
-    store.query_cc()
-    first_ms = (time.perf_counter() - q0) * 1e3
-    lat = []
-    for _ in range(max(1, args.query_reps)):
-        q0 = time.perf_counter()
-        _, ncc, rounds = store.query_cc()
-        lat.append((time.perf_counter() - q0) * 1e3)
-    log(f"CC query: {ncc} components, {rounds} rounds, p50 {np.percentile(lat, 50):.2f} ms, "
-        f"p99 {np.percentile(lat, 99):.2f} ms over {len(lat)} (first query {first_ms:.2f} ms)")
-
-    # algorithmic bytes of the aggregation kernel: 8 B per undirected edge (its pair, read once
-    # by the grouping) + 16 B read + 16 B written per device bucket the launch changed (counted
-    # by the kernel: flushed delta slots, deterministic buckets, updates XORed straight to HBM)
-    edges_per_launch = agg_updates / 2 / launches
-    slots_per_launch = (flushed + direct) / launches
-    alg_bytes = 8.0 * edges_per_launch + 32.0 * slots_per_launch
-    agg_s = agg_ms / launches / 1e3
-    achieved = alg_bytes / agg_s / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_wide_aggregate_rmat17_cubesketch.json" if cube
-                         else "traffic_aggregate_rmat17.json")
-    if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
-
-    result = {
-        "metric": ("edge updates/s at n=2^17, CubeSketch rule (7 columns per sample)" if cube else
-                   "edge updates/s at n=2^17 + p50/p99 CC query latency, served vs plain"),
-        "value": args.steps * BATCH / elapsed,
-        "unit": "edges/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u64",
-        "data": "synthetic",
-        "config": {"workload": "rmat17_cubesketch_cc" if cube else "rmat17_single_stream_cc",
-                   "n": N, "batch_edges": BATCH,
-                   "rmat_abcd": [*RMAT, round(1 - sum(RMAT), 2)], "del_frac": DEL_FRAC,
-                   "bkt_per_col": geo.bkt_per_col, "num_columns": geo.num_columns,
-                   "buckets_per_vertex": geo.num_buckets, "stream_updates": len(pairs)},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": kname, "kernel_ms": agg_ms / launches,
-                     "edges_per_launch": edges_per_launch,
-                     "alg_bytes_per_launch": alg_bytes,
-                     "alg_bytes_per_edge": alg_bytes / edges_per_launch,
-                     "buckets_changed_per_launch": slots_per_launch},
-        "cc_query_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
-                        "queries": len(lat), "first_query_with_scratch_setup": first_ms,
-                        "components": ncc, "rounds": rounds},
-        "pcie_inclusive": pcie,
-    }
-    stream.close()
-    store.close()
-    dbuf.free()
-    if not args.no_serving:
-        sv = run_serving(2000)
-        result["serving"] = {
-            "workload": f"{sv['tenants']} tenants, {sv['requests']} Poisson requests at "
-                        f"{sv['rate_req_per_s']:g}/s, {sv['updates']} updates, {sv['queries']} queries",
-            "plain": sv["plain"], "served": sv["served"],
-            "answers_identical": sv["answers_identical"],
-            "sketches_identical": sv["sketches_identical"],
-            "alone_equals_mix": sv["alone_equals_mix"]}
-    if not args.no_cpu:
-        threads = len(os.sched_getaffinity(0))
-        threads = max(1, min(threads, 16))
-        nb = min(args.cpu_batches, total_batches)
-        cb = cpu_baseline(pairs[: nb * BATCH], threads, cube)
-        cb.pop("seconds")
-        result["cpu_baseline"] = cb
-        log(f"cpu baseline {cb['value']:.3e} edges/s on {threads} threads")
-    print(json.dumps(result), flush=True)
+    print(json.dumps(bench_single(args)), flush=True)
 
 
 if __name__ == "__main__":

@@ -83,6 +83,7 @@ _sigs = {
     "gs_min_cut": ([_u32, _vp, _u64, _vp, _vp], _i32),
     "gs_stream_set_profiling": ([_vp, _i32], _i32),
     "gs_stream_timing": ([_vp, _vp, _vp, _vp], _i32),
+    "gs_stream_kernel_times": ([_vp, _vp, _vp, _vp, _vp], _i32),
     "gs_stream_traffic": ([_vp, _vp, _vp], _i32),
     "gs_stream_set_tuning": ([_vp, _vp], _i32),
     "gs_stream_get_tuning": ([_vp, _vp], _i32),
@@ -265,6 +266,14 @@ class Stream:
                                     ctypes.byref(upd)))
         return ms.value, launches.value, upd.value
 
+    def kernel_times(self):
+        """(grouping ms, aggregation ms, launches, directed updates) of the profiled launches
+        since the last call."""
+        g, ms, launches, upd = ctypes.c_double(), ctypes.c_double(), ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib.gs_stream_kernel_times(self.h, ctypes.byref(g), ctypes.byref(ms),
+                                          ctypes.byref(launches), ctypes.byref(upd)))
+        return g.value, ms.value, launches.value, upd.value
+
     def traffic(self):
         """(device buckets read-modify-written by flushes, updates XORed straight into HBM) of
         the profiled launches since the last call."""
@@ -355,6 +364,7 @@ class SketchStore:
         return out
 
     def close(self) -> None:
+        """Frees the store (the library refuses while a group holds it: close the group first)."""
         if self.h:
             _check(lib.gs_store_destroy(self.h))
             self.h = ctypes.c_void_p()

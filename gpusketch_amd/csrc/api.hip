@@ -105,6 +105,9 @@ void destroy_stream_resources(Stream& s) {
     (void)hipEventDestroy(p.first);
     (void)hipEventDestroy(p.second);
   }
+  for (hipEvent_t g : s.ev_group)
+    if (g) (void)hipEventDestroy(g);
+  s.ev_group.clear();
   if (s.stage_free) (void)hipEventDestroy(s.stage_free);
   if (s.copy_stream) (void)hipStreamDestroy(s.copy_stream);
   if (s.stream && s.own_stream) (void)hipStreamDestroy(s.stream);
@@ -236,12 +239,16 @@ int gs_store_create(uint32_t n, uint64_t seed, uint32_t num_samples, uint32_t co
 int gs_store_destroy(gs_store* h) {
   if (!h) return GS_OK;
   Store& st = h->impl;
-  hipDeviceSynchronize();
   std::vector<Stream*> ss;
   {
     std::lock_guard<std::mutex> lk(st.mu);
     ss = st.streams;
   }
+  // a group holds the store's rows and writes them from its own stream: it goes first
+  for (Stream* s : ss)
+    GS_CHECK(!s->group, GS_ERR_INVALID,
+             "gs_store_destroy: the store is a member of a group; destroy the group first");
+  hipDeviceSynchronize();
   for (Stream* s : ss) gs_stream_destroy(reinterpret_cast<gs_stream*>(s));
   free_query_scratch(&st);
   if (st.qstream) hipStreamDestroy(st.qstream);
@@ -414,8 +421,9 @@ int gs_flush(gs_stream* sh) {
     GS_HIP(hipMemcpy(&dropped, s.ws.d_counters + 3, sizeof(uint32_t), hipMemcpyDeviceToHost));
     if (dropped) {
       GS_HIP(hipMemset(s.ws.d_counters + 3, 0, sizeof(uint32_t)));
-      return fail(GS_ERR_INVALID, "%u device-submitted updates named a vertex >= n and were dropped",
-                  dropped);
+      return fail(GS_ERR_INVALID,
+                  "%u submitted updates named a vertex outside the store (or joined two members "
+                  "of a group) and were dropped", dropped);
     }
   }
   return GS_OK;
@@ -602,26 +610,41 @@ int gs_stream_set_profiling(gs_stream* sh, int enable) {
   return GS_OK;
 }
 
-int gs_stream_timing(gs_stream* sh, double* agg_ms, uint64_t* launches, uint64_t* updates) {
+int gs_stream_kernel_times(gs_stream* sh, double* grouping_ms, double* agg_ms,
+                           uint64_t* launches, uint64_t* updates) {
   GS_CHECK(sh, GS_ERR_INVALID, "null stream");
   Stream& s = sh->impl;
   GS_HIP(hipStreamSynchronize(s.stream));
-  for (auto& p : s.ev_pairs) {
+  for (size_t i = 0; i < s.ev_pairs.size(); i++) {
+    const auto& p = s.ev_pairs[i];
     float ms = 0;
     GS_HIP(hipEventElapsedTime(&ms, p.first, p.second));
     s.agg_ms += ms;
     s.agg_launches++;
+    if (s.ev_group[i]) {
+      float gms = 0;
+      GS_HIP(hipEventElapsedTime(&gms, s.ev_group[i], p.first));
+      s.grp_ms += gms;
+      hipEventDestroy(s.ev_group[i]);
+    }
     hipEventDestroy(p.first);
     hipEventDestroy(p.second);
   }
   s.ev_pairs.clear();
+  s.ev_group.clear();
+  if (grouping_ms) *grouping_ms = s.grp_ms;
   if (agg_ms) *agg_ms = s.agg_ms;
   if (launches) *launches = s.agg_launches;
   if (updates) *updates = s.agg_updates;
+  s.grp_ms = 0;
   s.agg_ms = 0;
   s.agg_launches = 0;
   s.agg_updates = 0;
   return GS_OK;
+}
+
+int gs_stream_timing(gs_stream* sh, double* agg_ms, uint64_t* launches, uint64_t* updates) {
+  return gs_stream_kernel_times(sh, nullptr, agg_ms, launches, updates);
 }
 
 int gs_stream_traffic(gs_stream* sh, uint64_t* flushed_buckets, uint64_t* direct_updates) {

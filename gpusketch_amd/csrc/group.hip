@@ -24,6 +24,8 @@ static void unregister_from_members(Group& g) {
 
 static void destroy_group(gs_group* h) {
   Group& g = h->impl;
+  // pending updates are applied before the group's stream goes away (as gs_stream_destroy does)
+  if (g.s && g.s->stream && g.s->d_pend[0]) (void)drain_stream(g.s);
   if (g.s && g.s->stream) (void)hipStreamSynchronize(g.s->stream);
   free_group_query(&g);
   if (g.s) {
@@ -143,8 +145,18 @@ int gs_group_submit_pairs(gs_group* h, const uint32_t* const* pairs, const uint6
   Group& g = h->impl;
   Stream* s = g.s;
   const uint32_t count = (uint32_t)g.members.size();
-  for (uint32_t i = 0; i < count; i++)
+  // every member's ids are checked before anything is submitted: a member-local id past the
+  // member's end would name another member's vertex once offset
+  for (uint32_t i = 0; i < count; i++) {
     GS_CHECK(pairs[i] || nedges[i] == 0, GS_ERR_INVALID, "member %u: null pairs", i);
+    const uint32_t n = g.members[i]->view.n;
+    uint32_t bad = 0;
+    for (uint64_t e = 0; e < 2 * nedges[i]; e++) bad |= pairs[i][e] >= n;
+    if (bad)
+      for (uint64_t e = 0; e < 2 * nedges[i]; e++)
+        GS_CHECK(pairs[i][e] < n, GS_ERR_INVALID, "member %u edge %llu: vertex %u out of range", i,
+                 (unsigned long long)(e / 2), pairs[i][e]);
+  }
   const uint64_t chunk = kGroupStageEdges;
   uint32_t mi = 0;
   uint64_t ei = 0;  // next edge of member mi
@@ -164,15 +176,12 @@ int gs_group_submit_pairs(gs_group* h, const uint32_t* const* pairs, const uint6
         ei = 0;
         continue;
       }
-      const uint32_t n = g.members[mi]->view.n, base = g.bases[mi];
+      const uint32_t base = g.bases[mi];
       const uint64_t take = std::min(chunk - m, nedges[mi] - ei);
       const uint32_t* p = pairs[mi] + 2 * ei;
       for (uint64_t i = 0; i < take; i++) {
-        const uint32_t u = p[2 * i], v = p[2 * i + 1];
-        GS_CHECK(u < n && v < n, GS_ERR_INVALID, "member %u edge %llu: vertex out of range", mi,
-                 (unsigned long long)(ei + i));
-        out[2 * (m + i)] = u + base;
-        out[2 * (m + i) + 1] = v + base;
+        out[2 * (m + i)] = p[2 * i] + base;
+        out[2 * (m + i) + 1] = p[2 * i + 1] + base;
       }
       m += take;
       ei += take;

@@ -184,6 +184,10 @@ struct Stream {
   bool profile = false;
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pairs;
+  // per aggregation launch fed by the edge-pair grouping: the event recorded before the grouping
+  // kernels (the launch's ev_pairs.first ends them); nullptr for vertex-batch launches
+  std::vector<hipEvent_t> ev_group;
+  double grp_ms = 0;
   double agg_ms = 0;
   uint64_t agg_launches = 0;
   uint64_t agg_updates = 0;

@@ -16,6 +16,7 @@
 // deterministic bucket is a wave XOR-reduction. Depths >= the LDS window go straight to HBM with
 // atomics (probability 2^-16 per column and update).
 #include <chrono>
+#include <cstring>
 
 #include "gs_internal.h"
 
@@ -93,14 +94,19 @@ __device__ __forceinline__ uint32_t hashed_id(const PartParams& pp, uint32_t v) 
 __device__ __forceinline__ uint32_t bucket_of(const PartParams& pp, uint32_t v) {
   return hashed_id(pp, v) >> pp.local_bits;
 }
-__device__ __forceinline__ bool entry_ok(uint2 uv, uint32_t n) {
-  return uv.x < n && uv.y < n && uv.x != uv.y;
+// a pair the grouping drops: a vertex outside the vertex space, or, in a group of stores
+// (vgraph: the member of every global vertex), endpoints in two different members
+__device__ __forceinline__ bool pair_invalid(uint2 uv, uint32_t n, const uint32_t* vgraph) {
+  return uv.x >= n || uv.y >= n || (vgraph && vgraph[uv.x] != vgraph[uv.y]);
+}
+__device__ __forceinline__ bool entry_ok(uint2 uv, uint32_t n, const uint32_t* vgraph) {
+  return uv.x != uv.y && !pair_invalid(uv, n, vgraph);
 }
 
 // The pairs of one launch: up to kMaxSegs segments (the fill buffer, then device batches read in
 // place), each covered by whole K1/K2 workgroups of kPartEdges edges; blk0[i] is segment i's first
 // workgroup, blk0[nseg] the total.
-constexpr int kMaxSegs = 32;
+constexpr int kMaxSegs = 48;  // a 2^25-edge launch of 2^20-edge device batches takes 32
 constexpr uint64_t kZeroCopyEdges = 1ULL << 18;  // device batches at least this long: in place
 struct PairSegs {
   const uint2* ptr[kMaxSegs];
@@ -132,11 +138,11 @@ __global__ void append_kernel(const uint2* src, uint2* dst, uint64_t nedges, uin
   }
 }
 
-// out-of-range pairs are counted (reported at the next flush) and skipped here and in K2
-// (entry_ok), never written: the pairs may be the caller's own device memory
+// invalid pairs (pair_invalid) are counted (reported at the next flush) and skipped here and in
+// K3 (entry_ok), never written: the pairs may be the caller's own device memory
 __global__ __launch_bounds__(kPartThreads) void bucket_count_kernel(
-    PairSegs sg, uint32_t n, PartParams pp, uint32_t nblk, uint32_t* __restrict__ hist,
-    uint32_t* __restrict__ counters) {
+    PairSegs sg, uint32_t n, const uint32_t* __restrict__ vgraph, PartParams pp, uint32_t nblk,
+    uint32_t* __restrict__ hist, uint32_t* __restrict__ counters) {
   extern __shared__ uint32_t h[];
   const uint32_t t = threadIdx.x;
   if (blockIdx.x == 0 && t < kClassBase + kClasses && t != 3)
@@ -151,7 +157,7 @@ __global__ __launch_bounds__(kPartThreads) void bucket_count_kernel(
     const uint64_t e = e0 + t + (uint64_t)k * kPartThreads;
     if (e >= e1) continue;
     const uint2 uv = pairs[e];
-    if (uv.x >= n || uv.y >= n) {  // never index past the store: drop and report at flush
+    if (pair_invalid(uv, n, vgraph)) {  // never index past the store: drop and report at flush
       atomicAdd(&counters[3], 1u);
       continue;
     }
@@ -234,7 +240,8 @@ __host__ __device__ __forceinline__ bool packed_entries_fit(const PartParams& pp
 // consecutive lanes write consecutive positions of one bucket's run (coalesced stores).
 template <bool kPacked>
 __global__ __launch_bounds__(kPartThreads) void bucket_scatter_kernel(
-    PairSegs sg, uint32_t n, PartParams pp, uint32_t nblk, const uint32_t* __restrict__ totals,
+    PairSegs sg, uint32_t n, const uint32_t* __restrict__ vgraph, PartParams pp, uint32_t nblk,
+    const uint32_t* __restrict__ totals,
     const uint32_t* __restrict__ hist, uint2* __restrict__ entries) {
   extern __shared__ __align__(16) uint32_t lds_u32[];
   uint2* s_stage = reinterpret_cast<uint2*>(lds_u32);  // 2 * kPartEdges entries
@@ -259,7 +266,7 @@ __global__ __launch_bounds__(kPartThreads) void bucket_scatter_kernel(
     uvs[k] = make_uint2(0, 0);
     if (e >= e1) continue;
     uint2 uv = pairs[e];
-    if (!entry_ok(uv, n)) continue;
+    if (!entry_ok(uv, n, vgraph)) continue;
     uvs[k] = uv;
     bk[2 * k] = bucket_of(pp, uv.x);
     bk[2 * k + 1] = bucket_of(pp, uv.y);
@@ -1145,6 +1152,7 @@ static int launch_wide_aggregate(Stream* s, const WorkItem* d_items, const ItemC
   if (s->profile) {
     GS_HIP(hipEventRecord(e1, s->stream));
     s->ev_pairs.emplace_back(e0, e1);
+    s->ev_group.push_back(nullptr);
     s->agg_updates += updates;
   }
   return GS_OK;
@@ -1205,6 +1213,7 @@ static int launch_aggregate(Stream* s, const WorkItem* d_items, const ItemClasse
   if (s->profile) {
     GS_HIP(hipEventRecord(e1, s->stream));
     s->ev_pairs.emplace_back(e0, e1);
+    s->ev_group.push_back(nullptr);
     s->agg_updates += updates;
   }
   return GS_OK;
@@ -1396,20 +1405,26 @@ static int process_pending(Stream* s) {
     GS_HIP(hipStreamWaitEvent(s->stream, s->copied, 0));
     s->copies_pending = false;
   }
+  hipEvent_t g0 = nullptr;  // start of the grouping kernels (measurement only)
+  if (s->profile) {
+    GS_HIP(hipEventCreate(&g0));
+    GS_HIP(hipEventRecord(g0, s->stream));
+  }
+  const uint32_t* vgraph = s->group ? s->group->d_vgraph : nullptr;
   hipLaunchKernelGGL(bucket_count_kernel, dim3(nblk), dim3(kPartThreads),
-                     sizeof(uint32_t) * pp.nbuckets, s->stream, sg, nverts, pp, nblk, ws.d_hist,
-                     ws.d_counters);
+                     sizeof(uint32_t) * pp.nbuckets, s->stream, sg, nverts, vgraph, pp, nblk,
+                     ws.d_hist, ws.d_counters);
   GS_HIP(hipGetLastError());
   hipLaunchKernelGGL(bucket_offsets_kernel, dim3(pp.nbuckets), dim3(kOffThreads), 0, s->stream,
                      ws.d_hist, nblk, ws.d_totals);
   GS_HIP(hipGetLastError());
   if (packed)
     hipLaunchKernelGGL(bucket_scatter_kernel<true>, dim3(nblk), dim3(kPartThreads),
-                       scatter_lds_bytes(pp.nbuckets), s->stream, sg, nverts, pp, nblk,
+                       scatter_lds_bytes(pp.nbuckets), s->stream, sg, nverts, vgraph, pp, nblk,
                        ws.d_totals, ws.d_hist, ws.d_entries);
   else
     hipLaunchKernelGGL(bucket_scatter_kernel<false>, dim3(nblk), dim3(kPartThreads),
-                       scatter_lds_bytes(pp.nbuckets), s->stream, sg, nverts, pp, nblk,
+                       scatter_lds_bytes(pp.nbuckets), s->stream, sg, nverts, vgraph, pp, nblk,
                        ws.d_totals, ws.d_hist, ws.d_entries);
   GS_HIP(hipGetLastError());
   GS_HIP(hipEventRecord(s->pend_free[b], s->stream));  // the fill buffer has been read
@@ -1430,8 +1445,14 @@ static int process_pending(Stream* s) {
   s->fill_n = 0;
   s->zsegs.clear();
   s->pend_buf ^= 1;
-  return launch_aggregate(s, ws.d_items, ws.cls, ws.d_counters + kClassBase, ws.d_sorted,
-                          2 * nedges);
+  const size_t nev = s->ev_pairs.size();
+  rc = launch_aggregate(s, ws.d_items, ws.cls, ws.d_counters + kClassBase, ws.d_sorted,
+                        2 * nedges);
+  if (g0) {
+    if (s->ev_pairs.size() == nev + 1) s->ev_group.back() = g0;
+    else (void)hipEventDestroy(g0);
+  }
+  return rc;
 }
 
 int submit_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uint32_t min_depth,
@@ -1458,9 +1479,10 @@ int submit_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uin
 }
 
 // Host batches: copied on the copy stream straight into the fill buffer (from the caller's
-// memory when it is pinned, else through the stream's pinned staging, range-checked on the way),
-// then appended in place on the compute stream. The call returns once the caller's memory has
-// been read.
+// memory when it is pinned, else through the stream's pinned staging), then appended in place on
+// the compute stream. The call returns once the caller's memory has been read. Pairs naming a
+// vertex outside the store are dropped by the grouping and reported by the next flush, as for
+// device batches, whatever memory they came from.
 int submit_pairs_host(Stream* s, const uint32_t* pairs, uint64_t nedges, uint32_t min_depth,
                       uint32_t depth_cap) {
   if (nedges == 0) return GS_OK;
@@ -1473,24 +1495,13 @@ int submit_pairs_host(Stream* s, const uint32_t* pairs, uint64_t nedges, uint32_
     pinned = attr.type == hipMemoryTypeHost;
   else
     (void)hipGetLastError();
-  const uint32_t n = stream_vertices(s);
   for (uint64_t e0 = 0; e0 < nedges;) {
     const uint64_t m = std::min(nedges - e0, s->pend_cap - s->pend_n);
     const size_t bytes = m * 2 * sizeof(uint32_t);
     const uint32_t* src = pairs + 2 * e0;
     if (!pinned) {
       if ((rc = ensure_stage(s, bytes, false))) return rc;
-      uint32_t bad = 0;
-      for (uint64_t i = 0; i < 2 * m; i++) {
-        const uint32_t v = src[i];
-        bad |= v >= n;
-        s->h_stage[i] = v;
-      }
-      if (bad) {
-        for (uint64_t i = 0; i < 2 * m; i++)
-          GS_CHECK(src[i] < n, GS_ERR_INVALID, "edge %llu: vertex %u out of range",
-                   (unsigned long long)(e0 + i / 2), src[i]);
-      }
+      std::memcpy(s->h_stage, src, bytes);
       src = s->h_stage;
     }
     // the fill buffer may still be read by the launch that processed it two buffers back

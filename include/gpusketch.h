@@ -83,6 +83,9 @@ int gs_hash_known_answer(const uint64_t* keys, const uint64_t* seeds, uint64_t c
  * num_samples * cols_per_sample columns, at most 4096. */
 int gs_store_create(uint32_t n, uint64_t seed, uint32_t num_samples, uint32_t cols_per_sample,
                     uint32_t flags, gs_store** out);
+/* Applies what its streams hold pending, then frees the store and its streams. Refused
+ * (GS_ERR_INVALID, nothing freed) while the store is a member of a group: destroy the group
+ * first. */
 int gs_store_destroy(gs_store* st);
 int gs_store_geometry(const gs_store* st, gs_geometry* out);
 /* Sketch::zero_contents for every vertex (src/sketch.cpp:102-108). */
@@ -114,7 +117,10 @@ int gs_submit_vertex_batches_filtered(gs_stream* s, const uint32_t* src, const u
 /* Raw undirected updates: pairs[2i], pairs[2i+1] toggles edge {u,v} in both endpoints'
  * sketches, as GraphSketchDriver inserts {u,v} and {v,u} (include/graph_sketch_driver.h:171-172)
  * and CCSketchAlg::update applies them (src/cc_sketch_alg.cpp:154-160). Grouping by vertex
- * happens on the device. Host memory; asynchronous after staging. */
+ * happens on the device. Host memory (pinned or pageable); asynchronous after staging. For every
+ * edge-pair submission (host or device memory): self loops are no-ops, and pairs naming a vertex
+ * >= n are skipped and make the stream's next gs_flush return GS_ERR_INVALID with their count;
+ * the other pairs of the submission are applied. */
 int gs_submit_edge_pairs(gs_stream* s, const uint32_t* pairs, uint64_t nedges);
 /* Same, input already resident in device memory. It must stay valid and unchanged until the
  * stream's next gs_flush (or gs_sync): long batches (2^18 edges and more) are read in place by
@@ -168,6 +174,12 @@ int gs_min_cut(uint32_t n, const uint32_t* pairs, uint64_t m, uint64_t* value,
  * milliseconds, the launch count and the directed updates those launches processed. */
 int gs_stream_set_profiling(gs_stream* s, int enable);
 int gs_stream_timing(gs_stream* s, double* agg_ms, uint64_t* launches, uint64_t* updates);
+/* The same, plus the summed milliseconds of the grouping kernels (count, offsets, scatter, sort)
+ * that preceded the profiled edge-pair launches. Like gs_stream_timing it returns and resets the
+ * sums. No counterpart in the reference (its kernel time is cudaEvent-timed in
+ * SKGPUSketchAlg::launch_gpu_kernel, src/sk_gpu_sketch_alg.cpp:156-179). */
+int gs_stream_kernel_times(gs_stream* s, double* grouping_ms, double* agg_ms, uint64_t* launches,
+                           uint64_t* updates);
 /* HBM work of the profiled aggregation launches, summed and reset: device buckets read-modify-
  * written by flushes (non-zero delta slots and deterministic buckets) and (vertex, bucket)
  * updates beyond the LDS window XORed straight into HBM. No counterpart in the reference (its
@@ -207,15 +219,19 @@ int gs_stream_autotune(gs_stream* s, const uint32_t* d_pairs, uint64_t nedges, g
  * store would give (each member's sketches bit-exact, partitions identical). */
 typedef struct gs_group gs_group;
 int gs_group_create(gs_store* const* stores, uint32_t count, gs_group** out);
+/* Applies the updates the group's stream holds pending, then frees the group (its members stay). */
 int gs_group_destroy(gs_group* g);
 int gs_group_info(const gs_group* g, uint32_t member, uint32_t* base, uint32_t* total);
 /* The group's stream, for gs_flush / gs_stream_set_profiling / gs_stream_timing (not for
  * gs_stream_destroy). */
 int gs_group_stream(gs_group* g, gs_stream** out);
 /* Host edge pairs of every member (member-local ids; NULL with nedges[i] = 0 skips a member),
- * ingested as one stream of the group's vertex space. */
+ * ingested as one stream of the group's vertex space. Every id is checked first: an id past its
+ * member's end fails the call (GS_ERR_INVALID) before anything is submitted. */
 int gs_group_submit_pairs(gs_group* g, const uint32_t* const* pairs, const uint64_t* nedges);
-/* Device-resident pairs already in global ids (base_i added); must stay valid until gs_flush. */
+/* Device-resident pairs already in global ids (base_i added); must stay valid until gs_flush.
+ * A pair whose endpoints lie in two different members is skipped and counted like an
+ * out-of-range pair (the group stream's next gs_flush returns GS_ERR_INVALID). */
 int gs_group_submit_pairs_device(gs_group* g, const uint32_t* d_pairs, uint64_t nedges);
 /* Allocates now what the group's first submission and first query would allocate (the stream's
  * fill buffers and grouping scratch at its current launch size, the pinned staging, the batched

@@ -178,3 +178,48 @@ def test_group_query_giants_with_pair_and_batch_fed_members():
     finally:
         for s in stores + singles:
             s.close()
+
+
+def test_cross_member_pairs_are_dropped_and_reported():
+    """A device submission in global ids whose pair joins two members is skipped (it would XOR an
+    index that never cancels into both members) and reported by the flush; the other pairs of the
+    submission are applied."""
+    rng = np.random.default_rng(7)
+    sizes, seeds = [300, 500, 200], [11, 12, 13]
+    stores = [gs.SketchStore(n, s) for n, s in zip(sizes, seeds)]
+    try:
+        with gs.StoreGroup(stores) as grp:
+            local = [member_pairs(rng, n, 4 * n) for n in sizes]
+            glob = np.concatenate([p + np.uint32(b) for p, b in zip(local, grp.bases)])
+            cross = np.array([[5, grp.bases[1] + 7], [grp.bases[2] + 1, grp.bases[1]],
+                              [grp.bases[1] - 1, grp.bases[1]]], dtype=np.uint32)
+            allp = np.concatenate([glob[:1000], cross, glob[1000:]])
+            buf = gs.DeviceBuffer(allp)
+            grp.submit_pairs_device(buf.ptr, len(allp))
+            with pytest.raises(gs.GsError, match="3 submitted updates"):
+                grp.flush()
+            buf.free()
+            for i, (n, sd) in enumerate(zip(sizes, seeds)):
+                assert np.array_equal(stores[i].export_buckets(), oracle_store(n, sd, local[i]))
+            # host submissions are checked member by member before anything is submitted
+            with pytest.raises(gs.GsError, match="out of range"):
+                grp.submit_pairs([local[0], np.array([[0, sizes[1]]], dtype=np.uint32), None])
+            grp.flush()
+            for i, (n, sd) in enumerate(zip(sizes, seeds)):
+                assert np.array_equal(stores[i].export_buckets(), oracle_store(n, sd, local[i]))
+    finally:
+        for st in stores:
+            st.close()
+
+
+def test_store_destroy_refused_while_in_group():
+    a, b = gs.SketchStore(100, 1), gs.SketchStore(200, 2)
+    grp = gs.StoreGroup([a, b])
+    grp.submit_pairs([np.array([[0, 1]], dtype=np.uint32), np.array([[2, 3]], dtype=np.uint32)])
+    with pytest.raises(gs.GsError, match="destroy the group first"):
+        a.close()
+    assert a.h  # still alive and usable
+    grp.close()
+    assert np.array_equal(a.export_buckets(), oracle_store(100, 1, np.array([[0, 1]], dtype=np.uint32)))
+    a.close()
+    b.close()

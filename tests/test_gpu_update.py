@@ -138,11 +138,26 @@ def test_empty_self_loops_and_toggle_cancel():
         assert not st.export_buckets().any()
 
 
-def test_out_of_range_vertex_is_rejected():
+def test_out_of_range_vertex_is_dropped_and_reported():
+    # host batches (pageable and pinned) behave as device batches: the pairs naming a vertex >= n
+    # are skipped, the rest is applied, and the next flush reports the dropped count
+    good = np.array([[1, 2], [3, 99], [0, 50]], dtype=np.uint32)
+    bad = np.array([[1, 100], [4000000000, 3]], dtype=np.uint32)
+    mixed = np.concatenate([good[:2], bad[:1], good[2:], bad[1:]])
     with gs.SketchStore(100, 5) as st:
         s = st.stream()
-        with pytest.raises(gs.GsError):
-            s.submit_pairs(np.array([[1, 100]], dtype=np.uint32))
+        s.submit_pairs(mixed)
+        with pytest.raises(gs.GsError, match="2 submitted updates"):
+            s.flush()
+        s.flush()  # reported once
+        assert np.array_equal(st.export_buckets(), oracle_store(100, 5, good))
+        pin = gs.PinnedBuffer(mixed.shape)
+        pin.array[:] = mixed
+        s.submit_pairs(pin.array)  # toggles the good pairs back out
+        with pytest.raises(gs.GsError, match="dropped"):
+            s.flush()
+        pin.free()
+        assert not st.export_buckets().any()
 
 
 def test_mixed_submissions_in_one_launch_and_caller_memory_untouched():
