@@ -80,6 +80,7 @@ _sigs = {
     "gs_import_buckets": ([_vp, _u32, _u32, _vp], _i32),
     "gs_query_cc": ([_vp, _vp, _vp, _vp, _vp, _u64, _vp], _i32),
     "gs_k_spanning_forests": ([_vp, _u32, _vp, _u64, _vp], _i32),
+    "gs_sample_vertices": ([_vp, _u32, _vp, _vp], _i32),
     "gs_min_cut": ([_u32, _vp, _u64, _vp, _vp], _i32),
     "gs_stream_set_profiling": ([_vp, _i32], _i32),
     "gs_stream_timing": ([_vp, _vp, _vp, _vp], _i32),
@@ -351,6 +352,14 @@ class SketchStore:
         if want_forest:
             return labels, int(ncc.value), int(rounds.value), forest[: flen.value].copy()
         return labels, int(ncc.value), int(rounds.value)
+
+    def sample_vertices(self, sample_idx: int):
+        """Every vertex sketch's sample at sample_idx as the device query's round 0 computes it:
+        (result codes 0 GOOD / 1 ZERO / 2 FAIL, sampled indices)."""
+        res = np.zeros(self.n, dtype=np.int32)
+        idx = np.zeros(self.n, dtype=np.uint64)
+        _check(lib.gs_sample_vertices(self.h, sample_idx, _p(res), _p(idx)))
+        return res, idx
 
     def k_spanning_forests(self, k: int):
         cap = k * self.n

@@ -264,11 +264,15 @@ __device__ __forceinline__ int wave_sample_wide(Load load, uint32_t lane, uint32
 
 // Round 0, sketches with several columns per sample: every vertex samples its own sample col0,
 // one vertex at a time per wave.
+// rec_res (test hook, else null): every vertex's sample result (0 GOOD, 1 ZERO, 2 FAIL) and
+// index are recorded there instead of being merged
 __global__ __launch_bounds__(kQThreads) void round0_wide_kernel(StoreView st, uint32_t cps,
                                                                 uint32_t col0, uint32_t* parent,
                                                                 uint32_t* vflags, uint32_t* flags,
                                                                 uint32_t* forest,
-                                                                uint64_t forest_cap) {
+                                                                uint64_t forest_cap,
+                                                                int32_t* rec_res,
+                                                                uint64_t* rec_idx) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64 * kR0Verts;
   if (v0 >= st.n) return;
@@ -289,7 +293,14 @@ __global__ __launch_bounds__(kQThreads) void round0_wide_kernel(StoreView st, ui
       my_idx = idx;
     }
   }
-  if (my_res >= 0) handle_sample(my_res, my_idx, v0 + lane, st.n, parent, vflags, flags, forest, forest_cap);
+  if (my_res >= 0) {
+    if (rec_res) {
+      rec_res[v0 + lane] = my_res;
+      rec_idx[v0 + lane] = my_idx;
+    } else {
+      handle_sample(my_res, my_idx, v0 + lane, st.n, parent, vflags, flags, forest, forest_cap);
+    }
+  }
 }
 
 // Round 0: every vertex samples its own column col0 (run_round_zero, src/cc_sketch_alg.cpp:
@@ -298,10 +309,12 @@ __global__ __launch_bounds__(kQThreads) void round0_wide_kernel(StoreView st, ui
 // the deterministic bucket and depths 0..15 only; a vertex with no good bucket among them reads
 // its deeper depths in a second pass (the scan order stays the reference's).
 constexpr uint32_t kRound0Depths = 16;
+// (rec_res, rec_idx: the test hook of round0_wide_kernel)
 __global__ __launch_bounds__(kQThreads) void round0_kernel(StoreView st, uint32_t col0,
                                                            uint32_t* parent, uint32_t* vflags,
                                                            uint32_t* flags, uint32_t* forest,
-                                                           uint64_t forest_cap) {
+                                                           uint64_t forest_cap, int32_t* rec_res,
+                                                           uint64_t* rec_idx) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t v0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64 * kR0Verts;
   if (v0 >= st.n) return;
@@ -342,7 +355,14 @@ __global__ __launch_bounds__(kQThreads) void round0_kernel(StoreView st, uint32_
       }
     }
   }
-  if (my_res >= 0) handle_sample(my_res, my_idx, v0 + lane, st.n, parent, vflags, flags, forest, forest_cap);
+  if (my_res >= 0) {
+    if (rec_res) {
+      rec_res[v0 + lane] = my_res;
+      rec_idx[v0 + lane] = my_idx;
+    } else {
+      handle_sample(my_res, my_idx, v0 + lane, st.n, parent, vflags, flags, forest, forest_cap);
+    }
+  }
 }
 
 // Big supernodes of a round, picked from kPickSamples sampled vertices (pick_big_kernel): up to kBig live
@@ -738,10 +758,10 @@ static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t
   // round 0
   if (wide)
     hipLaunchKernelGGL(round0_wide_kernel, g_r0, block, 0, q, v, cps, col0, st->d_parent,
-                       st->d_vflags, st->d_flags, forest, cap);
+                       st->d_vflags, st->d_flags, forest, cap, nullptr, nullptr);
   else
     hipLaunchKernelGGL(round0_kernel, g_r0, block, 0, q, v, col0 * cps, st->d_parent,
-                       st->d_vflags, st->d_flags, forest, cap);
+                       st->d_vflags, st->d_flags, forest, cap, nullptr, nullptr);
   hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
                      st->d_vflags, st->d_flags);
   hipLaunchKernelGGL(end_round_kernel, dim3(1), dim3(64), 0, q, st->d_flags, st->num_samples);
@@ -1214,4 +1234,42 @@ extern "C" int gs_k_spanning_forests(gs_store* h, uint32_t k, uint32_t* edges_ou
   GS_CHECK(h && counts && (edges_out || edge_cap == 0), GS_ERR_INVALID, "null argument");
   GS_CHECK(k >= 1, GS_ERR_INVALID, "k must be >= 1");
   return gs::k_forests(&h->impl, k, edges_out, edge_cap, counts);
+}
+
+// Test hook: Sketch::sample of every vertex sketch at sample index `sample_idx`, computed by the
+// round-0 samplers of the device Borůvka (the scan the query runs), recorded instead of merged.
+using namespace gs;
+extern "C" int gs_sample_vertices(gs_store* h, uint32_t sample_idx, int32_t* result,
+                                  uint64_t* idx_out) {
+  GS_CHECK(h && result && idx_out, GS_ERR_INVALID, "null argument");
+  Store* st = &h->impl;
+  GS_CHECK(sample_idx < st->num_samples, GS_ERR_OUT_OF_SAMPLES,
+           "sample index %u past the sketch's %u samples", sample_idx, st->num_samples);
+  int rc = sync_store(st);
+  if (rc) return rc;
+  const StoreView& v = st->view;
+  const uint32_t n = v.n;
+  int32_t* d_res = nullptr;
+  uint64_t* d_idx = nullptr;
+  if (hipMalloc((void**)&d_res, sizeof(int32_t) * n) != hipSuccess ||
+      hipMalloc((void**)&d_idx, sizeof(uint64_t) * n) != hipSuccess) {
+    (void)hipFree(d_res);
+    return fail(GS_ERR_OOM, "gs_sample_vertices: %u results", n);
+  }
+  hipStream_t q = st->qstream;
+  const dim3 g_r0 = wave_grid(n, kR0Verts), block(kQThreads);
+  if (st->cols_per_sample > 1)
+    hipLaunchKernelGGL(round0_wide_kernel, g_r0, block, 0, q, v, st->cols_per_sample, sample_idx,
+                       nullptr, nullptr, nullptr, nullptr, 0ull, d_res, d_idx);
+  else
+    hipLaunchKernelGGL(round0_kernel, g_r0, block, 0, q, v, sample_idx, nullptr, nullptr,
+                       nullptr, nullptr, 0ull, d_res, d_idx);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(q);
+  if (e == hipSuccess) e = hipMemcpy(result, d_res, sizeof(int32_t) * n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(idx_out, d_idx, sizeof(uint64_t) * n, hipMemcpyDeviceToHost);
+  (void)hipFree(d_res);
+  (void)hipFree(d_idx);
+  GS_HIP(e);
+  return GS_OK;
 }

@@ -115,6 +115,17 @@ struct PartParams {
   uint32_t nbits, local_bits, nbuckets, mask, mul, inv;
 };
 
+// The pairs of one launch: up to kMaxSegs segments (the fill buffer, then device batches read in
+// place), each covered by whole K1/K3 workgroups of 4096 edges; blk0[i] is segment i's first
+// workgroup, blk0[nseg] the total.
+constexpr int kMaxSegs = 48;  // a 2^25-edge launch of 2^20-edge device batches takes 32
+struct PairSegs {
+  const uint2* ptr[kMaxSegs];
+  uint64_t n[kMaxSegs];
+  uint32_t blk0[kMaxSegs + 1];
+  uint32_t nseg;
+};
+
 // per-stream scratch for the edge-pair pipeline, grown on demand
 // Work items are grouped into length classes (class = floor(log2(len)), len <= 1023) so that
 // the aggregation kernel can hand out the longest items first.
@@ -292,6 +303,15 @@ int ingest_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uin
                         uint32_t depth_cap);
 uint64_t stream_launch_edges(const Stream* s);
 int ensure_pair_scratch(Stream* s, uint64_t nedges);
+// the grouping scratch for `nedges` pairs carved from one buffer at `base` (256-byte aligned
+// pieces; base = nullptr only sizes it): sets ws's pointers and class layout, returns the bytes
+size_t carve_pair_scratch(const Stream* s, uint64_t nedges, char* base, PairScratch& ws);
+// append a device segment of m pairs to a launch
+void add_pair_segment(PairSegs& sg, uint32_t& nblk, const uint32_t* p, uint64_t m);
+// the grouping and aggregation kernels over `sg` on s's HIP stream with scratch `ws`
+int run_pair_pipeline(Stream* s, const PairSegs& sg, uint32_t nblk, uint64_t nedges,
+                      PairScratch& ws, hipEvent_t input_read);
+void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t stream);
 PartParams make_part_params(uint32_t n, int local_bits = -1);
 int autotune_stream(Stream* s, const uint32_t* d_pairs, uint64_t nedges, gs_tuning* best);
 void free_pair_scratch(PairScratch& ws);

@@ -68,6 +68,11 @@ __global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v) {
     p[i] = v;
 }
 
+void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t stream) {
+  const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 1024));
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(grid), dim3(256), 0, stream, p, n, v);
+}
+
 __global__ void fill_u64_kernel(uint64_t* p, uint64_t n, uint64_t v) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x)
@@ -103,17 +108,7 @@ __device__ __forceinline__ bool entry_ok(uint2 uv, uint32_t n, const uint32_t* v
   return uv.x != uv.y && !pair_invalid(uv, n, vgraph);
 }
 
-// The pairs of one launch: up to kMaxSegs segments (the fill buffer, then device batches read in
-// place), each covered by whole K1/K2 workgroups of kPartEdges edges; blk0[i] is segment i's first
-// workgroup, blk0[nseg] the total.
-constexpr int kMaxSegs = 48;  // a 2^25-edge launch of 2^20-edge device batches takes 32
 constexpr uint64_t kZeroCopyEdges = 1ULL << 18;  // device batches at least this long: in place
-struct PairSegs {
-  const uint2* ptr[kMaxSegs];
-  uint64_t n[kMaxSegs];
-  uint32_t blk0[kMaxSegs + 1];
-  uint32_t nseg;
-};
 // this workgroup's pairs: [e0, e1) of segment `p` (uniform per workgroup)
 __device__ __forceinline__ void block_pairs(const PairSegs& sg, uint32_t b, const uint2*& p,
                                             uint64_t& e0, uint64_t& e1) {
@@ -1253,8 +1248,49 @@ PartParams make_part_params(uint32_t n, int local_bits) {
   return p;
 }
 
+
 static uint32_t part_blocks(uint64_t nedges) {
   return (uint32_t)((nedges + kPartEdges - 1) / kPartEdges);
+}
+
+// Item capacity of every length class for a launch of `cap` edges over `verts` vertices: classes
+// that may hold full items (len >= 64, the smallest item size a stream can be tuned to; items of
+// one vertex may repeat) are bounded by the entries alone; shorter items are a vertex's last, at
+// most one per vertex. Returns the total.
+static uint64_t item_class_layout(uint64_t verts, uint64_t cap, ItemClasses& cls) {
+  uint64_t icap = 0;
+  for (int k = 0; k < kClasses; k++) {
+    cls.off[k] = icap;
+    uint64_t ck = (2 * cap) >> k;
+    if (k < 6) ck = std::min<uint64_t>(verts, ck);
+    icap += ck + 16;
+  }
+  return icap;
+}
+
+size_t carve_pair_scratch(const Stream* s, uint64_t nedges, char* base, PairScratch& ws) {
+  const PartParams& pp = stream_part(s);
+  const uint64_t verts = std::min<uint64_t>((uint64_t)stream_vertices(s), 2 * nedges);
+  ws = PairScratch{};
+  const uint64_t icap = item_class_layout(verts, nedges, ws.cls);
+  const uint64_t hist_words = (uint64_t)pp.nbuckets * (part_blocks(nedges) + kMaxSegs);
+  size_t off = 0;
+  auto take = [&](size_t bytes) -> char* {
+    char* p = base ? base + off : nullptr;
+    off += (bytes + 255) & ~(size_t)255;
+    return p;
+  };
+  ws.d_counters = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * kCounterWords));
+  ws.d_entries = reinterpret_cast<uint2*>(take(sizeof(uint2) * 2 * nedges));
+  ws.d_sorted = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * 2 * nedges));
+  ws.d_items = reinterpret_cast<WorkItem*>(take(sizeof(WorkItem) * icap));
+  ws.d_hist = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * hist_words));
+  ws.d_totals = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * pp.nbuckets));
+  ws.cap_edges = nedges;
+  ws.cap_items = icap;
+  ws.cap_hist = hist_words;
+  ws.cap_buckets = pp.nbuckets;
+  return off;
 }
 
 int ensure_pair_scratch(Stream* s, uint64_t nedges) {
@@ -1287,16 +1323,7 @@ int ensure_pair_scratch(Stream* s, uint64_t nedges) {
     ws.d_items = nullptr;
     uint64_t cap = nedges;
     const uint64_t verts = std::min<uint64_t>((uint64_t)stream_vertices(s), 2 * cap);
-    uint64_t icap = 0;
-    // classes that may hold full items (len >= 64, the smallest item size a stream can be tuned
-    // to; items of one vertex may repeat) are bounded by the entries alone; shorter items are a
-    // vertex's last, at most one per vertex
-    for (int k = 0; k < kClasses; k++) {
-      ws.cls.off[k] = icap;
-      uint64_t ck = (2 * cap) >> k;
-      if (k < 6) ck = std::min<uint64_t>(verts, ck);
-      icap += ck + 16;
-    }
+    const uint64_t icap = item_class_layout(verts, cap, ws.cls);
     GS_HIP(hipMalloc(&ws.d_entries, sizeof(uint2) * 2 * cap));
     GS_HIP(hipMalloc(&ws.d_sorted, sizeof(uint32_t) * 2 * cap));
     GS_HIP(hipMalloc(&ws.d_items, sizeof(WorkItem) * icap));
@@ -1370,26 +1397,14 @@ static int wait_fill_buffer(Stream* s) {
   return GS_OK;
 }
 
-// K1..K5 over the fill buffer; the other buffer becomes the fill buffer
-static int process_pending(Stream* s) {
-  if (s->pend_n == 0) return GS_OK;
-  PairScratch& ws = s->ws;
+// K1..K5 over the segments of `sg` (nblk K1/K3 workgroups, nedges pairs) on the stream's HIP
+// stream, with the grouping scratch `ws` (sized for nedges by ensure_pair_scratch, or carved
+// from a caller's fixed buffer). `input_read` (may be null) is recorded once the pairs have been
+// read (after K3).
+int run_pair_pipeline(Stream* s, const PairSegs& sg, uint32_t nblk, uint64_t nedges,
+                      PairScratch& ws, hipEvent_t input_read) {
   const PartParams& pp = stream_part(s);
   const uint32_t nverts = stream_vertices(s);
-  const uint64_t nedges = s->pend_n;
-  const int b = s->pend_buf;
-  PairSegs sg{};
-  uint32_t nblk = 0;
-  auto add_seg = [&](const uint32_t* p, uint64_t m) {
-    sg.ptr[sg.nseg] = reinterpret_cast<const uint2*>(p);
-    sg.n[sg.nseg] = m;
-    sg.blk0[sg.nseg] = nblk;
-    nblk += part_blocks(m);
-    sg.nseg++;
-  };
-  if (s->fill_n) add_seg(s->d_pend[b], s->fill_n);
-  for (const auto& z : s->zsegs) add_seg(z.first, z.second);
-  sg.blk0[sg.nseg] = nblk;
   const size_t sort_lds = sort_lds_bytes(pp.local_bits);
   int rc;
   const bool packed = packed_entries_fit(pp);
@@ -1401,10 +1416,6 @@ static int process_pending(Stream* s) {
       (rc = ensure_dyn_lds(scatter_fn, scatter_lds_bytes(pp.nbuckets))) ||
       (rc = ensure_dyn_lds(sort_fn, sort_lds)))
     return rc;
-  if (s->copies_pending) {  // every host copy into the buffer has landed
-    GS_HIP(hipStreamWaitEvent(s->stream, s->copied, 0));
-    s->copies_pending = false;
-  }
   hipEvent_t g0 = nullptr;  // start of the grouping kernels (measurement only)
   if (s->profile) {
     GS_HIP(hipEventCreate(&g0));
@@ -1427,8 +1438,7 @@ static int process_pending(Stream* s) {
                        scatter_lds_bytes(pp.nbuckets), s->stream, sg, nverts, vgraph, pp, nblk,
                        ws.d_totals, ws.d_hist, ws.d_entries);
   GS_HIP(hipGetLastError());
-  GS_HIP(hipEventRecord(s->pend_free[b], s->stream));  // the fill buffer has been read
-  s->pend_used[b] = true;
+  if (input_read) GS_HIP(hipEventRecord(input_read, s->stream));
   const uint32_t chunk = s->tune_chunk ? s->tune_chunk : (stream_wide(s) ? kWideChunk : kChunk);
   const uint32_t* bases = s->group ? (const uint32_t*)s->group->d_bases : nullptr;
   const uint32_t ngraphs = s->group ? (uint32_t)s->group->members.size() : 1u;
@@ -1441,10 +1451,6 @@ static int process_pending(Stream* s) {
                        s->stream, ws.d_entries, pp, ws.d_totals, ws.d_sorted, ws.d_items, ws.cls,
                        chunk, ws.d_counters + kClassBase, bases, ngraphs);
   GS_HIP(hipGetLastError());
-  s->pend_n = 0;
-  s->fill_n = 0;
-  s->zsegs.clear();
-  s->pend_buf ^= 1;
   const size_t nev = s->ev_pairs.size();
   rc = launch_aggregate(s, ws.d_items, ws.cls, ws.d_counters + kClassBase, ws.d_sorted,
                         2 * nedges);
@@ -1452,6 +1458,38 @@ static int process_pending(Stream* s) {
     if (s->ev_pairs.size() == nev + 1) s->ev_group.back() = g0;
     else (void)hipEventDestroy(g0);
   }
+  return rc;
+}
+
+void add_pair_segment(PairSegs& sg, uint32_t& nblk, const uint32_t* p, uint64_t m) {
+  sg.ptr[sg.nseg] = reinterpret_cast<const uint2*>(p);
+  sg.n[sg.nseg] = m;
+  sg.blk0[sg.nseg] = nblk;
+  nblk += part_blocks(m);
+  sg.nseg++;
+  sg.blk0[sg.nseg] = nblk;
+}
+
+// K1..K5 over the fill buffer and the device segments; the other buffer becomes the fill buffer
+static int process_pending(Stream* s) {
+  if (s->pend_n == 0) return GS_OK;
+  const uint64_t nedges = s->pend_n;
+  const int b = s->pend_buf;
+  PairSegs sg{};
+  uint32_t nblk = 0;
+  if (s->fill_n) add_pair_segment(sg, nblk, s->d_pend[b], s->fill_n);
+  for (const auto& z : s->zsegs) add_pair_segment(sg, nblk, z.first, z.second);
+  if (s->copies_pending) {  // every host copy into the buffer has landed
+    GS_HIP(hipStreamWaitEvent(s->stream, s->copied, 0));
+    s->copies_pending = false;
+  }
+  // the fill buffer is free again once K3 has read it
+  int rc = run_pair_pipeline(s, sg, nblk, nedges, s->ws, s->pend_free[b]);
+  s->pend_used[b] = true;
+  s->pend_n = 0;
+  s->fill_n = 0;
+  s->zsegs.clear();
+  s->pend_buf ^= 1;
   return rc;
 }
 

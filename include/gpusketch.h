@@ -156,6 +156,12 @@ int gs_import_buckets(gs_store* st, uint32_t v0, uint32_t count, const gs_bucket
 int gs_query_cc(gs_store* st, uint32_t* label_out, uint32_t* num_cc, uint32_t* rounds,
                 uint32_t* forest_out, uint64_t forest_cap, uint64_t* forest_len);
 
+/* Test hook: Sketch::sample (src/sketch.cpp:110-132) of every vertex sketch at sample index
+ * `sample_idx`, computed by the device Borůvka's round-0 samplers. result[v] = 0 GOOD (idx_out[v]
+ * = the sampled edge index), 1 ZERO, 2 FAIL; n entries each. GS_ERR_OUT_OF_SAMPLES when
+ * sample_idx >= num_samples (OutOfSamplesException, include/sketch.h:219-230). */
+int gs_sample_vertices(gs_store* st, uint32_t sample_idx, int32_t* result, uint64_t* idx_out);
+
 /* k edge-disjoint spanning forests (MCSketchAlg::calc_disjoint_spanning_forests,
  * src/mc_sketch_alg.cpp:889-920): forest i samples column i in round 0, its edges are toggled
  * out before forest i+1, and all are toggled back in before return. edges_out holds

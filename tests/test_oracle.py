@@ -193,9 +193,9 @@ def test_sample_zero_and_out_of_samples():
     assert orc.sample(sk, g, 3, sample_idx=1)[1] == orc.OUT_OF_SAMPLES
 
 
-def test_sample_crafted_fail_vector():
-    # test/sketch_test.cpp:14-64: build a vector whose updates only reach depths <= 1 with >= 2
-    # at depth 1 in every column; then no bucket is good and sample() must FAIL
+def crafted_fail_sketch():
+    """test/sketch_test.cpp:14-64: a vector whose updates only reach depths <= 1 with >= 2 at
+    depth 1 in every column, so that no bucket is good; returns (sketch buckets, geometry, seed)."""
     g = orc.Geometry.explicit(1, 8, 1, 7)  # Sketch(100, seed=0, 1, 7)
     seed, cols, guesses = 0, 7, 8
     vec_idx = [True] * (100 * 100)
@@ -233,6 +233,12 @@ def test_sample_crafted_fail_vector():
             applied += 1
             if applied >= total:
                 break
+    return sk, g, seed
+
+
+def test_sample_crafted_fail_vector():
+    sk, g, seed = crafted_fail_sketch()
+    assert sk.any()
     assert orc.sample(sk, g, seed)[1] == orc.FAIL
 
 
