@@ -2,7 +2,7 @@
 //   serve_bench [--tenants 64] [--requests 1000] [--rate 2000] [--budget-gb 200] [--seed 7]
 //               [--mc-tenants 8] [--k 4] [--min-n 4096] [--max-n 262144]
 //               [--max-edges 1000000] [--alone-check 0|1] [--evict 0|1]
-//               [--max-step-updates N] [--pinned-spill 0|1]
+//               [--max-step-updates N] [--pinned-spill 0|1] [--verify 0|1]
 // Tenants: n log-uniform in [min-n, max-n]. Requests arrive as a Poisson process of the given
 // rate (requests/s): 1% connectivity queries on random tenants, 0.5% min-cut queries (k) on the
 // first mc-tenants tenants, the rest update batches of log-uniform 1k..max-edges edges, 5% of
@@ -12,6 +12,9 @@
 // sketch is identical in both modes (and, with --alone-check, for a tenant served alone). With a
 // budget below the tenants' total, --evict 1 (the default) spills idle tenants to host memory to
 // serve the others; --evict 0 serves only the tenants admitted before the first request.
+// --verify 1 replays the trace on the host and checks every answer of both modes against the
+// exact graph: each CC answer against a union-find over the tenant's net graph at that point of
+// the trace (component count and labels), each min-cut answer against min(k, exact minimum cut).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -21,7 +24,9 @@
 #include <random>
 #include <string>
 #include <thread>
+#include <unordered_set>
 
+#include "gpusketch.h"
 #include "gpusketch/sketch_server.h"
 
 struct Req {
@@ -190,6 +195,91 @@ static bool same_answers(const RunStats& a, const RunStats& b, std::string* why)
   return true;
 }
 
+static uint64_t fnv1a(const uint32_t* p, size_t n) {  // the server's hash of a label array
+  uint64_t h = 1469598103934665603ull;
+  for (size_t i = 0; i < n; i++) {
+    h ^= p[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+static uint32_t uf_find(std::vector<uint32_t>& p, uint32_t x) {
+  while (p[x] != x) x = p[x] = p[p[x]];
+  return x;
+}
+
+struct Verified {
+  uint64_t cc = 0, mincut = 0, mismatches = 0;
+  std::string first;
+};
+
+// The net graph of every tenant replayed in trace order; at each query, the exact answer from a
+// union-find over it (labels: smallest vertex of each component) and, for min-cut queries,
+// min(k, the exact global minimum cut) -- compared with what each run answered.
+static Verified verify_answers(const Trace& tr, const std::vector<const RunStats*>& runs, uint32_t k) {
+  Verified out;
+  std::vector<std::unordered_set<uint64_t>> live(tr.tenants.size());
+  for (size_t q = 0; q < tr.reqs.size(); q++) {
+    const Req& r = tr.reqs[q];
+    auto& L = live[r.tenant];
+    if (r.kind == RequestKind::UPDATE) {
+      for (size_t e = 0; 2 * e < r.pairs.size(); e++) {
+        const uint32_t u = r.pairs[2 * e], v = r.pairs[2 * e + 1];
+        const uint64_t key = u < v ? ((uint64_t)u << 32 | v) : ((uint64_t)v << 32 | u);
+        if (!L.erase(key)) L.insert(key);
+      }
+      continue;
+    }
+    const uint32_t n = tr.tenants[r.tenant].num_vertices;
+    std::vector<uint32_t> p(n);
+    for (uint32_t v = 0; v < n; v++) p[v] = v;
+    for (uint64_t key : L) {
+      uint32_t a = uf_find(p, (uint32_t)(key >> 32)), b = uf_find(p, (uint32_t)key);
+      if (a != b) p[std::max(a, b)] = std::min(a, b);  // the smaller id stays the root
+    }
+    std::vector<uint32_t> lab(n);
+    uint32_t ncc = 0;
+    for (uint32_t v = 0; v < n; v++) {
+      lab[v] = uf_find(p, v);
+      ncc += lab[v] == v;
+    }
+    uint64_t cut = 0;
+    if (r.kind == RequestKind::MINCUT_QUERY && ncc == 1) {
+      std::vector<uint32_t> pairs;
+      pairs.reserve(2 * L.size());
+      for (uint64_t key : L) {
+        pairs.push_back((uint32_t)(key >> 32));
+        pairs.push_back((uint32_t)key);
+      }
+      uint64_t value = 0;
+      if (gs_min_cut(n, pairs.data(), L.size(), &value, nullptr) != GS_OK) {
+        out.mismatches++;
+        if (out.first.empty()) out.first = "exact min cut failed: " + std::string(gs_last_error());
+        continue;
+      }
+      cut = std::min<uint64_t>(value, k);
+    }
+    const uint64_t hash = fnv1a(lab.data(), n);
+    for (const RunStats* rs : runs) {
+      auto it = rs->by_index.find(q);
+      if (it == rs->by_index.end()) continue;  // not served in this run (tenant not admitted)
+      const RequestResult& a = it->second;
+      bool ok = a.status == 0;
+      if (r.kind == RequestKind::CC_QUERY) ok = ok && a.num_cc == ncc && a.partition_hash == hash;
+      else ok = ok && a.mincut == cut;
+      if (!ok) {
+        out.mismatches++;
+        if (out.first.empty())
+          out.first = "request " + std::to_string(q) + " (tenant " + std::to_string(r.tenant) +
+                      ") answered differently from the exact graph";
+      }
+    }
+    (r.kind == RequestKind::CC_QUERY ? out.cc : out.mincut)++;
+  }
+  return out;
+}
+
 static std::string stats_json(const RunStats& s) {
   std::string j = "{\"updates_per_s\": " + std::to_string(s.upd_per_s) +
                   ", \"makespan_s\": " + std::to_string(s.makespan) +
@@ -207,7 +297,7 @@ static std::string stats_json(const RunStats& s) {
 }
 
 int main(int argc, char** argv) {
-  int ntenants = 64, nreq = 1000, mc = 8, alone = 1;
+  int ntenants = 64, nreq = 1000, mc = 8, alone = 1, verify = 0;
   double rate = 2000, budget_gb = 200;
   uint64_t seed = 7, max_edges = 1000000;
   uint32_t k = 4, min_n = 4096, max_n = 262144;
@@ -228,6 +318,7 @@ int main(int argc, char** argv) {
     else if (a == "--max-step-updates") g_max_step_updates = std::stoull(v);
     else if (a == "--evict") g_evict = std::atoi(v) != 0;
     else if (a == "--pinned-spill") g_pinned_spill = std::atoi(v) != 0;
+    else if (a == "--verify") verify = std::atoi(v);
     else {
       std::fprintf(stderr, "unknown option %s\n", a.c_str());
       return 2;
@@ -261,14 +352,29 @@ int main(int argc, char** argv) {
   }
   uint64_t nq = 0;
   for (auto& r : tr.reqs) nq += r.kind != RequestKind::UPDATE;
+  std::string vjson = "null";
+  bool verified_ok = true;
+  if (verify) {
+    const auto v0 = std::chrono::steady_clock::now();
+    const Verified v = verify_answers(tr, {&plain, &served}, k);
+    verified_ok = v.mismatches == 0;
+    std::fprintf(stderr, "[serve] verify: %llu CC and %llu min-cut answers per mode against the "
+                 "exact graphs, %llu mismatches (%.1fs)\n", (unsigned long long)v.cc,
+                 (unsigned long long)v.mincut, (unsigned long long)v.mismatches,
+                 std::chrono::duration<double>(std::chrono::steady_clock::now() - v0).count());
+    vjson = "{\"cc_answers\": " + std::to_string(v.cc) + ", \"mincut_answers\": " +
+            std::to_string(v.mincut) + ", \"mismatches\": " + std::to_string(v.mismatches) +
+            ", \"first\": \"" + v.first + "\"}";
+  }
   std::printf("{\"tenants\": %d, \"requests\": %zu, \"queries\": %llu, \"updates\": %llu, "
               "\"rate_req_per_s\": %g, \"budget_gb\": %g, \"k\": %u, \"plain\": %s, "
               "\"served\": %s, \"answers_identical\": %s, \"sketches_identical\": %s, "
-              "\"alone_equals_mix\": %s, \"error\": \"%s\"}\n",
+              "\"alone_equals_mix\": %s, \"verified\": %s, \"error\": \"%s\"}\n",
               ntenants, tr.reqs.size(), (unsigned long long)nq,
               (unsigned long long)tr.total_updates, rate, budget_gb, k,
               stats_json(plain).c_str(), stats_json(served).c_str(), same ? "true" : "false",
-              same_sketch ? "true" : "false", alone_ok ? "true" : "false",
+              same_sketch ? "true" : "false", alone_ok ? "true" : "false", vjson.c_str(),
               (plain.error + served.error).c_str());
-  return (same && same_sketch && alone_ok && plain.error.empty() && served.error.empty()) ? 0 : 1;
+  return (same && same_sketch && alone_ok && verified_ok && plain.error.empty() &&
+          served.error.empty()) ? 0 : 1;
 }

@@ -223,3 +223,53 @@ def test_store_destroy_refused_while_in_group():
     assert np.array_equal(a.export_buckets(), oracle_store(100, 1, np.array([[0, 1]], dtype=np.uint32)))
     a.close()
     b.close()
+
+
+def batched_config_workload(graphs, seed):
+    """The batched-streams config: n = 2^s, s uniform in [10, 16] (log-uniform n in
+    [2^10, 2^16]), 4..64 updates per vertex, 5% deletions; even graphs Erdos-Renyi, odd graphs
+    R-MAT(0.57, 0.19, 0.19, 0.05)."""
+    rng = np.random.default_rng(seed)
+    scales = rng.integers(10, 17, size=graphs)
+    upv = rng.integers(4, 65, size=graphs)
+    out = []
+    for g in range(graphs):
+        n = 1 << int(scales[g])
+        m = int(upv[g]) * n // 2
+        if g % 2 == 0:
+            p = min(1.0, m / 1.05 / (n * (n - 1) / 2))
+            pairs = gs.gen_erdos_renyi(n, p, 0.05, seed + g)
+        else:
+            pairs = gs.gen_rmat(int(scales[g]), m, seed + g, del_frac=0.05)
+        out.append((n, 0xB0000000 + 7919 * g, pairs))
+    return out
+
+
+def test_batched_config_full_shape_256_members():
+    # 256 graphs in one group, all updates interleaved through one launch sequence; sampled
+    # members bit-exact with the oracle, every member's partition equal to a DSU of its net graph
+    work = batched_config_workload(256, 20261207)
+    stores = [gs.SketchStore(n, sd) for n, sd, _ in work]
+    try:
+        with gs.StoreGroup(stores) as grp:
+            glob = np.concatenate([p + np.uint32(b) for (_, _, p), b in zip(work, grp.bases)])
+            glob = glob[np.random.default_rng(2).permutation(len(glob))]
+            buf = gs.DeviceBuffer(glob)
+            for b in range(0, len(glob), 1 << 25):
+                grp.submit_pairs_device(buf.offset(b * 8), min(1 << 25, len(glob) - b))
+            grp.flush()
+            buf.free()
+            del glob
+            labels, ncc, rounds, status = grp.query_cc()
+            assert (status == 0).all()
+            sizes = np.array([n for n, _, _ in work])
+            sample = sorted({int(np.argmin(sizes)), int(np.argmax(sizes)), 1, 2, 127, 200})
+            for i in sample:
+                n, sd, p = work[i]
+                assert np.array_equal(stores[i].export_buckets(), oracle_store(n, sd, p)), f"member {i}"
+            for i, (n, _, p) in enumerate(work):
+                assert np.array_equal(labels[i], orc.dsu_components(n, net_edges(p))), f"member {i}"
+                assert ncc[i] == len(np.unique(labels[i]))
+    finally:
+        for st in stores:
+            st.close()

@@ -21,6 +21,30 @@ def net_edges(pairs):
     return np.stack([live >> 32, live & 0xFFFFFFFF], 1).astype(np.uint32)
 
 
+def edge_keys(e):
+    e = np.asarray(e, dtype=np.uint32).reshape(-1, 2)
+    lo, hi = e.min(1).astype(np.uint64), e.max(1).astype(np.uint64)
+    return (lo << np.uint64(32)) | hi
+
+
+def check_disjoint_forests(n, forests, net):
+    """Forest i must be a spanning forest of the net graph minus forests 0..i-1
+    (MCSketchAlg::calc_disjoint_spanning_forests, src/mc_sketch_alg.cpp:889-920): its edges lie
+    in that graph, it is acyclic, and its components are that graph's components."""
+    remaining = np.unique(edge_keys(net))  # sorted
+    for i, f in enumerate(forests):
+        fk = np.sort(edge_keys(f))
+        assert len(np.unique(fk)) == len(fk), f"forest {i} repeats an edge"
+        pos = np.searchsorted(remaining, fk)
+        inside = (pos < len(remaining)) & (remaining[np.minimum(pos, len(remaining) - 1)] == fk)
+        assert inside.all(), f"forest {i} holds an edge outside G - F_0..F_{i-1}"
+        g_i = np.stack([remaining >> np.uint64(32), remaining & np.uint64(0xFFFFFFFF)], 1).astype(np.uint32)
+        lab_f = orc.dsu_components(n, f)
+        assert len(f) == n - len(np.unique(lab_f)), f"forest {i} has a cycle"
+        assert np.array_equal(lab_f, orc.dsu_components(n, g_i)), f"forest {i} does not span"
+        remaining = np.delete(remaining, pos)
+
+
 def check_forest(n, forest, net, labels):
     netset = set(map(tuple, net.tolist()))
     assert all(tuple(e) in netset for e in forest.tolist())
@@ -191,13 +215,59 @@ def test_k_forests_and_planted_min_cut():
         forests = st.k_spanning_forests(k)
         after = st.export_buckets()
     assert np.array_equal(before, after)  # forests toggled back in
-    netset = set(map(tuple, net.tolist()))
-    seen = set()
-    for f in forests:
-        fs = set(map(tuple, f.tolist()))
-        assert fs <= netset and not (fs & seen)
-        seen |= fs
+    check_disjoint_forests(n, forests, net)
     cert = np.concatenate(forests)
     val, side = gs.min_cut(n, cert)
     assert val == exact
     assert orc.stoer_wagner(n, cert)[0] == exact
+
+
+def test_cc_rmat17_full_size_matches_dsu():
+    # the single-stream config's graph family at its real size: R-MAT n=2^17 (a,b,c,d) =
+    # (.57,.19,.19,.05), 2^23 updates (about 5% deletions) in 2^20-pair device batches; the
+    # partition must equal a DSU over the net graph and the forest must span it
+    n = 1 << 17
+    pairs = gs.gen_rmat(17, 1 << 23, 20261205, del_frac=0.05)
+    with gs.SketchStore(n, 0x5EEDC0DE) as st:
+        s = st.stream()
+        buf = gs.DeviceBuffer(pairs)
+        for b in range(0, len(pairs), 1 << 20):
+            s.submit_pairs_device(buf.offset(b * 8), min(1 << 20, len(pairs) - b))
+        s.flush()
+        labels, ncc, rounds, forest = st.query_cc(want_forest=True)
+        buf.free()
+    net = net_edges(pairs)
+    assert np.array_equal(labels, orc.dsu_components(n, net))
+    assert ncc > 1  # isolated vertices remain at this density
+    assert len(forest) == n - ncc
+    assert np.isin(edge_keys(forest), edge_keys(net)).all()
+    assert np.array_equal(orc.dsu_components(n, forest), labels)
+
+
+def test_mincut_config_full_shape():
+    # the min-cut config as stated: n=2^15, two ER(0.1) halves joined by exactly 5 crossing edges,
+    # shuffled, 10% insert/delete churn, k=8 sketch forests at the reference tool's sketch factor
+    # (tools/min_cut.cu:95-102,156); the certificate's minimum cut is the planted 5 and its side
+    # is one of the planted halves
+    n, k, cut = 1 << 15, 8, 5
+    pairs = gs.gen_planted_cut(n, 0.1, cut, 0.1, 20261206)
+    factor = k / np.log2(n) * 1.5
+    with gs.SketchStore(n, 0xFACADE, sketches_factor=factor) as st:
+        s = st.stream()
+        buf = gs.DeviceBuffer(pairs)
+        for b in range(0, len(pairs), 1 << 20):
+            s.submit_pairs_device(buf.offset(b * 8), min(1 << 20, len(pairs) - b))
+        s.flush()
+        buf.free()
+        before = st.export_buckets(0, 64)
+        forests = st.k_spanning_forests(k)
+        assert np.array_equal(st.export_buckets(0, 64), before)  # toggled back in
+    net = net_edges(pairs)
+    half = np.arange(n) < n // 2
+    assert int(np.sum(half[net[:, 0]] != half[net[:, 1]])) == cut
+    check_disjoint_forests(n, forests, net)
+    cert = np.concatenate(forests)
+    val, side = gs.min_cut(n, cert)
+    assert val == cut
+    side = side.astype(bool)
+    assert np.array_equal(side, half) or np.array_equal(side, ~half)

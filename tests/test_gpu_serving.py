@@ -1,6 +1,8 @@
 """Serving: the same request trace served one request at a time (plain) and with continuous
 batching must give identical answers (connectivity partitions, min cuts) and identical final
-sketches, and a tenant served alone must answer exactly as in the mix (program serve_bench)."""
+sketches, a tenant served alone must answer exactly as in the mix, and every answer must equal
+the exact one of the tenant's net graph at that point of the trace: a union-find for
+connectivity, min(k, exact minimum cut) for min-cut queries (program serve_bench --verify 1)."""
 import json
 import os
 import subprocess
@@ -18,12 +20,14 @@ def test_served_equals_plain_and_alone(rate):
         pytest.fail("serve_bench is not built; run `python gpusketch_amd/build.py`")
     r = subprocess.run([PROG, "--tenants", "10", "--requests", "300", "--rate", str(rate),
                         "--max-n", "30000", "--max-edges", "20000", "--seed", str(11 + rate),
-                        "--alone-check", "1"], capture_output=True, text=True, timeout=600)
+                        "--alone-check", "1", "--verify", "1"], capture_output=True, text=True,
+                       timeout=600)
     print(r.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
     out = json.loads(r.stdout.strip().splitlines()[-1])
     assert out["answers_identical"] and out["sketches_identical"] and out["alone_equals_mix"]
     assert out["queries"] > 0
+    assert out["verified"]["mismatches"] == 0 and out["verified"]["cc_answers"] > 0
 
 
 def test_eviction_spills_and_restores_bit_exact():
@@ -34,7 +38,7 @@ def test_eviction_spills_and_restores_bit_exact():
         pytest.fail("serve_bench is not built; run `python gpusketch_amd/build.py`")
     r = subprocess.run([PROG, "--tenants", "10", "--requests", "300", "--rate", "3000",
                         "--max-n", "30000", "--max-edges", "20000", "--seed", "23",
-                        "--budget-gb", "0.5", "--alone-check", "1"],
+                        "--budget-gb", "0.5", "--alone-check", "1", "--verify", "1"],
                        capture_output=True, text=True, timeout=600)
     print(r.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -42,3 +46,24 @@ def test_eviction_spills_and_restores_bit_exact():
     assert out["answers_identical"] and out["sketches_identical"] and out["alone_equals_mix"]
     assert out["served"]["evictions"] > 0 and out["plain"]["evictions"] > 0
     assert out["served"]["restores"] > 0
+    assert out["verified"]["mismatches"] == 0
+
+
+def test_serving_config_trace_answers_exact():
+    # the serving config as stated: 64 tenants with n log-uniform in [2^12, 2^18], 1000 Poisson
+    # requests (batches of 1k..1M edges, 5% deletions, 1% CC queries, k=4 min-cut queries on 8
+    # tenants), 200 GB budget; every answer of both modes checked against the exact graph
+    if not os.path.exists(PROG):
+        pytest.fail("serve_bench is not built; run `python gpusketch_amd/build.py`")
+    r = subprocess.run([PROG, "--tenants", "64", "--requests", "1000", "--rate", "2000",
+                        "--budget-gb", "200", "--k", "4", "--mc-tenants", "8",
+                        "--alone-check", "1", "--verify", "1"],
+                       capture_output=True, text=True, timeout=900)
+    print(r.stderr)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["tenants"] == 64 and out["updates"] > 100_000_000
+    assert out["answers_identical"] and out["sketches_identical"] and out["alone_equals_mix"]
+    v = out["verified"]
+    assert v["mismatches"] == 0, v["first"]
+    assert v["cc_answers"] > 0 and v["mincut_answers"] > 0
