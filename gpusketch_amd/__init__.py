@@ -97,6 +97,7 @@ _sigs = {
     "gs_group_submit_pairs_device": ([_vp, _vp, _u64], _i32),
     "gs_group_query_cc": ([_vp, _vp, _vp, _vp, _vp, _vp], _i32),
     "gs_group_prepare": ([_vp], _i32),
+    "gs_apply_edge_pairs_device_noalloc": ([_vp, _vp, _u64], _i32),
     "gs_host_alloc": ([_sz, _vp], _i32),
     "gs_host_free": ([_vp], _i32),
     "gs_device_alloc": ([_sz, _vp], _i32),
@@ -132,6 +133,8 @@ def _pairs(pairs) -> np.ndarray:
     return np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
 
 
+# most pairs per gs_apply_edge_pairs_device_noalloc call (GS_NOALLOC_MAX_EDGES)
+NOALLOC_MAX_EDGES = 1 << 20
 # gs_store_create flag: the CubeSketch (L0_SAMPLING) update rule
 STORE_CUBESKETCH = 1
 # columns per sample of the CubeSketch configuration (include/sketch.h:208-211)
@@ -331,6 +334,11 @@ class SketchStore:
 
     def zero(self) -> None:
         _check(lib.gs_store_zero(self.h))
+
+    def apply_pairs_device_noalloc(self, dptr, nedges: int) -> None:
+        """gs_apply_edge_pairs_device_noalloc: device pairs applied at once, no device allocation
+        (at most NOALLOC_MAX_EDGES per call)."""
+        _check(lib.gs_apply_edge_pairs_device_noalloc(self.h, dptr, nedges))
 
     def export_buckets(self, v0: int = 0, count: int | None = None) -> np.ndarray:
         count = self.n - v0 if count is None else count

@@ -251,6 +251,22 @@ int gs_group_prepare(gs_group* g);
 int gs_group_query_cc(gs_group* g, const uint8_t* member_mask, uint32_t* labels,
                       uint32_t* num_cc, uint32_t* rounds, int32_t* status);
 
+/* ---- Allocation-free update ----------------------------------------------------------------
+ * No counterpart in the reference: an entry point for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It applies device-resident edge pairs exactly as gs_submit_edge_pairs_device followed by
+ * gs_flush would (the same grouping and aggregation kernels), but takes every scratch buffer
+ * from a fixed device array of GS_NOALLOC_SCRATCH_BYTES that each GPU holds from the library's
+ * first use of it until the process ends, and calls no hipMalloc/hipFree. At most
+ * GS_NOALLOC_MAX_EDGES pairs per call (GS_ERR_INVALID beyond, or when the store's vertex count
+ * would need more scratch); it runs on the store's internal stream and returns when the updates
+ * are applied. Pairs naming a vertex >= n are skipped and reported (GS_ERR_INVALID) by the same
+ * call. The store's streams must have nothing pending or in flight (gs_sync first), and calls
+ * that overlap on one GPU race: they share that GPU's scratch array. */
+#define GS_NOALLOC_MAX_EDGES (1ull << 20)
+#define GS_NOALLOC_SCRATCH_BYTES (100ull << 20)
+int gs_apply_edge_pairs_device_noalloc(gs_store* st, const uint32_t* d_pairs, uint64_t nedges);
+
 /* Pinned host memory for staging buffers (replaces the cudaMallocHost calls of CudaStream and
  * SKGPUSketchAlg, include/cuda_stream.h:51-97, include/sk_gpu_sketch_alg.h:72). */
 int gs_host_alloc(size_t bytes, void** hptr);
