@@ -63,6 +63,10 @@ HOST_LIB = os.path.join(HERE, "libgpusketch_host.so")
 HOST_LIB_L0 = os.path.join(HERE, "libgpusketch_host_l0.so")
 HOST_FLAGS = ["-O2", "-std=c++17", "-pthread", "-Wall", "-Wextra", "-Wno-unused-parameter",
               "-I", os.path.join(ROOT, "include")]
+# programs built with a define of the reference's build: verify_facade as its VERIFY_SAMPLES_F
+# test builds are (the driver records the exact graph and the queries check against it; the
+# library itself does not depend on the define)
+PROG_DEFINES = {"verify_facade": ["-DVERIFY_SAMPLES_F"]}
 
 
 def build_host_programs() -> list[str]:
@@ -83,9 +87,11 @@ def build_host_programs() -> list[str]:
     outs = []
     cmds = []
     for p in sorted(glob.glob(os.path.join(HOST, "prog_*.cpp"))):
-        out = os.path.join(BIN, os.path.basename(p)[len("prog_"):-4])
+        name = os.path.basename(p)[len("prog_"):-4]
+        out = os.path.join(BIN, name)
         outs.append(out)
-        cmds.append(["g++", *HOST_FLAGS, p, "-o", out, f"-L{HERE}", "-lgpusketch_host",
+        defs = PROG_DEFINES.get(name, [])
+        cmds.append(["g++", *HOST_FLAGS, *defs, p, "-o", out, f"-L{HERE}", "-lgpusketch_host",
                      "-lgpusketch", "-Wl,-rpath,$ORIGIN/.."])
     for prog in ("facade_cpu", "facade_gpu"):
         p = os.path.join(HOST, f"prog_{prog}.cpp")

@@ -16,6 +16,7 @@
 #include "gpusketch/dsu.h"
 #include "gpusketch/edge_store.h"
 #include "gpusketch/graph_sketch_driver.h"
+#include "gpusketch/graph_verifier.h"
 #include "gpusketch/sketch.h"
 
 static int failures = 0;
@@ -455,6 +456,70 @@ static int sketch_dump(const char* pairs_path, node_id_t n, uint64_t seed, const
   return out ? 0 : 1;
 }
 
+// GraphVerifier (the reference's test/util/graph_verifier.cpp checks) and AsciiFileStream: the
+// cumulative text format, toggles, combine, and each check accepting a right answer and throwing
+// on a wrong one
+static void test_graph_verifier() {
+  const std::string path = work_dir + "/verifier_graph.txt";
+  {
+    std::ofstream f(path);
+    // two triangles {0,1,2}, {3,4,5}, a path 6-7, vertex 8 alone; the edge 2-3 listed twice
+    f << "9 10\n0 1\n1 2\n2 0\n3 4\n4 5\n5 3\n6 7\n2 3\n3 2\n8 8\n";
+  }
+  GraphVerifier g(9);
+  for (Edge e : std::vector<Edge>{{0, 1}, {1, 2}, {2, 0}, {3, 4}, {4, 5}, {5, 3}, {6, 7}}) g.edge_update(e);
+  GraphVerifier from_file(9, path);  // (8, 8) is a self loop: it toggles nothing but itself
+  from_file.edge_update({8, 8});
+  CHECK(from_file == g);
+  CHECK(g.get_num_kruskal_ccs() == 4);
+  // the text stream reads the same file as updates
+  {
+    AsciiFileStream as(path, false);
+    CHECK(as.vertices() == 9 && as.edges() == 10);
+    GraphStreamUpdate buf[16];
+    size_t got = as.get_update_buffer(buf, 16);
+    CHECK(got == 10 && buf[7].edge.src == 2 && buf[7].edge.dst == 3 && buf[7].type == INSERT);
+    CHECK(as.get_update_buffer(buf, 16) == 1 && buf[0].type == BREAKPOINT);
+  }
+  // connectivity: the right partition passes, a merged or split one throws
+  const node_id_t right[9] = {0, 0, 0, 3, 3, 3, 6, 6, 8};
+  g.verify_connected_components(ConnectedComponents(9, right));
+  const node_id_t merged[9] = {0, 0, 0, 0, 0, 0, 6, 6, 8};
+  const node_id_t split[9] = {0, 0, 2, 3, 3, 3, 6, 6, 8};
+  bool t1 = false, t2 = false;
+  try { g.verify_connected_components(ConnectedComponents(9, merged)); } catch (const IncorrectCCException&) { t1 = true; }
+  try { g.verify_connected_components(ConnectedComponents(9, split)); } catch (const IncorrectCCException&) { t2 = true; }
+  CHECK(t1 && t2);
+  // edges
+  g.verify_edge({1, 0});
+  bool t3 = false;
+  try { g.verify_edge({0, 3}); } catch (const BadEdgeException&) { t3 = true; }
+  CHECK(t3);
+  // forests: two disjoint spanning forests of the triangles' graph pass (the second spans the
+  // graph minus the first); a cycle, a non-edge, a repeated edge and a non-spanning forest throw
+  auto sf = [](std::vector<Edge> e) { return SpanningForest(9, std::move(e)); };
+  g.verify_spanning_forests({sf({{0, 1}, {1, 2}, {3, 4}, {4, 5}, {6, 7}}), sf({{2, 0}, {5, 3}})});
+  CHECK(g.num_edges() == 7);  // left unchanged
+  bool c = false, ne = false, rep = false, span = false;
+  try { g.verify_spanning_forests({sf({{0, 1}, {1, 2}, {2, 0}})}); } catch (const IncorrectForestException&) { c = true; }
+  try { g.verify_spanning_forests({sf({{0, 3}})}); } catch (const BadEdgeException&) { ne = true; }
+  try {  // the second forest reuses an edge the first one took
+    g.verify_spanning_forests({sf({{0, 1}, {1, 2}, {3, 4}, {4, 5}, {6, 7}}), sf({{0, 1}})});
+  } catch (const BadEdgeException&) { rep = true; }
+  try { g.verify_spanning_forests({sf({{0, 1}, {3, 4}, {4, 5}, {6, 7}})}); } catch (const IncorrectForestException&) { span = true; }
+  CHECK(c && ne && rep && span);
+  CHECK(g.num_edges() == 7);
+  // combine XORs the edge sets (two streams' graphs)
+  GraphVerifier h(9);
+  h.edge_update({6, 7});
+  h.edge_update({7, 8});
+  g.combine(h);
+  CHECK(g.num_edges() == 7 && g.get_num_kruskal_ccs() == 4);
+  auto m = g.extract_adj_matrix();
+  CHECK(m[7][1] && !m[6][1] && m[0][1] && m[0][2]);
+  CHECK(g != from_file);
+}
+
 int main(int argc, char** argv) {
   if (argc >= 2 && std::strcmp(argv[1], "sketch_dump") == 0 && argc == 6)
     return sketch_dump(argv[2], (node_id_t)std::stoul(argv[3]), std::stoull(argv[4]), argv[5]);
@@ -469,6 +534,7 @@ int main(int argc, char** argv) {
     run("driver_delivers_every_update", test_driver_delivers_every_update);
     run("edge_store", test_edge_store);
     run("mc_routing_with_driver", test_mc_routing_with_driver);
+    run("graph_verifier", test_graph_verifier);
     std::printf("%d failure(s)\n", failures);
     return failures ? 1 : 0;
   }

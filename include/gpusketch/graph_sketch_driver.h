@@ -3,7 +3,11 @@
 // both directions into the gutters, and lets worker threads hand full vertex batches to
 // Alg::apply_update_batch. The Alg contract is the reference's: allocate_worker_memory,
 // get_desired_updates_per_batch, get_num_vertices, pre_insert, apply_update_batch (thread-safe
-// across distinct thread ids), has_cached_query, print_configuration.
+// across distinct thread ids), has_cached_query, print_configuration, and, when VERIFY_SAMPLES_F
+// is defined, set_verifier: the driver then records the exact graph of the stream (a
+// GraphVerifier per stream thread, combined at the break point) and hands a copy to the algorithm
+// after each process_stream_until (include/graph_sketch_driver.h:64,119-189); check_verifier
+// compares it with an expected one.
 #pragma once
 #include <atomic>
 #include <chrono>
@@ -13,10 +17,16 @@
 #include <thread>
 #include <vector>
 
+#include <memory>
+#include <mutex>
+
 #include "driver_configuration.h"
 #include "graph_stream.h"
 #include "guttering.h"
 #include "worker_thread_group.h"
+#ifdef VERIFY_SAMPLES_F
+#include "graph_verifier.h"
+#endif
 
 class DriverException : public std::exception {
   std::string err_msg;
@@ -35,6 +45,10 @@ class GraphSketchDriver {
   size_t num_stream_threads;
   static constexpr size_t update_array_size = 4000;
   std::atomic<size_t> total_updates{0};
+#ifdef VERIFY_SAMPLES_F
+  std::unique_ptr<GraphVerifier> verifier;
+  std::mutex verifier_mtx;
+#endif
 
  public:
   GraphSketchDriver(Alg* alg, GraphStream* stream, DriverConfiguration config,
@@ -47,6 +61,9 @@ class GraphSketchDriver {
     gts = new GutteringSystem(sketching_alg->get_num_vertices(), config.get_worker_threads(),
                               batch, config.gutter_conf());
     worker_threads = new WorkerThreadGroup<Alg>(config.get_worker_threads(), this, gts);
+#ifdef VERIFY_SAMPLES_F
+    verifier.reset(new GraphVerifier(sketching_alg->get_num_vertices()));
+#endif
     if (this->num_stream_threads > 1 && !stream->get_update_is_thread_safe()) {
       std::cerr << "WARNING: stream get_update is not thread safe. Setting number of stream "
                    "threads to 1"
@@ -68,21 +85,44 @@ class GraphSketchDriver {
     worker_threads->resume_workers();
     auto task = [&](int thr_id) {
       std::vector<GraphStreamUpdate> buf(update_array_size);
+#ifdef VERIFY_SAMPLES_F
+      GraphVerifier local_verifier(sketching_alg->get_num_vertices());
+#endif
       for (;;) {
         size_t got = stream->get_update_buffer(buf.data(), buf.size());
         for (size_t i = 0; i < got; i++) {
-          if (buf[i].type == BREAKPOINT) return;
+          if (buf[i].type == BREAKPOINT) {
+#ifdef VERIFY_SAMPLES_F
+            std::lock_guard<std::mutex> lk(verifier_mtx);
+            verifier->combine(local_verifier);
+#endif
+            return;
+          }
           GraphUpdate upd{buf[i].edge, buf[i].type};
           sketching_alg->pre_insert(upd, thr_id);
           gts->insert({upd.edge.src, upd.edge.dst}, thr_id);
           gts->insert({upd.edge.dst, upd.edge.src}, thr_id);
+#ifdef VERIFY_SAMPLES_F
+          local_verifier.edge_update(upd.edge);
+#endif
         }
       }
     };
     std::vector<std::thread> threads;
     for (size_t i = 0; i < num_stream_threads; i++) threads.emplace_back(task, (int)i);
     for (auto& t : threads) t.join();
+#ifdef VERIFY_SAMPLES_F
+    sketching_alg->set_verifier(std::make_unique<GraphVerifier>(*verifier));
+#endif
   }
+
+#ifdef VERIFY_SAMPLES_F
+  // the graph the driver recorded must be `expected` (throws DriverException otherwise)
+  void check_verifier(const GraphVerifier& expected) {
+    if (*verifier != expected)
+      throw DriverException("Mismatch between driver verifier and expected verifier");
+  }
+#endif
 
   // Make the algorithm's state reflect every update processed so far.
   void prep_query(int query_code) {

@@ -6,6 +6,7 @@
 #pragma once
 #include <atomic>
 #include <cstdint>
+#include <fstream>
 #include <iostream>
 #include <string>
 
@@ -58,4 +59,25 @@ class BinaryFileStream : public GraphStream {
   std::atomic<edge_id_t> stream_off{0};  // next record to hand out
   std::atomic<edge_id_t> break_idx{0};
   edge_id_t write_off = 0;
+};
+
+// Text stream (the reference's tests read their res/*.txt graphs with StreamingUtilities'
+// AsciiFileStream, test/cc_alg_test.cpp:46): a "num_vertices num_edges" line, then one update per
+// line, "src dst" or, with has_type, "type src dst" (0 insert, 1 delete). Read by one thread.
+class AsciiFileStream : public GraphStream {
+ public:
+  explicit AsciiFileStream(std::string file_name, bool has_type = true);
+  AsciiFileStream(const AsciiFileStream&) = delete;
+  AsciiFileStream& operator=(const AsciiFileStream&) = delete;
+
+  size_t get_update_buffer(GraphStreamUpdate* upd_buf, size_t num_updates) override;
+  bool set_break_point(edge_id_t break_idx) override;
+  void serialize_metadata(std::ostream& out) override;
+
+ private:
+  std::string file_name;
+  bool has_type;
+  std::ifstream in;
+  edge_id_t stream_off = 0;
+  edge_id_t break_idx = 0;
 };

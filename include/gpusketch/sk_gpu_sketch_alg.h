@@ -1,8 +1,9 @@
 // SKGPUSketchAlg: the whole stream is buffered in pinned host memory during ingestion and then
 // applied on the device, either in one submission (launch_gpu_kernel) or in buffer-sized
 // submissions over the per-thread streams (launch_gpu_update). Keeps the reference's API
-// (include/sk_gpu_sketch_alg.h:11-138, src/sk_gpu_sketch_alg.cpp:7-257); queries are inherited
-// from CCGPUSketchAlg.
+// (include/sk_gpu_sketch_alg.h:11-138, src/sk_gpu_sketch_alg.cpp:7-257); it derives from
+// CCGPUSketchAlg (for its SketchParams constructor and timing) and so from CCSketchAlg, whose
+// queries it inherits.
 #pragma once
 #include <atomic>
 #include <mutex>

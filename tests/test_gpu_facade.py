@@ -26,3 +26,18 @@ def test_facade_gpu_program(tmp_path, prog):
     sys.stdout.write(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "0 failure(s)" in r.stdout
+
+
+def test_verify_samples_build_of_the_api(tmp_path):
+    """verify_facade is compiled with the reference's -DVERIFY_SAMPLES_F: the driver records
+    each stream's exact graph and the queries of CCSketchAlg / CCGPUSketchAlg / SKGPUSketchAlg /
+    MCSketchAlg / MCGPUSketchAlg (held through their base classes) check every answer against it;
+    the fixture's cumulative text file gives the expected graph (test/cc_alg_test.cpp:41-57)."""
+    path = os.path.join(os.path.dirname(PROG), "verify_facade")
+    if not os.path.exists(path):
+        pytest.fail("verify_facade is not built; run `python gpusketch_amd/build.py`")
+    txt = os.path.join(ROOT, "tests", "golden", "multiples_graph_1024.txt")
+    r = subprocess.run([path, txt, str(tmp_path)], capture_output=True, text=True, timeout=600)
+    sys.stdout.write(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "0 failure(s)" in r.stdout
