@@ -13,8 +13,8 @@
 // and once per column (seed + 5c), XORs {idx, checksum} into a per-wave LDS delta laid out like
 // the device rows (depth-major), then flushes only the non-zero buckets to HBM: a plain
 // read-xor-write when the item owns its vertex in this launch, 64-bit atomic XOR otherwise. The
-// deterministic bucket is a wave XOR-reduction. Depths >= the LDS window go straight to HBM with
-// atomics (probability 2^-16 per column and update).
+// deterministic bucket is a wave XOR-reduction. Depths >= the item's LDS window go straight to
+// HBM with atomics (probability 2^-9 per column and update for items of 32 or more entries).
 #include <chrono>
 #include <cstring>
 
@@ -525,27 +525,41 @@ __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
 // Its column keys and LDS addresses therefore stay in registers for the whole item, and an
 // entry's staged values are read from LDS once for T columns (8 lanes read one address).
 //
-// Per-wave LDS delta: rows of `cols` 16-byte buckets {alpha, gamma}. Row 7 + d holds depth d
-// (d < dl); depth 0 receives half of all (update, column) pairs, so entry slots e = 1..7 XOR
-// their depth-0 pairs into private copies (rows e - 1) and slot 0 into row 7: no two lanes of an
-// instruction ever hit one depth-0 address. The copies are folded into row 7 once per item.
-// Every LDS address is one select and one 24-bit multiply-add; only depths >= the item's window
-// (and the exact-hash fallback for a 16-bit tie) take a branch, to 64-bit HBM atomics.
-// After the delta, the chunk's stage: edge index, checksum, mix(swap(idx)), 64 of each.
+// Per-wave LDS delta: rows of `cols` 16-byte buckets {alpha, gamma}; row d holds depth d
+// (d < dl). A depth is the ctz of the column's low 16 hash bits with a sentinel bit at the item's
+// window depth dli, so it comes out clamped to the window: depth dli stands for every depth >=
+// dli and addresses a trash row just past the window (cleared after each item). A lane branches
+// only when one of its columns reads dli (probability 2^-dli), to find that pair's exact depth
+// (the full hash for a 16-bit tie) and XOR it into HBM with 64-bit atomics.
+// LDS banks: gfx950 serves a wave's 64-bit LDS atomics 16 lanes (entry slots 2q and 2q+1) at a
+// time over 32 four-byte banks, and a row of 8k buckets spans the banks a whole number of times,
+// so the two entry slots of a pass hit the same banks for one column whatever their depths. Odd
+// entry slots therefore XOR gamma first and alpha second: in each instruction one slot of the
+// pair writes alpha words and the other gamma words, 32 distinct banks, and no two lanes of a
+// pass ever share an address (lanes of different passes may, at depth 0 mostly: that costs
+// little and needs no private rows). The stage holds each entry's {idx, checksum} in that
+// order already, so an LDS address is one 24-bit multiply-add and the second one an XOR.
+// After the delta, the chunk's stage: for entry j, first = idx (j even) or checksum (j odd) and
+// second the other one, then mix(swap(idx)); 64 of each.
 // ------------------------------------------------------------------------------------------
-constexpr uint32_t kCopyRows = 7;  // private depth-0 rows of entry slots 1..7
 struct AggLayout {
   uint32_t cols, dl;
   __host__ __device__ AggLayout(uint32_t cols_, uint32_t dl_) : cols(cols_), dl(dl_) {}
-  __host__ __device__ uint32_t delta_bytes() const { return (kCopyRows + dl) * cols * 16u; }
+  // the window's dl depth rows and one more (the trash row of a window of dl depths)
+  __host__ __device__ uint32_t delta_bytes() const { return (dl + 1) * cols * 16u; }
   __host__ __device__ uint32_t wave_bytes() const { return delta_bytes() + 3 * 64 * 8; }
 };
 constexpr int kMaxColSteps = 8;  // T <= 8: up to 64 columns
-constexpr int kFlushRounds = 4;  // 64-slot rounds per flush step
-// windows of at most this many 64-slot rounds flush deferred: groups of stores see mostly short
-// items and run at 4 waves per SIMD either way; a single store keeps 5 with 2
+// 64-slot rounds per flush step, and windows of at most defer_rounds x 64 slots flush deferred
+// (that many rounds of stores stay live through the next item, 8 VGPRs each; a deferred window
+// is one flush step). A single store runs at 6 waves per SIMD with 2 and 2 (4 flush rounds
+// would cost a wave per SIMD); groups of stores see mostly short items and keep 4 and 4.
+template <bool kGroup>
+constexpr int flush_rounds() { return kGroup ? 4 : 2; }
 template <bool kGroup>
 constexpr int defer_rounds() { return kGroup ? 4 : 2; }
+static_assert(defer_rounds<false>() <= flush_rounds<false>() && defer_rounds<true>() <= flush_rounds<true>(),
+              "a deferred window must fit one flush step");
 constexpr uint32_t kMaxLdsDepths = 16;
 
 // 64-bit XOR at an LDS byte address (kept as a plain 32-bit address so that no base is re-added
@@ -560,8 +574,10 @@ __device__ __forceinline__ void lds_xor_u64(uint32_t addr, uint64_t v) {
 // the widest member (max_cols) and laid out with the current member's column count, which is
 // reloaded (with its geometry and keys) when a wave moves to an item of another member (the
 // delta is all zero between items). Otherwise every item belongs to the store `st`.
-// T = ceil(columns / 8) of the widest store the launch covers.
-template <bool kGroup, int T>
+// T = ceil(columns / 8) of the widest store the launch covers. kFull: a single store whose
+// column count is exactly 8 T (every column slot of every step is a column), so the T depth
+// hashes of a lane need no guard and run as independent chains before the LDS updates.
+template <bool kGroup, int T, bool kFull>
 __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
     StoreView st, const GraphDesc* __restrict__ descs, uint32_t max_cols,
     const WorkItem* __restrict__ items, ItemClasses cls,
@@ -581,22 +597,27 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
   const uint32_t wid = threadIdx.x / kWave;
   const uint32_t le = lane >> 3, ls = lane & 7;
   const uint32_t wbase = wid * L.wave_bytes();  // byte offset of this wave's delta
-  uint64_t* sidx = reinterpret_cast<uint64_t*>(lds + wbase + L.delta_bytes());
-  uint64_t* schk = sidx + kWave;
-  uint64_t* smix = schk + kWave;
-  const uint32_t row0 = le == 0 ? kCopyRows : le - 1;  // this lane's depth-0 row
+  uint64_t* sfirst = reinterpret_cast<uint64_t*>(lds + wbase + L.delta_bytes());
+  uint64_t* ssecond = sfirst + kWave;
+  uint64_t* smix = ssecond + kWave;
+  const uint32_t odd = lane & 1;  // of the staging lane's entry, and of the entry slot (le & 1)
+  // where this lane stages its entry's idx and checksum (swapped for odd entries)
+  uint64_t* const widx = odd ? ssecond + lane : sfirst + lane;
+  uint64_t* const wchk = odd ? sfirst + lane : ssecond + lane;
 
   uint64_t colk[T];
-  // LDS addresses of this lane's buckets: depth 0, and depth d minus d rows
+  uint64_t chk_key = 0;  // the checksum's key (sketch seed) in the split form
+  // LDS address of the half of this lane's depth-0 bucket it XORs first (gamma for odd entry
+  // slots); depth d adds d rows
   const uint32_t lbase = (uint32_t)(uintptr_t)lds + wbase;
-  uint32_t addr0[T], addrd[T];
+  uint32_t addr1[T];
   auto set_columns = [&]() {
+    chk_key = xxh3_seed_key(seed);
 #pragma unroll
     for (int t = 0; t < T; t++) {
       const uint32_t c = ls + 8 * t;
       colk[t] = c < cols ? xxh3_seed_key(seed + 5ULL * c) : 0;
-      addr0[t] = lbase + 16u * (row0 * cols + c);
-      addrd[t] = lbase + 16u * (kCopyRows * cols + c);
+      addr1[t] = lbase + 16u * c + 8u * (le & 1);
     }
   };
   if (!kGroup) set_columns();
@@ -645,6 +666,7 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
   };
   // deferred flush of the previous exclusive item (its window fits kDeferRounds x 64 slots)
   constexpr int kDeferRounds = defer_rounds<kGroup>();
+  constexpr int kFlushRounds = flush_rounds<kGroup>();
   bool pending = false;
   Bucket16* prows = nullptr;
   uint64_t pa[kDeferRounds], pg[kDeferRounds];
@@ -694,6 +716,8 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
     const uint32_t dli = min(dl, 35u - (uint32_t)__builtin_clz(len));
     const uint32_t islots = dli * cols;
     const uint32_t row_bytes = 16u * cols;
+    const uint32_t sent = 1u << dli;                 // dli <= 16: within the exact low bits
+    const uint32_t cap16 = 1u << (bpc < 16u ? bpc : 16u);
     uint64_t det_a = 0, det_g = 0;
     Bucket16* vrows = st.rows + (size_t)v * st.vstride;
 
@@ -704,12 +728,13 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
       if (lane < cnt) {
         const uint32_t other = dcur - vbase;
         uint64_t idx = concat_pair(v, other);
-        uint64_t chk = xxh3_len8(idx, seed);
+        uint64_t mx = xxh3_mix(xxh3_swap(idx));  // shared by the checksum and every column
+        uint64_t chk = xxh3_finish(mx ^ chk_key);  // = xxh3_len8(idx, seed)
         // subgraph filter: an update outside the store's subgraph becomes the (0, 0) no-op
-        if (min_depth && depth_of(chk, depth_cap) < min_depth) idx = chk = 0;
-        sidx[lane] = idx;
-        schk[lane] = chk;
-        smix[lane] = xxh3_mix(xxh3_swap(idx));
+        if (min_depth && depth_of(chk, depth_cap) < min_depth) idx = chk = mx = 0;
+        *widx = idx;
+        *wchk = chk;
+        smix[lane] = mx;
         det_a ^= idx;
         det_g ^= chk;
       }
@@ -719,20 +744,33 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
       for (uint32_t g = 0; g < ngroups; g++) {
         const uint32_t j = 8 * g + le;
         if (j < cnt) {
-          const uint64_t a = sidx[j], gm = schk[j], m = smix[j];
+          const uint64_t v1 = sfirst[j], v2 = ssecond[j], m = smix[j];
+          // the low 16 hash bits of each column and its depth clamped to the window: the sentinel
+          // bit at dli makes a depth >= dli read as dli, whose LDS row is the item's trash row
+          uint32_t h[T], dd[T];
 #pragma unroll
           for (int t = 0; t < T; t++) {
-            const uint32_t c = ls + 8 * t;
-            if (c < cols) {
-              const uint64_t x = m ^ colk[t];
-              uint32_t d = depth_low16(x, bpc);
-              const uint32_t offd = mul24_lo(d, row_bytes) + addrd[t];
-              if (d < dli) {
-                const uint32_t off = d ? offd : addr0[t];
-                lds_xor_u64(off, a);
-                lds_xor_u64(off + 8, gm);
-              } else {  // rare: deep buckets (and the 16-bit tie) straight to HBM
-                if (d == 16 && bpc > 16) d = depth_of(xxh3_finish(x), bpc);
+            h[t] = hash_low16(m ^ colk[t]);
+            dd[t] = (uint32_t)__builtin_ctz(h[t] | sent);
+          }
+          uint32_t dmax = 0;
+#pragma unroll
+          for (int t = 0; t < T; t++) {
+            if (kFull || ls + 8 * t < cols) {
+              const uint32_t off = mad24_lo(dd[t], row_bytes, addr1[t]);
+              lds_xor_u64(off, v1);
+              lds_xor_u64(off ^ 8u, v2);
+              dmax = max(dmax, dd[t]);
+            }
+          }
+          if (dmax == dli) {  // rare: depths >= the window straight to HBM (probability 2^-dli)
+            const uint64_t a = (le & 1) ? v2 : v1, gm = (le & 1) ? v1 : v2;
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+              const uint32_t c = ls + 8 * t;
+              if ((kFull || c < cols) && dd[t] == dli) {
+                uint32_t d = (uint32_t)__builtin_ctz(h[t] | cap16);
+                if (d == 16 && bpc > 16) d = depth_of(xxh3_finish(m ^ colk[t]), bpc);
                 if (d < bpc) {
                   Bucket16* b = &vrows[(size_t)d * cols + c];
                   atomic_xor_u64(&b->alpha, a);
@@ -749,27 +787,8 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
     dcur = dfirst_next;
     wcur = wnext;
     wnext = wafter;
-    // fold the private depth-0 rows of the entry slots that were used into row 7
-    const uint32_t ncopy = min(len, 8u) - 1;
-    if (lane < cols && ncopy) {
-      uint4 acc = make_uint4(0, 0, 0, 0);
-      for (uint32_t r = 0; r < ncopy; r++) {
-        uint4* p = reinterpret_cast<uint4*>(lds + wbase + r * row_bytes) + lane;
-        const uint4 x = *p;
-        acc.x ^= x.x;
-        acc.y ^= x.y;
-        acc.z ^= x.z;
-        acc.w ^= x.w;
-        *p = make_uint4(0, 0, 0, 0);
-      }
-      uint4* q = reinterpret_cast<uint4*>(lds + wbase + kCopyRows * row_bytes) + lane;
-      uint4 y = *q;
-      y.x ^= acc.x;
-      y.y ^= acc.y;
-      y.z ^= acc.z;
-      y.w ^= acc.w;
-      *q = y;
-    }
+    // the trash row took the LDS updates of the deep pairs
+    if (lane < cols) reinterpret_cast<Bucket16*>(lds + wbase + dli * row_bytes)[lane] = Bucket16{0, 0};
     det_a = wave_xor_reduce(det_a);
     det_g = wave_xor_reduce(det_g);
     if (lane == 0 && (det_a | det_g)) {
@@ -786,7 +805,7 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
     // leaves its stores pending: they complete after the next item, so the reads' latency is
     // hidden behind that item's work (the vertex is this item's alone; its deep buckets, written
     // by atomics, lie outside the window).
-    Bucket16* drow = reinterpret_cast<Bucket16*>(lds + wbase + kCopyRows * row_bytes);
+    Bucket16* drow = reinterpret_cast<Bucket16*>(lds + wbase);
     const bool defer = excl && islots <= kDeferRounds * kWave;
     if (!defer) complete_pending();
     for (uint32_t s0 = 0; s0 < islots; s0 += kFlushRounds * kWave) {
@@ -1024,12 +1043,24 @@ __global__ __launch_bounds__(kWideWaves* kWave) void wide_aggregate_kernel(
 // ------------------------------------------------------------------------------------------
 // launch helpers
 // ------------------------------------------------------------------------------------------
-// depths kept in the LDS delta by default; deeper updates (probability 2^-9 per column) go to
-// HBM directly
+// depths kept in the LDS delta of wide_aggregate_kernel by default; deeper updates
+// (probability 2^-9 per column) go to HBM directly
 constexpr uint32_t kDefaultLdsDepths = 9;
 
 static size_t agg_lds_bytes(uint32_t cols, uint32_t dl) {
   return (size_t)kAggWaves * AggLayout(cols, dl).wave_bytes();
+}
+
+// Default LDS window of aggregate_kernel: the most depths (up to 12) that still leave 6
+// workgroups per CU (its register use allows 6 waves per SIMD), and at least 9. Every depth
+// more halves the pairs that go to HBM atomics (2^-dl per column) while the flush scans one row
+// more per item: 12 depths measured fastest for 24 columns (R-MAT n=2^17 launches of 2^25
+// edges; 13 costs a workgroup per CU).
+static uint32_t default_agg_depths(uint32_t cols) {
+  constexpr size_t kLdsBytes = 160 * 1024, kWorkgroups = 6;
+  uint32_t dl = 9;
+  while (dl < 12 && agg_lds_bytes(cols, dl + 1) * kWorkgroups <= kLdsBytes) dl++;
+  return dl;
 }
 
 // persistent grid: as many workgroups as can be resident at once (the occupancy calculator
@@ -1087,26 +1118,34 @@ struct AggKernel {
   const void* fn;
   void (*launch)(dim3, dim3, size_t, hipStream_t, const AggArgs&);
 };
-template <bool G, int T>
+template <bool G, int T, bool F>
 static void agg_launch(dim3 grid, dim3 block, size_t lds, hipStream_t s, const AggArgs& a) {
-  hipLaunchKernelGGL((aggregate_kernel<G, T>), grid, block, lds, s, a.v, a.descs, a.cols, a.items,
-                     a.cls, a.class_counts, a.dsts, a.dl, a.exclusive_ok, a.min_depth, a.depth_cap,
-                     a.traffic);
+  hipLaunchKernelGGL((aggregate_kernel<G, T, F>), grid, block, lds, s, a.v, a.descs, a.cols,
+                     a.items, a.cls, a.class_counts, a.dsts, a.dl, a.exclusive_ok, a.min_depth,
+                     a.depth_cap, a.traffic);
 }
-template <bool G, int T>
+template <bool G, int T, bool F>
 static AggKernel agg_kernel_entry() {
-  return AggKernel{reinterpret_cast<const void*>(&aggregate_kernel<G, T>), &agg_launch<G, T>};
+  return AggKernel{reinterpret_cast<const void*>(&aggregate_kernel<G, T, F>), &agg_launch<G, T, F>};
 }
+// [group / single store with partial column steps / single store with full steps][T - 1]
 static const AggKernel& agg_kernel_for(bool group, uint32_t cols) {
-  static const AggKernel table[2][kMaxColSteps] = {
-      {agg_kernel_entry<false, 1>(), agg_kernel_entry<false, 2>(), agg_kernel_entry<false, 3>(),
-       agg_kernel_entry<false, 4>(), agg_kernel_entry<false, 5>(), agg_kernel_entry<false, 6>(),
-       agg_kernel_entry<false, 7>(), agg_kernel_entry<false, 8>()},
-      {agg_kernel_entry<true, 1>(), agg_kernel_entry<true, 2>(), agg_kernel_entry<true, 3>(),
-       agg_kernel_entry<true, 4>(), agg_kernel_entry<true, 5>(), agg_kernel_entry<true, 6>(),
-       agg_kernel_entry<true, 7>(), agg_kernel_entry<true, 8>()}};
+  static const AggKernel table[3][kMaxColSteps] = {
+      {agg_kernel_entry<false, 1, false>(), agg_kernel_entry<false, 2, false>(),
+       agg_kernel_entry<false, 3, false>(), agg_kernel_entry<false, 4, false>(),
+       agg_kernel_entry<false, 5, false>(), agg_kernel_entry<false, 6, false>(),
+       agg_kernel_entry<false, 7, false>(), agg_kernel_entry<false, 8, false>()},
+      {agg_kernel_entry<false, 1, true>(), agg_kernel_entry<false, 2, true>(),
+       agg_kernel_entry<false, 3, true>(), agg_kernel_entry<false, 4, true>(),
+       agg_kernel_entry<false, 5, true>(), agg_kernel_entry<false, 6, true>(),
+       agg_kernel_entry<false, 7, true>(), agg_kernel_entry<false, 8, true>()},
+      {agg_kernel_entry<true, 1, false>(), agg_kernel_entry<true, 2, false>(),
+       agg_kernel_entry<true, 3, false>(), agg_kernel_entry<true, 4, false>(),
+       agg_kernel_entry<true, 5, false>(), agg_kernel_entry<true, 6, false>(),
+       agg_kernel_entry<true, 7, false>(), agg_kernel_entry<true, 8, false>()}};
   const uint32_t t = (cols + 7) / 8;
-  return table[group ? 1 : 0][t - 1];
+  const int kind = group ? 2 : (cols == 8 * t ? 1 : 0);
+  return table[kind][t - 1];
 }
 
 // wide sketches (more than 64 columns, or the CubeSketch rule): wide_aggregate_kernel
@@ -1166,7 +1205,7 @@ static int launch_aggregate(Stream* s, const WorkItem* d_items, const ItemClasse
   int exclusive_ok = 1;
   if (grp) {
     cols = grp->max_cols;
-    dl = std::min(grp->min_bpc, s->tune_dl ? s->tune_dl : kDefaultLdsDepths);
+    dl = std::min(grp->min_bpc, s->tune_dl ? s->tune_dl : default_agg_depths(cols));
     device = grp->members[0]->device;
     for (Store* m : grp->members) {  // plain stores only if the group is every member's only writer
       std::lock_guard<std::mutex> lk(m->mu);
@@ -1175,7 +1214,7 @@ static int launch_aggregate(Stream* s, const WorkItem* d_items, const ItemClasse
   } else {
     v = s->store->view;
     cols = v.cols;
-    dl = std::min(v.bpc, s->tune_dl ? s->tune_dl : kDefaultLdsDepths);
+    dl = std::min(v.bpc, s->tune_dl ? s->tune_dl : default_agg_depths(cols));
     device = s->store->device;
     std::lock_guard<std::mutex> lk(s->store->mu);
     exclusive_ok = s->store->streams.size() == 1 ? 1 : 0;
@@ -1594,7 +1633,7 @@ int autotune_stream(Stream* s, const uint32_t* d_pairs, uint64_t nedges, gs_tuni
   const uint32_t min_bpc = s->group ? s->group->min_bpc : s->store->view.bpc;
   const PartParams base = s->group ? s->group->part : s->store->part;
   std::vector<uint32_t> dls, chunks = {256, 512, 1024}, bits;
-  for (uint32_t d : {8u, 9u, 10u, 12u})
+  for (uint32_t d : {9u, 10u, 11u, 12u})
     if (d <= min_bpc) dls.push_back(d);
   if (dls.empty()) dls.push_back(min_bpc);
   for (int b = (int)base.local_bits - 1; b <= (int)base.local_bits + 1; b++)

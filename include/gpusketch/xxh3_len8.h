@@ -47,9 +47,9 @@ GS_HD uint64_t xxh3_seed_key(uint64_t seed) { return xxh3_mix(xxh3_bitflip(seed)
 GS_HD uint32_t depth_of(uint64_t h, uint32_t max_depth) {
   return (uint32_t)__builtin_ctzll(h | (1ULL << max_depth));
 }
-// 32x32 -> low 32 bits of the product of the low 24 bits (full-rate v_mul_u32_u24 on gfx950)
+// 32x32 -> low 32 bits of the product of the low 24 bits (v_mul_u32_u24 on gfx950)
 // The device form is written as the instruction itself: with only the low bits of the product in
-// use, the compiler otherwise turns __umul24 back into a quarter-rate v_mul_lo_u32.
+// use, the compiler otherwise turns __umul24 back into v_mul_lo_u32, which cannot fuse an add.
 GS_HD uint32_t mul24_lo(uint32_t a, uint32_t b) {
 #if defined(__HIP_DEVICE_COMPILE__)
   uint32_t r;
@@ -57,6 +57,17 @@ GS_HD uint32_t mul24_lo(uint32_t a, uint32_t b) {
   return r;
 #else
   return (a & 0xFFFFFFu) * (b & 0xFFFFFFu);
+#endif
+}
+// (low 24 bits of a) * (low 24 bits of b) + c, low 32 bits (v_mad_u32_u24, one full-rate
+// instruction; the same asm form as mul24_lo keeps the compiler from widening it)
+GS_HD uint32_t mad24_lo(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t r;
+  asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+  return r;
+#else
+  return (a & 0xFFFFFFu) * (b & 0xFFFFFFu) + c;
 #endif
 }
 // low 32 bits of (hi:lo) >> s, 0 < s < 32 (v_alignbit_b32)
@@ -67,24 +78,28 @@ GS_HD uint32_t align_bits(uint32_t hi, uint32_t lo, uint32_t s) {
   return (uint32_t)((((uint64_t)hi << 32) | lo) >> s);
 #endif
 }
-// depth_of(xxh3_finish(x), max_depth) with a cheaper second multiply. The depth only needs the
-// low bits of h = x3 ^ (x3 >> 28): bits 0..15 of h come from bits 0..43 of x3 = x2 * M, whose
-// high word is needed mod 2^12 only, so its two cross products run as 24-bit multiplies. The first
-// product x1 = x * M is needed in full (x2 = x1 ^ ((x1 >> 35) + 8) feeds x1's high word into x2's
-// low word; the added term is below 2^30, so x2's high word equals x1's). When those 16 bits are
-// all zero (probability 2^-16) and max_depth > 16 the full hash decides.
-// min(depth, 16) part: ctz of the low 16 bits of h, capped at min(max_depth, 16)
-GS_HD uint32_t depth_low16(uint64_t x, uint32_t max_depth) {
+// The low 16 bits of xxh3_finish(x), computed with a cheaper second multiply (bits 16..31 of
+// the result are not those of the hash). Bits 0..15 of h = x3 ^ (x3 >> 28) come from bits 0..43
+// of x3 = x2 * M, whose high word is needed mod 2^12 only, so its two cross products run as
+// 24-bit multiply-adds. The first product x1 = x * M is needed in full (x2 = x1 ^ ((x1 >> 35) +
+// 8) feeds x1's high word into x2's low word; the added term is below 2^30, so x2's high word
+// equals x1's). 13 instructions on gfx950 for the XOR of x with a key and this.
+GS_HD uint32_t hash_low16(uint64_t x) {
   constexpr uint32_t kMlo = (uint32_t)kRrmxmxMul, kMhi = (uint32_t)(kRrmxmxMul >> 32);
   const uint32_t xlo = (uint32_t)x, xhi = (uint32_t)(x >> 32);
   const uint64_t p = (uint64_t)xlo * kMlo;
   const uint32_t x1hi = (uint32_t)(p >> 32) + xlo * kMhi + xhi * kMlo;
   const uint32_t x2lo = (uint32_t)p ^ ((x1hi >> 3) + 8u);
   const uint64_t q = (uint64_t)x2lo * kMlo;
-  const uint32_t x3hi = (uint32_t)(q >> 32) + mul24_lo(x2lo, kMhi) + mul24_lo(x1hi, kMlo);
-  const uint32_t h16 = ((uint32_t)q ^ align_bits(x3hi, (uint32_t)q, 28)) & 0xFFFFu;
+  const uint32_t x3hi = mad24_lo(x2lo, kMhi, mad24_lo(x1hi, kMlo, (uint32_t)(q >> 32)));
+  return (uint32_t)q ^ align_bits(x3hi, (uint32_t)q, 28);
+}
+// depth_of(xxh3_finish(x), max_depth) capped at 16: ctz of the low 16 bits of the hash, capped
+// at min(max_depth, 16). When those 16 bits are all zero (probability 2^-16) and max_depth > 16
+// the full hash decides (depth_fast).
+GS_HD uint32_t depth_low16(uint64_t x, uint32_t max_depth) {
   const uint32_t cap = max_depth < 16u ? max_depth : 16u;
-  return (uint32_t)__builtin_ctz(h16 | (1u << cap));
+  return (uint32_t)__builtin_ctz(hash_low16(x) | (1u << cap));
 }
 GS_HD uint32_t depth_fast(uint64_t x, uint32_t max_depth) {
   const uint32_t d = depth_low16(x, max_depth);
