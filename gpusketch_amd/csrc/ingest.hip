@@ -24,7 +24,7 @@ namespace gs {
 
 constexpr int kWave = 64;
 constexpr int kAggWaves = 4;                  // waves per aggregation workgroup
-constexpr uint32_t kChunk = 512;              // max destinations per work item
+constexpr uint32_t kChunk = 1024;             // max destinations per work item (the 11 item classes hold up to 2047)
 constexpr uint32_t kWideChunk = 1024;         // the same for wide stores
 constexpr int kPartThreads = 512;             // K1/K2 workgroup size
 constexpr int kPartEdges = 4096;              // undirected edges per K1/K2 workgroup
@@ -108,6 +108,17 @@ __device__ __forceinline__ bool entry_ok(uint2 uv, uint32_t n, const uint32_t* v
   return uv.x != uv.y && !pair_invalid(uv, n, vgraph);
 }
 
+// Partition block of workgroup `wg` of a K1/K3 launch of nblk workgroups. Workgroups are dealt to
+// the 8 XCDs round-robin (wg mod 8), and consecutive partition blocks write adjacent runs of
+// every bucket (K3) and adjacent counters of the histogram (K1): each XCD therefore takes one
+// contiguous range of blocks, so those partial cache lines fill up in one L2 instead of eight.
+// A bijection of [0, nblk): XCD x owns nblk / 8 blocks, one more if x < nblk mod 8.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t wg, uint32_t nblk) {
+  constexpr uint32_t kXcds = 8;
+  const uint32_t x = wg % kXcds, i = wg / kXcds, q = nblk / kXcds, r = nblk % kXcds;
+  return x * q + min(x, r) + i;
+}
+
 constexpr uint64_t kZeroCopyEdges = 1ULL << 18;  // device batches at least this long: in place
 // this workgroup's pairs: [e0, e1) of segment `p` (uniform per workgroup)
 __device__ __forceinline__ void block_pairs(const PairSegs& sg, uint32_t b, const uint2*& p,
@@ -144,9 +155,10 @@ __global__ __launch_bounds__(kPartThreads) void bucket_count_kernel(
     counters[t] = 0;  // class counts of this launch; [3] persists until a flush
   for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads) h[i] = 0;
   __syncthreads();
+  const uint32_t blk = xcd_block(blockIdx.x, nblk);
   const uint2* pairs;
   uint64_t e0, e1;
-  block_pairs(sg, blockIdx.x, pairs, e0, e1);
+  block_pairs(sg, blk, pairs, e0, e1);
 #pragma unroll
   for (int k = 0; k < kPartEdges / kPartThreads; k++) {
     const uint64_t e = e0 + t + (uint64_t)k * kPartThreads;
@@ -161,7 +173,7 @@ __global__ __launch_bounds__(kPartThreads) void bucket_count_kernel(
     atomicAdd(&h[bucket_of(pp, uv.y)], 1u);
   }
   __syncthreads();
-  for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads) hist[(size_t)i * nblk + blockIdx.x] = h[i];
+  for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads) hist[(size_t)i * nblk + blk] = h[i];
 }
 
 // exclusive prefix of counts[0..nb) into s_off (LDS; may be counts itself), returns the total;
@@ -248,9 +260,10 @@ __global__ __launch_bounds__(kPartThreads) void bucket_scatter_kernel(
   for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads) s_cnt[i] = 0;
   block_exclusive_scan(totals, pp.nbuckets, s_base, s_part);
 
+  const uint32_t blk = xcd_block(blockIdx.x, nblk);
   const uint2* pairs;
   uint64_t e0, e1;
-  block_pairs(sg, blockIdx.x, pairs, e0, e1);
+  block_pairs(sg, blk, pairs, e0, e1);
   constexpr int K = kPartEdges / kPartThreads;
   uint32_t bk[2 * K], rk[2 * K];
   uint2 uvs[K];
@@ -271,7 +284,7 @@ __global__ __launch_bounds__(kPartThreads) void bucket_scatter_kernel(
   __syncthreads();
   const uint32_t total = block_exclusive_scan(s_cnt, pp.nbuckets, s_loc, s_part);
   for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads)
-    s_base[i] += hist[(size_t)i * nblk + blockIdx.x];  // this block's start inside bucket i
+    s_base[i] += hist[(size_t)i * nblk + blk];  // this block's start inside bucket i
 #pragma unroll
   for (int k = 0; k < K; k++) {
     if (bk[2 * k] == kEmpty) continue;
@@ -332,7 +345,7 @@ constexpr uint32_t kTileBins = kSortThreads;               // local ids the tile
 static size_t sort_lds_bytes(uint32_t local_bits) {
   const size_t bins = (size_t)1 << local_bits;
   if (bins > kTileBins) return sizeof(uint32_t) * 3 * bins;
-  return sizeof(uint32_t) * (6 * bins + 2 * kSortTile);
+  return sizeof(uint32_t) * (5 * bins + 2 * kSortTile);
 }
 
 template <bool kPacked>
@@ -463,27 +476,31 @@ __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
   // Tiles of kSortTile entries: each tile is counting-sorted by local id in LDS, every id
   // reserves its run of the bucket's output (the id's cursor s_off advances by the tile's count),
   // and the tile is written out in LDS order, so consecutive lanes store consecutive positions of
-  // one id's run instead of scattering 4-byte stores over the whole bucket.
-  uint32_t* t_cnt = s_off + bins;          // per id: entries in this tile
+  // one id's run instead of scattering 4-byte stores over the whole bucket. An entry's rank among
+  // the tile's entries of its id is the old value its counting atomic returns, so one atomic pass
+  // serves both; the next tile's entries load while this one is sorted and written.
+  uint32_t* t_cnt = s_off + bins;          // per id: entries in this tile (zero between tiles)
   uint32_t* t_off = t_cnt + bins;          // per id: start of its run in the tile
-  uint32_t* t_cur = t_off + bins;          // per id: cursor inside the tile
-  uint32_t* t_gb = t_cur + bins;           // per id: start of its run in the bucket
+  uint32_t* t_gb = t_off + bins;           // per id: start of its run in the bucket
   uint32_t* st_val = t_gb + bins;          // the tile in id order: destination
   uint32_t* st_pos = st_val + kSortTile;   // and its position in the bucket
   __shared__ uint32_t s_wsum[kSortThreads / 64];
   const uint32_t lane = t & 63, wid = t >> 6;
+  for (uint32_t i = t; i < bins; i += kSortThreads) t_cnt[i] = 0;
+  uint32_t key[kSortBatch], val[kSortBatch];
+#pragma unroll
+  for (int k = 0; k < kSortBatch; k++) entry(min(t + k * kSortThreads, cnt - 1), key[k], val[k]);
+  __syncthreads();
   for (uint32_t tile0 = 0; tile0 < cnt; tile0 += kSortTile) {
     const uint32_t i0 = tile0 + t;
-    uint32_t key[kSortBatch], val[kSortBatch];
+    uint32_t rk[kSortBatch];
 #pragma unroll
-    for (int k = 0; k < kSortBatch; k++) entry(min(i0 + k * kSortThreads, cnt - 1), key[k], val[k]);
-    for (uint32_t i = t; i < bins; i += kSortThreads) t_cnt[i] = 0;
-    __syncthreads();
+    for (int k = 0; k < kSortBatch; k++)
+      rk[k] = i0 + k * kSortThreads < cnt ? wave_aggregated_add(t_cnt, key[k], 1u) : 0u;
+    uint32_t nkey[kSortBatch], nval[kSortBatch];
+    const uint32_t n0 = min(tile0 + kSortTile, cnt - 1) + t;  // next tile (clamped loads)
 #pragma unroll
-    for (int k = 0; k < kSortBatch; k++) {
-      if (i0 + k * kSortThreads >= cnt) break;
-      (void)wave_aggregated_add(t_cnt, key[k], 1u);
-    }
+    for (int k = 0; k < kSortBatch; k++) entry(min(n0 + k * kSortThreads, cnt - 1), nkey[k], nval[k]);
     __syncthreads();
     // exclusive scan of the tile counts (bins <= kSortThreads: one id per thread)
     const uint32_t c = t < bins ? t_cnt[t] : 0u;
@@ -498,22 +515,26 @@ __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
       uint32_t excl = x - c;
       for (uint32_t w = 0; w < wid; w++) excl += s_wsum[w];
       t_off[t] = excl;
-      t_cur[t] = excl;
       t_gb[t] = s_off[t];
       s_off[t] += c;
+      t_cnt[t] = 0;  // for the next tile's atomics (after two more barriers)
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kSortBatch; k++) {
       if (i0 + k * kSortThreads >= cnt) break;
-      const uint32_t lpos = wave_aggregated_add(t_cur, key[k], 1u);
+      const uint32_t lpos = t_off[key[k]] + rk[k];
       st_val[lpos] = val[k];
-      st_pos[lpos] = t_gb[key[k]] + (lpos - t_off[key[k]]);
+      st_pos[lpos] = t_gb[key[k]] + rk[k];
     }
     __syncthreads();
     const uint32_t ntile = min(kSortTile, cnt - tile0);
     for (uint32_t j = t; j < ntile; j += kSortThreads) sorted[base + st_pos[j]] = st_val[j];
-    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kSortBatch; k++) {
+      key[k] = nkey[k];
+      val[k] = nval[k];
+    }
   }
 }
 

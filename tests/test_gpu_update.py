@@ -304,7 +304,7 @@ def test_autotune_leaves_sketches_unchanged_and_keeps_parity():
         best = s.autotune(buf.offset(0), 100000)
         s.flush()
         assert np.array_equal(st.export_buckets(), before)  # every candidate applied twice
-        assert best.item_updates in (256, 512, 1024) and best.lds_depths in (8, 9, 10, 12)
+        assert best.item_updates in (256, 512, 1024) and best.lds_depths in (9, 10, 11, 12)
         t = s.get_tuning()
         assert (t.lds_depths, t.item_updates, t.bucket_bits) == (
             best.lds_depths, best.item_updates, best.bucket_bits)
