@@ -148,7 +148,8 @@ struct PairScratch {
   uint64_t cap_hist = 0;          // words of d_hist
   uint32_t* d_hist = nullptr;     // [bucket][partition block]: entries, then their offsets
   uint32_t cap_buckets = 0;
-  uint32_t* d_totals = nullptr;   // [bucket]: entries of the launch
+  uint32_t* d_totals = nullptr;   // [bucket]: entries of the launch, then [nbuckets + bucket]:
+                                  // start of the bucket's region
   ItemClasses cls{};
   uint32_t* d_counters = nullptr;  // [0..2] unused, [3]=dropped out-of-range entries since the
                                    // last flush, [kClassBase + k]=items of class k

@@ -208,7 +208,33 @@ __device__ uint32_t block_exclusive_scan(const uint32_t* counts, uint32_t nb, ui
   return total;
 }
 
+// exclusive prefix of in[0..n) into out (global memory), blockDim threads (a multiple of 64):
+// each thread scans a run of consecutive values
+__device__ void global_exclusive_scan(const uint32_t* in, uint32_t n, uint32_t* out,
+                                      uint32_t* s_part) {
+  const uint32_t t = threadIdx.x, nt = blockDim.x, lane = t & 63, wid = t >> 6;
+  const uint32_t per = (n + nt - 1) / nt;
+  const uint32_t b0 = min(n, t * per), b1 = min(n, b0 + per);
+  uint32_t sum = 0;
+  for (uint32_t b = b0; b < b1; b++) sum += in[b];
+  uint32_t x = sum;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, o);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == 63) s_part[wid] = x;
+  __syncthreads();
+  uint32_t run = x - sum;
+  for (uint32_t w = 0; w < wid; w++) run += s_part[w];
+  for (uint32_t b = b0; b < b1; b++) {
+    const uint32_t c = in[b];
+    out[b] = run;
+    run += c;
+  }
+}
+
 // one workgroup per bucket: the block counts of the bucket become exclusive offsets (in place)
+// and its total is written
 constexpr int kOffThreads = 256;
 __global__ __launch_bounds__(kOffThreads) void bucket_offsets_kernel(uint32_t* hist, uint32_t nblk,
                                                                      uint32_t* __restrict__ totals) {
@@ -236,6 +262,14 @@ __global__ __launch_bounds__(kOffThreads) void bucket_offsets_kernel(uint32_t* h
   if (t == kOffThreads - 1) totals[blockIdx.x] = s_part[t];
 }
 
+// one workgroup: the start of every bucket's region (exclusive prefix of the totals)
+__global__ __launch_bounds__(kOffThreads) void bucket_starts_kernel(const uint32_t* __restrict__ totals,
+                                                                    uint32_t nbuckets,
+                                                                    uint32_t* __restrict__ bstart) {
+  __shared__ uint32_t s_part[kOffThreads / 64];
+  global_exclusive_scan(totals, nbuckets, bstart, s_part);
+}
+
 // A directed entry of a bucket: the vertex (from which the sort takes its local id, the low
 // local_bits of the hashed id) and the other endpoint. kPacked (local_bits + nbits <= 32): one
 // u32, local id << nbits | other, half the bytes through K3 and K4; else a uint2 (vertex, other).
@@ -244,23 +278,32 @@ __host__ __device__ __forceinline__ bool packed_entries_fit(const PartParams& pp
 }
 
 // The block's entries are first placed in LDS in bucket order, then written out so that
-// consecutive lanes write consecutive positions of one bucket's run (coalesced stores).
+// consecutive lanes write consecutive positions of one bucket's run (coalesced stores). kPacked:
+// the stage holds each entry as its packed word and its bucket (6 bytes: 3 workgroups per CU);
+// otherwise as the pair (vertex, other) from which the bucket is recomputed.
 template <bool kPacked>
 __global__ __launch_bounds__(kPartThreads) void bucket_scatter_kernel(
     PairSegs sg, uint32_t n, const uint32_t* __restrict__ vgraph, PartParams pp, uint32_t nblk,
-    const uint32_t* __restrict__ totals,
+    const uint32_t* __restrict__ bstart,
     const uint32_t* __restrict__ hist, uint2* __restrict__ entries) {
   extern __shared__ __align__(16) uint32_t lds_u32[];
-  uint2* s_stage = reinterpret_cast<uint2*>(lds_u32);  // 2 * kPartEdges entries
-  uint32_t* s_base = lds_u32 + 4 * kPartEdges;         // nbuckets: global start
-  uint32_t* s_cnt = s_base + pp.nbuckets;              // nbuckets: block counts, then offsets
+  constexpr uint32_t kStage = 2 * kPartEdges;  // entries of a block
+  uint2* s_stage = reinterpret_cast<uint2*>(lds_u32);        // !kPacked: kStage pairs
+  uint32_t* s_word = lds_u32;                                 // kPacked: kStage words
+  uint16_t* s_bkt = reinterpret_cast<uint16_t*>(lds_u32 + kStage);  // kPacked: and buckets
+  uint32_t* s_base = lds_u32 + (kPacked ? kStage + kStage / 2 : 2 * kStage);  // nbuckets
+  uint32_t* s_cnt = s_base + pp.nbuckets;  // nbuckets: block counts, then offsets
   uint32_t* s_loc = s_cnt;
   __shared__ uint32_t s_part[kPartThreads];
   const uint32_t t = threadIdx.x;
-  for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads) s_cnt[i] = 0;
-  block_exclusive_scan(totals, pp.nbuckets, s_base, s_part);
-
   const uint32_t blk = xcd_block(blockIdx.x, nblk);
+  // this block's start inside every bucket (loads issued before the pairs are processed)
+  for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads) {
+    s_cnt[i] = 0;
+    s_base[i] = bstart[i] + hist[(size_t)i * nblk + blk];
+  }
+  __syncthreads();
+
   const uint2* pairs;
   uint64_t e0, e1;
   block_pairs(sg, blk, pairs, e0, e1);
@@ -283,32 +326,39 @@ __global__ __launch_bounds__(kPartThreads) void bucket_scatter_kernel(
   }
   __syncthreads();
   const uint32_t total = block_exclusive_scan(s_cnt, pp.nbuckets, s_loc, s_part);
-  for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads)
-    s_base[i] += hist[(size_t)i * nblk + blk];  // this block's start inside bucket i
+  const uint32_t lmask = (1u << pp.local_bits) - 1u;
 #pragma unroll
   for (int k = 0; k < K; k++) {
     if (bk[2 * k] == kEmpty) continue;
     const uint2 uv = uvs[k];
-    s_stage[s_loc[bk[2 * k]] + rk[2 * k]] = uv;
-    s_stage[s_loc[bk[2 * k + 1]] + rk[2 * k + 1]] = make_uint2(uv.y, uv.x);
+    const uint32_t i0 = s_loc[bk[2 * k]] + rk[2 * k], i1 = s_loc[bk[2 * k + 1]] + rk[2 * k + 1];
+    if (kPacked) {
+      s_word[i0] = ((hashed_id(pp, uv.x) & lmask) << pp.nbits) | uv.y;
+      s_bkt[i0] = (uint16_t)bk[2 * k];
+      s_word[i1] = ((hashed_id(pp, uv.y) & lmask) << pp.nbits) | uv.x;
+      s_bkt[i1] = (uint16_t)bk[2 * k + 1];
+    } else {
+      s_stage[i0] = uv;
+      s_stage[i1] = make_uint2(uv.y, uv.x);
+    }
   }
   __syncthreads();
   for (uint32_t i = t; i < total; i += kPartThreads) {
-    const uint2 en = s_stage[i];
-    const uint32_t h = hashed_id(pp, en.x);
-    const uint32_t b = h >> pp.local_bits;
-    const uint32_t pos = s_base[b] + (i - s_loc[b]);
-    if (kPacked)
-      reinterpret_cast<uint32_t*>(entries)[pos] =
-          ((h & ((1u << pp.local_bits) - 1u)) << pp.nbits) | en.y;
-    else
-      entries[pos] = en;
+    if (kPacked) {
+      const uint32_t b = s_bkt[i];
+      reinterpret_cast<uint32_t*>(entries)[s_base[b] + (i - s_loc[b])] = s_word[i];
+    } else {
+      const uint2 en = s_stage[i];
+      const uint32_t b = bucket_of(pp, en.x);
+      entries[s_base[b] + (i - s_loc[b])] = en;
+    }
   }
 }
 
 // LDS bytes of bucket_scatter_kernel's dynamic part
-static size_t scatter_lds_bytes(uint32_t nbuckets) {
-  return sizeof(uint2) * 2 * kPartEdges + sizeof(uint32_t) * 2 * nbuckets;
+static size_t scatter_lds_bytes(uint32_t nbuckets, bool packed) {
+  const size_t stage = 2 * (size_t)kPartEdges;
+  return (packed ? stage * 6 : stage * sizeof(uint2)) + sizeof(uint32_t) * 2 * nbuckets;
 }
 
 // member of a group that owns global vertex v: the last base <= v (bases ascending, bases[0] = 0)
@@ -351,6 +401,7 @@ static size_t sort_lds_bytes(uint32_t local_bits) {
 template <bool kPacked>
 __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
     const uint2* __restrict__ entries, PartParams pp, const uint32_t* __restrict__ counts,
+    const uint32_t* __restrict__ bstart,
     uint32_t* __restrict__ sorted, WorkItem* __restrict__ items, ItemClasses cls, uint32_t chunk,
     uint32_t* __restrict__ class_counts, const uint32_t* __restrict__ bases, uint32_t ngraphs) {
   extern __shared__ uint32_t lds_u32[];
@@ -360,17 +411,9 @@ __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
   __shared__ uint32_t s_part[kSortThreads];
   __shared__ uint32_t s_ccnt[kClasses], s_cbase[kClasses];
   const uint32_t t = threadIdx.x, b = blockIdx.x;
-  // region start of this bucket = sum of the counts of the buckets before it
-  uint32_t acc = 0;
-  for (uint32_t i = t; i < b; i += kSortThreads) acc += counts[i];
-  s_part[t] = acc;
+  // this bucket's region of the entries: [bstart[b], bstart[b] + counts[b])
   for (uint32_t i = t; i < bins; i += kSortThreads) s_cnt[i] = 0;
-  __syncthreads();
-  for (uint32_t o = kSortThreads / 2; o > 0; o >>= 1) {
-    if (t < o) s_part[t] += s_part[t + o];
-    __syncthreads();
-  }
-  const uint32_t base = s_part[0];
+  const uint32_t base = bstart[b];
   const uint32_t cnt = counts[b];
   __syncthreads();
   const uint32_t lmask = bins - 1;
@@ -1345,7 +1388,7 @@ size_t carve_pair_scratch(const Stream* s, uint64_t nedges, char* base, PairScra
   ws.d_sorted = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * 2 * nedges));
   ws.d_items = reinterpret_cast<WorkItem*>(take(sizeof(WorkItem) * icap));
   ws.d_hist = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * hist_words));
-  ws.d_totals = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * pp.nbuckets));
+  ws.d_totals = reinterpret_cast<uint32_t*>(take(sizeof(uint32_t) * 2 * pp.nbuckets));
   ws.cap_edges = nedges;
   ws.cap_items = icap;
   ws.cap_hist = hist_words;
@@ -1369,7 +1412,7 @@ int ensure_pair_scratch(Stream* s, uint64_t nedges) {
     const uint32_t nb = std::max(ws.cap_buckets, pp.nbuckets);
     const uint64_t hw = std::max(ws.cap_hist, hist_words);
     GS_HIP(hipMalloc(&ws.d_hist, sizeof(uint32_t) * hw));
-    GS_HIP(hipMalloc(&ws.d_totals, sizeof(uint32_t) * nb));
+    GS_HIP(hipMalloc(&ws.d_totals, sizeof(uint32_t) * 2 * nb));
     ws.cap_buckets = nb;
     ws.cap_hist = hw;
   }
@@ -1473,7 +1516,7 @@ int run_pair_pipeline(Stream* s, const PairSegs& sg, uint32_t nblk, uint64_t ned
   const void* sort_fn = packed ? (const void*)bucket_sort_kernel<true>
                                : (const void*)bucket_sort_kernel<false>;
   if ((rc = ensure_dyn_lds((const void*)bucket_count_kernel, sizeof(uint32_t) * pp.nbuckets)) ||
-      (rc = ensure_dyn_lds(scatter_fn, scatter_lds_bytes(pp.nbuckets))) ||
+      (rc = ensure_dyn_lds(scatter_fn, scatter_lds_bytes(pp.nbuckets, packed))) ||
       (rc = ensure_dyn_lds(sort_fn, sort_lds)))
     return rc;
   hipEvent_t g0 = nullptr;  // start of the grouping kernels (measurement only)
@@ -1486,17 +1529,21 @@ int run_pair_pipeline(Stream* s, const PairSegs& sg, uint32_t nblk, uint64_t ned
                      sizeof(uint32_t) * pp.nbuckets, s->stream, sg, nverts, vgraph, pp, nblk,
                      ws.d_hist, ws.d_counters);
   GS_HIP(hipGetLastError());
+  uint32_t* const bstart = ws.d_totals + pp.nbuckets;  // bucket region starts
   hipLaunchKernelGGL(bucket_offsets_kernel, dim3(pp.nbuckets), dim3(kOffThreads), 0, s->stream,
                      ws.d_hist, nblk, ws.d_totals);
   GS_HIP(hipGetLastError());
+  hipLaunchKernelGGL(bucket_starts_kernel, dim3(1), dim3(kOffThreads), 0, s->stream, ws.d_totals,
+                     pp.nbuckets, bstart);
+  GS_HIP(hipGetLastError());
   if (packed)
     hipLaunchKernelGGL(bucket_scatter_kernel<true>, dim3(nblk), dim3(kPartThreads),
-                       scatter_lds_bytes(pp.nbuckets), s->stream, sg, nverts, vgraph, pp, nblk,
-                       ws.d_totals, ws.d_hist, ws.d_entries);
+                       scatter_lds_bytes(pp.nbuckets, true), s->stream, sg, nverts, vgraph, pp,
+                       nblk, bstart, ws.d_hist, ws.d_entries);
   else
     hipLaunchKernelGGL(bucket_scatter_kernel<false>, dim3(nblk), dim3(kPartThreads),
-                       scatter_lds_bytes(pp.nbuckets), s->stream, sg, nverts, vgraph, pp, nblk,
-                       ws.d_totals, ws.d_hist, ws.d_entries);
+                       scatter_lds_bytes(pp.nbuckets, false), s->stream, sg, nverts, vgraph, pp,
+                       nblk, bstart, ws.d_hist, ws.d_entries);
   GS_HIP(hipGetLastError());
   if (input_read) GS_HIP(hipEventRecord(input_read, s->stream));
   const uint32_t chunk = s->tune_chunk ? s->tune_chunk : (stream_wide(s) ? kWideChunk : kChunk);
@@ -1504,12 +1551,12 @@ int run_pair_pipeline(Stream* s, const PairSegs& sg, uint32_t nblk, uint64_t ned
   const uint32_t ngraphs = s->group ? (uint32_t)s->group->members.size() : 1u;
   if (packed)
     hipLaunchKernelGGL(bucket_sort_kernel<true>, dim3(pp.nbuckets), dim3(kSortThreads), sort_lds,
-                       s->stream, ws.d_entries, pp, ws.d_totals, ws.d_sorted, ws.d_items, ws.cls,
-                       chunk, ws.d_counters + kClassBase, bases, ngraphs);
+                       s->stream, ws.d_entries, pp, ws.d_totals, bstart, ws.d_sorted, ws.d_items,
+                       ws.cls, chunk, ws.d_counters + kClassBase, bases, ngraphs);
   else
     hipLaunchKernelGGL(bucket_sort_kernel<false>, dim3(pp.nbuckets), dim3(kSortThreads), sort_lds,
-                       s->stream, ws.d_entries, pp, ws.d_totals, ws.d_sorted, ws.d_items, ws.cls,
-                       chunk, ws.d_counters + kClassBase, bases, ngraphs);
+                       s->stream, ws.d_entries, pp, ws.d_totals, bstart, ws.d_sorted, ws.d_items,
+                       ws.cls, chunk, ws.d_counters + kClassBase, bases, ngraphs);
   GS_HIP(hipGetLastError());
   const size_t nev = s->ev_pairs.size();
   rc = launch_aggregate(s, ws.d_items, ws.cls, ws.d_counters + kClassBase, ws.d_sorted,
