@@ -404,7 +404,7 @@ __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
     const uint32_t* __restrict__ bstart,
     uint32_t* __restrict__ sorted, WorkItem* __restrict__ items, ItemClasses cls, uint32_t chunk,
     uint32_t* __restrict__ class_counts, const uint32_t* __restrict__ bases, uint32_t ngraphs) {
-  extern __shared__ uint32_t lds_u32[];
+  extern __shared__ __align__(16) uint32_t lds_u32[];
   const uint32_t bins = 1u << pp.local_bits;
   uint32_t* s_cnt = lds_u32;              // per local id: count
   uint32_t* s_off = lds_u32 + bins;       // per local id: offset inside the bucket, then cursor
@@ -415,6 +415,9 @@ __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
   for (uint32_t i = t; i < bins; i += kSortThreads) s_cnt[i] = 0;
   const uint32_t base = bstart[b];
   const uint32_t cnt = counts[b];
+  // an empty bucket has no items; every load below is clamped to [0, cnt - 1] (cnt is uniform
+  // over the workgroup, so no barrier is skipped by part of it)
+  if (cnt == 0) return;
   __syncthreads();
   const uint32_t lmask = bins - 1;
   const uint32_t omask = kPacked ? (1u << pp.nbits) - 1u : 0u;
@@ -431,17 +434,28 @@ __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
       val = e.y;
     }
   };
-  // LDS atomics aggregated per wave for the key of the wave's first lane: a hub vertex fills whole
-  // waves with one key, which would otherwise serialize on one LDS address
-  // the entries are loaded kSortBatch at a time (independent loads, one wait per batch)
-  for (uint32_t i0 = t; i0 < cnt; i0 += kSortBatch * kSortThreads) {
-    uint32_t ka[kSortBatch], va[kSortBatch];
+  // Count pass: entries per local id. LDS atomics are aggregated per wave for the key of the
+  // wave's first lane (a hub vertex fills whole waves with one key, which would otherwise
+  // serialize on one LDS address). The entries are loaded kCountBatch per thread at a time, and
+  // the next batch is in flight while one is counted (clamped loads: no branch before the wait).
+  constexpr int kCountBatch = 16;
+  auto key_of = [&](uint32_t i) -> uint32_t {
+    uint32_t k, v;
+    entry(min(i, cnt - 1), k, v);
+    return k;
+  };
+  uint32_t kb[kCountBatch];
 #pragma unroll
-    for (int k = 0; k < kSortBatch; k++) entry(min(i0 + k * kSortThreads, cnt - 1), ka[k], va[k]);
+  for (int k = 0; k < kCountBatch; k++) kb[k] = key_of(t + k * kSortThreads);
+  for (uint32_t c0 = 0; c0 < cnt; c0 += kCountBatch * kSortThreads) {
+    uint32_t nk[kCountBatch];
 #pragma unroll
-    for (int k = 0; k < kSortBatch; k++) {
-      if (i0 + k * kSortThreads >= cnt) break;
-      const uint32_t key = ka[k];
+    for (int k = 0; k < kCountBatch; k++)
+      nk[k] = key_of(c0 + (kCountBatch + k) * kSortThreads + t);
+#pragma unroll
+    for (int k = 0; k < kCountBatch; k++) {
+      if (c0 + k * kSortThreads + t >= cnt) break;
+      const uint32_t key = kb[k];
       const uint32_t k0 = __builtin_amdgcn_readfirstlane(key);
       const uint64_t same = __ballot(key == k0);
       if (key != k0)
@@ -449,6 +463,8 @@ __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
       else if (lane_below(same) == 0)
         atomicAdd(&s_cnt[k0], (uint32_t)__popcll(same));
     }
+#pragma unroll
+    for (int k = 0; k < kCountBatch; k++) kb[k] = nk[k];
   }
   __syncthreads();
   // entry offsets per local id (exclusive prefix over the bucket)
@@ -522,11 +538,12 @@ __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
   // one id's run instead of scattering 4-byte stores over the whole bucket. An entry's rank among
   // the tile's entries of its id is the old value its counting atomic returns, so one atomic pass
   // serves both; the next tile's entries load while this one is sorted and written.
-  uint32_t* t_cnt = s_off + bins;          // per id: entries in this tile (zero between tiles)
-  uint32_t* t_off = t_cnt + bins;          // per id: start of its run in the tile
-  uint32_t* t_gb = t_off + bins;           // per id: start of its run in the bucket
-  uint32_t* st_val = t_gb + bins;          // the tile in id order: destination
-  uint32_t* st_pos = st_val + kSortTile;   // and its position in the bucket
+  // per id: {start of its run in the tile, start of its run in the bucket}, one 64-bit word so
+  // that an entry reads both at once; the tile in id order as {destination, position in the
+  // bucket}, one 64-bit store and one load per entry (2 * bins words lie before: 8-byte aligned)
+  uint2* t_og = reinterpret_cast<uint2*>(s_off + bins);
+  uint2* st = t_og + bins;
+  uint32_t* t_cnt = reinterpret_cast<uint32_t*>(st + kSortTile);  // per id: entries in this tile
   __shared__ uint32_t s_wsum[kSortThreads / 64];
   const uint32_t lane = t & 63, wid = t >> 6;
   for (uint32_t i = t; i < bins; i += kSortThreads) t_cnt[i] = 0;
@@ -557,8 +574,7 @@ __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
     if (t < bins) {
       uint32_t excl = x - c;
       for (uint32_t w = 0; w < wid; w++) excl += s_wsum[w];
-      t_off[t] = excl;
-      t_gb[t] = s_off[t];
+      t_og[t] = make_uint2(excl, s_off[t]);
       s_off[t] += c;
       t_cnt[t] = 0;  // for the next tile's atomics (after two more barriers)
     }
@@ -566,13 +582,15 @@ __global__ __launch_bounds__(kSortThreads) void bucket_sort_kernel(
 #pragma unroll
     for (int k = 0; k < kSortBatch; k++) {
       if (i0 + k * kSortThreads >= cnt) break;
-      const uint32_t lpos = t_off[key[k]] + rk[k];
-      st_val[lpos] = val[k];
-      st_pos[lpos] = t_gb[key[k]] + rk[k];
+      const uint2 og = t_og[key[k]];
+      st[og.x + rk[k]] = make_uint2(val[k], og.y + rk[k]);
     }
     __syncthreads();
     const uint32_t ntile = min(kSortTile, cnt - tile0);
-    for (uint32_t j = t; j < ntile; j += kSortThreads) sorted[base + st_pos[j]] = st_val[j];
+    for (uint32_t j = t; j < ntile; j += kSortThreads) {
+      const uint2 e = st[j];
+      sorted[base + e.y] = e.x;
+    }
 #pragma unroll
     for (int k = 0; k < kSortBatch; k++) {
       key[k] = nkey[k];
