@@ -75,6 +75,7 @@ _sigs = {
     "gs_submit_edge_pairs_device": ([_vp, _vp, _u64], _i32),
     "gs_submit_edge_pairs_filtered": ([_vp, _vp, _u64, _u32, _u32], _i32),
     "gs_flush": ([_vp], _i32),
+    "gs_flush_async": ([_vp], _i32),
     "gs_sync": ([_vp], _i32),
     "gs_export_buckets": ([_vp, _u32, _u32, _vp], _i32),
     "gs_import_buckets": ([_vp, _u32, _u32, _vp], _i32),
@@ -95,6 +96,7 @@ _sigs = {
     "gs_group_stream": ([_vp, _vp], _i32),
     "gs_group_submit_pairs": ([_vp, _vp, _vp], _i32),
     "gs_group_submit_pairs_device": ([_vp, _vp, _u64], _i32),
+    "gs_group_submit_member_pairs": ([_vp, _u32, _vp, _u64], _i32),
     "gs_group_query_cc": ([_vp, _vp, _vp, _vp, _vp, _vp], _i32),
     "gs_group_prepare": ([_vp], _i32),
     "gs_apply_edge_pairs_device_noalloc": ([_vp, _vp, _u64], _i32),
@@ -260,6 +262,10 @@ class Stream:
 
     def flush(self) -> None:
         _check(lib.gs_flush(self.h))
+
+    def flush_async(self) -> None:
+        """Launches the pending updates without waiting (a later flush/sync/query waits)."""
+        _check(lib.gs_flush_async(self.h))
 
     def set_profiling(self, on: bool) -> None:
         _check(lib.gs_stream_set_profiling(self.h, int(on)))
@@ -433,6 +439,12 @@ class StoreGroup:
 
     def submit_pairs_device(self, dptr, nedges: int) -> None:
         _check(lib.gs_group_submit_pairs_device(self.h, dptr, nedges))
+
+    def submit_member_pairs(self, member: int, pairs) -> None:
+        """One member's (m, 2) uint32 pairs in its own ids, rebased and range-checked on the
+        device (out-of-range pairs are dropped and reported by the next flush)."""
+        pairs = _pairs(pairs)
+        _check(lib.gs_group_submit_member_pairs(self.h, member, _p(pairs), pairs.shape[0]))
 
     def prepare(self) -> None:
         """Allocates the group's ingest buffers and query scratch now rather than on first use."""

@@ -429,6 +429,11 @@ int gs_flush(gs_stream* sh) {
   return GS_OK;
 }
 
+int gs_flush_async(gs_stream* sh) {
+  GS_CHECK(sh, GS_ERR_INVALID, "null stream");
+  return drain_stream(&sh->impl);
+}
+
 int gs_sync(gs_store* h) {
   GS_CHECK(h, GS_ERR_INVALID, "null store");
   return sync_store(&h->impl);

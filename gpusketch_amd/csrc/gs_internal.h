@@ -297,8 +297,13 @@ int launch_vertex_batches(Stream* s, const uint32_t* d_class_counts, const WorkI
 // drain_stream processes whatever is pending; ingest_pairs_device appends and drains at once
 int submit_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uint32_t min_depth,
                         uint32_t depth_cap);
+// host pairs of one group member in its own ids (n > 0): rebased by `base` and range-checked on
+// the device as they are appended
+struct MemberIds {
+  uint32_t base = 0, n = 0;
+};
 int submit_pairs_host(Stream* s, const uint32_t* pairs, uint64_t nedges, uint32_t min_depth,
-                      uint32_t depth_cap);
+                      uint32_t depth_cap, const MemberIds& mid = MemberIds{});
 int drain_stream(Stream* s);
 int ingest_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uint32_t min_depth,
                         uint32_t depth_cap);

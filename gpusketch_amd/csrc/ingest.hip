@@ -131,12 +131,20 @@ __device__ __forceinline__ void block_pairs(const PairSegs& sg, uint32_t b, cons
 }
 
 // copy of a submission into the fill buffer; with a subgraph filter (min_depth > 0) in-range
-// pairs outside it become (0, 0). src and dst may be the same buffer.
+// pairs outside it become (0, 0). With member_n > 0 the pairs are one group member's, in its own
+// ids: in-range pairs (both ids < member_n) are offset by the member's base into the group's
+// vertex space, the others become (kBadVertex, kBadVertex), dropped and counted by K1 like any
+// out-of-range pair. src and dst may be the same buffer.
+constexpr uint32_t kBadVertex = 0xFFFFFFFFu;
 __global__ void append_kernel(const uint2* src, uint2* dst, uint64_t nedges, uint32_t n,
-                              uint64_t seed, uint32_t min_depth, uint32_t depth_cap) {
+                              uint64_t seed, uint32_t min_depth, uint32_t depth_cap,
+                              uint32_t base, uint32_t member_n) {
   for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < nedges;
        e += (uint64_t)gridDim.x * blockDim.x) {
     uint2 uv = src[e];
+    if (member_n)
+      uv = uv.x < member_n && uv.y < member_n ? make_uint2(uv.x + base, uv.y + base)
+                                              : make_uint2(kBadVertex, kBadVertex);
     if (min_depth && uv.x < n && uv.y < n &&
         !edge_selected(concat_pair(uv.x, uv.y), seed, min_depth, depth_cap))
       uv = make_uint2(0, 0);
@@ -1494,15 +1502,15 @@ int prepare_stream(Stream* s) {
 // `m` pairs at src into the fill buffer behind what is pending (src may be that very place: a
 // host batch copied there; then only a subgraph filter needs a pass)
 static int append_pairs(Stream* s, const uint32_t* src, uint64_t m, uint32_t min_depth,
-                        uint32_t depth_cap, bool host_copy) {
+                        uint32_t depth_cap, bool host_copy, const MemberIds& mid = MemberIds{}) {
   GS_CHECK(!s->group || min_depth == 0, GS_ERR_INVALID, "no subgraph filter on group streams");
   uint2* dst = reinterpret_cast<uint2*>(s->d_pend[s->pend_buf]) + s->fill_n;
-  if (reinterpret_cast<const uint2*>(src) != dst || min_depth) {
+  if (reinterpret_cast<const uint2*>(src) != dst || min_depth || mid.n) {
     const uint64_t seed = s->group ? 0 : s->store->view.seed;
     const unsigned grid = (unsigned)std::min<uint64_t>((m + 255) / 256, 4096);
     hipLaunchKernelGGL(append_kernel, dim3(grid), dim3(256), 0, host_copy ? s->copy_stream : s->stream,
                        reinterpret_cast<const uint2*>(src), dst, m, stream_vertices(s), seed,
-                       min_depth, depth_cap);
+                       min_depth, depth_cap, mid.base, mid.n);
     GS_HIP(hipGetLastError());
   }
   if (host_copy) GS_HIP(hipEventRecord(s->copied, s->copy_stream));
@@ -1647,7 +1655,7 @@ int submit_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uin
 // vertex outside the store are dropped by the grouping and reported by the next flush, as for
 // device batches, whatever memory they came from.
 int submit_pairs_host(Stream* s, const uint32_t* pairs, uint64_t nedges, uint32_t min_depth,
-                      uint32_t depth_cap) {
+                      uint32_t depth_cap, const MemberIds& mid) {
   if (nedges == 0) return GS_OK;
   std::lock_guard<std::mutex> lk(s->mu);
   int rc = ensure_pending(s);
@@ -1675,7 +1683,7 @@ int submit_pairs_host(Stream* s, const uint32_t* pairs, uint64_t nedges, uint32_
       GS_HIP(hipEventRecord(s->stage_free, s->copy_stream));
       s->stage_pending = true;
     }
-    if ((rc = append_pairs(s, dst, m, min_depth, depth_cap, true))) return rc;  // records `copied`
+    if ((rc = append_pairs(s, dst, m, min_depth, depth_cap, true, mid))) return rc;  // records `copied`
     s->copies_pending = true;
     e0 += m;
     if (s->pend_n == s->pend_cap && (rc = process_pending(s))) return rc;

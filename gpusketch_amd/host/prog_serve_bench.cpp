@@ -3,9 +3,11 @@
 //               [--mc-tenants 8] [--k 4] [--min-n 4096] [--max-n 262144]
 //               [--max-edges 1000000] [--alone-check 0|1] [--evict 0|1]
 //               [--max-step-updates N] [--pinned-spill 0|1] [--verify 0|1]
+//               [--cc-frac 0.01] [--mc-frac 0.005] [--pinned-payloads 0|1]
 // Tenants: n log-uniform in [min-n, max-n]. Requests arrive as a Poisson process of the given
-// rate (requests/s): 1% connectivity queries on random tenants, 0.5% min-cut queries (k) on the
-// first mc-tenants tenants, the rest update batches of log-uniform 1k..max-edges edges, 5% of
+// rate (requests/s): a cc-frac share of connectivity queries on random tenants, mc-frac of
+// min-cut queries (k) on the first mc-tenants tenants, the rest update batches of log-uniform
+// 1k..max-edges edges, 5% of
 // which delete an earlier inserted edge of the tenant. The same trace runs twice: plain (one
 // request at a time on per-tenant streams) and served (continuous batching). Prints one JSON
 // line: updates/s and p50/p99 latencies per mode, and whether every answer and every final
@@ -15,6 +17,10 @@
 // --verify 1 replays the trace on the host and checks every answer of both modes against the
 // exact graph: each CC answer against a union-find over the tenant's net graph at that point of
 // the trace (component count and labels), each min-cut answer against min(k, exact minimum cut).
+// With --pinned-payloads 1 (the default) the update payloads are placed in pinned host memory
+// before the clock starts and handed over without a copy (SketchServer::submit_update_pinned),
+// as a server that receives requests into pinned buffers would; with 0 the server copies each
+// payload at arrival (submit_update). Both modes of a run take the payloads the same way.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -34,7 +40,11 @@ struct Req {
   uint32_t tenant;
   RequestKind kind;
   std::vector<uint32_t> pairs;
+  const uint32_t* pinned = nullptr;  // the same pairs in the trace's pinned arena
 };
+
+static double g_cc_frac = 0.01, g_mc_frac = 0.005;
+static bool g_pinned_payloads = true;
 
 struct Trace {
   std::vector<TenantSpec> tenants;
@@ -62,10 +72,10 @@ static Trace make_trace(int ntenants, int nreq, double rate, uint64_t seed, int 
     Req r;
     r.t = t;
     const double x = U(rng);
-    if (x < 0.01) {
+    if (x < g_cc_frac) {
       r.kind = RequestKind::CC_QUERY;
       r.tenant = (uint32_t)(rng() % ntenants);
-    } else if (x < 0.015 && mc_tenants > 0) {
+    } else if (x < g_cc_frac + g_mc_frac && mc_tenants > 0) {
       r.kind = RequestKind::MINCUT_QUERY;
       r.tenant = (uint32_t)(rng() % std::min(mc_tenants, ntenants));
     } else {
@@ -105,6 +115,7 @@ struct RunStats {
   std::map<std::string, std::pair<double, double>> p50p99;  // kind -> (p50 ms, p99 ms)
   uint64_t steps = 0, evictions = 0, restores = 0;
   SketchServer::MoveStats moves;
+  SketchServer::PhaseStats phases;
   std::map<uint64_t, RequestResult> by_index;  // trace index -> result
   std::vector<std::vector<uint64_t>> exports;
   std::string error;
@@ -146,7 +157,9 @@ static RunStats run(const Trace& tr, bool batching, uint64_t budget, uint32_t k,
     if (due > now) std::this_thread::sleep_for(std::chrono::duration<double>(due - now));
     uint64_t id = 0;
     const uint32_t t = (uint32_t)tid[r.tenant];
-    if (r.kind == RequestKind::UPDATE) id = srv.submit_update(t, r.pairs.data(), r.pairs.size() / 2);
+    if (r.kind == RequestKind::UPDATE)
+      id = r.pinned ? srv.submit_update_pinned(t, r.pinned, r.pairs.size() / 2)
+                    : srv.submit_update(t, r.pairs.data(), r.pairs.size() / 2);
     else if (r.kind == RequestKind::CC_QUERY) id = srv.submit_cc_query(t);
     else id = srv.submit_mincut_query(t, k);
     id_to_index[id] = q;
@@ -174,6 +187,7 @@ static RunStats run(const Trace& tr, bool batching, uint64_t budget, uint32_t k,
   st.evictions = srv.evictions();
   st.restores = srv.restores();
   st.moves = srv.move_stats();
+  st.phases = srv.phase_stats();
   if (keep_exports)
     for (size_t i = 0; i < tr.tenants.size(); i++)
       if (tid[i] >= 0 && srv.has_sketch((uint32_t)tid[i]))
@@ -289,7 +303,11 @@ static std::string stats_json(const RunStats& s) {
                   ", \"spill_gb\": " + std::to_string(s.moves.spill_bytes / 1e9) +
                   ", \"spill_s\": " + std::to_string(s.moves.spill_s) +
                   ", \"restore_gb\": " + std::to_string(s.moves.restore_bytes / 1e9) +
-                  ", \"restore_s\": " + std::to_string(s.moves.restore_s);
+                  ", \"restore_s\": " + std::to_string(s.moves.restore_s) +
+                  ", \"worker_s\": {\"updates\": " + std::to_string(s.phases.update_s) +
+                  ", \"cc_queries\": " + std::to_string(s.phases.cc_s) +
+                  ", \"mincut_forests\": " + std::to_string(s.phases.mincut_s) +
+                  ", \"cc_batches\": " + std::to_string(s.phases.cc_batches) + "}";
   for (auto& kv : s.p50p99)
     j += ", \"" + kv.first + "_ms\": {\"p50\": " + std::to_string(kv.second.first) +
          ", \"p99\": " + std::to_string(kv.second.second) + "}";
@@ -319,6 +337,9 @@ int main(int argc, char** argv) {
     else if (a == "--evict") g_evict = std::atoi(v) != 0;
     else if (a == "--pinned-spill") g_pinned_spill = std::atoi(v) != 0;
     else if (a == "--verify") verify = std::atoi(v);
+    else if (a == "--cc-frac") g_cc_frac = std::atof(v);
+    else if (a == "--mc-frac") g_mc_frac = std::atof(v);
+    else if (a == "--pinned-payloads") g_pinned_payloads = std::atoi(v) != 0;
     else {
       std::fprintf(stderr, "unknown option %s\n", a.c_str());
       return 2;
@@ -330,6 +351,20 @@ int main(int argc, char** argv) {
   std::fprintf(stderr, "[serve] trace: %d tenants, %zu requests, %llu updates (%.1fs)\n",
                ntenants, tr.reqs.size(), (unsigned long long)tr.total_updates,
                std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+  void* arena = nullptr;
+  if (g_pinned_payloads && tr.total_updates) {
+    if (gs_host_alloc(tr.total_updates * 2 * sizeof(uint32_t), &arena) != GS_OK) {
+      std::fprintf(stderr, "pinned arena: %s\n", gs_last_error());
+      return 2;
+    }
+    uint32_t* w = static_cast<uint32_t*>(arena);
+    for (auto& r : tr.reqs) {
+      if (r.kind != RequestKind::UPDATE) continue;
+      std::memcpy(w, r.pairs.data(), r.pairs.size() * sizeof(uint32_t));
+      r.pinned = w;
+      w += r.pairs.size();
+    }
+  }
   RunStats plain = run(tr, false, budget, k, {}, true);
   std::fprintf(stderr, "[serve] plain: %.3e updates/s over %.3fs\n", plain.upd_per_s, plain.makespan);
   RunStats served = run(tr, true, budget, k, {}, true);
@@ -366,12 +401,17 @@ int main(int argc, char** argv) {
             std::to_string(v.mincut) + ", \"mismatches\": " + std::to_string(v.mismatches) +
             ", \"first\": \"" + v.first + "\"}";
   }
-  std::printf("{\"tenants\": %d, \"requests\": %zu, \"queries\": %llu, \"updates\": %llu, "
+  uint64_t ncc = 0;
+  for (auto& r : tr.reqs) ncc += r.kind == RequestKind::CC_QUERY;
+  if (arena) gs_host_free(arena);
+  std::printf("{\"tenants\": %d, \"requests\": %zu, \"queries\": %llu, \"cc_queries\": %llu, "
+              "\"pinned_payloads\": %s, \"updates\": %llu, "
               "\"rate_req_per_s\": %g, \"budget_gb\": %g, \"k\": %u, \"plain\": %s, "
               "\"served\": %s, \"answers_identical\": %s, \"sketches_identical\": %s, "
               "\"alone_equals_mix\": %s, \"verified\": %s, \"error\": \"%s\"}\n",
-              ntenants, tr.reqs.size(), (unsigned long long)nq,
-              (unsigned long long)tr.total_updates, rate, budget_gb, k,
+              ntenants, tr.reqs.size(), (unsigned long long)nq, (unsigned long long)ncc,
+              g_pinned_payloads ? "true" : "false", (unsigned long long)tr.total_updates, rate,
+              budget_gb, k,
               stats_json(plain).c_str(), stats_json(served).c_str(), same ? "true" : "false",
               same_sketch ? "true" : "false", alone_ok ? "true" : "false", vjson.c_str(),
               (plain.error + served.error).c_str());

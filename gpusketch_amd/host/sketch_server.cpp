@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 
@@ -48,6 +49,9 @@ SketchServer::~SketchServer() {
     give_back(t->spill);
   }
   for (auto& kv : pinned_free) gs_host_free(kv.second);
+  for (auto& r : queue)
+    if (r.owned) gs_host_free(r.owned);
+  for (auto& kv : payload_free) gs_host_free(kv.second);
 }
 
 double SketchServer::now_s() const {
@@ -101,41 +105,94 @@ uint64_t SketchServer::admitted_bytes() {
   return admitted;
 }
 
-uint64_t SketchServer::submit_update(uint32_t tenant, const uint32_t* pairs, uint64_t nedges) {
-  Request r{0, tenant, RequestKind::UPDATE, 0, std::vector<uint32_t>(pairs, pairs + 2 * nedges),
-            now_s()};
+uint64_t SketchServer::enqueue(Request r) {
   std::lock_guard<std::mutex> lk(mu);
-  if (tenant >= tenants.size() || tenants[tenant]->removing)
+  if (r.tenant >= tenants.size() || tenants[r.tenant]->removing) {
+    release_payload_locked(r);
     throw std::invalid_argument("unknown or removed tenant");
+  }
   r.id = next_id++;
-  tenants[tenant]->queued++;
-  queue.push_back(std::move(r));
+  tenants[r.tenant]->queued++;
+  queue.push_back(r);
   cv.notify_all();
   return next_id - 1;
+}
+
+uint64_t SketchServer::submit_update(uint32_t tenant, const uint32_t* pairs, uint64_t nedges) {
+  Request r;
+  r.tenant = tenant;
+  r.arrival_s = now_s();
+  r.nedges = nedges;
+  const size_t words = 2 * (size_t)nedges;
+  if (words) {
+    {  // a free pool buffer of at least `words` words (at most twice that)
+      std::lock_guard<std::mutex> lk(mu);
+      auto it = payload_free.lower_bound(words);
+      if (it != payload_free.end() && it->first <= 2 * words) {
+        r.owned = it->second;
+        r.owned_words = it->first;
+        payload_free.erase(it);
+      }
+    }
+    if (!r.owned) {
+      size_t cap = 1u << 14;
+      while (cap < words) cap *= 2;
+      void* p = nullptr;
+      gs_throw_if(gs_host_alloc(cap * sizeof(uint32_t), &p));
+      r.owned = static_cast<uint32_t*>(p);
+      r.owned_words = cap;
+    }
+    std::memcpy(r.owned, pairs, words * sizeof(uint32_t));
+    r.pairs = r.owned;
+  }
+  return enqueue(r);
+}
+
+uint64_t SketchServer::submit_update_pinned(uint32_t tenant, const uint32_t* pairs,
+                                            uint64_t nedges) {
+  Request r;
+  r.tenant = tenant;
+  r.arrival_s = now_s();
+  r.pairs = pairs;
+  r.nedges = nedges;
+  return enqueue(r);
 }
 
 uint64_t SketchServer::submit_cc_query(uint32_t tenant) {
-  Request r{0, tenant, RequestKind::CC_QUERY, 0, {}, now_s()};
-  std::lock_guard<std::mutex> lk(mu);
-  if (tenant >= tenants.size() || tenants[tenant]->removing)
-    throw std::invalid_argument("unknown or removed tenant");
-  r.id = next_id++;
-  tenants[tenant]->queued++;
-  queue.push_back(std::move(r));
-  cv.notify_all();
-  return next_id - 1;
+  Request r;
+  r.tenant = tenant;
+  r.kind = RequestKind::CC_QUERY;
+  r.arrival_s = now_s();
+  return enqueue(r);
 }
 
 uint64_t SketchServer::submit_mincut_query(uint32_t tenant, uint32_t k) {
-  Request r{0, tenant, RequestKind::MINCUT_QUERY, k, {}, now_s()};
-  std::lock_guard<std::mutex> lk(mu);
-  if (tenant >= tenants.size() || tenants[tenant]->removing)
-    throw std::invalid_argument("unknown or removed tenant");
-  r.id = next_id++;
-  tenants[tenant]->queued++;
-  queue.push_back(std::move(r));
-  cv.notify_all();
-  return next_id - 1;
+  Request r;
+  r.tenant = tenant;
+  r.kind = RequestKind::MINCUT_QUERY;
+  r.k = k;
+  r.arrival_s = now_s();
+  return enqueue(r);
+}
+
+void SketchServer::release_payload_locked(Request& r) {
+  if (r.owned) payload_free.emplace(r.owned_words, r.owned);
+  r.owned = nullptr;
+  r.pairs = nullptr;
+}
+
+void SketchServer::fail_locked(Request& r, int status) {
+  RequestResult res;
+  res.id = r.id;
+  res.tenant = r.tenant;
+  res.kind = r.kind;
+  res.arrival_s = r.arrival_s;
+  res.done_s = now_s();
+  res.status = status;
+  release_payload_locked(r);
+  tenants[r.tenant]->queued--;
+  answered++;
+  done.push_back(std::move(res));
 }
 
 void SketchServer::drain() {
@@ -341,6 +398,7 @@ void SketchServer::finish(Request& r, RequestResult res) {
   res.arrival_s = r.arrival_s;
   res.done_s = now_s();
   std::lock_guard<std::mutex> lk(mu);
+  release_payload_locked(r);
   tenants[r.tenant]->queued--;
   answered++;
   done.push_back(std::move(res));
@@ -348,36 +406,41 @@ void SketchServer::finish(Request& r, RequestResult res) {
 
 void SketchServer::apply_updates(std::vector<Request*>& upd) {
   if (upd.empty()) return;
+  const double t_start = now_s();
   if (cfg.batching) {
     ensure_group();
     const size_t G = group_members.size();
-    std::vector<std::vector<uint32_t>> per(G);
     std::vector<size_t> slot(tenants.size(), (size_t)-1);
     for (size_t i = 0; i < G; i++) slot[group_members[i]] = i;
-    for (Request* r : upd) {
-      auto& v = per[slot[r->tenant]];
-      v.insert(v.end(), r->pairs.begin(), r->pairs.end());
-    }
-    std::vector<const uint32_t*> ptr(G);
-    std::vector<uint64_t> cnt(G);
-    for (size_t i = 0; i < G; i++) {
-      ptr[i] = per[i].empty() ? nullptr : per[i].data();
-      cnt[i] = per[i].size() / 2;
-    }
+    // each payload by DMA as it arrived (member ids; rebase and range check on the device)
+    for (Request* r : upd)
+      gs_throw_if(gs_group_submit_member_pairs(group, (uint32_t)slot[r->tenant], r->pairs,
+                                               r->nedges));
     gs_stream* gsm = nullptr;
     gs_throw_if(gs_group_stream(group, &gsm));
-    gs_throw_if(gs_group_submit_pairs(group, ptr.data(), cnt.data()));
-    gs_throw_if(gs_flush(gsm));
+    // the launches are not waited for while more requests are queued (a query waits for its
+    // tenant's store); the last step before the queue runs dry waits, and so reports pairs the
+    // device dropped (out of range) in error()
+    bool more = false;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      more = !queue.empty();
+    }
+    gs_throw_if(more ? gs_flush_async(gsm) : gs_flush(gsm));
   } else {
     for (Request* r : upd) {
       Tenant& t = *tenants[r->tenant];
-      gs_throw_if(gs_submit_edge_pairs(t.stream, r->pairs.data(), r->pairs.size() / 2));
+      gs_throw_if(gs_submit_edge_pairs(t.stream, r->pairs, r->nedges));
       gs_throw_if(gs_flush(t.stream));
     }
   }
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    phases.update_s += now_s() - t_start;
+  }
   for (Request* r : upd) {
     RequestResult res;
-    res.updates = r->pairs.size() / 2;
+    res.updates = r->nedges;
     finish(*r, std::move(res));
   }
 }
@@ -397,6 +460,7 @@ void SketchServer::run_step(std::vector<Request>& step) {
   // 1. the updates the step's queries depend on: those of the queried tenants
   apply_updates(upd_q);
   // 2. connectivity queries: one batched Boruvka over the queried tenants
+  const double t_cc = now_s();
   if (!cc.empty()) {
     if (cfg.batching) {
       const size_t G = group_members.size();
@@ -438,6 +502,7 @@ void SketchServer::run_step(std::vector<Request>& step) {
       }
     }
   }
+  const double t_mc = now_s();
   // 3. min-cut queries: k spanning forests on the device; the exact cut of their union is a
   //    host computation of up to tens of milliseconds, done by the cutter thread meanwhile
   for (Request* r : mc) {
@@ -458,6 +523,15 @@ void SketchServer::run_step(std::vector<Request>& step) {
     }
     if (res.status != GS_ERR_OUT_OF_SAMPLES) gs_throw_if(res.status);
     finish(*r, std::move(res));
+  }
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    if (!cc.empty()) {
+      phases.cc_s += t_mc - t_cc;
+      phases.cc_batches += cfg.batching ? 1 : cc.size();
+    }
+    phases.mincut_s += now_s() - t_mc;
+    phases.mincut_queries += mc.size();
   }
   // 4. every other tenant's updates
   apply_updates(upd_o);
@@ -521,7 +595,7 @@ void SketchServer::loop() {
             continue;
           }
           if (it->kind == RequestKind::UPDATE) {
-            const uint64_t m = it->pairs.size() / 2;
+            const uint64_t m = it->nedges;
             if (nupd > 0 && nupd + m > cfg.max_step_updates) break;  // step full
             nupd += m;
           } else {
@@ -567,16 +641,7 @@ void SketchServer::loop() {
             ++it;
             continue;
           }
-          RequestResult res;
-          res.id = it->id;
-          res.tenant = it->tenant;
-          res.kind = it->kind;
-          res.arrival_s = it->arrival_s;
-          res.done_s = now_s();
-          res.status = GS_ERR_HIP;
-          tenants[need]->queued--;
-          answered++;
-          done.push_back(std::move(res));
+          fail_locked(*it, GS_ERR_HIP);
           it = queue.erase(it);
         }
       }
@@ -593,17 +658,11 @@ void SketchServer::loop() {
               std::find(step_handed.begin(), step_handed.end(), r.id) != step_handed.end();
           for (auto& d : done)
             if (d.id == r.id) answered_already = true;
-          if (answered_already) continue;
-          RequestResult res;
-          res.id = r.id;
-          res.tenant = r.tenant;
-          res.kind = r.kind;
-          res.arrival_s = r.arrival_s;
-          res.done_s = now_s();
-          res.status = GS_ERR_HIP;
-          tenants[r.tenant]->queued--;
-          answered++;
-          done.push_back(std::move(res));
+          if (answered_already) {
+            release_payload_locked(r);
+            continue;
+          }
+          fail_locked(r, GS_ERR_HIP);
         }
       }
     }

@@ -136,6 +136,11 @@ int gs_submit_edge_pairs_filtered(gs_stream* s, const uint32_t* pairs, uint64_t 
 /* Wait until all work submitted on the stream is finished (CudaStream::flush_buffers,
  * include/cuda_stream.h:215-276). */
 int gs_flush(gs_stream* s);
+/* Launch the updates the stream holds pending (a partly filled batch) without waiting for them:
+ * the half of gs_flush that the reference's flush_buffers performs before its wait. A later
+ * gs_flush, gs_sync or query of the store waits for them; dropped pairs are reported by the next
+ * gs_flush. For callers that overlap their own host work with the device's (a serving loop). */
+int gs_flush_async(gs_stream* s);
 /* Wait for every stream of the store (CCGPUSketchAlg::flush_buffers,
  * src/cc_gpu_sketch_alg.cpp:21-31). */
 int gs_sync(gs_store* st);
@@ -235,6 +240,15 @@ int gs_group_stream(gs_group* g, gs_stream** out);
  * ingested as one stream of the group's vertex space. Every id is checked first: an id past its
  * member's end fails the call (GS_ERR_INVALID) before anything is submitted. */
 int gs_group_submit_pairs(gs_group* g, const uint32_t* const* pairs, const uint64_t* nedges);
+/* Host edge pairs of one member in its own ids, taken as they are: the DMA reads them straight
+ * from the caller's memory when it is pinned (gs_host_alloc), and the device adds the member's
+ * base and checks the range while appending them. A pair with an id >= the member's vertex count
+ * is skipped and counted like an out-of-range pair (the group stream's next gs_flush returns
+ * GS_ERR_INVALID); the other pairs are applied. Returns once the pairs have been read, so the
+ * buffer may be reused. No host pass over the pairs (a serving loop's payloads go to the device
+ * as they arrived). No counterpart in the reference. */
+int gs_group_submit_member_pairs(gs_group* g, uint32_t member, const uint32_t* pairs,
+                                 uint64_t nedges);
 /* Device-resident pairs already in global ids (base_i added); must stay valid until gs_flush.
  * A pair whose endpoints lie in two different members is skipped and counted like an
  * out-of-range pair (the group stream's next gs_flush returns GS_ERR_INVALID). */

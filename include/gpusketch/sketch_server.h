@@ -5,7 +5,10 @@
 // every admitted tenant up to and including that tenant's first pending query, applies all
 // gathered updates of the queried tenants in one device submission (a store group), answers all
 // gathered connectivity queries with one batched Boruvka, then the min-cut queries, then applies
-// every other tenant's gathered updates in one more submission. A query
+// every other tenant's gathered updates in one more submission. Update payloads stay in pinned
+// host memory from arrival: each goes by DMA straight to the device, where the member's base is
+// added and the ids range-checked (no host pass), and a step's launches are not waited for: the
+// next step's host work overlaps them, and a query waits for its tenant's store. A query
 // therefore sees exactly the updates of its tenant that arrived before it, so every answer is
 // the one the tenant would get alone. With batching off, requests are served one at a time on
 // per-tenant streams (the plain single-stream path). No counterpart in the reference, whose
@@ -79,8 +82,12 @@ class SketchServer {
   void wait_admissions();
   uint64_t admitted_bytes();
 
-  // enqueue (thread-safe, returns the request id); pairs are copied
+  // enqueue (thread-safe, returns the request id); pairs are copied (into a pinned buffer of the
+  // server's pool)
   uint64_t submit_update(uint32_t tenant, const uint32_t* pairs, uint64_t nedges);
+  // the same without the copy: `pairs` must be pinned (gs_host_alloc) and stay valid and
+  // unchanged until the request is answered (drain / results)
+  uint64_t submit_update_pinned(uint32_t tenant, const uint32_t* pairs, uint64_t nedges);
   uint64_t submit_cc_query(uint32_t tenant);
   uint64_t submit_mincut_query(uint32_t tenant, uint32_t k);
 
@@ -102,6 +109,16 @@ class SketchServer {
     std::lock_guard<std::mutex> lk(mu);
     return moves;
   }
+  // worker seconds per phase of the steps (after drain): submitting updates, answering
+  // connectivity queries, computing min-cut forests
+  struct PhaseStats {
+    double update_s = 0, cc_s = 0, mincut_s = 0;
+    uint64_t cc_batches = 0, mincut_queries = 0;
+  };
+  PhaseStats phase_stats() {
+    std::lock_guard<std::mutex> lk(mu);
+    return phases;
+  }
   // message of the last device error that failed a step ("" if none)
   std::string error() {
     std::lock_guard<std::mutex> lk(mu);
@@ -110,12 +127,15 @@ class SketchServer {
 
  private:
   struct Request {
-    uint64_t id;
-    uint32_t tenant;
-    RequestKind kind;
-    uint32_t k;
-    std::vector<uint32_t> pairs;
-    double arrival_s;
+    uint64_t id = 0;
+    uint32_t tenant = 0;
+    RequestKind kind = RequestKind::UPDATE;
+    uint32_t k = 0;
+    const uint32_t* pairs = nullptr;  // UPDATE: 2 * nedges ids, pinned
+    uint64_t nedges = 0;
+    double arrival_s = 0;
+    uint32_t* owned = nullptr;  // the pool buffer holding `pairs` (submit_update), or null
+    size_t owned_words = 0;     // its capacity
   };
   struct SpillBuf {
     uint64_t* p = nullptr;
@@ -145,6 +165,10 @@ class SketchServer {
   void apply_updates(std::vector<Request*>& upd);
   void ensure_group();
   void finish(Request& r, RequestResult res);
+  uint64_t enqueue(Request r);
+  void release_payload_locked(Request& r);  // mu held
+  // answers a request that failed (mu held)
+  void fail_locked(Request& r, int status);
   void cut_loop();
 
   // a min-cut query whose forests are on the host: its exact cut is computed off the worker
@@ -169,7 +193,9 @@ class SketchServer {
   std::vector<RequestResult> done;
   std::atomic<uint64_t> num_steps{0}, num_evictions{0}, num_restores{0};
   MoveStats moves;
+  PhaseStats phases;
   std::multimap<size_t, uint64_t*> pinned_free;  // pinned spill buffers by capacity (words)
+  std::multimap<size_t, uint32_t*> payload_free;  // pinned payload buffers by capacity (words)
   std::string last_error;
   // store group over the admitted tenants (batching mode), rebuilt when membership changes
   gs_group* group = nullptr;

@@ -212,6 +212,50 @@ def test_cross_member_pairs_are_dropped_and_reported():
             st.close()
 
 
+def test_member_pairs_rebased_and_checked_on_device():
+    """gs_group_submit_member_pairs: a member's pairs in its own ids go to the device as they are
+    (pinned: DMA from the caller's buffer; pageable: staged), the device adds the member's base
+    and drops pairs with an id past the member's end (one that a host-side rebase would have
+    turned into another member's vertex), reported with their count by the next flush. The
+    sketches equal the oracle's of the valid pairs. gs_flush_async launches without waiting, and
+    an export (like a query) waits for the launches."""
+    rng = np.random.default_rng(23)
+    sizes, seeds = [300, 500, 200], [31, 32, 33]
+    stores = [gs.SketchStore(n, s) for n, s in zip(sizes, seeds)]
+    try:
+        with gs.StoreGroup(stores) as grp:
+            local = [member_pairs(rng, n, 5 * n) for n in sizes]
+            bad = np.array([[0, sizes[1]], [sizes[1] + 5, 3], [sizes[1], sizes[1] + 1]],
+                           dtype=np.uint32)  # member 1: id n_1 and past it (member 2's range)
+            pinned = gs.PinnedBuffer((len(local[0]), 2))
+            pinned.array[:] = local[0]
+            try:
+                half = len(local[1]) // 2
+                grp.submit_member_pairs(1, local[1][:half])
+                grp.submit_member_pairs(0, pinned.array)  # pinned: no staging copy
+                grp.submit_member_pairs(1, np.concatenate([bad, local[1][half:]]))
+                grp.submit_member_pairs(2, local[2])
+                with pytest.raises(gs.GsError, match="3 submitted updates"):
+                    grp.flush()
+            finally:
+                pinned.free()
+            for i, (n, sd) in enumerate(zip(sizes, seeds)):
+                assert np.array_equal(stores[i].export_buckets(), oracle_store(n, sd, local[i]))
+            more = [member_pairs(rng, n, 2 * n) for n in sizes]
+            for i in (2, 0, 1):
+                grp.submit_member_pairs(i, more[i])
+            grp.stream.flush_async()
+            for i, (n, sd) in enumerate(zip(sizes, seeds)):
+                both = np.concatenate([local[i], more[i]])
+                assert np.array_equal(stores[i].export_buckets(), oracle_store(n, sd, both))
+            grp.flush()  # nothing dropped since the last flush
+            with pytest.raises(gs.GsError, match="member 3"):
+                grp.submit_member_pairs(3, local[0])
+    finally:
+        for st in stores:
+            st.close()
+
+
 def test_store_destroy_refused_while_in_group():
     a, b = gs.SketchStore(100, 1), gs.SketchStore(200, 2)
     grp = gs.StoreGroup([a, b])
