@@ -336,7 +336,8 @@ def bench_single(args) -> dict:
     store = gs.SketchStore(N, SEED_SKETCH, cubesketch=cube)
     geo = store.geom
     stream = store.stream()
-    stream.set_tuning(launch_edges=LAUNCH_EDGES)
+    stream.set_tuning(lds_depths=args.lds_depths, item_updates=args.item_updates,
+                      launch_edges=LAUNCH_EDGES)
     dbuf = gs.DeviceBuffer(pairs)
     log(f"store {geo.device_bytes / 1e9:.3f} GB, B={geo.bkt_per_col} C={geo.num_columns}; "
         f"input resident in HBM ({dbuf.nbytes / 1e9:.2f} GB)")
@@ -529,6 +530,10 @@ def main() -> None:
     ap.add_argument("--graphs", type=int, default=256, help="batched mode: independent graphs")
     ap.add_argument("--query-reps", type=int, default=0,
                     help="query repetitions (default: 3 in batched mode, 100 in single mode)")
+    ap.add_argument("--lds-depths", type=int, default=0,
+                    help="single mode: LDS depth window of the aggregation kernel (0: the library's)")
+    ap.add_argument("--item-updates", type=int, default=0,
+                    help="single mode: destinations per work item (0: the library's)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the known-size copy kernel (HBM counter calibration)")
     args = ap.parse_args()

@@ -441,7 +441,7 @@ int gs_sync(gs_store* h) {
 
 // ---- reference-order transfers ----------------------------------------------------------------
 // A vertex's buckets in reference order are [c*B + d] for column c, depth d, then the deterministic
-// bucket; the device keeps them depth-major ([d*C + c], row stride vstride) with the deterministic
+// bucket; the device keeps them as bucket_slot orders them (row stride vstride) with the deterministic
 // buckets in their own array. One thread per reference-order bucket of a chunk of vertices: the
 // reference-order side is read or written coalesced, the device side gathered within a row.
 __global__ void to_reference_order_kernel(const Bucket16* __restrict__ rows,
@@ -453,7 +453,7 @@ __global__ void to_reference_order_kernel(const Bucket16* __restrict__ rows,
        i += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t v = (uint32_t)(i / nb), j = (uint32_t)(i % nb);
     const uint32_t c = j / B, d = j % B;
-    out[i] = j == nb - 1 ? det[v0 + v] : rows[(uint64_t)(v0 + v) * vstride + (uint64_t)d * C + c];
+    out[i] = j == nb - 1 ? det[v0 + v] : rows[(uint64_t)(v0 + v) * vstride + bucket_slot(d, c, C, B)];
   }
 }
 
@@ -466,7 +466,7 @@ __global__ void from_reference_order_kernel(const Bucket16* __restrict__ in, uin
     const uint32_t v = (uint32_t)(i / nb), j = (uint32_t)(i % nb);
     const uint32_t c = j / B, d = j % B;
     if (j == nb - 1) det[v0 + v] = in[i];
-    else rows[(uint64_t)(v0 + v) * vstride + (uint64_t)d * C + c] = in[i];
+    else rows[(uint64_t)(v0 + v) * vstride + bucket_slot(d, c, C, B)] = in[i];
   }
 }
 

@@ -86,12 +86,14 @@ __device__ __forceinline__ bool unite(uint32_t* parent, uint32_t a, uint32_t b) 
 }
 
 // result of sampling: 0 GOOD, 1 ZERO, 2 FAIL
-__device__ __forceinline__ int sample_column(const Bucket16 det, const Bucket16* col, uint32_t stride,
-                                             uint32_t bpc, uint64_t seed, uint64_t* idx) {
+// load(d) reads depth d of the sampled column
+template <class Load>
+__device__ __forceinline__ int sample_column(const Bucket16 det, Load load, uint32_t bpc,
+                                             uint64_t seed, uint64_t* idx) {
   if (det.alpha == 0 && det.gamma == 0) return 1;
   if (det.gamma == xxh3_len8(det.alpha, seed)) { *idx = det.alpha; return 0; }
   for (uint32_t d = 0; d < bpc; d++) {
-    Bucket16 b = col[(size_t)d * stride];
+    Bucket16 b = load(d);
     if (b.gamma == xxh3_len8(b.alpha, seed)) { *idx = b.alpha; return 0; }
   }
   return 2;
@@ -200,15 +202,15 @@ __device__ __forceinline__ int wave_sample(const Bucket16 b, uint32_t lane, uint
   return 0;
 }
 
-// Sample slots of a sketch with cps columns per sample, in storage order: slot j < sw = cps*B of
-// sample r is column r*cps + j % cps at depth j / cps (depth-major like the rows, so the cps
-// columns of one depth are adjacent in memory), slot sw is the deterministic bucket. Offset of
-// slot j inside a vertex's rows, and its position in the scan order of Sketch::sample
-// (src/sketch.cpp:121-128: column by column, depths ascending).
+// Sample slots of a sketch with cps columns per sample: slot j < sw = cps*B of sample r is
+// column r*cps + j % cps at depth j / cps (in the depth rows the cps columns of one depth are
+// adjacent in memory), slot sw is the deterministic bucket. Offset of slot j inside a vertex's
+// rows, and its position in the scan order of Sketch::sample (src/sketch.cpp:121-128: column by
+// column, depths ascending).
 __device__ __forceinline__ size_t slot_offset(const StoreView& st, uint32_t cps, uint32_t r,
                                               uint32_t j) {
   const uint32_t d = j / cps;
-  return (size_t)d * st.cols + (size_t)r * cps + (j - d * cps);
+  return bucket_slot(d, r * cps + (j - d * cps), st.cols, st.bpc);
 }
 __device__ __forceinline__ uint32_t slot_scan_pos(uint32_t cps, uint32_t bpc, uint32_t j) {
   const uint32_t d = j / cps;
@@ -305,9 +307,9 @@ __global__ __launch_bounds__(kQThreads) void round0_wide_kernel(StoreView st, ui
 
 // Round 0: every vertex samples its own column col0 (run_round_zero, src/cc_sketch_alg.cpp:
 // 250-271). A wave takes kR0Verts consecutive vertices, four at a time, one bucket per lane. Each
-// bucket of the column is a 128-byte line of its own (depth-major rows), so the first pass reads
-// the deterministic bucket and depths 0..15 only; a vertex with no good bucket among them reads
-// its deeper depths in a second pass (the scan order stays the reference's).
+// bucket of the column's depth rows is a 128-byte line of its own, so the first pass reads the
+// deterministic bucket and depths 0..15 only; a vertex with no good bucket among them reads its
+// deeper depths in a second pass (the scan order stays the reference's).
 constexpr uint32_t kRound0Depths = 16;
 // (rec_res, rec_idx: the test hook of round0_wide_kernel)
 __global__ __launch_bounds__(kQThreads) void round0_kernel(StoreView st, uint32_t col0,
@@ -328,7 +330,7 @@ __global__ __launch_bounds__(kQThreads) void round0_kernel(StoreView st, uint32_
     for (int q = 0; q < 4; q++) {
       const uint32_t vi = v0 + min(i0 + q, cnt - 1);
       bv[q] = lane > nd    ? Bucket16{0, 0}
-              : lane < nd ? st.rows[(size_t)vi * st.vstride + (size_t)lane * st.cols + col0]
+              : lane < nd ? st.rows[(size_t)vi * st.vstride + bucket_slot(lane, col0, st.cols, st.bpc)]
                           : st.det[vi];
     }
 #pragma unroll
@@ -339,7 +341,7 @@ __global__ __launch_bounds__(kQThreads) void round0_kernel(StoreView st, uint32_
       if (res == 2 && nd < st.bpc) {  // wave-uniform: depths nd..B-1, one per lane
         const uint32_t dd = nd + lane;
         const Bucket16 b2 = dd < st.bpc
-            ? st.rows[(size_t)(v0 + i0 + q) * st.vstride + (size_t)dd * st.cols + col0]
+            ? st.rows[(size_t)(v0 + i0 + q) * st.vstride + bucket_slot(dd, col0, st.cols, st.bpc)]
             : Bucket16{0, 0};
         const uint64_t good = __ballot(dd < st.bpc && b2.gamma == xxh3_len8(b2.alpha, st.seed));
         if (good) {
@@ -370,9 +372,11 @@ __global__ __launch_bounds__(kQThreads) void round0_kernel(StoreView st, uint32_
 // largest first. Their column sums are formed from per-wave partial sums instead of atomics
 // into one accumulator row (which thousands of waves would contend for). When the store's
 // updates were all symmetric (edge pairs: every update XORs the same bucket of both endpoints)
-// the XOR of a column over all vertices is zero, and a supernode holding more than half of the
-// samples is the giant (flags[3]): its sum is the XOR over all vertices outside it, so its
-// members are not even read.
+// the XOR of a column over all vertices is zero, so any supernode's sum is the XOR over all
+// vertices outside it: the supernode holding the most samples is the giant (flags[3]) and its
+// members are not even read, when it holds at least kMinGiant samples and more than the finished
+// supernodes do (those are read for the complement and are skipped otherwise).
+constexpr uint32_t kMinGiant = 8;
 
 __global__ __launch_bounds__(kPickSamples) void pick_big_kernel(uint32_t n,
                                                                 const uint32_t* __restrict__ comp,
@@ -381,13 +385,17 @@ __global__ __launch_bounds__(kPickSamples) void pick_big_kernel(uint32_t n,
   if (flags[kFlagDone]) return;
   __shared__ uint32_t s_r[kPickSamples];
   __shared__ uint32_t s_cand[kPickSamples];
-  __shared__ uint32_t s_ncand;
+  __shared__ uint32_t s_ncand, s_max, s_done;
   const uint32_t t = threadIdx.x;
   const uint32_t v = (uint32_t)(((uint64_t)t * n) / kPickSamples + n / (2 * kPickSamples));
   uint32_t r = comp[v];
   if (vflags[r] & kVDone) r = kNoGiant;
   s_r[t] = r;
-  if (t == 0) s_ncand = 0;
+  if (t == 0) {
+    s_ncand = 0;
+    s_max = 0;
+    s_done = 0;
+  }
   __syncthreads();
   // samples in this supernode, counted at its first sample only; a candidate key orders by
   // count, then by sample position (unique)
@@ -398,12 +406,19 @@ __global__ __launch_bounds__(kPickSamples) void pick_big_kernel(uint32_t n,
     if (same && i < first) first = i;
   }
   const uint32_t key = (r != kNoGiant && first == t && cnt >= kMinBig) ? (cnt << 16) | t : 0u;
-  if (key) s_cand[atomicAdd(&s_ncand, 1u)] = key;  // at most kPickSamples / kMinBig
+  if (key) {
+    s_cand[atomicAdd(&s_ncand, 1u)] = key;  // at most kPickSamples / kMinBig
+    atomicMax(&s_max, key);
+  }
+  if (r == kNoGiant) atomicAdd(&s_done, 1u);
   __syncthreads();
   if (t >= 64) return;
-  // wave 0: bitonic sort of (the first 64 of) the candidates over 64 lanes, largest first. More
-  // than 64 candidates leave no giant (one holds over half of the samples, leaving at most
-  // kPickSamples / 2 / kMinBig - 1 = 63 others), so a giant is always among the sorted ones.
+  // the giant: the candidate of most samples (over all candidates)
+  const uint32_t gkey = s_max;
+  const bool giant = symmetric && gkey && (gkey >> 16) >= kMinGiant && (gkey >> 16) > s_done;
+  const uint32_t groot = giant ? s_r[gkey & 0xFFFFu] : kNoGiant;
+  // wave 0: bitonic sort of (the first 64 of) the candidates over 64 lanes, largest first; the
+  // big supernodes are the first kBig of them other than the giant
   const uint32_t nc = s_ncand;
   uint32_t x = t < nc ? s_cand[t] : 0u;
   for (uint32_t k = 2; k <= 64; k <<= 1)
@@ -413,13 +428,8 @@ __global__ __launch_bounds__(kPickSamples) void pick_big_kernel(uint32_t n,
     }
   if (t >= (uint32_t)kBig) return;
   const uint32_t root = x ? s_r[x & 0xFFFFu] : kNoGiant;
-  if (t == 0) {
-    const bool giant = x && symmetric && (x >> 16) > kPickSamples / 2;
-    flags[3] = giant ? root : kNoGiant;
-    flags[kFlagBig] = giant ? kNoGiant : root;  // the giant is formed from the rest
-  } else {
-    flags[kFlagBig + t] = root;
-  }
+  if (t == 0) flags[3] = groot;
+  flags[kFlagBig + t] = root == groot ? kNoGiant : root;  // the giant is formed from the rest
 }
 
 // Rounds >= 1: the sample sums (sample r: its cps columns, depths 0..B-1, plus the deterministic
@@ -917,8 +927,10 @@ __global__ __launch_bounds__(kQThreads) void g_round0_kernel(
   const GraphDesc d = descs[g];
   const uint32_t v = V - d.base;
   uint64_t idx = 0;
-  const int res = sample_column(d.det[v], d.rows + (size_t)v * d.vstride + col0, d.cols, d.bpc,
-                                d.seed, &idx);
+  const Bucket16* vrow = d.rows + (size_t)v * d.vstride;
+  const int res = sample_column(
+      d.det[v], [&](uint32_t k) { return vrow[bucket_slot(k, col0, d.cols, d.bpc)]; }, d.bpc,
+      d.seed, &idx);
   handle_group_sample(res, idx, V, d, parent, vflags, &gstate[g]);
 }
 
@@ -991,7 +1003,7 @@ __global__ __launch_bounds__(kQThreads) void g_acc_kernel(
   const bool cgrp = cmp && gg == gg0;
   const uint32_t w = rd ? d.bpc + 1 : 0;
   const uint32_t v = rd ? V - d.base : 0;
-  const Bucket16* vrow = rd ? d.rows + (size_t)v * d.vstride + col : nullptr;
+  const Bucket16* vrow = rd ? d.rows + (size_t)v * d.vstride : nullptr;
   Bucket16* a0 = acc + (size_t)r0 * acc_stride;
   Bucket16* ar = acc + (size_t)r * acc_stride;
   Bucket16* c0 = acc + (size_t)gg0 * acc_stride;
@@ -1002,7 +1014,7 @@ __global__ __launch_bounds__(kQThreads) void g_acc_kernel(
     for (int k = 0; k < kAccBatch; k++) {
       const uint32_t j = j0 + k;
       b[k] = Bucket16{0, 0};
-      if (j < w) b[k] = j < d.bpc ? vrow[(size_t)j * d.cols] : d.det[v];
+      if (j < w) b[k] = j < d.bpc ? vrow[bucket_slot(j, col, d.cols, d.bpc)] : d.det[v];
     }
 #pragma unroll
     for (int k = 0; k < kAccBatch; k++) {
@@ -1048,7 +1060,7 @@ __global__ __launch_bounds__(kQThreads) void g_sample_kernel(
   const GraphDesc d = descs[g];
   Bucket16* a = acc + (size_t)V * acc_stride;
   uint64_t idx = 0;
-  const int res = sample_column(a[d.bpc], a, 1, d.bpc, d.seed, &idx);
+  const int res = sample_column(a[d.bpc], [&](uint32_t k) { return a[k]; }, d.bpc, d.seed, &idx);
   for (uint32_t j = 0; j <= d.bpc; j++) a[j] = Bucket16{0, 0};
   handle_group_sample(res, idx, V, d, parent, vflags, &gstate[g]);
 }

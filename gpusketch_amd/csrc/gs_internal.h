@@ -1,10 +1,11 @@
 // Internal definitions shared by the HIP translation units of libgpusketch.
 //
 // Device layout of a sketch store (gfx950-first, see the store section below):
-//   rows : per vertex, bkt_per_col depth rows, each num_columns buckets of {alpha, gamma}
-//          (depth-major, so the low depths that almost every update touches are contiguous and
-//          a wave's 64 lanes flush consecutive 16-byte buckets); the per-vertex stride is padded
-//          to a multiple of 8 buckets (128 bytes) so every vertex starts on an L2 line.
+//   rows : per vertex, bkt_per_col x num_columns buckets of {alpha, gamma}: depths 0..11 as depth
+//          rows of num_columns buckets (the low depths that almost every update touches are
+//          contiguous and a wave's 64 lanes flush consecutive 16-byte buckets), then each
+//          column's deeper depths as one run (bucket_slot); the per-vertex stride is padded to a
+//          multiple of 8 buckets (128 bytes) so every vertex starts on an L2 line.
 //   det  : the deterministic bucket of every vertex, in a separate dense array.
 // Reference numbering (src/sketch.cpp:8-37) is bucket [column*bkt_per_col + depth] with the
 // deterministic bucket last; export/import convert.
@@ -79,9 +80,16 @@ __device__ __forceinline__ uint64_t wave_xor_reduce(uint64_t x) {
 
 #endif
 
-__host__ __device__ __forceinline__ size_t row_index(const StoreView& s, uint32_t v, uint32_t d,
-                                                     uint32_t c) {
-  return (size_t)v * s.vstride + (size_t)d * s.cols + c;
+// Bucket (depth d, column c) inside a vertex's row of B x C buckets. Depths below kRowDepths
+// are depth-major rows of C buckets (the LDS window of the aggregation kernel, where almost every
+// update lands: a flush writes consecutive buckets); the deeper depths of each column follow as
+// one run of B - kRowDepths buckets per column (reached by about 2^-12 of the column updates, and
+// read by a query about 3 lines at a time instead of one 128-byte line per depth). A query reads
+// one column per vertex and round: kRowDepths + 3..4 lines instead of B.
+constexpr uint32_t kRowDepths = 12;
+__host__ __device__ __forceinline__ uint32_t bucket_slot(uint32_t d, uint32_t c, uint32_t cols,
+                                                         uint32_t bpc) {
+  return d < kRowDepths ? d * cols + c : kRowDepths * cols + c * (bpc - kRowDepths) + (d - kRowDepths);
 }
 
 // work item of the aggregation kernel: `len` destinations of vertex `vertex` starting at

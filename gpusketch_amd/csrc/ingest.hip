@@ -660,6 +660,13 @@ __device__ __forceinline__ void lds_xor_u64(uint32_t addr, uint64_t v) {
                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Waves per SIMD the register allocation of aggregate_kernel must allow (the occupancy its
+// register use reaches on gfx950 without spills).
+template <bool kGroup, int T>
+constexpr int agg_min_waves() {
+  return kGroup ? (T <= 3 ? 4 : T <= 7 ? 3 : 2) : (T <= 1 ? 7 : T <= 3 ? 6 : T == 4 ? 5 : T <= 7 ? 4 : 3);
+}
+
 // kGroup: items belong to the members of a group (descs[item.graph]); the LDS delta is sized for
 // the widest member (max_cols) and laid out with the current member's column count, which is
 // reloaded (with its geometry and keys) when a wave moves to an item of another member (the
@@ -668,7 +675,7 @@ __device__ __forceinline__ void lds_xor_u64(uint32_t addr, uint64_t v) {
 // column count is exactly 8 T (every column slot of every step is a column), so the T depth
 // hashes of a lane need no guard and run as independent chains before the LDS updates.
 template <bool kGroup, int T, bool kFull>
-__global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
+__global__ __launch_bounds__(kAggWaves* kWave, (agg_min_waves<kGroup, T>())) void aggregate_kernel(
     StoreView st, const GraphDesc* __restrict__ descs, uint32_t max_cols,
     const WorkItem* __restrict__ items, ItemClasses cls,
     const uint32_t* __restrict__ class_counts, const uint32_t* __restrict__ dsts, uint32_t dl,
@@ -808,6 +815,7 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
     const uint32_t row_bytes = 16u * cols;
     const uint32_t sent = 1u << dli;                 // dli <= 16: within the exact low bits
     const uint32_t cap16 = 1u << (bpc < 16u ? bpc : 16u);
+    const uint32_t dtail = bpc - kRowDepths, dbase = kRowDepths * cols - kRowDepths;
     uint64_t det_a = 0, det_g = 0;
     Bucket16* vrows = st.rows + (size_t)v * st.vstride;
 
@@ -862,7 +870,9 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
                 uint32_t d = (uint32_t)__builtin_ctz(h[t] | cap16);
                 if (d == 16 && bpc > 16) d = depth_of(xxh3_finish(m ^ colk[t]), bpc);
                 if (d < bpc) {
-                  Bucket16* b = &vrows[(size_t)d * cols + c];
+                  // bucket_slot with the deep runs' start folded in: depth rows below
+                  // kRowDepths, then run c of B - kRowDepths buckets
+                  Bucket16* b = &vrows[d < kRowDepths ? d * cols + c : c * dtail + d + dbase];
                   atomic_xor_u64(&b->alpha, a);
                   atomic_xor_u64(&b->gamma, gm);
                   n_deep++;
@@ -891,10 +901,10 @@ __global__ __launch_bounds__(kAggWaves* kWave) void aggregate_kernel(
     // flush, kFlushRounds x 64 slots at a time: all LDS reads first, then the HBM reads of the
     // non-zero slots together, one wait, then the stores (exclusive item: plain read-xor-write;
     // otherwise 64-bit atomics, which need no read). Slot s = depth * cols + column is LDS row
-    // 7 + depth and device bucket s of the vertex. An exclusive item whose window fits one step
-    // leaves its stores pending: they complete after the next item, so the reads' latency is
-    // hidden behind that item's work (the vertex is this item's alone; its deep buckets, written
-    // by atomics, lie outside the window).
+    // 7 + depth and device bucket s of the vertex (the window lies in the depth rows of
+    // bucket_slot). An exclusive item whose window fits one step leaves its stores pending: they
+    // complete after the next item, so the reads' latency is hidden behind that item's work (the
+    // vertex is this item's alone; its deep buckets, written by atomics, lie outside the window).
     Bucket16* drow = reinterpret_cast<Bucket16*>(lds + wbase);
     const bool defer = excl && islots <= kDeferRounds * kWave;
     if (!defer) complete_pending();
@@ -1071,7 +1081,7 @@ __global__ __launch_bounds__(kWideWaves* kWave) void wide_aggregate_kernel(
             if (d == 16 && st.bpc > 16) d = depth_of(xxh3_finish(x), st.bpc);
             if (d < st.bpc) {
               for (uint32_t k = kL0 ? dli : d; k <= d; k++) {
-                Bucket16* b = &vrows[(size_t)k * st.cols + c];
+                Bucket16* b = &vrows[bucket_slot(k, c, st.cols, st.bpc)];
                 atomic_xor_u64(&b->alpha, a);
                 atomic_xor_u64(&b->gamma, gm);
                 n_deep++;
@@ -1110,7 +1120,7 @@ __global__ __launch_bounds__(kWideWaves* kWave) void wide_aggregate_kernel(
         xg = rg;
       }
       if (active && (xa | xg)) {
-        Bucket16* b = &vrows[(size_t)k * st.cols + c];
+        Bucket16* b = &vrows[bucket_slot(k, c, st.cols, st.bpc)];
         atomic_xor_u64(&b->alpha, xa);
         atomic_xor_u64(&b->gamma, xg);
         n_flush++;
@@ -1309,7 +1319,8 @@ static int launch_aggregate(Stream* s, const WorkItem* d_items, const ItemClasse
     std::lock_guard<std::mutex> lk(s->store->mu);
     exclusive_ok = s->store->streams.size() == 1 ? 1 : 0;
   }
-  dl = std::min(dl, kMaxLdsDepths);  // the 16-bit depth needs no exact hash inside the window
+  // the window stays in the depth rows of bucket_slot, where a flush slot is the device bucket
+  dl = std::min(dl, kRowDepths);
   GS_CHECK(cols >= 1 && cols <= 8 * kMaxColSteps, GS_ERR_UNSUPPORTED,
            "sketches with %u columns (more than %d) are not supported by the update kernel", cols,
            8 * kMaxColSteps);

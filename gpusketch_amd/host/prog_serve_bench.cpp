@@ -3,7 +3,7 @@
 //               [--mc-tenants 8] [--k 4] [--min-n 4096] [--max-n 262144]
 //               [--max-edges 1000000] [--alone-check 0|1] [--evict 0|1]
 //               [--max-step-updates N] [--pinned-spill 0|1] [--verify 0|1]
-//               [--cc-frac 0.01] [--mc-frac 0.005] [--pinned-payloads 0|1]
+//               [--cc-frac 0.01] [--mc-frac 0.005] [--pinned-payloads 0|1] [--dump PREFIX]
 // Tenants: n log-uniform in [min-n, max-n]. Requests arrive as a Poisson process of the given
 // rate (requests/s): a cc-frac share of connectivity queries on random tenants, mc-frac of
 // min-cut queries (k) on the first mc-tenants tenants, the rest update batches of log-uniform
@@ -21,6 +21,8 @@
 // before the clock starts and handed over without a copy (SketchServer::submit_update_pinned),
 // as a server that receives requests into pinned buffers would; with 0 the server copies each
 // payload at arrival (submit_update). Both modes of a run take the payloads the same way.
+// --dump PREFIX writes every answered request of each mode to PREFIX_plain.csv and
+// PREFIX_served.csv (trace index, kind, tenant, edges, arrival and answer times, latency, step).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -130,6 +132,27 @@ static double pct(std::vector<double> v, double p) {
 }
 
 static uint64_t g_max_step_updates = 0;  // 0: the server's default
+static std::string g_dump;                 // --dump prefix ("" = none)
+
+static void dump_csv(const RunStats& s, const char* mode) {
+  if (g_dump.empty()) return;
+  const std::string path = g_dump + "_" + mode + ".csv";
+  FILE* f = std::fopen(path.c_str(), "w");
+  if (!f) {
+    std::fprintf(stderr, "[serve] cannot write %s\n", path.c_str());
+    return;
+  }
+  std::fprintf(f, "index,kind,tenant,edges,arrival_ms,done_ms,latency_ms,step\n");
+  for (auto& kv : s.by_index) {
+    const RequestResult& r = kv.second;
+    const char* kind = r.kind == RequestKind::UPDATE ? "update"
+                       : r.kind == RequestKind::CC_QUERY ? "cc" : "mincut";
+    std::fprintf(f, "%llu,%s,%u,%llu,%.4f,%.4f,%.4f,%llu\n", (unsigned long long)kv.first, kind,
+                 r.tenant, (unsigned long long)r.updates, r.arrival_s * 1e3, r.done_s * 1e3,
+                 (r.done_s - r.arrival_s) * 1e3, (unsigned long long)r.step);
+  }
+  std::fclose(f);
+}
 static bool g_evict = true, g_pinned_spill = true;
 
 static RunStats run(const Trace& tr, bool batching, uint64_t budget, uint32_t k,
@@ -340,6 +363,7 @@ int main(int argc, char** argv) {
     else if (a == "--cc-frac") g_cc_frac = std::atof(v);
     else if (a == "--mc-frac") g_mc_frac = std::atof(v);
     else if (a == "--pinned-payloads") g_pinned_payloads = std::atoi(v) != 0;
+    else if (a == "--dump") g_dump = v;
     else {
       std::fprintf(stderr, "unknown option %s\n", a.c_str());
       return 2;
@@ -367,9 +391,11 @@ int main(int argc, char** argv) {
   }
   RunStats plain = run(tr, false, budget, k, {}, true);
   std::fprintf(stderr, "[serve] plain: %.3e updates/s over %.3fs\n", plain.upd_per_s, plain.makespan);
+  dump_csv(plain, "plain");
   RunStats served = run(tr, true, budget, k, {}, true);
   std::fprintf(stderr, "[serve] served: %.3e updates/s over %.3fs in %llu steps\n",
                served.upd_per_s, served.makespan, (unsigned long long)served.steps);
+  dump_csv(served, "served");
   std::string why;
   bool same = same_answers(plain, served, &why);
   bool same_sketch = plain.exports == served.exports;

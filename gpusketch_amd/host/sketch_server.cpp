@@ -397,6 +397,7 @@ void SketchServer::finish(Request& r, RequestResult res) {
   res.kind = r.kind;
   res.arrival_s = r.arrival_s;
   res.done_s = now_s();
+  res.step = num_steps.load();
   std::lock_guard<std::mutex> lk(mu);
   release_payload_locked(r);
   tenants[r.tenant]->queued--;

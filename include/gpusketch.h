@@ -199,7 +199,8 @@ int gs_stream_traffic(gs_stream* s, uint64_t* flushed_buckets, uint64_t* direct_
 
 /* ---- Tuning ----------------------------------------------------------------------------------
  * Launch parameters of a stream's update path (0 = default): lds_depths = depths of every column
- * kept in the aggregation kernel's LDS delta (1..16, default 9), item_updates = destinations per work
+ * kept in the aggregation kernel's LDS delta (1..16, default 9 to 12 by width; at most 12 are used
+ * for sketches of up to 64 columns), item_updates = destinations per work
  * item (64..1024, default 512; 1024 for wide sketches), bucket_bits = vertex-id bits resolved
  * inside one grouping bucket (default log2(n) - 8, at most 13), launch_edges = submitted edge
  * pairs coalesced into one launch of the grouping and aggregation kernels (2^12..2^28; default

@@ -60,6 +60,7 @@ struct RequestResult {
   int status = 0;            // gs_status of the answer (e.g. out of samples)
   uint32_t num_cc = 0;       // CC_QUERY
   uint64_t partition_hash = 0;
+  uint64_t step = 0;         // the scheduling step that was running when it was answered
   uint64_t mincut = 0;       // MINCUT_QUERY: the cut if below k, else k
   std::vector<node_id_t> labels;  // CC_QUERY with keep_labels
 };
