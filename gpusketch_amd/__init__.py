@@ -465,7 +465,9 @@ class StoreGroup:
                                    _p(rounds), _p(status))
         if rc != GS_OK and not (allow_out_of_samples and rc == GS_ERR_OUT_OF_SAMPLES):
             _check(rc)
-        per = [labels[b:b + s.n].copy() for b, s in zip(self.bases, self.stores)]
+        # members outside the mask: the library leaves their entries alone; report identity labels
+        per = [labels[b:b + s.n].copy() if m is None or m[i] else np.arange(s.n, dtype=np.uint32)
+               for i, (b, s) in enumerate(zip(self.bases, self.stores))]
         return per, ncc, rounds, status
 
     def close(self) -> None:

@@ -36,7 +36,6 @@ constexpr int kFlagDone = kFlagBig + kBig;
 constexpr int kFlagRound = kFlagDone + 1;
 constexpr int kFlagRounds = kFlagDone + 2;
 constexpr int kFlagWords = kFlagDone + 4;
-constexpr int kAccBatch = 8;  // buckets loaded together per lane by the group accumulation
 // Vertices per wave of the per-vertex kernels of the single-store query (one lane per vertex for
 // the vertex attributes, the wave's vertices then taken a few at a time, one bucket per lane).
 // Fewer vertices per wave put more waves in flight (64 per wave is 2 waves per SIMD at n = 2^17)
@@ -890,8 +889,16 @@ int k_forests(Store* st, uint32_t k, uint32_t* edges_out, uint64_t edge_cap, uin
 // Rounds >= 1 are one captured HIP graph: the round number lives in device memory, so the same
 // graph replays round after round and the host only reads back whether any member is active.
 // gstate: [0,G) modified this round, [G,2G) status (0 active, 1 done, 2 out of samples),
-// [2G,3G) rounds used, [3G] round counter, [3G+1] any member active after this round.
+// [2G,3G) rounds used, [3G] round counter, [3G+1] any member active after this round, [3G+2] the
+// number M of vertices of the queried members. The per-vertex kernels walk only those: vmap
+// lists their global ids member after member, and a fixed grid strides over [0, M) (M read from
+// device memory), so one captured round serves any selection of members and a query of a few
+// small members costs about what their own single-store queries would.
 // ------------------------------------------------------------------------------------------
+constexpr uint32_t kSelGrid = 2048;  // workgroups of the per-vertex kernels of a group query
+__device__ __forceinline__ uint32_t sel_count(const uint32_t* gstate, uint32_t G) {
+  return gstate[3 * G + 2];
+}
 constexpr uint32_t kGActive = 0, kGDone = 1, kGOutOfSamples = 2;
 
 __device__ __forceinline__ void handle_group_sample(int res, uint64_t idx, uint32_t self,
@@ -917,13 +924,36 @@ __global__ void g_begin_kernel(uint32_t* gstate, uint32_t G) {
   if (threadIdx.x == 0 && blockIdx.x == 0) gstate[3 * G + 1] = 0;
 }
 
+// the queried members' vertices of this thread (grid-stride over [0, M))
+#define FOR_SELECTED(V, vmap, gstate, G)                                                        \
+  for (uint32_t _t = blockIdx.x * blockDim.x + threadIdx.x, _m = sel_count(gstate, G); _t < _m; \
+       _t += gridDim.x * blockDim.x)                                                            \
+    if (const uint32_t V = vmap[_t]; true)
+
+__global__ void g_init_kernel(const uint32_t* __restrict__ vmap, uint32_t G,
+                              const uint32_t* __restrict__ gstate, uint32_t* parent,
+                              uint32_t* comp, uint32_t* vflags) {
+  FOR_SELECTED(V, vmap, gstate, G) {
+    parent[V] = V;
+    comp[V] = V;
+    vflags[V] = 0;
+  }
+}
+
+// vmap: block row j lists queried member j's vertices (sel: its base and first position)
+__global__ void g_vmap_kernel(const uint32_t* __restrict__ sel, uint32_t* __restrict__ vmap) {
+  const uint32_t j = blockIdx.x, base = sel[2 * j], p0 = sel[2 * j + 1], p1 = sel[2 * j + 3];
+  for (uint32_t t = blockIdx.y * blockDim.x + threadIdx.x; p0 + t < p1; t += gridDim.y * blockDim.x)
+    vmap[p0 + t] = base + t;
+}
+
 __global__ __launch_bounds__(kQThreads) void g_round0_kernel(
-    const GraphDesc* __restrict__ descs, const uint32_t* __restrict__ vgraph, uint32_t N,
-    uint32_t G, uint32_t col0, uint32_t* parent, uint32_t* vflags, uint32_t* gstate) {
-  const uint32_t V = blockIdx.x * blockDim.x + threadIdx.x;
-  if (V >= N) return;
+    const GraphDesc* __restrict__ descs, const uint32_t* __restrict__ vgraph,
+    const uint32_t* __restrict__ vmap, uint32_t G, uint32_t col0, uint32_t* parent,
+    uint32_t* vflags, uint32_t* gstate) {
+  FOR_SELECTED(V, vmap, gstate, G) {
   const uint32_t g = vgraph[V];
-  if (gstate[G + g] != kGActive) return;
+  if (gstate[G + g] != kGActive) continue;
   const GraphDesc d = descs[g];
   const uint32_t v = V - d.base;
   uint64_t idx = 0;
@@ -932,6 +962,7 @@ __global__ __launch_bounds__(kQThreads) void g_round0_kernel(
       d.det[v], [&](uint32_t k) { return vrow[bucket_slot(k, col0, d.cols, d.bpc)]; }, d.bpc,
       d.seed, &idx);
   handle_group_sample(res, idx, V, d, parent, vflags, &gstate[g]);
+  }
 }
 
 // Per member and round: the supernode whose sum is formed from the rest of the member (the
@@ -964,105 +995,145 @@ __global__ void g_pick_giant_kernel(const GraphDesc* __restrict__ descs, uint32_
   giant[g] = pick;
 }
 
-// the group form of acc_kernel: lanes are global vertices of possibly different members. A
-// vertex of a live supernode adds its column into that supernode's sum; with a giant picked for
-// its member (g_pick_giant_kernel), a vertex inside it is not read at all and every other vertex
-// of the member (finished supernodes included) also adds into the giant's sum. Lanes sharing the
-// wave's first root (or first giant) are XOR-reduced into one atomic per bucket.
+// The group form of acc_kernel. A wave takes 64 consecutive queried vertices (lane = vertex for
+// their attributes) and walks the ones it must read, four at a time, one bucket per lane (bucket
+// j < B of the round's column, lane B the deterministic bucket: one vertex is one load instruction
+// and its row of atomics one instruction over consecutive addresses). A vertex of a live
+// supernode adds its column into that supernode's sum; with a giant picked for its member
+// (g_pick_giant_kernel), a vertex inside it is not read at all and every other vertex of the
+// member (finished supernodes included) also adds into the giant's sum. Both sums are held in
+// registers while consecutive vertices share the target (the giant: every read vertex of the
+// member) and go out as one row of atomics when it changes.
 __global__ __launch_bounds__(kQThreads) void g_acc_kernel(
-    const GraphDesc* __restrict__ descs, const uint32_t* __restrict__ vgraph, uint32_t N,
-    uint32_t G, const uint32_t* __restrict__ comp, const uint32_t* __restrict__ vflags,
-    const uint32_t* __restrict__ gstate, const uint32_t* __restrict__ giant,
-    Bucket16* __restrict__ acc, uint32_t acc_stride, uint32_t max_w) {
+    const GraphDesc* __restrict__ descs, const uint32_t* __restrict__ vgraph,
+    const uint32_t* __restrict__ vmap, uint32_t G, const uint32_t* __restrict__ comp,
+    const uint32_t* __restrict__ vflags, const uint32_t* __restrict__ gstate,
+    const uint32_t* __restrict__ giant, Bucket16* __restrict__ acc, uint32_t acc_stride) {
   const uint32_t col = gstate[3 * G];
-  const uint32_t V = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63;
-  bool inm = V < N;
-  uint32_t g = 0, r = 0, gg = kNoGiant;
-  bool done = false;
-  if (inm) {
-    g = vgraph[V];
-    inm = gstate[G + g] == kGActive;
-  }
-  if (inm) {
-    r = comp[V];
-    done = (vflags[r] & kVDone) != 0;
-    gg = giant[g];
-  }
-  const bool ingiant = inm && gg != kNoGiant && r == gg;
-  const bool act = inm && !done && !ingiant;         // into its own supernode's sum
-  const bool cmp = inm && gg != kNoGiant && !ingiant;  // into its member's giant
-  const bool rd = act || cmp;
-  GraphDesc d{};
-  if (rd) d = descs[g];
-  const uint64_t am = __ballot(act), cm = __ballot(cmp);
-  if ((am | cm) == 0) return;  // wave-uniform
-  const uint32_t r0 = am ? (uint32_t)__shfl((int)r, __builtin_ctzll(am)) : 0u;
-  const uint32_t gg0 = cm ? (uint32_t)__shfl((int)gg, __builtin_ctzll(cm)) : 0u;
-  const bool grp = act && r == r0;
-  const bool cgrp = cmp && gg == gg0;
-  const uint32_t w = rd ? d.bpc + 1 : 0;
-  const uint32_t v = rd ? V - d.base : 0;
-  const Bucket16* vrow = rd ? d.rows + (size_t)v * d.vstride : nullptr;
-  Bucket16* a0 = acc + (size_t)r0 * acc_stride;
-  Bucket16* ar = acc + (size_t)r * acc_stride;
-  Bucket16* c0 = acc + (size_t)gg0 * acc_stride;
-  Bucket16* cr = acc + (size_t)(cmp ? gg : 0u) * acc_stride;
-  for (uint32_t j0 = 0; j0 < max_w; j0 += kAccBatch) {
-    Bucket16 b[kAccBatch];
-#pragma unroll
-    for (int k = 0; k < kAccBatch; k++) {
-      const uint32_t j = j0 + k;
-      b[k] = Bucket16{0, 0};
-      if (j < w) b[k] = j < d.bpc ? vrow[bucket_slot(j, col, d.cols, d.bpc)] : d.det[v];
+  const uint32_t M = sel_count(gstate, G);
+  // running sums: of supernode sr (row width sw) and of giant gr (row width gw)
+  uint32_t sr = kNoGiant, sw = 0, gr = kNoGiant, gw = 0;
+  uint64_t sa = 0, sg = 0, ga = 0, gg = 0;
+  auto flush_row = [&](uint32_t row, uint32_t w, uint64_t& a, uint64_t& g) {
+    if (row != kNoGiant && lane < w && (a | g)) {
+      Bucket16* dst = acc + (size_t)row * acc_stride + lane;
+      __hip_atomic_fetch_xor(&dst->alpha, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_xor(&dst->gamma, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    a = g = 0;
+  };
+  // whole waves stride over [0, M) (the ballots below need every lane of the wave)
+  for (uint32_t t0 = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; t0 < M;
+       t0 += gridDim.x * blockDim.x) {
+    bool inm = t0 + lane < M;
+    const uint32_t V = inm ? vmap[t0 + lane] : 0u;
+    uint32_t g = 0, r = 0, gt = kNoGiant;
+    bool done = false;
+    if (inm) {
+      g = vgraph[V];
+      inm = gstate[G + g] == kGActive;
+    }
+    if (inm) {
+      r = comp[V];
+      done = (vflags[r] & kVDone) != 0;
+      gt = giant[g];
+    }
+    const bool ingiant = inm && gt != kNoGiant && r == gt;
+    const bool act = inm && !done && !ingiant;         // into its own supernode's sum
+    const bool cmp = inm && gt != kNoGiant && !ingiant;  // into its member's giant
+    const uint64_t actm = __ballot(act), cmpm = __ballot(cmp);
+    uint64_t m = actm | cmpm;
+    while (m) {
+      uint32_t ii[4];
 #pragma unroll
-    for (int k = 0; k < kAccBatch; k++) {
-      const uint32_t j = j0 + k;
-      if (j >= max_w) break;
-      const bool nz = (b[k].alpha | b[k].gamma) != 0;
-      if (am) {
-        const uint64_t xa = wave_xor_reduce(grp ? b[k].alpha : 0);
-        const uint64_t xg = wave_xor_reduce(grp ? b[k].gamma : 0);
-        if (lane == 0 && (xa | xg)) {
-          __hip_atomic_fetch_xor(&a0[j].alpha, xa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __hip_atomic_fetch_xor(&a0[j].gamma, xg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (int q = 0; q < 4; q++) {
+        ii[q] = m ? (uint32_t)__builtin_ctzll(m) : ii[0];
+        m &= m ? m - 1 : 0;
+      }
+      Bucket16 bv[4];
+      uint32_t wq[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const uint32_t Vi = __builtin_amdgcn_readlane((int)V, ii[q]);
+        const uint32_t gi = __builtin_amdgcn_readlane((int)g, ii[q]);
+        const GraphDesc& d = descs[gi];
+        const uint32_t v = Vi - d.base, bpc = d.bpc;
+        wq[q] = bpc + 1;
+        bv[q] = lane < bpc    ? d.rows[(size_t)v * d.vstride + bucket_slot(lane, col, d.cols, bpc)]
+                : lane == bpc ? d.det[v]
+                              : Bucket16{0, 0};
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        if (q > 0 && ii[q] == ii[0]) break;  // padding of the last group
+        const uint32_t i = ii[q];
+        if ((actm >> i) & 1) {
+          const uint32_t ri = __builtin_amdgcn_readlane((int)r, i);
+          if (ri != sr) {
+            flush_row(sr, sw, sa, sg);
+            sr = ri;
+            sw = wq[q];
+          }
+          sa ^= bv[q].alpha;
+          sg ^= bv[q].gamma;
         }
-        if (act && !grp && nz) {
-          __hip_atomic_fetch_xor(&ar[j].alpha, b[k].alpha, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __hip_atomic_fetch_xor(&ar[j].gamma, b[k].gamma, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((cmpm >> i) & 1) {
+          const uint32_t gi = __builtin_amdgcn_readlane((int)gt, i);
+          if (gi != gr) {
+            flush_row(gr, gw, ga, gg);
+            gr = gi;
+            gw = wq[q];
+          }
+          ga ^= bv[q].alpha;
+          gg ^= bv[q].gamma;
         }
       }
-      if (cm) {
-        const uint64_t xa = wave_xor_reduce(cgrp ? b[k].alpha : 0);
-        const uint64_t xg = wave_xor_reduce(cgrp ? b[k].gamma : 0);
-        if (lane == 0 && (xa | xg)) {
-          __hip_atomic_fetch_xor(&c0[j].alpha, xa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __hip_atomic_fetch_xor(&c0[j].gamma, xg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        if (cmp && !cgrp && nz) {
-          __hip_atomic_fetch_xor(&cr[j].alpha, b[k].alpha, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          __hip_atomic_fetch_xor(&cr[j].gamma, b[k].gamma, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-      }
     }
   }
+  flush_row(sr, sw, sa, sg);
+  flush_row(gr, gw, ga, gg);
 }
 
 __global__ __launch_bounds__(kQThreads) void g_sample_kernel(
-    const GraphDesc* __restrict__ descs, const uint32_t* __restrict__ vgraph, uint32_t N,
-    uint32_t G, const uint32_t* __restrict__ comp, Bucket16* __restrict__ acc, uint32_t acc_stride,
-    uint32_t* parent, uint32_t* vflags, uint32_t* gstate) {
-  const uint32_t V = blockIdx.x * blockDim.x + threadIdx.x;
-  if (V >= N || comp[V] != V || (vflags[V] & kVDone)) return;
+    const GraphDesc* __restrict__ descs, const uint32_t* __restrict__ vgraph,
+    const uint32_t* __restrict__ vmap, uint32_t G, const uint32_t* __restrict__ comp,
+    Bucket16* __restrict__ acc, uint32_t acc_stride, uint32_t* parent, uint32_t* vflags,
+    uint32_t* gstate) {
+  FOR_SELECTED(V, vmap, gstate, G) {
+  if (comp[V] != V || (vflags[V] & kVDone)) continue;
   const uint32_t g = vgraph[V];
-  if (gstate[G + g] != kGActive) return;
+  if (gstate[G + g] != kGActive) continue;
   const GraphDesc d = descs[g];
   Bucket16* a = acc + (size_t)V * acc_stride;
   uint64_t idx = 0;
   const int res = sample_column(a[d.bpc], [&](uint32_t k) { return a[k]; }, d.bpc, d.seed, &idx);
   for (uint32_t j = 0; j <= d.bpc; j++) a[j] = Bucket16{0, 0};
   handle_group_sample(res, idx, V, d, parent, vflags, &gstate[g]);
+  }
+}
+
+// compress_kernel and finalize_parent_kernel over the queried members' vertices
+__global__ __launch_bounds__(kQThreads) void g_compress_kernel(const uint32_t* __restrict__ vmap,
+                                                               uint32_t G,
+                                                               const uint32_t* __restrict__ gstate,
+                                                               uint32_t* parent, uint32_t* comp,
+                                                               uint32_t* vflags) {
+  FOR_SELECTED(v, vmap, gstate, G) {
+    const uint32_t r = find_root(parent, v);
+    comp[v] = r;
+    if (r == v) {
+      const uint32_t f = vflags[v];
+      vflags[v] = ((f & (kVZero | kVDone)) && !(f & kVMerged)) ? kVDone : 0u;
+    }
+  }
+}
+__global__ __launch_bounds__(kQThreads) void g_finalize_kernel(const uint32_t* __restrict__ vmap,
+                                                               uint32_t G,
+                                                               const uint32_t* __restrict__ gstate,
+                                                               uint32_t* parent,
+                                                               const uint32_t* comp) {
+  FOR_SELECTED(v, vmap, gstate, G) parent[v] = comp[v];
 }
 
 // after a round: settle every active member and advance the round counter (one workgroup)
@@ -1095,8 +1166,8 @@ void free_group_query(Group* g) {
   (void)hipFree(g->d_gstate);
   (void)hipFree(g->d_acc);
   (void)hipFree(g->d_giant);
-  (void)hipFree(g->d_gsym);
-  g->d_giant = g->d_gsym = nullptr;
+  (void)hipFree(g->d_vmap);
+  g->d_giant = g->d_gsym = g->d_vmap = nullptr;  // d_gsym lies inside d_gstate
   if (g->h_gstate) (void)hipHostFree(g->h_gstate);
   if (g->round_exec) (void)hipGraphExecDestroy(g->round_exec);
   g->d_parent = g->d_comp = g->d_vflags = g->d_gstate = g->h_gstate = nullptr;
@@ -1106,22 +1177,27 @@ void free_group_query(Group* g) {
 
 // the kernels of one round >= 1, launched on `q` (eagerly or under capture)
 static void launch_group_round(Group* g, hipStream_t q) {
-  const uint32_t N = g->n_total, G = (uint32_t)g->members.size();
-  const dim3 grid((N + kQThreads - 1) / kQThreads), block(kQThreads);
+  const uint32_t G = (uint32_t)g->members.size();
+  const dim3 grid(kSelGrid), block(kQThreads);
   hipLaunchKernelGGL(g_begin_kernel, dim3(1), dim3(64), 0, q, g->d_gstate, G);
   hipLaunchKernelGGL(g_pick_giant_kernel, dim3((G + 255) / 256), dim3(256), 0, q, g->d_descs, G,
                      g->d_comp, g->d_vflags, g->d_gsym, g->d_gstate, g->d_giant);
-  hipLaunchKernelGGL(g_acc_kernel, grid, block, 0, q, g->d_descs, g->d_vgraph, N, G, g->d_comp,
-                     g->d_vflags, g->d_gstate, g->d_giant, g->d_acc, (uint32_t)g->acc_stride,
-                     g->max_bpc + 1);
-  hipLaunchKernelGGL(g_sample_kernel, grid, block, 0, q, g->d_descs, g->d_vgraph, N, G, g->d_comp,
-                     g->d_acc, (uint32_t)g->acc_stride, g->d_parent, g->d_vflags, g->d_gstate);
-  hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, N, g->d_parent, g->d_comp, g->d_vflags,
-                     nullptr);
+  hipLaunchKernelGGL(g_acc_kernel, grid, block, 0, q, g->d_descs, g->d_vgraph, g->d_vmap, G,
+                     g->d_comp, g->d_vflags, g->d_gstate, g->d_giant, g->d_acc,
+                     (uint32_t)g->acc_stride);
+  hipLaunchKernelGGL(g_sample_kernel, grid, block, 0, q, g->d_descs, g->d_vgraph, g->d_vmap, G,
+                     g->d_comp, g->d_acc, (uint32_t)g->acc_stride, g->d_parent, g->d_vflags,
+                     g->d_gstate);
+  hipLaunchKernelGGL(g_compress_kernel, grid, block, 0, q, g->d_vmap, G, g->d_gstate, g->d_parent,
+                     g->d_comp, g->d_vflags);
   hipLaunchKernelGGL(g_settle_kernel, dim3(1), dim3(1024), 0, q, g->d_descs, G, g->d_gstate);
-  hipLaunchKernelGGL(finalize_parent_kernel, grid, block, 0, q, N, g->d_parent, g->d_comp,
-                     nullptr);
+  hipLaunchKernelGGL(g_finalize_kernel, grid, block, 0, q, g->d_vmap, G, g->d_gstate, g->d_parent,
+                     g->d_comp);
 }
+
+// words of d_gstate / h_gstate: the state (3G + 3), the selection (2 per member + 2), then
+// per member whether its vertex sketches XOR to zero (d_gsym points there)
+static size_t gstate_words(uint32_t G) { return 3 * (size_t)G + 3 + 2 * ((size_t)G + 1) + G; }
 
 // the batched query's scratch and its captured round, made once per group
 int prepare_group_query(Group* g) {
@@ -1132,12 +1208,13 @@ int prepare_group_query(Group* g) {
     GS_HIP(hipMalloc(&g->d_parent, sizeof(uint32_t) * N));
     GS_HIP(hipMalloc(&g->d_comp, sizeof(uint32_t) * N));
     GS_HIP(hipMalloc(&g->d_vflags, sizeof(uint32_t) * N));
-    GS_HIP(hipMalloc(&g->d_gstate, sizeof(uint32_t) * (3 * (size_t)G + 2)));
-    GS_HIP(hipHostMalloc((void**)&g->h_gstate, sizeof(uint32_t) * (3 * (size_t)G + 2),
+    GS_HIP(hipMalloc(&g->d_gstate, sizeof(uint32_t) * gstate_words(G)));
+    GS_HIP(hipHostMalloc((void**)&g->h_gstate, sizeof(uint32_t) * gstate_words(G),
                          hipHostMallocDefault));
+    GS_HIP(hipMalloc(&g->d_vmap, sizeof(uint32_t) * N));
     GS_HIP(hipMalloc(&g->d_acc, sizeof(Bucket16) * (size_t)N * g->acc_stride));
     GS_HIP(hipMalloc(&g->d_giant, sizeof(uint32_t) * G));
-    GS_HIP(hipMalloc(&g->d_gsym, sizeof(uint32_t) * G));
+    g->d_gsym = g->d_gstate + 3 * (size_t)G + 3 + 2 * ((size_t)G + 1);  // part of d_gstate
     hipLaunchKernelGGL(zero_acc_kernel, dim3(2048), dim3(256), 0, q, g->d_acc,
                        (uint64_t)N * g->acc_stride);
     GS_HIP(hipGetLastError());
@@ -1156,30 +1233,46 @@ int prepare_group_query(Group* g) {
 
 int group_query_cc(Group* g, const uint8_t* mask, uint32_t* labels, uint32_t* num_cc,
                    uint32_t* rounds, int32_t* status) {
-  const uint32_t N = g->n_total, G = (uint32_t)g->members.size();
+  const uint32_t G = (uint32_t)g->members.size();
   hipStream_t q = g->s->stream;
   int prc = prepare_group_query(g);
   if (prc) return prc;
-  // which members' sketches XOR to zero (the giant of such a member is summed from the rest)
-  std::vector<uint32_t> sym(G);
-  for (uint32_t i = 0; i < G; i++) sym[i] = g->members[i]->symmetric.load() ? 1u : 0u;
-  GS_HIP(hipMemcpy(g->d_gsym, sym.data(), sizeof(uint32_t) * G, hipMemcpyHostToDevice));
-  // host-initialised state: nothing modified, members outside the mask already done, round 0
-  std::vector<uint32_t> init(3 * (size_t)G + 2, 0);
-  for (uint32_t i = 0; i < G; i++) init[G + i] = (mask && !mask[i]) ? kGDone : kGActive;
-  std::memcpy(g->h_gstate, init.data(), init.size() * sizeof(uint32_t));
-  GS_HIP(hipMemcpyAsync(g->d_gstate, g->h_gstate, init.size() * sizeof(uint32_t),
+  // host-initialised state: nothing modified, members outside the mask already done, round 0;
+  // the queried members' bases and positions in vmap; which members' sketches XOR to zero (the
+  // giant of such a member is summed from the rest)
+  std::fill(g->h_gstate, g->h_gstate + gstate_words(G), 0u);
+  uint32_t* sel = g->h_gstate + 3 * (size_t)G + 3;
+  uint32_t* sym = sel + 2 * ((size_t)G + 1);
+  uint32_t nsel = 0, M = 0;
+  for (uint32_t i = 0; i < G; i++) {
+    sym[i] = g->members[i]->symmetric.load() ? 1u : 0u;
+    const bool on = !mask || mask[i];
+    g->h_gstate[G + i] = on ? kGActive : kGDone;
+    if (!on) continue;
+    sel[2 * nsel] = g->bases[i];
+    sel[2 * nsel + 1] = M;
+    M += g->members[i]->view.n;
+    nsel++;
+  }
+  sel[2 * nsel + 1] = M;
+  g->h_gstate[3 * G + 2] = M;
+  GS_HIP(hipMemcpyAsync(g->d_gstate, g->h_gstate, gstate_words(G) * sizeof(uint32_t),
                         hipMemcpyHostToDevice, q));
-  const dim3 grid((N + kQThreads - 1) / kQThreads), block(kQThreads);
-  hipLaunchKernelGGL(init_query_kernel, grid, block, 0, q, N, g->d_parent, g->d_comp, g->d_vflags);
+  if (nsel) {
+    hipLaunchKernelGGL(g_vmap_kernel, dim3(nsel, 16), dim3(256), 0, q, g->d_gstate + 3 * G + 3,
+                       g->d_vmap);
+  }
+  const dim3 grid(kSelGrid), block(kQThreads);
+  hipLaunchKernelGGL(g_init_kernel, grid, block, 0, q, g->d_vmap, G, g->d_gstate, g->d_parent,
+                     g->d_comp, g->d_vflags);
   hipLaunchKernelGGL(g_begin_kernel, dim3(1), dim3(64), 0, q, g->d_gstate, G);
-  hipLaunchKernelGGL(g_round0_kernel, grid, block, 0, q, g->d_descs, g->d_vgraph, N, G, 0u,
+  hipLaunchKernelGGL(g_round0_kernel, grid, block, 0, q, g->d_descs, g->d_vgraph, g->d_vmap, G, 0u,
                      g->d_parent, g->d_vflags, g->d_gstate);
-  hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, N, g->d_parent, g->d_comp, g->d_vflags,
-                     nullptr);
+  hipLaunchKernelGGL(g_compress_kernel, grid, block, 0, q, g->d_vmap, G, g->d_gstate, g->d_parent,
+                     g->d_comp, g->d_vflags);
   hipLaunchKernelGGL(g_settle_kernel, dim3(1), dim3(1024), 0, q, g->d_descs, G, g->d_gstate);
-  hipLaunchKernelGGL(finalize_parent_kernel, grid, block, 0, q, N, g->d_parent, g->d_comp,
-                     nullptr);
+  hipLaunchKernelGGL(g_finalize_kernel, grid, block, 0, q, g->d_vmap, G, g->d_gstate, g->d_parent,
+                     g->d_comp);
   GS_HIP(hipGetLastError());
   uint32_t max_samples = 0;
   for (Store* m : g->members) max_samples = std::max(max_samples, m->num_samples);
@@ -1205,8 +1298,7 @@ int group_query_cc(Group* g, const uint8_t* mask, uint32_t* labels, uint32_t* nu
   int rc = GS_OK;
   for (uint32_t i = 0; i < G; i++) {
     const uint32_t base = g->bases[i], n = g->members[i]->view.n;
-    if (mask && !mask[i]) {  // not queried: identity labels
-      for (uint32_t v = 0; v < n; v++) labels[base + v] = v;
+    if (mask && !mask[i]) {  // not queried: its labels are left as they are
       if (num_cc) num_cc[i] = n;
       if (rounds) rounds[i] = 0;
       if (status) status[i] = GS_OK;

@@ -262,12 +262,16 @@ struct Group {
   uint32_t* d_comp = nullptr;
   uint32_t* d_vflags = nullptr;
   uint32_t* d_gstate = nullptr;  // [0,G) modified this round, [G,2G) status, [2G,3G) rounds,
-                                 // [3G] round counter, [3G+1] any member still active
+                                 // [3G] round counter, [3G+1] any member still active,
+                                 // [3G+2] vertices of the queried members, then per queried
+                                 // member (base, first position in d_vmap), and the total
+  uint32_t* d_vmap = nullptr;    // the queried members' global vertices, member after member
   uint32_t* h_gstate = nullptr;  // pinned mirror
   Bucket16* d_acc = nullptr;
   uint64_t acc_stride = 0;       // buckets per vertex in d_acc (max_bpc + 1)
   uint32_t* d_giant = nullptr;   // per member, each round: the supernode summed from the rest
   uint32_t* d_gsym = nullptr;    // per member, each query: its vertex sketches XOR to zero
+                                 // (inside d_gstate)
   hipGraphExec_t round_exec = nullptr;  // one captured Boruvka round over every member
 };
 

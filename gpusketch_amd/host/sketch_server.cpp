@@ -1,7 +1,9 @@
 // SketchServer (see sketch_server.h): admission, arrival-ordered scheduling, continuous batching
 // of many tenants' updates into one store-group submission and of their connectivity queries
-// into one batched Boruvka. Every device call is made from the server's worker thread; the host
-// part of a min-cut query (the exact cut of k forests) runs on a second thread meanwhile.
+// into one batched Boruvka. The worker thread makes the device calls of updates and connectivity
+// queries; min-cut queries go to a second thread (the cutter): with batching it computes the k
+// forests on the device and the exact cut of their union while the worker serves the other
+// tenants, without batching only the exact cut.
 #include "gpusketch/sketch_server.h"
 
 #include <algorithm>
@@ -301,7 +303,8 @@ void SketchServer::give_back(SpillBuf& b) {
 // spills `evict` to host memory, then gives `need` a store (restored from its spill, or new)
 void SketchServer::make_resident(uint32_t need, const std::vector<uint32_t>& evict) {
   if (!evict.empty() && group) {
-    // drop the group before the stores it refers to
+    // drop the group before the stores it refers to (once no forests are being computed)
+    std::unique_lock<std::shared_mutex> glk(group_mu);
     gs_throw_if(gs_group_destroy(group));
     group = nullptr;
     group_members.clear();
@@ -373,7 +376,10 @@ void SketchServer::ensure_group() {
       }
   }
   if (group && members == group_members) return;
-  if (group) gs_throw_if(gs_group_destroy(group));
+  if (group) {
+    std::unique_lock<std::shared_mutex> glk(group_mu);  // no forests being computed
+    gs_throw_if(gs_group_destroy(group));
+  }
   group = nullptr;
   group_members = members;
   if (!members.empty()) {
@@ -471,7 +477,9 @@ void SketchServer::run_step(std::vector<Request>& step) {
       for (Request* r : cc) mask[slot[r->tenant]] = 1;
       uint32_t total = 0, base = 0;
       gs_throw_if(gs_group_info(group, 0, &base, &total));
-      std::vector<uint32_t> labels(total), ncc(G), rounds(G);
+      if (cc_labels.size() < total) cc_labels.resize(total);  // kept across steps
+      std::vector<uint32_t>& labels = cc_labels;
+      std::vector<uint32_t> ncc(G), rounds(G);
       std::vector<int32_t> status(G);
       const int rc = gs_group_query_cc(group, mask.data(), labels.data(), ncc.data(),
                                        rounds.data(), status.data());
@@ -504,11 +512,21 @@ void SketchServer::run_step(std::vector<Request>& step) {
     }
   }
   const double t_mc = now_s();
-  // 3. min-cut queries: k spanning forests on the device; the exact cut of their union is a
-  //    host computation of up to tens of milliseconds, done by the cutter thread meanwhile
+  // 3. min-cut queries: k spanning forests on the device and the exact cut of their union (a host
+  //    computation of up to tens of milliseconds). With batching both go to the cutter thread and
+  //    the tenant's later requests wait for it (its store must not change under the forests);
+  //    otherwise the forests are computed here and only the cut goes to the cutter.
   for (Request* r : mc) {
     Tenant& t = *tenants[r->tenant];
     const uint32_t n = t.spec.num_vertices, k = std::max<uint32_t>(1, r->k);
+    if (cfg.batching) {
+      std::lock_guard<std::mutex> lk(mu);
+      t.mc_busy = true;
+      step_handed.push_back(r->id);
+      cut_jobs.push_back(CutJob{*r, n, k, {}, 0, t.store});
+      cut_cv.notify_one();
+      continue;
+    }
     std::vector<uint32_t> edges(2 * (size_t)k * n);
     std::vector<uint64_t> counts(k);
     RequestResult res;
@@ -549,10 +567,36 @@ void SketchServer::cut_loop() {
       cut_jobs.pop_front();
     }
     RequestResult res;
+    if (job.store) {
+      // the forests, on the tenant's store: its sync drains the group's stream, which the worker
+      // must not destroy meanwhile
+      std::shared_lock<std::shared_mutex> glk(group_mu);
+      job.edges.resize(2 * (size_t)job.k * job.n);
+      std::vector<uint64_t> counts(job.k);
+      res.status = gs_k_spanning_forests(job.store, job.k, job.edges.data(),
+                                         (uint64_t)job.k * job.n, counts.data());
+      job.m = 0;
+      for (uint64_t c : counts) job.m += c;
+      if (res.status != GS_OK) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (res.status != GS_ERR_OUT_OF_SAMPLES) last_error = gs_last_error();
+      }
+    }
     uint64_t value = 0;
-    res.status = gs_min_cut(job.n, job.edges.data(), job.m, &value, nullptr);
-    if (res.status == GS_OK) res.mincut = std::min<uint64_t>(value, job.k);
+    if (res.status == GS_OK) {
+      res.status = gs_min_cut(job.n, job.edges.data(), job.m, &value, nullptr);
+      if (res.status == GS_OK) res.mincut = std::min<uint64_t>(value, job.k);
+    }
+    const uint32_t tenant = job.req.tenant;
+    const bool forests = job.store != nullptr;
     finish(job.req, std::move(res));
+    if (forests) {
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        tenants[tenant]->mc_busy = false;
+      }
+      cv.notify_all();  // the tenant's later requests can run
+    }
     idle_cv.notify_all();
   }
 }
@@ -566,7 +610,7 @@ void SketchServer::loop() {
       std::unique_lock<std::mutex> lk(mu);
       auto runnable = [&] {
         for (auto& r : queue)
-          if (tenants[r.tenant]->admitted) return true;
+          if (tenants[r.tenant]->admitted && !tenants[r.tenant]->mc_busy) return true;
         return false;
       };
       auto removable = [&] {
@@ -591,7 +635,7 @@ void SketchServer::loop() {
         uint64_t nupd = 0;
         for (auto it = queue.begin(); it != queue.end();) {
           const uint32_t t = it->tenant;
-          if (!tenants[t]->admitted || blocked[t]) {
+          if (!tenants[t]->admitted || tenants[t]->mc_busy || blocked[t]) {
             ++it;
             continue;
           }
@@ -611,8 +655,11 @@ void SketchServer::loop() {
       step_handed.clear();
     }
     if (!release.empty()) {
-      // drop the group before the stores it refers to
-      if (group) gs_throw_if(gs_group_destroy(group));
+      // drop the group before the stores it refers to (once no forests are being computed)
+      if (group) {
+        std::unique_lock<std::shared_mutex> glk(group_mu);
+        gs_throw_if(gs_group_destroy(group));
+      }
       group = nullptr;
       group_members.clear();
       std::lock_guard<std::mutex> lk(mu);

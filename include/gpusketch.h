@@ -260,9 +260,11 @@ int gs_group_submit_pairs_device(gs_group* g, const uint32_t* d_pairs, uint64_t 
  * Call after gs_stream_set_tuning on the group's stream. No counterpart in the reference. */
 int gs_group_prepare(gs_group* g);
 /* Connected components of the members selected by member_mask (NULL: all) in one batched
- * Borůvka whose rounds >= 1 replay a captured HIP graph. labels: total entries, member-local
- * smallest vertex of each component; num_cc / rounds / status: one entry per member, status
- * GS_OK or GS_ERR_OUT_OF_SAMPLES (the call then also returns GS_ERR_OUT_OF_SAMPLES). */
+ * Borůvka whose rounds >= 1 replay a captured HIP graph; its kernels walk the selected members'
+ * vertices only. labels: total entries, member-local smallest vertex of each component for the
+ * selected members (the entries of the others are left unchanged); num_cc / rounds / status: one
+ * entry per member (n / 0 / GS_OK for members not selected), status GS_OK or
+ * GS_ERR_OUT_OF_SAMPLES (the call then also returns GS_ERR_OUT_OF_SAMPLES). */
 int gs_group_query_cc(gs_group* g, const uint8_t* member_mask, uint32_t* labels,
                       uint32_t* num_cc, uint32_t* rounds, int32_t* status);
 

@@ -8,9 +8,10 @@
 // every other tenant's gathered updates in one more submission. Update payloads stay in pinned
 // host memory from arrival: each goes by DMA straight to the device, where the member's base is
 // added and the ids range-checked (no host pass), and a step's launches are not waited for: the
-// next step's host work overlaps them, and a query waits for its tenant's store. A query
-// therefore sees exactly the updates of its tenant that arrived before it, so every answer is
-// the one the tenant would get alone. With batching off, requests are served one at a time on
+// next step's host work overlaps them, and a query waits for its tenant's store. A min-cut
+// query's forests are computed on a second thread (its tenant's later requests wait for it, the
+// other tenants' requests go on meanwhile). A query therefore sees exactly the updates of its
+// tenant that arrived before it, so every answer is the one the tenant would get alone. With batching off, requests are served one at a time on
 // per-tenant streams (the plain single-stream path). No counterpart in the reference, whose
 // algorithm objects each hold one graph and are driven by one stream.
 #pragma once
@@ -21,6 +22,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -150,6 +152,7 @@ class SketchServer {
     gs_store* store = nullptr;
     gs_stream* stream = nullptr;  // plain mode
     bool admitted = false, removing = false, removed = false;
+    bool mc_busy = false;         // a min-cut query of it is on the cutter thread (batching)
     uint64_t queued = 0;          // requests not yet answered
     bool spilled = false;         // buckets held in `spill` (reference order), no store
     SpillBuf spill;
@@ -172,12 +175,15 @@ class SketchServer {
   void fail_locked(Request& r, int status);
   void cut_loop();
 
-  // a min-cut query whose forests are on the host: its exact cut is computed off the worker
+  // a min-cut query handed to the cutter thread: with `store`, the cutter first computes its k
+  // forests on the device (batching mode: the tenant's later requests wait, the worker goes on
+  // with the others); otherwise the forests are `edges` and only the exact cut is left
   struct CutJob {
     Request req;
     uint32_t n, k;
     std::vector<uint32_t> edges;
     uint64_t m;
+    gs_store* store = nullptr;
   };
 
   ServerConfig cfg;
@@ -201,6 +207,7 @@ class SketchServer {
   // store group over the admitted tenants (batching mode), rebuilt when membership changes
   gs_group* group = nullptr;
   std::vector<uint32_t> group_members;
+  std::vector<uint32_t> cc_labels;  // labels buffer of the batched queries (worker only)
   std::thread worker;
   // host min-cut jobs, run by `cutter` while the worker goes on with the device
   std::deque<CutJob> cut_jobs;
@@ -208,4 +215,7 @@ class SketchServer {
   bool cut_stop = false;
   std::condition_variable cut_cv;
   std::thread cutter;
+  // held shared by the cutter while it computes forests on a member store of the group (whose
+  // stream its sync drains) and exclusively by the worker while it destroys the group
+  std::shared_mutex group_mu;
 };
