@@ -79,6 +79,9 @@ _sigs = {
     "gs_sync": ([_vp], _i32),
     "gs_export_buckets": ([_vp, _u32, _u32, _vp], _i32),
     "gs_import_buckets": ([_vp, _u32, _u32, _vp], _i32),
+    "gs_store_snapshot_bytes": ([_vp, _vp], _i32),
+    "gs_store_snapshot": ([_vp, _vp], _i32),
+    "gs_store_restore": ([_vp, _vp], _i32),
     "gs_query_cc": ([_vp, _vp, _vp, _vp, _vp, _u64, _vp], _i32),
     "gs_k_spanning_forests": ([_vp, _u32, _vp, _u64, _vp], _i32),
     "gs_sample_vertices": ([_vp, _u32, _vp, _vp], _i32),
@@ -355,6 +358,19 @@ class SketchStore:
     def import_buckets(self, data: np.ndarray, v0: int = 0) -> None:
         data = np.ascontiguousarray(data, dtype=np.uint64)
         _check(lib.gs_import_buckets(self.h, v0, data.shape[0], _p(data)))
+
+    def snapshot(self) -> np.ndarray:
+        """gs_store_snapshot: the store's device-order snapshot (bytes)."""
+        nbytes = ctypes.c_uint64()
+        _check(lib.gs_store_snapshot_bytes(self.h, ctypes.byref(nbytes)))
+        out = np.zeros(nbytes.value, dtype=np.uint8)
+        _check(lib.gs_store_snapshot(self.h, _p(out)))
+        return out
+
+    def restore(self, snap: np.ndarray) -> None:
+        """gs_store_restore from a snapshot of a store of the same geometry and seed."""
+        snap = np.ascontiguousarray(snap, dtype=np.uint8)
+        _check(lib.gs_store_restore(self.h, _p(snap)))
 
     def query_cc(self, want_forest: bool = False):
         labels = np.zeros(self.n, dtype=np.uint32)

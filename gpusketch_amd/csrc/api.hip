@@ -567,6 +567,76 @@ int gs_import_buckets(gs_store* h, uint32_t v0, uint32_t count, const gs_bucket*
   return GS_OK;
 }
 
+// ---- device-order snapshots --------------------------------------------------------------------
+// A 64-byte header (magic, geometry, seed, flags, whether the vertex sketches XOR to zero), then
+// the rows and the deterministic buckets exactly as the device holds them: one DMA each way.
+namespace {
+constexpr uint64_t kSnapMagic = 0x31504e5348534753ULL;  // "GSHSNP1"
+struct SnapHeader {
+  uint64_t magic;
+  uint64_t seed;
+  uint32_t n, bpc, cols, vstride;
+  uint32_t flags, symmetric, num_samples, cols_per_sample;
+  uint64_t pad[2];
+};
+static_assert(sizeof(SnapHeader) == 64, "snapshot header is 64 bytes");
+uint64_t snapshot_bytes(const Store& st) {
+  return sizeof(SnapHeader) + ((uint64_t)st.view.n * st.view.vstride + st.view.n) * sizeof(Bucket16);
+}
+}  // namespace
+
+int gs_store_snapshot_bytes(gs_store* h, uint64_t* bytes) {
+  GS_CHECK(h && bytes, GS_ERR_INVALID, "null argument");
+  *bytes = snapshot_bytes(h->impl);
+  return GS_OK;
+}
+
+int gs_store_snapshot(gs_store* h, void* dst) {
+  GS_CHECK(h && dst, GS_ERR_INVALID, "null argument");
+  Store& st = h->impl;
+  int rc = gs_sync(h);
+  if (rc) return rc;
+  SnapHeader hd{};
+  hd.magic = kSnapMagic;
+  hd.seed = st.view.seed;
+  hd.n = st.view.n;
+  hd.bpc = st.view.bpc;
+  hd.cols = st.view.cols;
+  hd.vstride = st.view.vstride;
+  hd.flags = st.flags;
+  hd.symmetric = st.symmetric.load() ? 1u : 0u;
+  hd.num_samples = st.num_samples;
+  hd.cols_per_sample = st.cols_per_sample;
+  unsigned char* p = static_cast<unsigned char*>(dst);
+  std::memcpy(p, &hd, sizeof(hd));
+  const size_t rows = (size_t)st.view.n * st.view.vstride * sizeof(Bucket16);
+  GS_HIP(hipMemcpy(p + sizeof(hd), st.view.rows, rows, hipMemcpyDeviceToHost));
+  GS_HIP(hipMemcpy(p + sizeof(hd) + rows, st.view.det, (size_t)st.view.n * sizeof(Bucket16),
+                   hipMemcpyDeviceToHost));
+  return GS_OK;
+}
+
+int gs_store_restore(gs_store* h, const void* src) {
+  GS_CHECK(h && src, GS_ERR_INVALID, "null argument");
+  Store& st = h->impl;
+  SnapHeader hd;
+  std::memcpy(&hd, src, sizeof(hd));
+  GS_CHECK(hd.magic == kSnapMagic, GS_ERR_INVALID, "not a store snapshot");
+  GS_CHECK(hd.seed == st.view.seed && hd.n == st.view.n && hd.bpc == st.view.bpc &&
+               hd.cols == st.view.cols && hd.vstride == st.view.vstride && hd.flags == st.flags &&
+               hd.num_samples == st.num_samples && hd.cols_per_sample == st.cols_per_sample,
+           GS_ERR_INVALID, "snapshot of a store of another geometry or seed");
+  int rc = gs_sync(h);
+  if (rc) return rc;
+  const unsigned char* p = static_cast<const unsigned char*>(src);
+  const size_t rows = (size_t)st.view.n * st.view.vstride * sizeof(Bucket16);
+  GS_HIP(hipMemcpy(st.view.rows, p + sizeof(hd), rows, hipMemcpyHostToDevice));
+  GS_HIP(hipMemcpy(st.view.det, p + sizeof(hd) + rows, (size_t)st.view.n * sizeof(Bucket16),
+                   hipMemcpyHostToDevice));
+  st.symmetric = hd.symmetric != 0;
+  return GS_OK;
+}
+
 int gs_stream_set_tuning(gs_stream* sh, const gs_tuning* t) {
   GS_CHECK(sh && t, GS_ERR_INVALID, "null argument");
   GS_CHECK(t->lds_depths <= 16, GS_ERR_INVALID, "lds_depths must be <= 16");

@@ -4,6 +4,7 @@
 //               [--max-edges 1000000] [--alone-check 0|1] [--evict 0|1]
 //               [--max-step-updates N] [--pinned-spill 0|1] [--verify 0|1]
 //               [--cc-frac 0.01] [--mc-frac 0.005] [--pinned-payloads 0|1] [--dump PREFIX]
+//               [--reserve-spill 0|1]
 // Tenants: n log-uniform in [min-n, max-n]. Requests arrive as a Poisson process of the given
 // rate (requests/s): a cc-frac share of connectivity queries on random tenants, mc-frac of
 // min-cut queries (k) on the first mc-tenants tenants, the rest update batches of log-uniform
@@ -13,7 +14,9 @@
 // line: updates/s and p50/p99 latencies per mode, and whether every answer and every final
 // sketch is identical in both modes (and, with --alone-check, for a tenant served alone). With a
 // budget below the tenants' total, --evict 1 (the default) spills idle tenants to host memory to
-// serve the others; --evict 0 serves only the tenants admitted before the first request.
+// serve the others (into pinned memory reserved before the clock starts, about 1.25x the
+// tenants' bytes beyond the budget, unless --reserve-spill 0); --evict 0 serves only the
+// tenants admitted before the first request.
 // --verify 1 replays the trace on the host and checks every answer of both modes against the
 // exact graph: each CC answer against a union-find over the tenant's net graph at that point of
 // the trace (component count and labels), each min-cut answer against min(k, exact minimum cut).
@@ -153,7 +156,7 @@ static void dump_csv(const RunStats& s, const char* mode) {
   }
   std::fclose(f);
 }
-static bool g_evict = true, g_pinned_spill = true;
+static bool g_evict = true, g_pinned_spill = true, g_reserve_spill = true;
 
 static RunStats run(const Trace& tr, bool batching, uint64_t budget, uint32_t k,
                     const std::vector<uint32_t>& only_tenants, bool keep_exports) {
@@ -169,6 +172,18 @@ static RunStats run(const Trace& tr, bool batching, uint64_t budget, uint32_t k,
     if (only_tenants.empty() ||
         std::find(only_tenants.begin(), only_tenants.end(), (uint32_t)i) != only_tenants.end())
       tid[i] = (int)srv.add_tenant(tr.tenants[i]);
+  if (g_evict && g_reserve_spill) {
+    // pinned spill space for the tenants beyond the budget (with slack for fragmentation),
+    // reserved before the clock starts as a server would at start-up
+    uint64_t total = 0;
+    for (size_t i = 0; i < tr.tenants.size(); i++) {
+      if (tid[i] < 0) continue;
+      gs_geometry g{};
+      if (gs_geometry_for(tr.tenants[i].num_vertices, tr.tenants[i].sketches_factor, 1, &g) == GS_OK)
+        total += g.device_bytes + 64;
+    }
+    if (total > budget) srv.reserve_spill((total - budget) + (total - budget) / 4);
+  }
   srv.wait_admissions();  // the clock then measures requests only
   std::map<uint64_t, uint64_t> id_to_index;
   const double start = srv.now_s();
@@ -330,7 +345,9 @@ static std::string stats_json(const RunStats& s) {
                   ", \"worker_s\": {\"updates\": " + std::to_string(s.phases.update_s) +
                   ", \"cc_queries\": " + std::to_string(s.phases.cc_s) +
                   ", \"mincut_forests\": " + std::to_string(s.phases.mincut_s) +
-                  ", \"cc_batches\": " + std::to_string(s.phases.cc_batches) + "}";
+                  ", \"cc_batches\": " + std::to_string(s.phases.cc_batches) +
+                  ", \"group_builds\": " + std::to_string(s.phases.group_builds) +
+                  ", \"group_build_s\": " + std::to_string(s.phases.group_s) + "}";
   for (auto& kv : s.p50p99)
     j += ", \"" + kv.first + "_ms\": {\"p50\": " + std::to_string(kv.second.first) +
          ", \"p99\": " + std::to_string(kv.second.second) + "}";
@@ -364,6 +381,7 @@ int main(int argc, char** argv) {
     else if (a == "--mc-frac") g_mc_frac = std::atof(v);
     else if (a == "--pinned-payloads") g_pinned_payloads = std::atoi(v) != 0;
     else if (a == "--dump") g_dump = v;
+    else if (a == "--reserve-spill") g_reserve_spill = std::atoi(v) != 0;
     else {
       std::fprintf(stderr, "unknown option %s\n", a.c_str());
       return 2;

@@ -7,6 +7,7 @@
 #include "gpusketch/sketch_server.h"
 
 #include <algorithm>
+#include <iterator>
 #include <chrono>
 #include <cstring>
 #include <stdexcept>
@@ -51,6 +52,7 @@ SketchServer::~SketchServer() {
     give_back(t->spill);
   }
   for (auto& kv : pinned_free) gs_host_free(kv.second);
+  if (arena) gs_host_free(arena);
   for (auto& r : queue)
     if (r.owned) gs_host_free(r.owned);
   for (auto& kv : payload_free) gs_host_free(kv.second);
@@ -211,12 +213,26 @@ std::vector<uint64_t> SketchServer::export_tenant(uint32_t t) {
   drain();
   std::lock_guard<std::mutex> lk(mu);
   Tenant& tn = *tenants.at(t);
-  if (tn.spilled) return std::vector<uint64_t>(tn.spill.p, tn.spill.p + tn.spill.words);
-  if (!tn.store) throw std::invalid_argument("tenant has no store");
+  gs_store* st = tn.store;
+  if (tn.spilled) {  // the snapshot through a temporary store of the same geometry
+    st = nullptr;
+    const uint32_t samples =
+        (uint32_t)Sketch::calc_cc_samples(tn.spec.num_vertices, tn.spec.sketches_factor);
+    gs_throw_if(gs_store_create(tn.spec.num_vertices, tn.spec.seed, samples, 1, 0, &st));
+    const int rc = gs_store_restore(st, tn.spill.p);
+    if (rc != GS_OK) gs_store_destroy(st);
+    gs_throw_if(rc);
+  }
+  if (!st) throw std::invalid_argument("tenant has no store");
   gs_geometry g{};
-  gs_throw_if(gs_store_geometry(tn.store, &g));
-  std::vector<uint64_t> out((size_t)g.num_vertices * g.num_buckets * 2);
-  gs_throw_if(gs_export_buckets(tn.store, 0, g.num_vertices, reinterpret_cast<gs_bucket*>(out.data())));
+  std::vector<uint64_t> out;
+  int rc = gs_store_geometry(st, &g);
+  if (rc == GS_OK) {
+    out.resize((size_t)g.num_vertices * g.num_buckets * 2);
+    rc = gs_export_buckets(st, 0, g.num_vertices, reinterpret_cast<gs_bucket*>(out.data()));
+  }
+  if (tn.spilled) gs_store_destroy(st);
+  gs_throw_if(rc);
   return out;
 }
 
@@ -269,9 +285,32 @@ bool SketchServer::plan_room_locked(std::vector<uint32_t>* evict, uint32_t* need
 
 // a host buffer of at least `words` words: a free pinned one of at most twice that, else a new
 // one (worker thread)
+void SketchServer::reserve_spill(uint64_t bytes) {
+  std::lock_guard<std::mutex> lk(mu);
+  if (arena || bytes == 0) return;
+  void* p = nullptr;
+  const size_t words = (size_t)((bytes + 7) / 8);
+  gs_throw_if(gs_host_alloc(words * sizeof(uint64_t), &p));
+  arena = static_cast<uint64_t*>(p);
+  arena_words = words;
+  arena_free.emplace(0, words);
+}
+
 SketchServer::SpillBuf SketchServer::take_spill_buffer(size_t words) {
   SpillBuf b;
   b.words = words;
+  // first fit in the reserved arena
+  for (auto it = arena_free.begin(); it != arena_free.end(); ++it) {
+    if (it->second < words) continue;
+    const size_t off = it->first, len = it->second;
+    arena_free.erase(it);
+    if (len > words) arena_free.emplace(off + words, len - words);
+    b.p = arena + off;
+    b.cap = words;
+    b.pinned = true;
+    b.in_arena = true;
+    return b;
+  }
   if (!cfg.pinned_spill) {
     b.p = new uint64_t[words];  // uninitialised: the export writes every word
     b.cap = words;
@@ -293,7 +332,23 @@ SketchServer::SpillBuf SketchServer::take_spill_buffer(size_t words) {
 }
 
 void SketchServer::give_back(SpillBuf& b) {
-  if (b.p) {
+  if (b.p && b.in_arena) {
+    size_t off = (size_t)(b.p - arena), len = b.cap;
+    auto next = arena_free.lower_bound(off);
+    if (next != arena_free.end() && off + len == next->first) {  // merge with the range after
+      len += next->second;
+      next = arena_free.erase(next);
+    }
+    if (next != arena_free.begin()) {  // and with the one before
+      auto prev = std::prev(next);
+      if (prev->first + prev->second == off) {
+        off = prev->first;
+        len += prev->second;
+        arena_free.erase(prev);
+      }
+    }
+    arena_free.emplace(off, len);
+  } else if (b.p) {
     if (b.pinned) pinned_free.emplace(b.cap, b.p);
     else delete[] b.p;
   }
@@ -312,12 +367,11 @@ void SketchServer::make_resident(uint32_t need, const std::vector<uint32_t>& evi
   for (uint32_t i : evict) {
     const double t_start = now_s();
     Tenant& t = *tenants[i];
-    const uint64_t n = t.spec.num_vertices;
-    gs_geometry g{};
-    gs_throw_if(gs_store_geometry(t.store, &g));
-    const size_t words = n * g.num_buckets * 2;
+    uint64_t bytes = 0;
+    gs_throw_if(gs_store_snapshot_bytes(t.store, &bytes));
+    const size_t words = (size_t)((bytes + 7) / 8);
     SpillBuf buf = take_spill_buffer(words);
-    int rc = gs_export_buckets(t.store, 0, (uint32_t)n, reinterpret_cast<gs_bucket*>(buf.p));
+    int rc = gs_store_snapshot(t.store, buf.p);
     if (rc != GS_OK) give_back(buf);
     gs_throw_if(rc);
     if (t.stream) gs_throw_if(gs_stream_destroy(t.stream));
@@ -342,7 +396,7 @@ void SketchServer::make_resident(uint32_t need, const std::vector<uint32_t>& evi
   gs_throw_if(gs_store_create(t.spec.num_vertices, t.spec.seed, samples, 1, 0, &st));
   int rc = GS_OK;
   if (t.spilled)
-    rc = gs_import_buckets(st, 0, t.spec.num_vertices, reinterpret_cast<const gs_bucket*>(t.spill.p));
+    rc = gs_store_restore(st, t.spill.p);
   if (rc == GS_OK && !cfg.batching) rc = gs_stream_create(st, &stream);
   if (rc != GS_OK) {
     gs_store_destroy(st);  // the spill stays, for a later attempt
@@ -376,6 +430,7 @@ void SketchServer::ensure_group() {
       }
   }
   if (group && members == group_members) return;
+  const double t_build = now_s();
   if (group) {
     std::unique_lock<std::shared_mutex> glk(group_mu);  // no forests being computed
     gs_throw_if(gs_group_destroy(group));
@@ -395,6 +450,9 @@ void SketchServer::ensure_group() {
     // buffers and query scratch now, not in the first step that updates or queries
     gs_throw_if(gs_group_prepare(group));
   }
+  std::lock_guard<std::mutex> lk(mu);
+  phases.group_s += now_s() - t_build;
+  phases.group_builds++;
 }
 
 void SketchServer::finish(Request& r, RequestResult res) {
@@ -630,7 +688,18 @@ void SketchServer::loop() {
           release.push_back(i);
         }
       if (release.empty()) admit_locked();
-      if (release.empty() && !plan_room_locked(&evict, &need)) {
+      bool room = release.empty() && plan_room_locked(&evict, &need);
+      // with batching, making a waiting tenant resident (spills, a restore) and serving the
+      // resident tenants take turns, so neither waits for all of the other
+      if (room && cfg.batching && runnable()) {
+        step_turn = !step_turn;
+        if (step_turn) {
+          room = false;
+          evict.clear();
+          need = UINT32_MAX;
+        }
+      }
+      if (release.empty() && !room) {
         std::vector<char> blocked(tenants.size(), 0);
         uint64_t nupd = 0;
         for (auto it = queue.begin(); it != queue.end();) {

@@ -152,6 +152,16 @@ int gs_export_buckets(gs_store* st, uint32_t v0, uint32_t count, gs_bucket* out)
 /* Overwrite `count` vertex sketches from reference-ordered buckets (the istream constructor,
  * src/sketch.cpp:39-51, used by CCSketchAlg::construct_from_serialized_data :57-91). */
 int gs_import_buckets(gs_store* st, uint32_t v0, uint32_t count, const gs_bucket* in);
+/* Device-order snapshot of a whole store (its buckets as the device holds them, and whether its
+ * vertex sketches XOR to zero) for spilling it to host memory: one DMA each way, straight from or
+ * into pinned memory (gs_host_alloc). gs_store_snapshot waits for the store's pending updates;
+ * gs_store_restore refuses a snapshot of another geometry or seed (GS_ERR_INVALID) and replaces
+ * the store's contents. The format is this library's own, not the reference order
+ * (gs_export_buckets writes that). No counterpart in the reference; the serving loop uses it to
+ * evict and restore tenants. */
+int gs_store_snapshot_bytes(gs_store* st, uint64_t* bytes);
+int gs_store_snapshot(gs_store* st, void* host_dst);
+int gs_store_restore(gs_store* st, const void* host_src);
 
 /* Connected components by Borůvka emulation on the device (CCSketchAlg::boruvka_emulation +
  * connected_components, src/cc_sketch_alg.cpp:491-584). label_out[v] = smallest vertex id of

@@ -83,6 +83,9 @@ class SketchServer {
   bool has_sketch(uint32_t tenant);
   // wait until every waiting tenant that fits the budget has been admitted
   void wait_admissions();
+  // pins `bytes` of host memory now for spilled tenants (carved per spill, returned on restore),
+  // so that evictions do not pin memory while requests wait; spills beyond it pin their own
+  void reserve_spill(uint64_t bytes);
   uint64_t admitted_bytes();
 
   // enqueue (thread-safe, returns the request id); pairs are copied (into a pinned buffer of the
@@ -117,6 +120,8 @@ class SketchServer {
   struct PhaseStats {
     double update_s = 0, cc_s = 0, mincut_s = 0;
     uint64_t cc_batches = 0, mincut_queries = 0;
+    double group_s = 0;       // building store groups (batching: after membership changes)
+    uint64_t group_builds = 0;
   };
   PhaseStats phase_stats() {
     std::lock_guard<std::mutex> lk(mu);
@@ -145,6 +150,7 @@ class SketchServer {
     size_t words = 0;  // in use
     size_t cap = 0;    // allocated
     bool pinned = false;
+    bool in_arena = false;  // a range of the reserved arena
   };
   struct Tenant {
     TenantSpec spec;
@@ -197,11 +203,16 @@ class SketchServer {
   uint64_t next_id = 0, answered = 0;
   bool stop = false;
   bool busy = false;
+  bool step_turn = false;  // batching: a step's turn next (it alternates with residency work)
   std::vector<RequestResult> done;
   std::atomic<uint64_t> num_steps{0}, num_evictions{0}, num_restores{0};
   MoveStats moves;
   PhaseStats phases;
   std::multimap<size_t, uint64_t*> pinned_free;  // pinned spill buffers by capacity (words)
+  // the reserved spill arena (reserve_spill): its free ranges, offset -> words, merged on return
+  uint64_t* arena = nullptr;
+  size_t arena_words = 0;
+  std::map<size_t, size_t> arena_free;
   std::multimap<size_t, uint32_t*> payload_free;  // pinned payload buffers by capacity (words)
   std::string last_error;
   // store group over the admitted tenants (batching mode), rebuilt when membership changes
