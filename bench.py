@@ -337,7 +337,7 @@ def bench_single(args) -> dict:
     geo = store.geom
     stream = store.stream()
     stream.set_tuning(lds_depths=args.lds_depths, item_updates=args.item_updates,
-                      launch_edges=LAUNCH_EDGES)
+                      bucket_bits=args.bucket_bits, launch_edges=LAUNCH_EDGES)
     dbuf = gs.DeviceBuffer(pairs)
     log(f"store {geo.device_bytes / 1e9:.3f} GB, B={geo.bkt_per_col} C={geo.num_columns}; "
         f"input resident in HBM ({dbuf.nbytes / 1e9:.2f} GB)")
@@ -534,6 +534,8 @@ def main() -> None:
                     help="single mode: LDS depth window of the aggregation kernel (0: the library's)")
     ap.add_argument("--item-updates", type=int, default=0,
                     help="single mode: destinations per work item (0: the library's)")
+    ap.add_argument("--bucket-bits", type=int, default=0,
+                    help="single mode: vertex-id bits resolved inside one grouping bucket (0: the library's)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the known-size copy kernel (HBM counter calibration)")
     args = ap.parse_args()

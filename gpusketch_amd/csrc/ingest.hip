@@ -29,7 +29,7 @@ constexpr uint32_t kWideChunk = 1024;         // the same for wide stores
 constexpr int kPartThreads = 512;             // K1/K2 workgroup size
 constexpr int kPartEdges = 4096;              // undirected edges per K1/K2 workgroup
 constexpr int kSortThreads = 1024;            // K4 workgroup size (one workgroup per bucket)
-constexpr int kSortBatch = 8;                 // entries per thread loaded together in K4
+constexpr int kSortBatch = 16;                // entries per thread loaded together in K4
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 
 __host__ __device__ __forceinline__ uint32_t item_class(uint32_t len) {
@@ -99,15 +99,6 @@ __device__ __forceinline__ uint32_t hashed_id(const PartParams& pp, uint32_t v) 
 __device__ __forceinline__ uint32_t bucket_of(const PartParams& pp, uint32_t v) {
   return hashed_id(pp, v) >> pp.local_bits;
 }
-// a pair the grouping drops: a vertex outside the vertex space, or, in a group of stores
-// (vgraph: the member of every global vertex), endpoints in two different members
-__device__ __forceinline__ bool pair_invalid(uint2 uv, uint32_t n, const uint32_t* vgraph) {
-  return uv.x >= n || uv.y >= n || (vgraph && vgraph[uv.x] != vgraph[uv.y]);
-}
-__device__ __forceinline__ bool entry_ok(uint2 uv, uint32_t n, const uint32_t* vgraph) {
-  return uv.x != uv.y && !pair_invalid(uv, n, vgraph);
-}
-
 // Partition block of workgroup `wg` of a K1/K3 launch of nblk workgroups. Workgroups are dealt to
 // the 8 XCDs round-robin (wg mod 8), and consecutive partition blocks write adjacent runs of
 // every bucket (K3) and adjacent counters of the histogram (K1): each XCD therefore takes one
@@ -152,8 +143,41 @@ __global__ void append_kernel(const uint2* src, uint2* dst, uint64_t nedges, uin
   }
 }
 
-// invalid pairs (pair_invalid) are counted (reported at the next flush) and skipped here and in
-// K3 (entry_ok), never written: the pairs may be the caller's own device memory
+// The K pairs of this thread in a K1/K3 workgroup (edges e0 + t + k * kPartThreads below e1), all
+// loads issued before any is used, then (groups) the members of their endpoints, also together.
+// ok[k]: the slot holds a pair of the launch; a pair the grouping drops (an endpoint outside the
+// vertex space or, in a group of stores, endpoints in two members) comes back as
+// (kBadVertex, kBadVertex).
+template <int K>
+__device__ __forceinline__ void load_block_pairs(const uint2* __restrict__ pairs, uint64_t e0,
+                                                 uint64_t e1, uint32_t n,
+                                                 const uint32_t* __restrict__ vgraph, uint2 (&uv)[K],
+                                                 bool (&ok)[K]) {
+  const uint32_t t = threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const uint64_t e = e0 + t + (uint64_t)k * kPartThreads;
+    ok[k] = e < e1;
+    uv[k] = pairs[ok[k] ? e : e0];  // e0 < e1: every workgroup holds at least one pair
+  }
+  if (vgraph) {
+    uint32_t ga[K], gb[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {  // clamped ids: an out-of-range pair is dropped by its ids
+      ga[k] = vgraph[min(uv[k].x, n - 1u)];
+      gb[k] = vgraph[min(uv[k].y, n - 1u)];
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++)
+      if (ga[k] != gb[k]) uv[k] = make_uint2(kBadVertex, kBadVertex);
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++)
+    if (uv[k].x >= n || uv[k].y >= n) uv[k] = make_uint2(kBadVertex, kBadVertex);
+}
+
+// dropped pairs are counted (reported at the next flush) and skipped here and in K3, never
+// written: the pairs may be the caller's own device memory
 __global__ __launch_bounds__(kPartThreads) void bucket_count_kernel(
     PairSegs sg, uint32_t n, const uint32_t* __restrict__ vgraph, PartParams pp, uint32_t nblk,
     uint32_t* __restrict__ hist, uint32_t* __restrict__ counters) {
@@ -167,18 +191,20 @@ __global__ __launch_bounds__(kPartThreads) void bucket_count_kernel(
   const uint2* pairs;
   uint64_t e0, e1;
   block_pairs(sg, blk, pairs, e0, e1);
+  constexpr int K = kPartEdges / kPartThreads;
+  uint2 uv[K];
+  bool ok[K];
+  load_block_pairs<K>(pairs, e0, e1, n, vgraph, uv, ok);
 #pragma unroll
-  for (int k = 0; k < kPartEdges / kPartThreads; k++) {
-    const uint64_t e = e0 + t + (uint64_t)k * kPartThreads;
-    if (e >= e1) continue;
-    const uint2 uv = pairs[e];
-    if (pair_invalid(uv, n, vgraph)) {  // never index past the store: drop and report at flush
+  for (int k = 0; k < K; k++) {
+    if (!ok[k]) continue;
+    if (uv[k].x == kBadVertex) {  // never index past the store: drop and report at flush
       atomicAdd(&counters[3], 1u);
       continue;
     }
-    if (uv.x == uv.y) continue;
-    atomicAdd(&h[bucket_of(pp, uv.x)], 1u);
-    atomicAdd(&h[bucket_of(pp, uv.y)], 1u);
+    if (uv[k].x == uv[k].y) continue;
+    atomicAdd(&h[bucket_of(pp, uv[k].x)], 1u);
+    atomicAdd(&h[bucket_of(pp, uv[k].y)], 1u);
   }
   __syncthreads();
   for (uint32_t i = t; i < pp.nbuckets; i += kPartThreads) hist[(size_t)i * nblk + blk] = h[i];
@@ -242,32 +268,70 @@ __device__ void global_exclusive_scan(const uint32_t* in, uint32_t n, uint32_t* 
 }
 
 // one workgroup per bucket: the block counts of the bucket become exclusive offsets (in place)
-// and its total is written
+// and its total is written. The counts go in rounds of kOffThreads x 16: each thread holds a run
+// of 16 consecutive counts in registers (four 16-byte loads when the row is 16-byte aligned),
+// scans it, the runs are scanned across the wave by shuffles and across the waves through LDS,
+// and the offsets go back the same way; a round's carry feeds the next.
 constexpr int kOffThreads = 256;
+constexpr int kOffRun = 16;
 __global__ __launch_bounds__(kOffThreads) void bucket_offsets_kernel(uint32_t* hist, uint32_t nblk,
                                                                      uint32_t* __restrict__ totals) {
-  __shared__ uint32_t s_part[kOffThreads];
+  __shared__ uint32_t s_wave[kOffThreads / 64];
   uint32_t* col = hist + (size_t)blockIdx.x * nblk;
-  const uint32_t t = threadIdx.x;
-  const uint32_t per = (nblk + kOffThreads - 1) / kOffThreads;
-  const uint32_t b0 = min(nblk, t * per), b1 = min(nblk, b0 + per);
-  uint32_t sum = 0;
-  for (uint32_t b = b0; b < b1; b++) sum += col[b];
-  s_part[t] = sum;
-  __syncthreads();
-  for (uint32_t o = 1; o < kOffThreads; o <<= 1) {
-    const uint32_t add = t >= o ? s_part[t - o] : 0;
+  const uint32_t t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const bool vec = (((uintptr_t)col) & 15u) == 0;
+  uint32_t carry = 0;
+  for (uint32_t r0 = 0; r0 < nblk; r0 += kOffThreads * kOffRun) {
+    const uint32_t b0 = r0 + t * kOffRun;
+    uint32_t v[kOffRun];
+    if (vec && b0 + kOffRun <= nblk) {
+#pragma unroll
+      for (int q = 0; q < kOffRun / 4; q++) {
+        const uint4 x = reinterpret_cast<const uint4*>(col + b0)[q];
+        v[4 * q] = x.x;
+        v[4 * q + 1] = x.y;
+        v[4 * q + 2] = x.z;
+        v[4 * q + 3] = x.w;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < kOffRun; q++) v[q] = b0 + q < nblk ? col[b0 + q] : 0u;
+    }
+    uint32_t sum = 0;
+#pragma unroll
+    for (int q = 0; q < kOffRun; q++) {
+      const uint32_t c = v[q];
+      v[q] = sum;
+      sum += c;
+    }
+    uint32_t x = sum;  // inclusive scan of the run totals over the wave
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)x, o);
+      if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63) s_wave[wid] = x;
     __syncthreads();
-    s_part[t] += add;
-    __syncthreads();
+    uint32_t base = carry + x - sum, round_total = 0;
+#pragma unroll
+    for (int w = 0; w < kOffThreads / 64; w++) {
+      const uint32_t p = s_wave[w];
+      base += (uint32_t)w < wid ? p : 0u;
+      round_total += p;
+    }
+    __syncthreads();  // s_wave is rewritten by the next round
+    if (vec && b0 + kOffRun <= nblk) {
+#pragma unroll
+      for (int q = 0; q < kOffRun / 4; q++)
+        reinterpret_cast<uint4*>(col + b0)[q] =
+            make_uint4(base + v[4 * q], base + v[4 * q + 1], base + v[4 * q + 2], base + v[4 * q + 3]);
+    } else {
+#pragma unroll
+      for (int q = 0; q < kOffRun; q++)
+        if (b0 + q < nblk) col[b0 + q] = base + v[q];
+    }
+    carry += round_total;
   }
-  uint32_t run = t ? s_part[t - 1] : 0;
-  for (uint32_t b = b0; b < b1; b++) {
-    const uint32_t c = col[b];
-    col[b] = run;
-    run += c;
-  }
-  if (t == kOffThreads - 1) totals[blockIdx.x] = s_part[t];
+  if (t == 0) totals[blockIdx.x] = carry;
 }
 
 // one workgroup: the start of every bucket's region (exclusive prefix of the totals)
@@ -318,15 +382,13 @@ __global__ __launch_bounds__(kPartThreads) void bucket_scatter_kernel(
   constexpr int K = kPartEdges / kPartThreads;
   uint32_t bk[2 * K], rk[2 * K];
   uint2 uvs[K];
+  bool okk[K];
+  load_block_pairs<K>(pairs, e0, e1, n, vgraph, uvs, okk);
 #pragma unroll
   for (int k = 0; k < K; k++) {
-    uint64_t e = e0 + t + (uint64_t)k * kPartThreads;
     bk[2 * k] = bk[2 * k + 1] = kEmpty;
-    uvs[k] = make_uint2(0, 0);
-    if (e >= e1) continue;
-    uint2 uv = pairs[e];
-    if (!entry_ok(uv, n, vgraph)) continue;
-    uvs[k] = uv;
+    const uint2 uv = uvs[k];
+    if (!okk[k] || uv.x == kBadVertex || uv.x == uv.y) continue;
     bk[2 * k] = bucket_of(pp, uv.x);
     bk[2 * k + 1] = bucket_of(pp, uv.y);
     rk[2 * k] = atomicAdd(&s_cnt[bk[2 * k]], 1u);
