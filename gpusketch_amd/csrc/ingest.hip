@@ -962,8 +962,8 @@ __global__ __launch_bounds__(kAggWaves* kWave, (agg_min_waves<kGroup, T>())) voi
     wave_sync_lds();
     // flush, kFlushRounds x 64 slots at a time: all LDS reads first, then the HBM reads of the
     // non-zero slots together, one wait, then the stores (exclusive item: plain read-xor-write;
-    // otherwise 64-bit atomics, which need no read). Slot s = depth * cols + column is LDS row
-    // 7 + depth and device bucket s of the vertex (the window lies in the depth rows of
+    // otherwise 64-bit atomics, which need no read). Slot s = depth * cols + column is in LDS
+    // row `depth` and is device bucket s of the vertex (the window lies in the depth rows of
     // bucket_slot). An exclusive item whose window fits one step leaves its stores pending: they
     // complete after the next item, so the reads' latency is hidden behind that item's work (the
     // vertex is this item's alone; its deep buckets, written by atomics, lie outside the window).
