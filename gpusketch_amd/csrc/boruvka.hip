@@ -28,7 +28,7 @@ constexpr uint32_t kNoGiant = 0xFFFFFFFFu;
 // big supernodes per round (pick_big_kernel) and the share of kPickSamples sampled vertices one
 // must hold: 2 of 256 at n = 2^17 is about 1000 members (R-MAT, plain CC query: 1.10 ms against
 // 1.22 ms with 8 of at least 4 samples; 1024 samples cost more to pick than they save)
-constexpr int kBig = 16;
+constexpr int kBig = 64;
 constexpr int kPickSamples = 256;
 constexpr uint32_t kMinBig = 2;
 constexpr int kFlagBig = 4;
@@ -168,7 +168,7 @@ __global__ void end_round_kernel(uint32_t* flags, uint32_t num_samples) {
 }
 
 __global__ void clear_all_flags_kernel(uint32_t* flags) {
-  if (threadIdx.x < kFlagWords) flags[threadIdx.x] = 0;
+  for (uint32_t i = threadIdx.x; i < (uint32_t)kFlagWords; i += blockDim.x) flags[i] = 0;
 }
 
 __global__ void zero_acc_kernel(Bucket16* acc, uint64_t count) {
@@ -368,8 +368,10 @@ __global__ __launch_bounds__(kQThreads) void round0_kernel(StoreView st, uint32_
 
 // Big supernodes of a round, picked from kPickSamples sampled vertices (pick_big_kernel): up to kBig live
 // supernodes holding at least kMinBig of the samples each (flags[4 + k], kNoGiant when unused),
-// largest first. Their column sums are formed from per-wave partial sums instead of atomics
-// into one accumulator row (which thousands of waves would contend for). When the store's
+// largest first. Their column sums are formed from per-block partial sums (acc_kernel sums them in
+// LDS) instead of atomics into one accumulator row (which thousands of waves would contend for:
+// with 16 such supernodes, the medium supernodes of rounds 2-3 of an R-MAT query took most of
+// their accumulation time in contended atomics). When the store's
 // updates were all symmetric (edge pairs: every update XORs the same bucket of both endpoints)
 // the XOR of a column over all vertices is zero, so any supernode's sum is the XOR over all
 // vertices outside it: the supernode holding the most samples is the giant (flags[3]) and its
@@ -411,33 +413,37 @@ __global__ __launch_bounds__(kPickSamples) void pick_big_kernel(uint32_t n,
   }
   if (r == kNoGiant) atomicAdd(&s_done, 1u);
   __syncthreads();
-  if (t >= 64) return;
   // the giant: the candidate of most samples (over all candidates)
   const uint32_t gkey = s_max;
   const bool giant = symmetric && gkey && (gkey >> 16) >= kMinGiant && (gkey >> 16) > s_done;
   const uint32_t groot = giant ? s_r[gkey & 0xFFFFu] : kNoGiant;
-  // wave 0: bitonic sort of (the first 64 of) the candidates over 64 lanes, largest first; the
-  // big supernodes are the first kBig of them other than the giant
-  const uint32_t nc = s_ncand;
-  uint32_t x = t < nc ? s_cand[t] : 0u;
-  for (uint32_t k = 2; k <= 64; k <<= 1)
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      const uint32_t y = (uint32_t)__shfl_xor((int)x, (int)j);
-      x = (((t & j) == 0) == ((t & k) == 0)) ? max(x, y) : min(x, y);
-    }
-  if (t >= (uint32_t)kBig) return;
-  const uint32_t root = x ? s_r[x & 0xFFFFu] : kNoGiant;
   if (t == 0) flags[3] = groot;
-  flags[kFlagBig + t] = root == groot ? kNoGiant : root;  // the giant is formed from the rest
+  if (t < (uint32_t)kBig) flags[kFlagBig + t] = kNoGiant;
+  __syncthreads();
+  // the big supernodes: the other candidates of most samples, each placed at its rank (keys are
+  // unique: samples, then first sample position)
+  const uint32_t nc = s_ncand;
+  if (t < nc) {
+    const uint32_t x = s_cand[t];
+    if (!(giant && x == gkey)) {
+      uint32_t rank = 0;
+      for (uint32_t u = 0; u < nc; u++) {
+        const uint32_t y = s_cand[u];
+        rank += y > x && !(giant && y == gkey);
+      }
+      if (rank < (uint32_t)kBig) flags[kFlagBig + rank] = s_r[x & 0xFFFFu];
+    }
+  }
 }
 
 // Rounds >= 1: the sample sums (sample r: its cps columns, depths 0..B-1, plus the deterministic
 // bucket; w = cps*B + 1 slots) of the live supernodes. A wave takes 64 consecutive vertices and
 // walks the ones it must read, four at a time; lane l holds slot 64*blockIdx.y + l (so one
 // vertex is one load instruction, and its row of atomics is one instruction over consecutive
-// addresses). Big supernodes accumulate in registers and leave per-block partials
-// bpart[block][k] (the block's waves summed through LDS); with a giant, every vertex outside it
-// (finished supernodes included) also adds into cpart[block]; the other supernodes take atomics.
+// addresses). Big supernodes accumulate into LDS rows of the block (LDS atomics) and leave
+// per-block partials bpart[block][k]; with a giant, every vertex outside it (finished supernodes
+// included) also adds into cpart[block] (registers, the block's waves summed through LDS); the
+// other supernodes take atomics.
 __global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st, uint32_t cps,
                                                         uint32_t* __restrict__ parent,
                                                         const uint32_t* __restrict__ comp,
@@ -460,9 +466,13 @@ __global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st, uint32_t c
   const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
   const uint32_t v0 = gwave * kAccVerts;
   const uint32_t giant = flags[3];
-  uint32_t big[kBig];
-#pragma unroll
-  for (int k = 0; k < kBig; k++) big[k] = flags[kFlagBig + k];
+  // the big supernodes' sums of this block: LDS rows of rw slots, one per big supernode
+  extern __shared__ Bucket16 s_big[];
+  __shared__ uint32_t s_bigr[kBig];
+  const uint32_t rw = min(64u, w);
+  for (uint32_t i = threadIdx.x; i < kBig * rw; i += blockDim.x) s_big[i] = Bucket16{0, 0};
+  if (threadIdx.x < (uint32_t)kBig) s_bigr[threadIdx.x] = flags[kFlagBig + threadIdx.x];
+  __syncthreads();
   const bool split = giant != kNoGiant;
   const uint32_t v = v0 + lane;
   const bool inr = lane < kAccVerts && v < st.n;
@@ -471,17 +481,14 @@ __global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st, uint32_t c
   const bool rd = inr && (split ? r != giant : !done);  // this vertex's column is read
   const bool act = rd && !done;                         // and belongs to a live supernode
   int kb = -1;
-#pragma unroll
-  for (int k = kBig - 1; k >= 0; k--)
-    if (r == big[k]) kb = k;
+  for (int k = 0; k < kBig; k++)
+    kb = (kb < 0 && r == s_bigr[k]) ? k : kb;
   const uint64_t rdm = __ballot(rd), actm = __ballot(act);
   const bool lj = slot < w;
   const uint32_t jj = lj ? slot : 0;
   const bool isdet = jj == sw;
   const size_t soff = isdet ? 0 : slot_offset(st, cps, col, jj);
-  uint64_t ca = 0, cg = 0, ba[kBig], bg[kBig];
-#pragma unroll
-  for (int k = 0; k < kBig; k++) ba[k] = bg[k] = 0;
+  uint64_t ca = 0, cg = 0;
   uint64_t m = rdm;
   while (m) {
     uint32_t ii[4];
@@ -508,12 +515,11 @@ __global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st, uint32_t c
       if (!((actm >> i) & 1)) continue;
       const int kbi = __builtin_amdgcn_readlane(kb, i);
       if (kbi >= 0) {
-#pragma unroll
-        for (int k = 0; k < kBig; k++)
-          if (kbi == k) {
-            ba[k] ^= xa;
-            bg[k] = bg[k] ^ xg;
-          }
+        if (lj && (xa | xg)) {
+          Bucket16* dst = &s_big[(uint32_t)kbi * rw + lane];
+          __hip_atomic_fetch_xor(&dst->alpha, xa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_fetch_xor(&dst->gamma, xg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
       } else if (lj && (xa | xg)) {
         Bucket16* dst = acc + (size_t)__builtin_amdgcn_readlane((int)r, i) * w + jj;
         __hip_atomic_fetch_xor(&dst->alpha, xa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -538,12 +544,11 @@ __global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st, uint32_t c
     block_sum(ca, cg);
     if (wv == 0 && lj) cpart[(size_t)blockIdx.x * w + slot] = Bucket16{ca, cg};
   }
-#pragma unroll
-  for (int k = 0; k < kBig; k++)
-    if (big[k] != kNoGiant) {
-      block_sum(ba[k], bg[k]);
-      if (wv == 0 && lj) bpart[((size_t)blockIdx.x * kBig + k) * w + slot] = Bucket16{ba[k], bg[k]};
-    }
+  __syncthreads();  // every wave's LDS sums are in
+  for (uint32_t i = threadIdx.x; i < kBig * rw; i += blockDim.x) {
+    const uint32_t k = i / rw, sl = blockIdx.y * 64 + (i - k * rw);
+    if (s_bigr[k] != kNoGiant && sl < w) bpart[((size_t)blockIdx.x * kBig + k) * w + sl] = s_big[i];
+  }
 }
 
 // accumulators of the big supernodes (blockIdx.y = k < kBig) and of the giant (blockIdx.y =
@@ -780,7 +785,8 @@ static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t
     for (uint32_t k = 0; k < kRoundsAhead; k++) {
       hipLaunchKernelGGL(pick_big_kernel, dim3(1), dim3(kPickSamples), 0, q, n, st->d_comp,
                          st->d_vflags, symmetric, st->d_flags);
-      hipLaunchKernelGGL(acc_kernel, dim3(g_acc.x, (w + 63) / 64), block, 0, q, v, cps,
+      hipLaunchKernelGGL(acc_kernel, dim3(g_acc.x, (w + 63) / 64), block,
+                         sizeof(Bucket16) * kBig * std::min(64u, w), q, v, cps,
                          st->d_parent, st->d_comp, st->d_vflags, st->d_flags, st->d_acc,
                          st->d_cpart, st->d_bpart);
       hipLaunchKernelGGL(big_acc_kernel, dim3(w, kBig + 1), dim3(kQThreads), 0, q, st->d_flags,
