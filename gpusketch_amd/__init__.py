@@ -101,6 +101,7 @@ _sigs = {
     "gs_group_submit_pairs_device": ([_vp, _vp, _u64], _i32),
     "gs_group_submit_member_pairs": ([_vp, _u32, _vp, _u64], _i32),
     "gs_group_query_cc": ([_vp, _vp, _vp, _vp, _vp, _vp], _i32),
+    "gs_group_fence": ([_vp], _i32),
     "gs_group_prepare": ([_vp], _i32),
     "gs_apply_edge_pairs_device_noalloc": ([_vp, _vp, _u64], _i32),
     "gs_host_alloc": ([_sz, _vp], _i32),
@@ -461,6 +462,10 @@ class StoreGroup:
         device (out-of-range pairs are dropped and reported by the next flush)."""
         pairs = _pairs(pairs)
         _check(lib.gs_group_submit_member_pairs(self.h, member, _p(pairs), pairs.shape[0]))
+
+    def fence(self) -> None:
+        """gs_group_fence: the next query waits for the updates submitted so far only."""
+        _check(lib.gs_group_fence(self.h))
 
     def prepare(self) -> None:
         """Allocates the group's ingest buffers and query scratch now rather than on first use."""

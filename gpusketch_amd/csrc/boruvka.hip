@@ -1176,6 +1176,11 @@ void free_group_query(Group* g) {
   g->d_giant = g->d_gsym = g->d_vmap = nullptr;  // d_gsym lies inside d_gstate
   if (g->h_gstate) (void)hipHostFree(g->h_gstate);
   if (g->round_exec) (void)hipGraphExecDestroy(g->round_exec);
+  if (g->qstream) (void)hipStreamDestroy(g->qstream);
+  if (g->ev_fence) (void)hipEventDestroy(g->ev_fence);
+  g->qstream = nullptr;
+  g->ev_fence = nullptr;
+  g->fenced = false;
   g->d_parent = g->d_comp = g->d_vflags = g->d_gstate = g->h_gstate = nullptr;
   g->d_acc = nullptr;
   g->round_exec = nullptr;
@@ -1208,7 +1213,8 @@ static size_t gstate_words(uint32_t G) { return 3 * (size_t)G + 3 + 2 * ((size_t
 // the batched query's scratch and its captured round, made once per group
 int prepare_group_query(Group* g) {
   const uint32_t N = g->n_total, G = (uint32_t)g->members.size();
-  hipStream_t q = g->s->stream;
+  if (!g->qstream) GS_HIP(hipStreamCreateWithFlags(&g->qstream, hipStreamNonBlocking));
+  hipStream_t q = g->qstream;
   if (!g->d_parent) {
     g->acc_stride = g->max_bpc + 1;
     GS_HIP(hipMalloc(&g->d_parent, sizeof(uint32_t) * N));
@@ -1240,9 +1246,12 @@ int prepare_group_query(Group* g) {
 int group_query_cc(Group* g, const uint8_t* mask, uint32_t* labels, uint32_t* num_cc,
                    uint32_t* rounds, int32_t* status) {
   const uint32_t G = (uint32_t)g->members.size();
-  hipStream_t q = g->s->stream;
   int prc = prepare_group_query(g);
   if (prc) return prc;
+  // the query runs on its own stream: after the group stream's fence (gs_group_fence), or after
+  // everything (the caller synchronized the members' writers)
+  hipStream_t q = g->qstream;
+  if (g->fenced) GS_HIP(hipStreamWaitEvent(q, g->ev_fence, 0));
   // host-initialised state: nothing modified, members outside the mask already done, round 0;
   // the queried members' bases and positions in vmap; which members' sketches XOR to zero (the
   // giant of such a member is summed from the rest)

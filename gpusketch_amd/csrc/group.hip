@@ -224,16 +224,50 @@ int gs_group_prepare(gs_group* h) {
   return GS_OK;
 }
 
+int gs_group_fence(gs_group* h) {
+  GS_CHECK(h, GS_ERR_INVALID, "null group");
+  Group& g = h->impl;
+  int rc = drain_stream(g.s);  // launch what is pending
+  if (rc) return rc;
+  if (!g.ev_fence) GS_HIP(hipEventCreateWithFlags(&g.ev_fence, hipEventDisableTiming));
+  GS_HIP(hipEventRecord(g.ev_fence, g.s->stream));
+  g.fenced = true;
+  return GS_OK;
+}
+
 int gs_group_query_cc(gs_group* h, const uint8_t* member_mask, uint32_t* labels,
                       uint32_t* num_cc, uint32_t* rounds, int32_t* status) {
   GS_CHECK(h && labels, GS_ERR_INVALID, "null argument");
   Group& g = h->impl;
-  // every writer of every member must be done before the sketches are read
-  for (Store* m : g.members) {
-    int rc = sync_store(m);
-    if (rc) return rc;
+  if (g.fenced) {
+    // the group stream up to the fence (waited for on the device), every other writer of the
+    // queried members now
+    for (size_t i = 0; i < g.members.size(); i++) {
+      if (member_mask && !member_mask[i]) continue;
+      Store* m = g.members[i];
+      std::vector<Stream*> ss;
+      {
+        std::lock_guard<std::mutex> lk(m->mu);
+        ss = m->streams;
+      }
+      for (Stream* s : ss) {
+        if (s == g.s) continue;
+        int rc = drain_stream(s);
+        if (rc) return rc;
+        GS_HIP(hipStreamSynchronize(s->stream));
+      }
+      GS_HIP(hipStreamSynchronize(m->qstream));
+    }
+  } else {
+    // every writer of every member must be done before the sketches are read
+    for (Store* m : g.members) {
+      int rc = sync_store(m);
+      if (rc) return rc;
+    }
   }
-  return group_query_cc(&g, member_mask, labels, num_cc, rounds, status);
+  const int rc = group_query_cc(&g, member_mask, labels, num_cc, rounds, status);
+  g.fenced = false;
+  return rc;
 }
 
 }  // extern "C"

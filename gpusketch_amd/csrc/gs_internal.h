@@ -273,6 +273,9 @@ struct Group {
   uint32_t* d_gsym = nullptr;    // per member, each query: its vertex sketches XOR to zero
                                  // (inside d_gstate)
   hipGraphExec_t round_exec = nullptr;  // one captured Boruvka round over every member
+  hipStream_t qstream = nullptr;  // the batched queries' stream
+  hipEvent_t ev_fence = nullptr;  // gs_group_fence: the updates the next query waits for
+  bool fenced = false;
 };
 
 // vertex space and partition of whatever a stream updates

@@ -469,7 +469,7 @@ void SketchServer::finish(Request& r, RequestResult res) {
   done.push_back(std::move(res));
 }
 
-void SketchServer::apply_updates(std::vector<Request*>& upd) {
+void SketchServer::apply_updates(std::vector<Request*>& upd, bool may_wait) {
   if (upd.empty()) return;
   const double t_start = now_s();
   if (cfg.batching) {
@@ -486,8 +486,8 @@ void SketchServer::apply_updates(std::vector<Request*>& upd) {
     // the launches are not waited for while more requests are queued (a query waits for its
     // tenant's store); the last step before the queue runs dry waits, and so reports pairs the
     // device dropped (out of range) in error()
-    bool more = false;
-    {
+    bool more = !may_wait;
+    if (!more) {
       std::lock_guard<std::mutex> lk(mu);
       more = !queue.empty();
     }
@@ -524,6 +524,15 @@ void SketchServer::run_step(std::vector<Request>& step) {
   }
   // 1. the updates the step's queries depend on: those of the queried tenants
   apply_updates(upd_q);
+  // with batching and connectivity queries: a fence after those, then every other tenant's
+  // updates are launched at once and run beside the batched query (its own stream, waiting for
+  // the fence only)
+  bool others_applied = false;
+  if (cfg.batching && !cc.empty()) {
+    gs_throw_if(gs_group_fence(group));
+    apply_updates(upd_o, false);
+    others_applied = true;
+  }
   // 2. connectivity queries: one batched Boruvka over the queried tenants
   const double t_cc = now_s();
   if (!cc.empty()) {
@@ -610,8 +619,8 @@ void SketchServer::run_step(std::vector<Request>& step) {
     phases.mincut_s += now_s() - t_mc;
     phases.mincut_queries += mc.size();
   }
-  // 4. every other tenant's updates
-  apply_updates(upd_o);
+  // 4. every other tenant's updates (unless they went in before the queries)
+  if (!others_applied) apply_updates(upd_o);
 }
 
 void SketchServer::cut_loop() {

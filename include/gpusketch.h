@@ -277,6 +277,14 @@ int gs_group_prepare(gs_group* g);
  * GS_ERR_OUT_OF_SAMPLES (the call then also returns GS_ERR_OUT_OF_SAMPLES). */
 int gs_group_query_cc(gs_group* g, const uint8_t* member_mask, uint32_t* labels,
                       uint32_t* num_cc, uint32_t* rounds, int32_t* status);
+/* Launches the group's pending updates and marks the point its next gs_group_query_cc waits
+ * for: that query then runs on the group's query stream after the updates submitted before the
+ * fence (and after the queried members' other writers), while group updates submitted after the
+ * fence may run beside it. The caller must not submit updates of the queried members between
+ * the fence and the query. Without a fence the query waits for every writer of every member.
+ * No counterpart in the reference; the serving loop overlaps a step's queries with the updates of
+ * the tenants it does not query. */
+int gs_group_fence(gs_group* g);
 
 /* ---- Allocation-free update ----------------------------------------------------------------
  * No counterpart in the reference: an entry point for callers that cannot allocate device

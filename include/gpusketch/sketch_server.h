@@ -3,15 +3,15 @@
 // memory is released). Requests (update batches, connectivity queries, min-cut queries) are
 // taken in arrival order; with batching on, one scheduling step gathers every pending request of
 // every admitted tenant up to and including that tenant's first pending query, applies all
-// gathered updates of the queried tenants in one device submission (a store group), answers all
-// gathered connectivity queries with one batched Boruvka, then the min-cut queries, then applies
-// every other tenant's gathered updates in one more submission. Update payloads stay in pinned
-// host memory from arrival: each goes by DMA straight to the device, where the member's base is
-// added and the ids range-checked (no host pass), and a step's launches are not waited for: the
-// next step's host work overlaps them, and a query waits for its tenant's store. A min-cut
-// query's forests are computed on a second thread (its tenant's later requests wait for it, the
-// other tenants' requests go on meanwhile). A query therefore sees exactly the updates of its
-// tenant that arrived before it, so every answer is the one the tenant would get alone. With batching off, requests are served one at a time on
+// gathered updates of the queried tenants in one device submission (a store group), sets a
+// fence, launches every other tenant's gathered updates, and answers all gathered connectivity
+// queries with one batched Boruvka that waits for the fence only (so it runs beside those other
+// updates); min-cut queries go to a second thread, which computes their forests and cuts (the
+// tenant's later requests wait for it, the others go on). Update payloads stay in pinned host
+// memory from arrival: each goes by DMA straight to the device, where the member's base is added
+// and the ids range-checked (no host pass), and a step's launches are not waited for. A query
+// therefore sees exactly the updates of its tenant that arrived before it, so every answer is
+// the one the tenant would get alone. With batching off, requests are served one at a time on
 // per-tenant streams (the plain single-stream path). No counterpart in the reference, whose
 // algorithm objects each hold one graph and are driven by one stream.
 #pragma once
@@ -172,7 +172,8 @@ class SketchServer {
   SpillBuf take_spill_buffer(size_t words);
   void give_back(SpillBuf& b);
   void run_step(std::vector<Request>& step);
-  void apply_updates(std::vector<Request*>& upd);
+  // may_wait: the launches may be waited for when no request is queued (reporting dropped pairs)
+  void apply_updates(std::vector<Request*>& upd, bool may_wait = true);
   void ensure_group();
   void finish(Request& r, RequestResult res);
   uint64_t enqueue(Request r);

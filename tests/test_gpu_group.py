@@ -317,3 +317,37 @@ def test_batched_config_full_shape_256_members():
     finally:
         for st in stores:
             st.close()
+
+
+def test_fenced_query_sees_updates_before_the_fence():
+    """gs_group_fence: a masked query after a fence answers for the updates submitted before it,
+    while updates of other members submitted after the fence are applied beside it."""
+    rng = np.random.default_rng(77)
+    seeds = [3000 + 11 * i for i in range(len(SIZES))]
+    stores = [gs.SketchStore(n, s) for n, s in zip(SIZES, seeds)]
+    try:
+        with gs.StoreGroup(stores) as grp:
+            grp.prepare()
+            first = [member_pairs(rng, n, 4 * n) for n in SIZES]
+            later = [member_pairs(rng, n, 8 * n) for n in SIZES]
+            mask = np.array([1, 0, 1, 0, 1, 0], dtype=np.uint8)
+            for rep in range(3):
+                grp.submit_pairs(first)
+                grp.fence()
+                # the unqueried members' updates go in after the fence, without a flush
+                grp.submit_pairs([None if mask[i] else later[i] for i in range(len(SIZES))])
+                labels, ncc, _, status = grp.query_cc(mask)
+                for i in range(len(SIZES)):
+                    if not mask[i]:
+                        continue
+                    applied = np.concatenate([first[i]] * (rep + 1))
+                    want = orc.dsu_components(SIZES[i], net_edges(applied))
+                    assert status[i] == 0 and np.array_equal(labels[i], want), (rep, i)
+                grp.flush()
+                for i in range(len(SIZES)):
+                    seq = [first[i]] * (rep + 1) + ([] if mask[i] else [later[i]] * (rep + 1))
+                    got = stores[i].export_buckets()
+                    assert np.array_equal(got, oracle_store(SIZES[i], seeds[i], np.concatenate(seq))), (rep, i)
+    finally:
+        for s in stores:
+            s.close()
