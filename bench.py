@@ -304,6 +304,21 @@ def host_cpu() -> dict:
             "cgroup_quota_cores": quota, "usable": usable}
 
 
+def build_provenance() -> dict:
+    """Digest of the sources the loaded libgpusketch.so was compiled from, and whether it is the
+    digest of the sources in this tree."""
+    import importlib.util
+
+    import gpusketch_amd as gs
+    spec = importlib.util.spec_from_file_location(
+        "gpusketch_amd_build", os.path.join(ROOT, "gpusketch_amd", "build.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    loaded = gs.source_digest()
+    return {"library": "gpusketch_amd/libgpusketch.so", "source_digest": loaded[:16],
+            "matches_tree": loaded == mod.source_digest()}
+
+
 def load_profile(name: str, edges_per_launch: float):
     """A committed counter summary (profiles/<name>), used only when it was measured at the launch
     size this run times."""
@@ -484,6 +499,7 @@ def bench_single(args) -> dict:
                         "queries": len(lat), "first_query_with_scratch_setup": first_ms,
                         "components": ncc, "rounds": rounds, "cpu_boruvka": cpu_query},
         "pcie_inclusive": pcie,
+        "build": build_provenance(),
     }
     stream.close()
     store.close()

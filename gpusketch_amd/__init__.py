@@ -60,6 +60,7 @@ _u32, _u64, _i32, _vp, _dbl, _sz = (ctypes.c_uint32, ctypes.c_uint64, ctypes.c_i
                                     ctypes.c_void_p, ctypes.c_double, ctypes.c_size_t)
 _sigs = {
     "gs_last_error": ([], ctypes.c_char_p),
+    "gs_source_digest": ([_vp, _sz], _i32),
     "gs_geometry_for": ([_u32, _dbl, _u32, _vp], _i32),
     "gs_geometry_for_sketch": ([_u32, _dbl, _u32, _u32, _vp], _i32),
     "gs_hash_known_answer": ([_vp, _vp, _u64, _vp], _i32),
@@ -145,6 +146,13 @@ NOALLOC_MAX_EDGES = 1 << 20
 STORE_CUBESKETCH = 1
 # columns per sample of the CubeSketch configuration (include/sketch.h:208-211)
 CUBESKETCH_COLS_PER_SAMPLE = 7
+
+
+def source_digest() -> str:
+    """SHA-256 of the sources the loaded library was built from (gs_source_digest)."""
+    buf = ctypes.create_string_buffer(80)
+    _check(lib.gs_source_digest(buf, len(buf)))
+    return buf.value.decode()
 
 
 def geometry_for(n: int, sketches_factor: float = 1.0, cols_per_sample: int = 1,

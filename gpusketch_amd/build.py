@@ -7,6 +7,7 @@ Run ``python gpusketch_amd/build.py`` from the repository root (or call ``build(
 from __future__ import annotations
 
 import glob
+import hashlib
 import os
 import subprocess
 import sys
@@ -32,6 +33,31 @@ HIP_FLAGS = [
 ]
 
 
+def library_sources() -> list[str]:
+    """Every file that goes into libgpusketch.so: the sources under csrc and the headers they
+    include (include/gpusketch.h, include/gpusketch/*.h), sorted."""
+    pats = [os.path.join(CSRC, "*"), os.path.join(ROOT, "include", "gpusketch.h"),
+            os.path.join(ROOT, "include", "gpusketch", "*.h")]
+    files = []
+    for p in pats:
+        files += [f for f in glob.glob(p) if os.path.isfile(f)]
+    return sorted(files)
+
+
+def source_digest() -> str:
+    """SHA-256 over the library's sources (relative path and bytes of each) and its compile
+    flags. Compiled into the library (gs_source_digest), so a loaded library can be checked
+    against the tree it runs from."""
+    h = hashlib.sha256()
+    h.update(" ".join(HIP_FLAGS[:-2]).encode())
+    for f in library_sources():
+        h.update(os.path.relpath(f, ROOT).encode() + b"\0")
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+        h.update(b"\0")
+    return h.hexdigest()
+
+
 def _run(cmd):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -45,11 +71,13 @@ def build_library(jobs: int = 8) -> str:
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
     objs = []
     cmds = []
+    digest = [f"-DGS_SOURCE_DIGEST=\"{source_digest()}\""]
     for s in srcs:
         o = os.path.join(OBJ, os.path.basename(s) + ".o")
         objs.append(o)
         lang = ["-x", "hip"] if s.endswith(".hip") else []
-        cmds.append([HIPCC, *HIP_FLAGS, *lang, "-c", s, "-o", o])
+        extra = digest if os.path.basename(s) == "api.hip" else []
+        cmds.append([HIPCC, *HIP_FLAGS, *extra, *lang, "-c", s, "-o", o])
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"

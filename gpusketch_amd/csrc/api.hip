@@ -151,6 +151,17 @@ extern "C" {
 
 const char* gs_last_error(void) { return g_last_error.c_str(); }
 
+#ifndef GS_SOURCE_DIGEST
+#define GS_SOURCE_DIGEST "unknown"
+#endif
+
+int gs_source_digest(char* out, size_t cap) {
+  static const char digest[] = GS_SOURCE_DIGEST;
+  GS_CHECK(out && cap >= sizeof(digest), GS_ERR_INVALID, "gs_source_digest: buffer too small");
+  std::memcpy(out, digest, sizeof(digest));
+  return GS_OK;
+}
+
 int gs_geometry_for(uint32_t n, double sketches_factor, uint32_t cols_per_sample,
                     gs_geometry* out) {
   return gs_geometry_for_sketch(n, sketches_factor, cols_per_sample, 0, out);

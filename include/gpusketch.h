@@ -62,6 +62,11 @@ typedef struct gs_geometry {
 /* Thread-local message of the last failing call ("" if none). */
 const char* gs_last_error(void);
 
+/* Build provenance (no counterpart in the reference): writes the SHA-256 (64 hex digits and a
+ * NUL, so cap >= 65) that the build recipe took over the library's sources and compile flags, so
+ * a caller can check that the loaded library was built from the sources beside it. */
+int gs_source_digest(char* out, size_t cap);
+
 /* Sizing: Sketch::calc_vector_length / calc_cc_samples / calc_bkt_per_col
  * (include/sketch.h:63-77,198-206) for n vertices and a CCAlgConfiguration sketches_factor. */
 int gs_geometry_for(uint32_t n, double sketches_factor, uint32_t cols_per_sample,

@@ -26,6 +26,22 @@ def test_header_symbols_exported():
     assert sorted(decl) == gs.EXPORTED
 
 
+def test_library_built_from_these_sources():
+    """The loaded libgpusketch.so carries the digest of the sources it was compiled from; it must
+    be the digest of the sources in this tree (a stale build fails here)."""
+    build = _build_recipe()
+    assert gs.source_digest() == build.source_digest()
+
+
+def _build_recipe():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "gpusketch_amd_build", os.path.join(ROOT, "gpusketch_amd", "build.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
 def test_geometry_matches_oracle():
     for n in (2, 3, 100, 1024, 8192, 1 << 15, 1 << 17, 1 << 18):
         g = gs.geometry_for(n)
