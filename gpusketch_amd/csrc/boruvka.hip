@@ -21,11 +21,11 @@ namespace gs {
 
 constexpr int kQThreads = 256;
 // query flags: [0] modified this round, [1] out of samples, [2] forest edges, [3] giant
-// supernode of the round, [kFlagBig + k] big supernode k (see pick_big_kernel), [kFlagDone] the
+// supernode of the round, [kFlagBig + k] big supernode k (see pick_big), [kFlagDone] the
 // partition is final (or out of samples): every later round's kernels return at once,
 // [kFlagRound] index of the round being run, [kFlagRounds] rounds the query took
 constexpr uint32_t kNoGiant = 0xFFFFFFFFu;
-// big supernodes per round (pick_big_kernel) and the share of kPickSamples sampled vertices one
+// big supernodes per round (pick_big) and the share of kPickSamples sampled vertices one
 // must hold: 2 of 256 at n = 2^17 is about 1000 members (R-MAT, plain CC query: 1.10 ms against
 // 1.22 ms with 8 of at least 4 samples; 1024 samples cost more to pick than they save)
 constexpr int kBig = 64;
@@ -70,6 +70,12 @@ __device__ __forceinline__ uint32_t find_root(uint32_t* parent, uint32_t x) {
 constexpr uint32_t kVZero = 1;    // sampled ZERO this round (no edge leaves the supernode)
 constexpr uint32_t kVMerged = 2;  // absorbed another supernode this round
 constexpr uint32_t kVDone = 4;    // ZERO and not merged since: the supernode can never change
+
+// A root's state for the next round: a root that sampled ZERO and absorbed nothing has no
+// leaving edge and stays as it is in every later round. settle(settle(f)) == settle(f).
+__device__ __forceinline__ uint32_t settle(uint32_t f) {
+  return ((f & (kVZero | kVDone)) && !(f & kVMerged)) ? kVDone : 0u;
+}
 
 __device__ __forceinline__ bool unite(uint32_t* parent, uint32_t a, uint32_t b) {
   for (;;) {
@@ -138,21 +144,23 @@ __device__ __forceinline__ void handle_sample(int res, uint64_t idx, uint32_t se
   }
 }
 
-__global__ void init_query_kernel(uint32_t n, uint32_t* parent, uint32_t* comp, uint32_t* vflags) {
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+__global__ void init_query_kernel(uint32_t n, uint32_t* parent, uint32_t* comp, uint32_t* vflags,
+                                  uint32_t* flags) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (flags && t < (uint32_t)kFlagWords) flags[t] = 0;
+  for (uint32_t v = t; v < n; v += gridDim.x * blockDim.x) {
     parent[v] = v;
     comp[v] = v;
     vflags[v] = 0;
   }
 }
 
-// Rounds are enqueued several at a time without looking at the host; this kernel ends a round
-// on the device (after compress): a round that modified nothing leaves the partition final (the
-// query took `round` rounds); a modified round needs another, and past the last sample that is
-// out of samples (the reference throws when it samples past num_samples,
+// Rounds are enqueued several at a time without looking at the host; the round ends on the
+// device (the last workgroup of compress_kernel): a round that modified nothing leaves the
+// partition final (the query took `round` rounds); a modified round needs another, and past the
+// last sample that is out of samples (the reference throws when it samples past num_samples,
 // src/sketch.cpp:110-114); otherwise the next round starts with its modified flag clear.
-__global__ void end_round_kernel(uint32_t* flags, uint32_t num_samples) {
-  if (threadIdx.x != 0 || flags[kFlagDone]) return;
+__device__ __forceinline__ void end_round(uint32_t* flags, uint32_t num_samples) {
   const uint32_t round = flags[kFlagRound];
   if (!flags[0]) {
     flags[kFlagDone] = 1;
@@ -165,10 +173,6 @@ __global__ void end_round_kernel(uint32_t* flags, uint32_t num_samples) {
     flags[kFlagRound] = round + 1;
     flags[0] = 0;
   }
-}
-
-__global__ void clear_all_flags_kernel(uint32_t* flags) {
-  for (uint32_t i = threadIdx.x; i < (uint32_t)kFlagWords; i += blockDim.x) flags[i] = 0;
 }
 
 __global__ void zero_acc_kernel(Bucket16* acc, uint64_t count) {
@@ -366,7 +370,7 @@ __global__ __launch_bounds__(kQThreads) void round0_kernel(StoreView st, uint32_
   }
 }
 
-// Big supernodes of a round, picked from kPickSamples sampled vertices (pick_big_kernel): up to kBig live
+// Big supernodes of a round, picked from kPickSamples sampled vertices (pick_big): up to kBig live
 // supernodes holding at least kMinBig of the samples each (flags[4 + k], kNoGiant when unused),
 // largest first. Their column sums are formed from per-block partial sums (acc_kernel sums them in
 // LDS) instead of atomics into one accumulator row (which thousands of waves would contend for:
@@ -379,18 +383,16 @@ __global__ __launch_bounds__(kQThreads) void round0_kernel(StoreView st, uint32_
 // supernodes do (those are read for the complement and are skipped otherwise).
 constexpr uint32_t kMinGiant = 8;
 
-__global__ __launch_bounds__(kPickSamples) void pick_big_kernel(uint32_t n,
-                                                                const uint32_t* __restrict__ comp,
-                                                                const uint32_t* __restrict__ vflags,
-                                                                int symmetric, uint32_t* flags) {
-  if (flags[kFlagDone]) return;
+// pick_big: one workgroup of kPickSamples threads, after the round's compress
+__device__ void pick_big(uint32_t n, uint32_t* parent, const uint32_t* vflags, int symmetric,
+                         uint32_t* flags) {
   __shared__ uint32_t s_r[kPickSamples];
   __shared__ uint32_t s_cand[kPickSamples];
   __shared__ uint32_t s_ncand, s_max, s_done;
   const uint32_t t = threadIdx.x;
   const uint32_t v = (uint32_t)(((uint64_t)t * n) / kPickSamples + n / (2 * kPickSamples));
-  uint32_t r = comp[v];
-  if (vflags[r] & kVDone) r = kNoGiant;
+  uint32_t r = find_root(parent, v);
+  if (settle(vflags[r]) & kVDone) r = kNoGiant;
   s_r[t] = r;
   if (t == 0) {
     s_ncand = 0;
@@ -661,20 +663,34 @@ __global__ __launch_bounds__(kQThreads) void sample_wide_kernel(StoreView st, ui
   if (my_res >= 0) handle_sample(my_res, my_idx, v, st.n, parent, vflags, flags, forest, forest_cap);
 }
 
-// Publishes the roots after a round's unions and settles the root states: a root that sampled
-// ZERO and absorbed nothing has no leaving edge and stays as it is in every later round.
-// `flags` (null in the group query) skips the work once the partition is final.
+// Publishes the roots after a round's unions and settles the root states (settle).
+// Workgroup 0 also ends the round (end_round) and, when another round follows, picks its big
+// supernodes (pick_big): one launch instead of three per round. It reads nothing the other
+// workgroups write but root states, and settle() maps a root state before and after its
+// workgroup settled it to the same answer. Other workgroups may see the done flag it sets and
+// return: the done flag follows a round without unions (compress would change no root) or runs
+// out of samples (the partition is discarded).
 __global__ __launch_bounds__(kQThreads) void compress_kernel(uint32_t n, uint32_t* parent,
                                                              uint32_t* comp, uint32_t* vflags,
-                                                             const uint32_t* __restrict__ flags) {
-  uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= n || (flags && flags[kFlagDone])) return;
-  uint32_t r = find_root(parent, v);
-  comp[v] = r;
-  if (r == v) {
-    const uint32_t f = vflags[v];
-    vflags[v] = ((f & (kVZero | kVDone)) && !(f & kVMerged)) ? kVDone : 0u;
+                                                             uint32_t* flags, uint32_t num_samples,
+                                                             int symmetric) {
+  static_assert(kQThreads == kPickSamples, "pick_big runs on one compress workgroup");
+  if (flags[kFlagDone]) return;
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) {
+    const uint32_t r = find_root(parent, v);
+    comp[v] = r;
+    if (r == v) vflags[v] = settle(vflags[v]);
   }
+  if (blockIdx.x != 0) return;
+  __shared__ uint32_t s_done;
+  __syncthreads();  // every lane of the workgroup has read the done flag
+  if (threadIdx.x == 0) {
+    end_round(flags, num_samples);
+    s_done = flags[kFlagDone];
+  }
+  __syncthreads();
+  if (!s_done) pick_big(n, parent, vflags, symmetric, flags);
 }
 
 __global__ __launch_bounds__(kQThreads) void finalize_parent_kernel(uint32_t n, uint32_t* parent,
@@ -767,8 +783,7 @@ static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t
   uint64_t cap = record_forest ? st->forest_cap - forest_base : 0;
   if (col0 >= st->num_samples) return fail(GS_ERR_OUT_OF_SAMPLES, "sketch out of samples");
   hipLaunchKernelGGL(init_query_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
-                     st->d_vflags);
-  hipLaunchKernelGGL(clear_all_flags_kernel, dim3(1), dim3(64), 0, q, st->d_flags);
+                     st->d_vflags, st->d_flags);
   // round 0
   if (wide)
     hipLaunchKernelGGL(round0_wide_kernel, g_r0, block, 0, q, v, cps, col0, st->d_parent,
@@ -776,15 +791,12 @@ static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t
   else
     hipLaunchKernelGGL(round0_kernel, g_r0, block, 0, q, v, col0 * cps, st->d_parent,
                        st->d_vflags, st->d_flags, forest, cap, nullptr, nullptr);
-  hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
-                     st->d_vflags, st->d_flags);
-  hipLaunchKernelGGL(end_round_kernel, dim3(1), dim3(64), 0, q, st->d_flags, st->num_samples);
-  GS_HIP(hipGetLastError());
   const int symmetric = st->symmetric.load() ? 1 : 0;
+  hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
+                     st->d_vflags, st->d_flags, st->num_samples, symmetric);
+  GS_HIP(hipGetLastError());
   for (uint32_t enqueued = 1;; enqueued += kRoundsAhead) {
     for (uint32_t k = 0; k < kRoundsAhead; k++) {
-      hipLaunchKernelGGL(pick_big_kernel, dim3(1), dim3(kPickSamples), 0, q, n, st->d_comp,
-                         st->d_vflags, symmetric, st->d_flags);
       hipLaunchKernelGGL(acc_kernel, dim3(g_acc.x, (w + 63) / 64), block,
                          sizeof(Bucket16) * kBig * std::min(64u, w), q, v, cps,
                          st->d_parent, st->d_comp, st->d_vflags, st->d_flags, st->d_acc,
@@ -798,8 +810,7 @@ static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t
         hipLaunchKernelGGL(sample_kernel, g_smp, block, 0, q, v, st->d_comp, st->d_acc,
                            st->d_parent, st->d_vflags, st->d_flags, forest, cap);
       hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
-                         st->d_vflags, st->d_flags);
-      hipLaunchKernelGGL(end_round_kernel, dim3(1), dim3(64), 0, q, st->d_flags, st->num_samples);
+                         st->d_vflags, st->d_flags, st->num_samples, symmetric);
     }
     GS_HIP(hipGetLastError());
     GS_HIP(hipMemcpyAsync(st->h_flags, st->d_flags, sizeof(uint32_t) * kFlagWords,
@@ -1130,7 +1141,7 @@ __global__ __launch_bounds__(kQThreads) void g_compress_kernel(const uint32_t* _
     comp[v] = r;
     if (r == v) {
       const uint32_t f = vflags[v];
-      vflags[v] = ((f & (kVZero | kVDone)) && !(f & kVMerged)) ? kVDone : 0u;
+      vflags[v] = settle(f);
     }
   }
 }

