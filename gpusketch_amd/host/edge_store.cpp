@@ -16,15 +16,22 @@ EdgeStore::EdgeStore(size_t seed, node_id_t n, size_t sketch_bytes, size_t num_s
       adjlist(n),
       vertex_contracted(n, 1),
       max_edges(sketch_bytes / store_edge_bytes),
-      adj_mutex(new std::mutex[n]) {}
+      adj_mutex(new std::mutex[n]) {
+  needs_contraction = n;  // no contraction running: nothing to claim
+}
 
 EdgeStore::~EdgeStore() = default;
 
+// Tags with the deepest subgraph index num_subgraphs - 1 as the cap, the cap the caller's routing
+// uses once sketch subgraphs exist (the reference caps these first inserts at num_subgraphs,
+// src/edge_store.cpp:54 against src/mc_gpu_sketch_alg.cpp:160, so an edge at the deepest level
+// inserted before the first contraction and deleted after it would be stored as two entries
+// that never cancel).
 TaggedUpdateBatch EdgeStore::insert_adj_edges(node_id_t src, const std::vector<node_id_t>& dsts) {
   std::vector<SubgraphTaggedUpdate> tagged(dsts.size());
   for (size_t i = 0; i < dsts.size(); i++) {
     const vec_t idx = concat_pairing_fn(src, dsts[i]);
-    tagged[i] = {(node_id_t)Bucket_Boruvka::get_index_depth(idx, (long)seed, num_subgraphs),
+    tagged[i] = {(node_id_t)Bucket_Boruvka::get_index_depth(idx, (long)seed, num_subgraphs - 1),
                  dsts[i]};
   }
   return insert_adj_edges(src, 0, tagged.data(), tagged.size());
@@ -100,21 +107,34 @@ std::vector<SubgraphTaggedUpdate> EdgeStore::vertex_contract(node_id_t src) {
   return ret;
 }
 
-TaggedUpdateBatch EdgeStore::vertex_advance_subgraph(node_id_t cur_first_es) {
-  const node_id_t lo = cur_first_es - 1;
+// Hands back the next uncontracted vertex's entries during a contraction. The contraction ends
+// (true_min_subgraph catches up) only when every vertex has been handled, not merely claimed: a
+// thread that claimed a vertex and waits for its lock still belongs to this contraction, so the
+// next one cannot start under it (it would reset the vertex's flag and move the first stored
+// subgraph past entries that are then handed to the older subgraph only). The epoch is read
+// after the claim, which pins it until this vertex is handled.
+TaggedUpdateBatch EdgeStore::vertex_advance_subgraph(node_id_t) {
   for (;;) {
-    const node_id_t src = needs_contraction.fetch_add(1);
+    // every vertex claimed: no claim past them either (callers may poll while the last ones are
+    // handled, and the counter must not wrap around)
+    node_id_t src = needs_contraction.load();
+    if (src < num_vertices) src = needs_contraction.fetch_add(1);
     if (src >= num_vertices) {
-      if (src == num_vertices) {
-        std::lock_guard<std::mutex> lk(contract_lock);
-        ++true_min_subgraph;  // every vertex has been handed back once
-      }
-      return {0, lo, cur_first_es, {}};
+      const node_id_t first = (node_id_t)cur_subgraph.load();
+      return {0, first - 1, first, {}};
     }
-    std::lock_guard<std::mutex> lk(adj_mutex[src]);
-    if (!adjlist[src].empty() && !vertex_contracted[src])
-      return {src, lo, cur_first_es, vertex_contract(src)};
-    vertex_contracted[src] = 1;
+    const node_id_t first = (node_id_t)cur_subgraph.load();
+    TaggedUpdateBatch out{src, first - 1, first, {}};
+    {
+      std::lock_guard<std::mutex> lk(adj_mutex[src]);
+      if (!adjlist[src].empty() && !vertex_contracted[src]) out.dsts_data = vertex_contract(src);
+      vertex_contracted[src] = 1;
+    }
+    if (vertices_handled.fetch_add(1) + 1 == num_vertices) {
+      std::lock_guard<std::mutex> lk(contract_lock);
+      ++true_min_subgraph;  // every vertex has been handed back once
+    }
+    if (!out.dsts_data.empty()) return out;
   }
 }
 
@@ -122,7 +142,10 @@ void EdgeStore::check_if_too_big() {
   if (num_edges < max_edges) return;
   std::lock_guard<std::mutex> lk(contract_lock);
   if (true_min_subgraph < cur_subgraph) return;  // a contraction is already running
+  // no vertex can be claimed until the first stored subgraph has moved (needs_contraction is
+  // still past the last vertex), so every claim of this contraction sees its epoch
   std::fill(vertex_contracted.begin(), vertex_contracted.end(), 0);
-  needs_contraction = 0;
+  vertices_handled = 0;
   ++cur_subgraph;
+  needs_contraction = 0;
 }

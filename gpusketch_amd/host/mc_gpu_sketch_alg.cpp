@@ -75,6 +75,7 @@ void MCGPUSketchAlg::apply_flush_updates() {
         TaggedUpdateBatch more = edge_store.vertex_advance_subgraph(
             (node_id_t)edge_store.get_first_store_subgraph());
         if (!more.dsts_data.empty()) complete_update_batch(t, more);
+        else std::this_thread::yield();  // the last claimed vertices are being handled
       }
     });
   for (auto& th : threads) th.join();

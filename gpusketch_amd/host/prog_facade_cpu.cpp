@@ -302,7 +302,7 @@ static void test_edge_store() {
   for (auto& kv : live) {
     if (!kv.second) continue;
     const vec_t idx = concat_pairing_fn(kv.first.first, kv.first.second);
-    const size_t tag = Bucket_Boruvka::get_index_depth(idx, 123, subgraphs);
+    const size_t tag = Bucket_Boruvka::get_index_depth(idx, 123, subgraphs - 1);
     if (tag >= es.get_first_store_subgraph()) expect.insert(kv.first);
   }
   std::set<std::pair<node_id_t, node_id_t>> got;
@@ -357,14 +357,15 @@ struct RoutingAlg {
   }
 };
 
-static void test_mc_routing_with_driver() {
+// workers driver threads, a store budget of budget_edges entries, updates random toggles
+static void mc_routing_case(uint64_t stream_seed, int workers, size_t budget_edges, int updates) {
   const node_id_t n = 400;
   const size_t nsub = 12;
-  const std::string path = work_dir + "/facade_routing.data";
-  std::mt19937_64 rng(29);
+  const std::string path = work_dir + "/facade_routing_" + std::to_string(stream_seed) + ".data";
+  std::mt19937_64 rng(stream_seed);
   std::vector<GraphStreamUpdate> ups;
   std::map<std::pair<node_id_t, node_id_t>, int> live;
-  for (int i = 0; i < 30000; i++) {
+  for (int i = 0; i < updates; i++) {
     node_id_t u = rng() % n, v = rng() % n;
     if (u == v) continue;
     ups.push_back({{u, v}, INSERT});
@@ -375,10 +376,10 @@ static void test_mc_routing_with_driver() {
     w.write_header(n, ups.size());
     w.write_updates(ups.data(), ups.size());
   }
-  RoutingAlg alg(n, nsub, 31, 3000 * sizeof(SubgraphTaggedUpdate));
+  RoutingAlg alg(n, nsub, 31, budget_edges * sizeof(SubgraphTaggedUpdate));
   {
     BinaryFileStream s(path);
-    GraphSketchDriver<RoutingAlg> drv(&alg, &s, DriverConfiguration().worker_threads(3), 2);
+    GraphSketchDriver<RoutingAlg> drv(&alg, &s, DriverConfiguration().worker_threads(workers), 2);
     drv.process_stream_until(END_OF_STREAM);
     drv.prep_query(1);
   }
@@ -426,6 +427,15 @@ static void test_mc_routing_with_driver() {
   CHECK(alg.es.get_num_edges() == stored);
   CHECK(alg.es.get_edges().size() == stored);
   std::remove(path.c_str());
+}
+
+static void test_mc_routing_with_driver() { mc_routing_case(29, 3, 3000, 30000); }
+
+// Contractions back to back under many workers: a budget barely above what one contraction
+// leaves makes the next contraction start as soon as the previous one ends, so a vertex claimed
+// by one contraction and handled late would meet the next one's state.
+static void test_mc_routing_contractions_back_to_back() {
+  for (uint64_t seed = 1; seed <= 12; seed++) mc_routing_case(1000 + seed, 8, 300, 20000);
 }
 
 static int sketch_dump(const char* pairs_path, node_id_t n, uint64_t seed, const char* out_path) {
@@ -534,6 +544,7 @@ int main(int argc, char** argv) {
     run("driver_delivers_every_update", test_driver_delivers_every_update);
     run("edge_store", test_edge_store);
     run("mc_routing_with_driver", test_mc_routing_with_driver);
+    run("mc_routing_contractions_back_to_back", test_mc_routing_contractions_back_to_back);
     run("graph_verifier", test_graph_verifier);
     std::printf("%d failure(s)\n", failures);
     return failures ? 1 : 0;

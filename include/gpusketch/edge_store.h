@@ -19,8 +19,8 @@ class EdgeStore {
             size_t start_subgraph = 0);
   ~EdgeStore();
 
-  // tags each destination with get_index_depth(concat(src, dst), seed, num_subgraphs); for use
-  // while no sketch subgraph exists
+  // tags each destination with get_index_depth(concat(src, dst), seed, num_subgraphs - 1); for
+  // use while no sketch subgraph exists
   TaggedUpdateBatch insert_adj_edges(node_id_t src, const std::vector<node_id_t>& dst_vertices);
   // pre-tagged destinations; caller_first_es_subgraph is the first stored subgraph the caller saw
   TaggedUpdateBatch insert_adj_edges(node_id_t src, node_id_t caller_first_es_subgraph,
@@ -46,7 +46,8 @@ class EdgeStore {
   std::atomic<size_t> cur_subgraph;       // subgraph at which edges enter the store
   std::atomic<size_t> true_min_subgraph;  // smallest subgraph still present in the store
   std::atomic<edge_id_t> num_edges{0};
-  std::atomic<node_id_t> needs_contraction{0};
+  std::atomic<node_id_t> needs_contraction{0};  // next vertex to claim in a contraction
+  std::atomic<node_id_t> vertices_handled{0};   // vertices of the contraction handled so far
   std::vector<std::vector<SubgraphTaggedUpdate>> adjlist;
   std::vector<char> vertex_contracted;
   size_t max_edges;
