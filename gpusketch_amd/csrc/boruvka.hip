@@ -35,6 +35,7 @@ constexpr int kFlagBig = 4;
 constexpr int kFlagDone = kFlagBig + kBig;
 constexpr int kFlagRound = kFlagDone + 1;
 constexpr int kFlagRounds = kFlagDone + 2;
+constexpr int kFlagRoots = kFlagDone + 3;  // components, counted after the last round
 constexpr int kFlagWords = kFlagDone + 4;
 // Vertices per wave of the per-vertex kernels of the single-store query (one lane per vertex for
 // the vertex attributes, the wave's vertices then taken a few at a time, one bucket per lane).
@@ -75,6 +76,17 @@ constexpr uint32_t kVDone = 4;    // ZERO and not merged since: the supernode ca
 // leaving edge and stays as it is in every later round. settle(settle(f)) == settle(f).
 __device__ __forceinline__ uint32_t settle(uint32_t f) {
   return ((f & (kVZero | kVDone)) && !(f & kVMerged)) ? kVDone : 0u;
+}
+
+// find_root where no union runs during the kernel (compress_kernel): plain loads, served by the
+// caches, instead of the device-coherent loads the concurrent unions of sample_kernel need
+__device__ __forceinline__ uint32_t find_root_settled(const uint32_t* parent, uint32_t x) {
+  uint32_t p = parent[x];
+  while (p != x) {
+    x = p;
+    p = parent[x];
+  }
+  return x;
 }
 
 __device__ __forceinline__ bool unite(uint32_t* parent, uint32_t a, uint32_t b) {
@@ -384,14 +396,14 @@ __global__ __launch_bounds__(kQThreads) void round0_kernel(StoreView st, uint32_
 constexpr uint32_t kMinGiant = 8;
 
 // pick_big: one workgroup of kPickSamples threads, after the round's compress
-__device__ void pick_big(uint32_t n, uint32_t* parent, const uint32_t* vflags, int symmetric,
+__device__ void pick_big(uint32_t n, const uint32_t* parent, const uint32_t* vflags, int symmetric,
                          uint32_t* flags) {
   __shared__ uint32_t s_r[kPickSamples];
   __shared__ uint32_t s_cand[kPickSamples];
   __shared__ uint32_t s_ncand, s_max, s_done;
   const uint32_t t = threadIdx.x;
   const uint32_t v = (uint32_t)(((uint64_t)t * n) / kPickSamples + n / (2 * kPickSamples));
-  uint32_t r = find_root(parent, v);
+  uint32_t r = find_root_settled(parent, v);
   if (settle(vflags[r]) & kVDone) r = kNoGiant;
   s_r[t] = r;
   if (t == 0) {
@@ -678,7 +690,7 @@ __global__ __launch_bounds__(kQThreads) void compress_kernel(uint32_t n, uint32_
   if (flags[kFlagDone]) return;
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v < n) {
-    const uint32_t r = find_root(parent, v);
+    const uint32_t r = find_root_settled(parent, v);
     comp[v] = r;
     if (r == v) vflags[v] = settle(vflags[v]);
   }
@@ -691,6 +703,15 @@ __global__ __launch_bounds__(kQThreads) void compress_kernel(uint32_t n, uint32_
   }
   __syncthreads();
   if (!s_done) pick_big(n, parent, vflags, symmetric, flags);
+}
+
+// components of the final partition: roots (comp[v] == v) counted per wave, one atomic each
+__global__ __launch_bounds__(kQThreads) void count_roots_kernel(uint32_t n,
+                                                                const uint32_t* __restrict__ comp,
+                                                                uint32_t* count) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t m = __ballot(v < n && comp[v] == v);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(count, (uint32_t)__builtin_popcountll(m));
 }
 
 __global__ __launch_bounds__(kQThreads) void finalize_parent_kernel(uint32_t n, uint32_t* parent,
@@ -711,6 +732,8 @@ void free_query_scratch(Store* st) {
   (void)hipFree(st->d_flags);
   (void)hipFree(st->d_forest);
   if (st->h_flags) (void)hipHostFree(st->h_flags);
+  if (st->h_labels) (void)hipHostFree(st->h_labels);
+  st->h_labels = nullptr;
   st->d_parent = st->d_comp = st->d_vflags = st->d_flags = st->d_forest = st->h_flags = nullptr;
   st->d_acc = nullptr;
   st->forest_cap = 0;
@@ -735,6 +758,7 @@ static int ensure_query_scratch(Store* st, uint64_t forest_pairs) {
     GS_HIP(hipMalloc(&st->d_bpart, sizeof(Bucket16) * w * nparts * kBig));
     GS_HIP(hipMalloc(&st->d_flags, sizeof(uint32_t) * kFlagWords));
     GS_HIP(hipHostMalloc((void**)&st->h_flags, sizeof(uint32_t) * kFlagWords, hipHostMallocDefault));
+    GS_HIP(hipHostMalloc((void**)&st->h_labels, sizeof(uint32_t) * n, hipHostMallocDefault));
     uint64_t cnt = (uint64_t)n * w;
     hipLaunchKernelGGL(zero_acc_kernel, dim3(2048), dim3(256), 0, st->qstream, st->d_acc, cnt);
     GS_HIP(hipGetLastError());
@@ -764,8 +788,11 @@ static int ensure_query_scratch(Store* st, uint64_t forest_pairs) {
 
 // One Borůvka pass. Forest edges are appended at d_forest[2*forest_base...]; returns the pass's
 // forest size in *forest_len.
-// Rounds are enqueued speculatively, kRoundsAhead at a time, and the host reads the flags once
-// per batch: a round after the final one only launches kernels that return at once.
+// Rounds are enqueued speculatively and the host reads the flags once per batch of rounds: a
+// round after the final one only launches kernels that return at once (about 8 us of device
+// time), a host round trip between batches costs several times that. The first batch covers
+// the rounds the store's previous pass took plus one (at least kRoundsAhead), later batches
+// kRoundsAhead rounds each.
 constexpr uint32_t kRoundsAhead = 4;
 static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t* rounds_out,
                         uint64_t* forest_len, bool record_forest = true) {
@@ -795,8 +822,9 @@ static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t
   hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
                      st->d_vflags, st->d_flags, st->num_samples, symmetric);
   GS_HIP(hipGetLastError());
-  for (uint32_t enqueued = 1;; enqueued += kRoundsAhead) {
-    for (uint32_t k = 0; k < kRoundsAhead; k++) {
+  uint32_t batch = std::max(kRoundsAhead, std::min(st->last_rounds + 1, st->num_samples));
+  for (uint32_t enqueued = 1;; enqueued += batch, batch = kRoundsAhead) {
+    for (uint32_t k = 0; k < batch; k++) {
       hipLaunchKernelGGL(acc_kernel, dim3(g_acc.x, (w + 63) / 64), block,
                          sizeof(Bucket16) * kBig * std::min(64u, w), q, v, cps,
                          st->d_parent, st->d_comp, st->d_vflags, st->d_flags, st->d_acc,
@@ -821,6 +849,7 @@ static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t
              "Boruvka rounds did not settle");
   }
   const uint32_t round = st->h_flags[kFlagRounds];
+  st->last_rounds = round;
   if (st->h_flags[1])
     return fail(GS_ERR_OUT_OF_SAMPLES,
                 "This sketch (seed=%llu, max samples=%u) cannot be sampled more times (cur idx=%u)!",
@@ -846,17 +875,23 @@ int boruvka_query(Store* st, const uint32_t*, uint32_t* label_out, uint32_t* num
   rc = boruvka_pass(st, 0, 0, rounds, &fl, record);
   if (rc) return rc;
   const uint32_t n = st->view.n;
-  if (label_out) GS_HIP(hipMemcpy(label_out, st->d_comp, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
-  if (num_cc || (forest_len && !record)) {
-    std::vector<uint32_t> lab;
-    const uint32_t* l = label_out;
-    if (!l) {
-      lab.resize(n);
-      GS_HIP(hipMemcpy(lab.data(), st->d_comp, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
-      l = lab.data();
-    }
-    uint32_t c = 0;
-    for (uint32_t i = 0; i < n; i++) c += l[i] == i;
+  hipStream_t q = st->qstream;
+  const bool count = num_cc || (forest_len && !record);
+  if (count) {  // flags[kFlagRoots] is zero: the init kernel cleared it and no round uses it
+    const dim3 grid((n + kQThreads - 1) / kQThreads), block(kQThreads);
+    hipLaunchKernelGGL(count_roots_kernel, grid, block, 0, q, n, st->d_comp,
+                       st->d_flags + kFlagRoots);
+    GS_HIP(hipGetLastError());
+    GS_HIP(hipMemcpyAsync(st->h_flags + kFlagRoots, st->d_flags + kFlagRoots, sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, q));
+  }
+  // labels through the store's pinned staging (one DMA straight from the device), then copied
+  if (label_out) GS_HIP(hipMemcpyAsync(st->h_labels, st->d_comp, sizeof(uint32_t) * n,
+                                       hipMemcpyDeviceToHost, q));
+  if (count || label_out) GS_HIP(hipStreamSynchronize(q));
+  if (label_out) std::memcpy(label_out, st->h_labels, sizeof(uint32_t) * n);
+  if (count) {
+    const uint32_t c = st->h_flags[kFlagRoots];
     if (num_cc) *num_cc = c;
     if (!record) fl = n - c;  // a spanning forest has one edge per union: n - components
   }

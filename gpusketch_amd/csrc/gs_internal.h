@@ -238,6 +238,8 @@ struct Store {
   uint32_t* d_forest = nullptr;      // forest edges (min,max) pairs, capacity k * n
   uint64_t forest_cap = 0;           // pairs
   uint32_t* h_flags = nullptr;       // pinned mirror of d_flags
+  uint32_t* h_labels = nullptr;      // pinned staging of the query's labels (n words)
+  uint32_t last_rounds = 0;          // rounds the store's last Boruvka pass took
   hipStream_t qstream = nullptr;
   Stream* qs = nullptr;              // internal stream for forest toggles (not in `streams`)
   size_t bytes() const {
