@@ -706,9 +706,13 @@ __global__ __launch_bounds__(kQThreads) void compress_kernel(uint32_t n, uint32_
 }
 
 // components of the final partition: roots (comp[v] == v) counted per wave, one atomic each
+// (enqueued after every batch of rounds: counts only once the partition is final, which happens
+// in exactly one of them)
 __global__ __launch_bounds__(kQThreads) void count_roots_kernel(uint32_t n,
                                                                 const uint32_t* __restrict__ comp,
+                                                                const uint32_t* flags,
                                                                 uint32_t* count) {
+  if (!flags[kFlagDone]) return;
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t m = __ballot(v < n && comp[v] == v);
   if ((threadIdx.x & 63) == 0 && m) atomicAdd(count, (uint32_t)__builtin_popcountll(m));
@@ -794,8 +798,15 @@ static int ensure_query_scratch(Store* st, uint64_t forest_pairs) {
 // the rounds the store's previous pass took plus one (at least kRoundsAhead), later batches
 // kRoundsAhead rounds each.
 constexpr uint32_t kRoundsAhead = 4;
+// tail (optional): enqueued after every batch of rounds, before the host waits for the flags --
+// the root count and a copy of the labels into the store's pinned staging, valid after the
+// batch that ends the pass
+struct PassTail {
+  bool count_roots = false;
+  bool labels = false;
+};
 static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t* rounds_out,
-                        uint64_t* forest_len, bool record_forest = true) {
+                        uint64_t* forest_len, bool record_forest = true, PassTail tail = {}) {
   const StoreView& v = st->view;
   const uint32_t n = v.n;
   const dim3 grid((n + kQThreads - 1) / kQThreads), block(kQThreads);
@@ -840,9 +851,15 @@ static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t
       hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
                          st->d_vflags, st->d_flags, st->num_samples, symmetric);
     }
+    if (tail.count_roots)  // flags[kFlagRoots] is zero until the final batch's count
+      hipLaunchKernelGGL(count_roots_kernel, grid, block, 0, q, n, st->d_comp, st->d_flags,
+                         st->d_flags + kFlagRoots);
     GS_HIP(hipGetLastError());
     GS_HIP(hipMemcpyAsync(st->h_flags, st->d_flags, sizeof(uint32_t) * kFlagWords,
                           hipMemcpyDeviceToHost, q));
+    if (tail.labels)
+      GS_HIP(hipMemcpyAsync(st->h_labels, st->d_comp, sizeof(uint32_t) * n,
+                            hipMemcpyDeviceToHost, q));
     GS_HIP(hipStreamSynchronize(q));
     if (st->h_flags[kFlagDone]) break;
     GS_CHECK(enqueued <= st->num_samples + kRoundsAhead, GS_ERR_INVALID,
@@ -872,23 +889,13 @@ int boruvka_query(Store* st, const uint32_t*, uint32_t* label_out, uint32_t* num
   if (rc) return rc;
   uint64_t fl = 0;
   const bool record = forest_out && forest_cap;
-  rc = boruvka_pass(st, 0, 0, rounds, &fl, record);
+  const bool count = num_cc || (forest_len && !record);
+  PassTail tail;
+  tail.count_roots = count;
+  tail.labels = label_out != nullptr;  // through the store's pinned staging, one DMA
+  rc = boruvka_pass(st, 0, 0, rounds, &fl, record, tail);
   if (rc) return rc;
   const uint32_t n = st->view.n;
-  hipStream_t q = st->qstream;
-  const bool count = num_cc || (forest_len && !record);
-  if (count) {  // flags[kFlagRoots] is zero: the init kernel cleared it and no round uses it
-    const dim3 grid((n + kQThreads - 1) / kQThreads), block(kQThreads);
-    hipLaunchKernelGGL(count_roots_kernel, grid, block, 0, q, n, st->d_comp,
-                       st->d_flags + kFlagRoots);
-    GS_HIP(hipGetLastError());
-    GS_HIP(hipMemcpyAsync(st->h_flags + kFlagRoots, st->d_flags + kFlagRoots, sizeof(uint32_t),
-                          hipMemcpyDeviceToHost, q));
-  }
-  // labels through the store's pinned staging (one DMA straight from the device), then copied
-  if (label_out) GS_HIP(hipMemcpyAsync(st->h_labels, st->d_comp, sizeof(uint32_t) * n,
-                                       hipMemcpyDeviceToHost, q));
-  if (count || label_out) GS_HIP(hipStreamSynchronize(q));
   if (label_out) std::memcpy(label_out, st->h_labels, sizeof(uint32_t) * n);
   if (count) {
     const uint32_t c = st->h_flags[kFlagRoots];
