@@ -79,7 +79,11 @@ int init_stream_objects(Stream* s) {
   }
   if (hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess)
     return fail(GS_ERR_HIP, "HIP copy stream creation failed");
-  hipEvent_t* evs[] = {&s->stage_free, &s->pend_free[0], &s->pend_free[1], &s->copied};
+  if (hipStreamCreateWithFlags(&s->gstream, hipStreamNonBlocking) != hipSuccess)
+    return fail(GS_ERR_HIP, "HIP grouping stream creation failed");
+  hipEvent_t* evs[] = {&s->stage_free,    &s->pend_free[0],  &s->pend_free[1], &s->copied,
+                       &s->grouped[0],    &s->grouped[1],    &s->aggregated[0],
+                       &s->aggregated[1]};
   for (hipEvent_t* e : evs)
     if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess)
       return fail(GS_ERR_HIP, "HIP event creation failed");
@@ -88,8 +92,16 @@ int init_stream_objects(Stream* s) {
 
 void destroy_stream_resources(Stream& s) {
   if (s.stream) (void)hipStreamSynchronize(s.stream);
+  if (s.gstream) (void)hipStreamSynchronize(s.gstream);
   if (s.copy_stream) (void)hipStreamSynchronize(s.copy_stream);
   free_pair_scratch(s.ws);
+  free_pair_scratch(s.ws_alt);
+  for (int i = 0; i < 2; i++) {
+    if (s.grouped[i]) (void)hipEventDestroy(s.grouped[i]);
+    if (s.aggregated[i]) (void)hipEventDestroy(s.aggregated[i]);
+    s.grouped[i] = s.aggregated[i] = nullptr;
+    s.agg_used[i] = false;
+  }
   if (s.h_stage) (void)hipHostFree(s.h_stage);
   if (s.d_stage) (void)hipFree(s.d_stage);
   for (int b = 0; b < 2; b++) {
@@ -105,11 +117,15 @@ void destroy_stream_resources(Stream& s) {
     (void)hipEventDestroy(p.first);
     (void)hipEventDestroy(p.second);
   }
-  for (hipEvent_t g : s.ev_group)
-    if (g) (void)hipEventDestroy(g);
+  for (auto& g : s.ev_group) {
+    if (g.first) (void)hipEventDestroy(g.first);
+    if (g.second) (void)hipEventDestroy(g.second);
+  }
   s.ev_group.clear();
   if (s.stage_free) (void)hipEventDestroy(s.stage_free);
   if (s.copy_stream) (void)hipStreamDestroy(s.copy_stream);
+  if (s.gstream) (void)hipStreamDestroy(s.gstream);
+  s.gstream = nullptr;
   if (s.stream && s.own_stream) (void)hipStreamDestroy(s.stream);
   s.h_stage = nullptr;
   s.d_stage = nullptr;
@@ -427,16 +443,18 @@ int gs_flush(gs_stream* sh) {
   int rc = drain_stream(&s);
   if (rc) return rc;
   GS_HIP(hipStreamSynchronize(s.stream));
-  if (s.ws.d_counters) {
-    uint32_t dropped = 0;
-    GS_HIP(hipMemcpy(&dropped, s.ws.d_counters + 3, sizeof(uint32_t), hipMemcpyDeviceToHost));
-    if (dropped) {
-      GS_HIP(hipMemset(s.ws.d_counters + 3, 0, sizeof(uint32_t)));
-      return fail(GS_ERR_INVALID,
-                  "%u submitted updates named a vertex outside the store (or joined two members "
-                  "of a group) and were dropped", dropped);
-    }
+  uint32_t dropped = 0;
+  for (PairScratch* ws : {&s.ws, &s.ws_alt}) {  // (both scratch sets of the pipeline)
+    if (!ws->d_counters) continue;
+    uint32_t d = 0;
+    GS_HIP(hipMemcpy(&d, ws->d_counters + 3, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (d) GS_HIP(hipMemset(ws->d_counters + 3, 0, sizeof(uint32_t)));
+    dropped += d;
   }
+  if (dropped)
+    return fail(GS_ERR_INVALID,
+                "%u submitted updates named a vertex outside the store (or joined two members of "
+                "a group) and were dropped", dropped);
   return GS_OK;
 }
 
@@ -707,11 +725,12 @@ int gs_stream_kernel_times(gs_stream* sh, double* grouping_ms, double* agg_ms,
     GS_HIP(hipEventElapsedTime(&ms, p.first, p.second));
     s.agg_ms += ms;
     s.agg_launches++;
-    if (s.ev_group[i]) {
+    if (s.ev_group[i].first) {
       float gms = 0;
-      GS_HIP(hipEventElapsedTime(&gms, s.ev_group[i], p.first));
+      GS_HIP(hipEventElapsedTime(&gms, s.ev_group[i].first, s.ev_group[i].second));
       s.grp_ms += gms;
-      hipEventDestroy(s.ev_group[i]);
+      hipEventDestroy(s.ev_group[i].first);
+      hipEventDestroy(s.ev_group[i].second);
     }
     hipEventDestroy(p.first);
     hipEventDestroy(p.second);

@@ -170,7 +170,18 @@ struct Stream {
   hipStream_t stream = nullptr;
   bool own_stream = true;  // false: `stream` belongs to someone else (a store's query stream)
   hipStream_t copy_stream = nullptr;
+  // Edge-pair launches are pipelined: the appends into the fill buffer and the grouping kernels
+  // run on gstream, the aggregation kernel on `stream` (after an event of its grouping), so the
+  // grouping of launch k+1 runs while launch k aggregates. The two launches use two grouping
+  // scratch sets: ws, and ws_alt, allocated the first time a launch finds ws still read by an
+  // aggregation. grouped[i] / aggregated[i]: set i's grouping done / aggregation done.
+  hipStream_t gstream = nullptr;
   PairScratch ws;
+  PairScratch ws_alt;
+  int ws_next = 0;  // set of the next launch (0: ws, 1: ws_alt)
+  hipEvent_t grouped[2] = {nullptr, nullptr};
+  hipEvent_t aggregated[2] = {nullptr, nullptr};
+  bool agg_used[2] = {false, false};
   // host staging (pinned) and matching device buffers for vertex-batch submissions
   uint32_t* h_stage = nullptr;
   size_t h_stage_bytes = 0;
@@ -204,9 +215,9 @@ struct Stream {
   bool profile = false;
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pairs;
-  // per aggregation launch fed by the edge-pair grouping: the event recorded before the grouping
-  // kernels (the launch's ev_pairs.first ends them); nullptr for vertex-batch launches
-  std::vector<hipEvent_t> ev_group;
+  // per aggregation launch fed by the edge-pair grouping: the events recorded before and after
+  // the grouping kernels (on gstream); nullptrs for vertex-batch launches
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_group;
   double grp_ms = 0;
   double agg_ms = 0;
   uint64_t agg_launches = 0;
@@ -325,15 +336,17 @@ int drain_stream(Stream* s);
 int ingest_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uint32_t min_depth,
                         uint32_t depth_cap);
 uint64_t stream_launch_edges(const Stream* s);
-int ensure_pair_scratch(Stream* s, uint64_t nedges);
+int ensure_pair_scratch(Stream* s, PairScratch& ws, uint64_t nedges);
 // the grouping scratch for `nedges` pairs carved from one buffer at `base` (256-byte aligned
 // pieces; base = nullptr only sizes it): sets ws's pointers and class layout, returns the bytes
 size_t carve_pair_scratch(const Stream* s, uint64_t nedges, char* base, PairScratch& ws);
 // append a device segment of m pairs to a launch
 void add_pair_segment(PairSegs& sg, uint32_t& nblk, const uint32_t* p, uint64_t m);
-// the grouping and aggregation kernels over `sg` on s's HIP stream with scratch `ws`
+// the grouping kernels over `sg` with scratch `ws` on s's gstream, then the aggregation kernel
+// on s's stream (set: ws is s->ws (0) or s->ws_alt (1), the events of the pipeline); set -1:
+// every kernel on s's stream, no events (a caller's scratch)
 int run_pair_pipeline(Stream* s, const PairSegs& sg, uint32_t nblk, uint64_t nedges,
-                      PairScratch& ws, hipEvent_t input_read);
+                      PairScratch& ws, hipEvent_t input_read, int set);
 void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t stream);
 PartParams make_part_params(uint32_t n, int local_bits = -1);
 int autotune_stream(Stream* s, const uint32_t* d_pairs, uint64_t nedges, gs_tuning* best);

@@ -1348,7 +1348,7 @@ static int launch_wide_aggregate(Stream* s, const WorkItem* d_items, const ItemC
   if (s->profile) {
     GS_HIP(hipEventRecord(e1, s->stream));
     s->ev_pairs.emplace_back(e0, e1);
-    s->ev_group.push_back(nullptr);
+    s->ev_group.emplace_back(nullptr, nullptr);
     s->agg_updates += updates;
   }
   return GS_OK;
@@ -1410,7 +1410,7 @@ static int launch_aggregate(Stream* s, const WorkItem* d_items, const ItemClasse
   if (s->profile) {
     GS_HIP(hipEventRecord(e1, s->stream));
     s->ev_pairs.emplace_back(e0, e1);
-    s->ev_group.push_back(nullptr);
+    s->ev_group.emplace_back(nullptr, nullptr);
     s->agg_updates += updates;
   }
   return GS_OK;
@@ -1495,16 +1495,23 @@ size_t carve_pair_scratch(const Stream* s, uint64_t nedges, char* base, PairScra
   return off;
 }
 
-int ensure_pair_scratch(Stream* s, uint64_t nedges) {
-  PairScratch& ws = s->ws;
+// (scratch being replaced may still be read by a grouping on gstream or an aggregation on stream)
+static int sync_pipeline(Stream* s) {
+  GS_HIP(hipStreamSynchronize(s->stream));
+  if (s->gstream) GS_HIP(hipStreamSynchronize(s->gstream));
+  return GS_OK;
+}
+
+int ensure_pair_scratch(Stream* s, PairScratch& ws, uint64_t nedges) {
   const PartParams& pp = stream_part(s);
   if (!ws.d_counters) {
     GS_HIP(hipMalloc(&ws.d_counters, sizeof(uint32_t) * kCounterWords));
-    GS_HIP(hipMemsetAsync(ws.d_counters, 0, sizeof(uint32_t) * kCounterWords, s->stream));
+    GS_HIP(hipMemsetAsync(ws.d_counters, 0, sizeof(uint32_t) * kCounterWords,
+                          s->gstream ? s->gstream : s->stream));
   }
   const uint64_t hist_words = (uint64_t)pp.nbuckets * (part_blocks(nedges) + kMaxSegs);
   if (ws.cap_buckets < pp.nbuckets || ws.cap_hist < hist_words) {
-    GS_HIP(hipStreamSynchronize(s->stream));
+    if (int rc = sync_pipeline(s)) return rc;
     (void)hipFree(ws.d_hist);
     (void)hipFree(ws.d_totals);
     ws.d_hist = ws.d_totals = nullptr;
@@ -1516,7 +1523,7 @@ int ensure_pair_scratch(Stream* s, uint64_t nedges) {
     ws.cap_hist = hw;
   }
   if (ws.cap_edges < nedges) {
-    GS_HIP(hipStreamSynchronize(s->stream));
+    if (int rc = sync_pipeline(s)) return rc;
     (void)hipFree(ws.d_entries);
     (void)hipFree(ws.d_sorted);
     (void)hipFree(ws.d_items);
@@ -1550,9 +1557,9 @@ uint64_t stream_launch_edges(const Stream* s) {
 // the fill buffers, sized for the stream's launch size (grown on demand, never while pending)
 static int ensure_pending(Stream* s) {
   const uint64_t want = stream_launch_edges(s);
-  if (s->pend_cap >= want && s->d_pend[0]) return ensure_pair_scratch(s, s->pend_cap);
+  if (s->pend_cap >= want && s->d_pend[0]) return ensure_pair_scratch(s, s->ws, s->pend_cap);
   GS_CHECK(s->pend_n == 0, GS_ERR_INVALID, "launch size changed with updates pending");
-  GS_HIP(hipStreamSynchronize(s->stream));
+  if (int rc = sync_pipeline(s)) return rc;
   if (s->copy_stream) GS_HIP(hipStreamSynchronize(s->copy_stream));
   for (int b = 0; b < 2; b++) {
     (void)hipFree(s->d_pend[b]);
@@ -1564,7 +1571,11 @@ static int ensure_pending(Stream* s) {
       return fail(GS_ERR_OOM, "edge buffers of %llu edges unavailable", (unsigned long long)want);
   s->pend_cap = want;
   s->pend_buf = 0;
-  return ensure_pair_scratch(s, want);
+  if (s->ws_alt.d_counters) {  // the second set grows on its next use
+    int rc = ensure_pair_scratch(s, s->ws_alt, want);
+    if (rc) return rc;
+  }
+  return ensure_pair_scratch(s, s->ws, want);
 }
 
 int prepare_stream(Stream* s) {
@@ -1581,7 +1592,7 @@ static int append_pairs(Stream* s, const uint32_t* src, uint64_t m, uint32_t min
   if (reinterpret_cast<const uint2*>(src) != dst || min_depth || mid.n) {
     const uint64_t seed = s->group ? 0 : s->store->view.seed;
     const unsigned grid = (unsigned)std::min<uint64_t>((m + 255) / 256, 4096);
-    hipLaunchKernelGGL(append_kernel, dim3(grid), dim3(256), 0, host_copy ? s->copy_stream : s->stream,
+    hipLaunchKernelGGL(append_kernel, dim3(grid), dim3(256), 0, host_copy ? s->copy_stream : s->gstream,
                        reinterpret_cast<const uint2*>(src), dst, m, stream_vertices(s), seed,
                        min_depth, depth_cap, mid.base, mid.n);
     GS_HIP(hipGetLastError());
@@ -1604,7 +1615,9 @@ static int wait_fill_buffer(Stream* s) {
 // from a caller's fixed buffer). `input_read` (may be null) is recorded once the pairs have been
 // read (after K3).
 int run_pair_pipeline(Stream* s, const PairSegs& sg, uint32_t nblk, uint64_t nedges,
-                      PairScratch& ws, hipEvent_t input_read) {
+                      PairScratch& ws, hipEvent_t input_read, int set) {
+  const bool piped = set >= 0 && s->gstream;
+  hipStream_t gst = piped ? s->gstream : s->stream;  // the grouping kernels' stream
   const PartParams& pp = stream_part(s);
   const uint32_t nverts = stream_vertices(s);
   const size_t sort_lds = sort_lds_bytes(pp.local_bits);
@@ -1618,51 +1631,67 @@ int run_pair_pipeline(Stream* s, const PairSegs& sg, uint32_t nblk, uint64_t ned
       (rc = ensure_dyn_lds(scatter_fn, scatter_lds_bytes(pp.nbuckets, packed))) ||
       (rc = ensure_dyn_lds(sort_fn, sort_lds)))
     return rc;
-  hipEvent_t g0 = nullptr;  // start of the grouping kernels (measurement only)
+  // the scratch set's previous aggregation must be done before its grouping overwrites it
+  if (piped && s->agg_used[set]) GS_HIP(hipStreamWaitEvent(gst, s->aggregated[set], 0));
+  hipEvent_t g0 = nullptr, g1 = nullptr;  // around the grouping kernels (measurement only)
   if (s->profile) {
     GS_HIP(hipEventCreate(&g0));
-    GS_HIP(hipEventRecord(g0, s->stream));
+    GS_HIP(hipEventCreate(&g1));
+    GS_HIP(hipEventRecord(g0, gst));
   }
   const uint32_t* vgraph = s->group ? s->group->d_vgraph : nullptr;
   hipLaunchKernelGGL(bucket_count_kernel, dim3(nblk), dim3(kPartThreads),
-                     sizeof(uint32_t) * pp.nbuckets, s->stream, sg, nverts, vgraph, pp, nblk,
+                     sizeof(uint32_t) * pp.nbuckets, gst, sg, nverts, vgraph, pp, nblk,
                      ws.d_hist, ws.d_counters);
   GS_HIP(hipGetLastError());
   uint32_t* const bstart = ws.d_totals + pp.nbuckets;  // bucket region starts
-  hipLaunchKernelGGL(bucket_offsets_kernel, dim3(pp.nbuckets), dim3(kOffThreads), 0, s->stream,
+  hipLaunchKernelGGL(bucket_offsets_kernel, dim3(pp.nbuckets), dim3(kOffThreads), 0, gst,
                      ws.d_hist, nblk, ws.d_totals);
   GS_HIP(hipGetLastError());
-  hipLaunchKernelGGL(bucket_starts_kernel, dim3(1), dim3(kOffThreads), 0, s->stream, ws.d_totals,
+  hipLaunchKernelGGL(bucket_starts_kernel, dim3(1), dim3(kOffThreads), 0, gst, ws.d_totals,
                      pp.nbuckets, bstart);
   GS_HIP(hipGetLastError());
   if (packed)
     hipLaunchKernelGGL(bucket_scatter_kernel<true>, dim3(nblk), dim3(kPartThreads),
-                       scatter_lds_bytes(pp.nbuckets, true), s->stream, sg, nverts, vgraph, pp,
+                       scatter_lds_bytes(pp.nbuckets, true), gst, sg, nverts, vgraph, pp,
                        nblk, bstart, ws.d_hist, ws.d_entries);
   else
     hipLaunchKernelGGL(bucket_scatter_kernel<false>, dim3(nblk), dim3(kPartThreads),
-                       scatter_lds_bytes(pp.nbuckets, false), s->stream, sg, nverts, vgraph, pp,
+                       scatter_lds_bytes(pp.nbuckets, false), gst, sg, nverts, vgraph, pp,
                        nblk, bstart, ws.d_hist, ws.d_entries);
   GS_HIP(hipGetLastError());
-  if (input_read) GS_HIP(hipEventRecord(input_read, s->stream));
+  if (input_read) GS_HIP(hipEventRecord(input_read, gst));
   const uint32_t chunk = s->tune_chunk ? s->tune_chunk : (stream_wide(s) ? kWideChunk : kChunk);
   const uint32_t* bases = s->group ? (const uint32_t*)s->group->d_bases : nullptr;
   const uint32_t ngraphs = s->group ? (uint32_t)s->group->members.size() : 1u;
   if (packed)
     hipLaunchKernelGGL(bucket_sort_kernel<true>, dim3(pp.nbuckets), dim3(kSortThreads), sort_lds,
-                       s->stream, ws.d_entries, pp, ws.d_totals, bstart, ws.d_sorted, ws.d_items,
+                       gst, ws.d_entries, pp, ws.d_totals, bstart, ws.d_sorted, ws.d_items,
                        ws.cls, chunk, ws.d_counters + kClassBase, bases, ngraphs);
   else
     hipLaunchKernelGGL(bucket_sort_kernel<false>, dim3(pp.nbuckets), dim3(kSortThreads), sort_lds,
-                       s->stream, ws.d_entries, pp, ws.d_totals, bstart, ws.d_sorted, ws.d_items,
+                       gst, ws.d_entries, pp, ws.d_totals, bstart, ws.d_sorted, ws.d_items,
                        ws.cls, chunk, ws.d_counters + kClassBase, bases, ngraphs);
   GS_HIP(hipGetLastError());
+  if (g1) GS_HIP(hipEventRecord(g1, gst));
+  if (piped) {  // the aggregation waits for this grouping only
+    GS_HIP(hipEventRecord(s->grouped[set], gst));
+    GS_HIP(hipStreamWaitEvent(s->stream, s->grouped[set], 0));
+  }
   const size_t nev = s->ev_pairs.size();
   rc = launch_aggregate(s, ws.d_items, ws.cls, ws.d_counters + kClassBase, ws.d_sorted,
                         2 * nedges);
+  if (!rc && piped) {
+    GS_HIP(hipEventRecord(s->aggregated[set], s->stream));
+    s->agg_used[set] = true;
+  }
   if (g0) {
-    if (s->ev_pairs.size() == nev + 1) s->ev_group.back() = g0;
-    else (void)hipEventDestroy(g0);
+    if (s->ev_pairs.size() == nev + 1) {
+      s->ev_group.back() = {g0, g1};
+    } else {
+      (void)hipEventDestroy(g0);
+      (void)hipEventDestroy(g1);
+    }
   }
   return rc;
 }
@@ -1686,11 +1715,26 @@ static int process_pending(Stream* s) {
   if (s->fill_n) add_pair_segment(sg, nblk, s->d_pend[b], s->fill_n);
   for (const auto& z : s->zsegs) add_pair_segment(sg, nblk, z.first, z.second);
   if (s->copies_pending) {  // every host copy into the buffer has landed
-    GS_HIP(hipStreamWaitEvent(s->stream, s->copied, 0));
+    GS_HIP(hipStreamWaitEvent(s->gstream, s->copied, 0));
     s->copies_pending = false;
   }
+  // the scratch set: the one the previous launch did not use, unless that is the second set,
+  // not yet allocated, and the first is already free (a stream that launches now and then
+  // never allocates the second)
+  int set = s->ws_next;
+  if (set == 1 && !s->ws_alt.d_counters) {
+    const hipError_t q = s->agg_used[0] ? hipEventQuery(s->aggregated[0]) : hipSuccess;
+    if (q == hipSuccess) {
+      set = 0;
+    } else {
+      if (q != hipErrorNotReady) GS_HIP(q);
+      int rc = ensure_pair_scratch(s, s->ws_alt, s->pend_cap);
+      if (rc) return rc;
+    }
+  }
+  s->ws_next = set ^ 1;
   // the fill buffer is free again once K3 has read it
-  int rc = run_pair_pipeline(s, sg, nblk, nedges, s->ws, s->pend_free[b]);
+  int rc = run_pair_pipeline(s, sg, nblk, nedges, set ? s->ws_alt : s->ws, s->pend_free[b], set);
   s->pend_used[b] = true;
   s->pend_n = 0;
   s->fill_n = 0;

@@ -96,7 +96,7 @@ extern "C" int gs_apply_edge_pairs_device_noalloc(gs_store* h, const uint32_t* d
   PairSegs sg{};
   uint32_t nblk = 0;
   add_pair_segment(sg, nblk, d_pairs, nedges);
-  rc = run_pair_pipeline(&s, sg, nblk, nedges, ws, nullptr);
+  rc = run_pair_pipeline(&s, sg, nblk, nedges, ws, nullptr, -1);
   if (rc) return rc;
   uint32_t dropped = 0;
   GS_HIP(hipMemcpyAsync(&dropped, ws.d_counters + 3, sizeof(uint32_t), hipMemcpyDeviceToHost,
