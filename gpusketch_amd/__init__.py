@@ -101,6 +101,8 @@ _sigs = {
     "gs_group_submit_pairs": ([_vp, _vp, _vp], _i32),
     "gs_group_submit_pairs_device": ([_vp, _vp, _u64], _i32),
     "gs_group_submit_member_pairs": ([_vp, _u32, _vp, _u64], _i32),
+    "gs_group_submit_member_pairs_async": ([_vp, _u32, _vp, _u64], _i32),
+    "gs_stream_wait_copies": ([_vp], _i32),
     "gs_group_query_cc": ([_vp, _vp, _vp, _vp, _vp, _vp], _i32),
     "gs_group_fence": ([_vp], _i32),
     "gs_group_prepare": ([_vp], _i32),

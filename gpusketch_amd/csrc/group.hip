@@ -207,6 +207,24 @@ int gs_group_submit_member_pairs(gs_group* h, uint32_t member, const uint32_t* p
   return submit_pairs_host(g.s, pairs, nedges, 0, 0, mid);
 }
 
+int gs_group_submit_member_pairs_async(gs_group* h, uint32_t member, const uint32_t* pairs,
+                                       uint64_t nedges) {
+  GS_CHECK(h, GS_ERR_INVALID, "null group");
+  Group& g = h->impl;
+  GS_CHECK(member < g.members.size(), GS_ERR_INVALID, "member %u of a group of %zu", member,
+           g.members.size());
+  GS_CHECK(pairs || nedges == 0, GS_ERR_INVALID, "null pairs");
+  MemberIds mid;
+  mid.base = g.bases[member];
+  mid.n = g.members[member]->view.n;
+  return submit_pairs_host(g.s, pairs, nedges, 0, 0, mid, false);
+}
+
+int gs_stream_wait_copies(gs_stream* sh) {
+  GS_CHECK(sh, GS_ERR_INVALID, "null stream");
+  return wait_copies(&sh->impl);
+}
+
 int gs_group_submit_pairs_device(gs_group* h, const uint32_t* d_pairs, uint64_t nedges) {
   GS_CHECK(h, GS_ERR_INVALID, "null group");
   GS_CHECK(d_pairs || nedges == 0, GS_ERR_INVALID, "null pairs");

@@ -330,8 +330,14 @@ int submit_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uin
 struct MemberIds {
   uint32_t base = 0, n = 0;
 };
+// wait_read: return only once the caller's memory has been read; else pinned memory is left to
+// the DMA (the caller keeps it unchanged until wait_copies) and other memory is staged before
+// the call returns
 int submit_pairs_host(Stream* s, const uint32_t* pairs, uint64_t nedges, uint32_t min_depth,
-                      uint32_t depth_cap, const MemberIds& mid = MemberIds{});
+                      uint32_t depth_cap, const MemberIds& mid = MemberIds{},
+                      bool wait_read = true);
+// every host batch submitted so far has been read by its DMA
+int wait_copies(Stream* s);
 int drain_stream(Stream* s);
 int ingest_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uint32_t min_depth,
                         uint32_t depth_cap);

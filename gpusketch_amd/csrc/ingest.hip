@@ -1772,7 +1772,7 @@ int submit_pairs_device(Stream* s, const uint32_t* d_pairs, uint64_t nedges, uin
 // vertex outside the store are dropped by the grouping and reported by the next flush, as for
 // device batches, whatever memory they came from.
 int submit_pairs_host(Stream* s, const uint32_t* pairs, uint64_t nedges, uint32_t min_depth,
-                      uint32_t depth_cap, const MemberIds& mid) {
+                      uint32_t depth_cap, const MemberIds& mid, bool wait_read) {
   if (nedges == 0) return GS_OK;
   std::lock_guard<std::mutex> lk(s->mu);
   int rc = ensure_pending(s);
@@ -1805,7 +1805,7 @@ int submit_pairs_host(Stream* s, const uint32_t* pairs, uint64_t nedges, uint32_
     e0 += m;
     if (s->pend_n == s->pend_cap && (rc = process_pending(s))) return rc;
   }
-  if (pinned) {  // the caller may reuse its buffer once the DMA has read it
+  if (pinned && wait_read) {  // the caller may reuse its buffer once the DMA has read it
     // poll first: the copy of a batch takes ~0.15 ms, and a blocking wait's wake-up delay would
     // otherwise idle the link between one submission and the next; block if it runs long
     const auto t0 = std::chrono::steady_clock::now();
@@ -1816,6 +1816,12 @@ int submit_pairs_host(Stream* s, const uint32_t* pairs, uint64_t nedges, uint32_
     if (q == hipErrorNotReady) q = hipEventSynchronize(s->copied);
     GS_HIP(q);
   }
+  return GS_OK;
+}
+
+int wait_copies(Stream* s) {
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->copy_stream) GS_HIP(hipStreamSynchronize(s->copy_stream));
   return GS_OK;
 }
 

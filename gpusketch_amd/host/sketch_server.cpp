@@ -45,7 +45,8 @@ SketchServer::~SketchServer() {
   }
   cut_cv.notify_all();
   cutter.join();
-  if (group) gs_group_destroy(group);
+  if (group) gs_group_destroy(group);  // (waits for its copies)
+  for (auto& b : payload_inflight) gs_host_free(b.second);
   for (auto& t : tenants) {
     if (t->stream) gs_stream_destroy(t->stream);
     if (t->store) gs_store_destroy(t->store);
@@ -477,12 +478,21 @@ void SketchServer::apply_updates(std::vector<Request*>& upd, bool may_wait) {
     const size_t G = group_members.size();
     std::vector<size_t> slot(tenants.size(), (size_t)-1);
     for (size_t i = 0; i < G; i++) slot[group_members[i]] = i;
-    // each payload by DMA as it arrived (member ids; rebase and range check on the device)
-    for (Request* r : upd)
-      gs_throw_if(gs_group_submit_member_pairs(group, (uint32_t)slot[r->tenant], r->pairs,
-                                               r->nedges));
     gs_stream* gsm = nullptr;
     gs_throw_if(gs_group_stream(group, &gsm));
+    release_inflight(gsm);
+    // each payload by DMA as it arrived (member ids; rebase and range check on the device),
+    // enqueued back to back without waiting for the copies: a pool buffer stays out of the pool
+    // until they have drained
+    for (Request* r : upd) {
+      gs_throw_if(gs_group_submit_member_pairs_async(group, (uint32_t)slot[r->tenant], r->pairs,
+                                                     r->nedges));
+      if (r->owned) {
+        std::lock_guard<std::mutex> lk(mu);
+        payload_inflight.emplace_back(r->owned_words, r->owned);
+        r->owned = nullptr;
+      }
+    }
     // the launches are not waited for while more requests are queued (a query waits for its
     // tenant's store); the last step before the queue runs dry waits, and so reports pairs the
     // device dropped (out of range) in error()
@@ -508,6 +518,17 @@ void SketchServer::apply_updates(std::vector<Request*>& upd, bool may_wait) {
     res.updates = r->nedges;
     finish(*r, std::move(res));
   }
+}
+
+void SketchServer::release_inflight(gs_stream* gsm) {
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    if (payload_inflight.empty()) return;
+  }
+  gs_throw_if(gs_stream_wait_copies(gsm));
+  std::lock_guard<std::mutex> lk(mu);
+  for (auto& b : payload_inflight) payload_free.emplace(b.first, b.second);
+  payload_inflight.clear();
 }
 
 void SketchServer::run_step(std::vector<Request>& step) {
@@ -668,6 +689,63 @@ void SketchServer::cut_loop() {
   }
 }
 
+// The requests of the next step. Plain: the first runnable request. Batching: for each runnable
+// tenant (admitted, no min-cut on the cutter), in the order of its oldest queued request, its
+// run -- its queued requests up to and including its first query, so that a query sees exactly
+// the updates that arrived before it -- taken whole while the step holds at most
+// max_step_updates updates (a run that alone exceeds it is cut there). Whole runs rather than
+// slices of every tenant's queue: a launch reads and writes the sketch buckets its updates
+// touch, about as many per vertex for a few updates as for many, so each tenant's updates should
+// reach the device in as few launches as its queries allow.
+void SketchServer::take_step_locked(std::vector<Request>& step) {
+  auto runnable = [&](uint32_t t) { return tenants[t]->admitted && !tenants[t]->mc_busy; };
+  if (!cfg.batching) {
+    for (auto it = queue.begin(); it != queue.end(); ++it)
+      if (runnable(it->tenant)) {
+        step.push_back(std::move(*it));
+        queue.erase(it);
+        return;
+      }
+    return;
+  }
+  // each runnable tenant's run: positions in the queue (ascending) and updates
+  std::vector<std::vector<size_t>> runs(tenants.size());
+  std::vector<uint64_t> run_updates(tenants.size(), 0);
+  std::vector<char> closed(tenants.size(), 0);
+  std::vector<uint32_t> order;  // tenants by their oldest queued request
+  size_t pos = 0;
+  for (auto it = queue.begin(); it != queue.end(); ++it, ++pos) {
+    const uint32_t t = it->tenant;
+    if (!runnable(t) || closed[t]) continue;
+    if (runs[t].empty()) order.push_back(t);
+    runs[t].push_back(pos);
+    if (it->kind == RequestKind::UPDATE) run_updates[t] += it->nedges;
+    else closed[t] = 1;  // the run ends at the tenant's first query
+  }
+  std::vector<char> take(queue.size(), 0);
+  uint64_t nupd = 0;
+  for (uint32_t t : order) {
+    if (nupd + run_updates[t] <= cfg.max_step_updates) {
+      for (size_t p : runs[t]) take[p] = 1;
+      nupd += run_updates[t];
+    } else if (nupd == 0) {  // the first run alone exceeds the cap: its prefix up to the cap
+      for (size_t p : runs[t]) {
+        const Request& r = queue[p];
+        if (nupd > 0 && r.kind == RequestKind::UPDATE && nupd + r.nedges > cfg.max_step_updates)
+          break;
+        take[p] = 1;
+        if (r.kind == RequestKind::UPDATE) nupd += r.nedges;
+      }
+      break;
+    }
+    if (nupd >= cfg.max_step_updates) break;  // (else a smaller run of a later tenant may fit)
+  }
+  std::deque<Request> rest;
+  for (size_t p = 0; p < queue.size(); p++)
+    (take[p] ? step.emplace_back(std::move(queue[p])) : rest.emplace_back(std::move(queue[p])));
+  queue.swap(rest);
+}
+
 void SketchServer::loop() {
   for (;;) {
     std::vector<Request> step;
@@ -708,27 +786,7 @@ void SketchServer::loop() {
           need = UINT32_MAX;
         }
       }
-      if (release.empty() && !room) {
-        std::vector<char> blocked(tenants.size(), 0);
-        uint64_t nupd = 0;
-        for (auto it = queue.begin(); it != queue.end();) {
-          const uint32_t t = it->tenant;
-          if (!tenants[t]->admitted || tenants[t]->mc_busy || blocked[t]) {
-            ++it;
-            continue;
-          }
-          if (it->kind == RequestKind::UPDATE) {
-            const uint64_t m = it->nedges;
-            if (nupd > 0 && nupd + m > cfg.max_step_updates) break;  // step full
-            nupd += m;
-          } else {
-            blocked[t] = 1;  // this tenant's later requests wait for the next step
-          }
-          step.push_back(std::move(*it));
-          it = queue.erase(it);
-          if (!cfg.batching) break;  // plain: one request per step
-        }
-      }
+      if (release.empty() && !room) take_step_locked(step);
       busy = true;
       step_handed.clear();
     }

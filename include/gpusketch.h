@@ -265,6 +265,17 @@ int gs_group_submit_pairs(gs_group* g, const uint32_t* const* pairs, const uint6
  * as they arrived). No counterpart in the reference. */
 int gs_group_submit_member_pairs(gs_group* g, uint32_t member, const uint32_t* pairs,
                                  uint64_t nedges);
+/* gs_group_submit_member_pairs without the wait for the DMA: pinned pairs (gs_host_alloc) are
+ * read by a DMA that may still run after the call returns, so the caller keeps them unchanged
+ * until gs_stream_wait_copies on the group's stream (or its gs_flush / gs_sync) has returned;
+ * other memory is staged before the call returns. A serving loop enqueues a step's payloads back
+ * to back this way. No counterpart in the reference. */
+int gs_group_submit_member_pairs_async(gs_group* g, uint32_t member, const uint32_t* pairs,
+                                       uint64_t nedges);
+/* Waits until every host batch submitted on the stream so far has been read by its DMA (the
+ * buffers of gs_group_submit_member_pairs_async may then be reused). No counterpart in the
+ * reference. */
+int gs_stream_wait_copies(gs_stream* s);
 /* Device-resident pairs already in global ids (base_i added); must stay valid until gs_flush.
  * A pair whose endpoints lie in two different members is skipped and counted like an
  * out-of-range pair (the group stream's next gs_flush returns GS_ERR_INVALID). */

@@ -174,6 +174,8 @@ class SketchServer {
   void run_step(std::vector<Request>& step);
   // may_wait: the launches may be waited for when no request is queued (reporting dropped pairs)
   void apply_updates(std::vector<Request*>& upd, bool may_wait = true);
+  void release_inflight(gs_stream* gsm);
+  void take_step_locked(std::vector<Request>& step);
   void ensure_group();
   void finish(Request& r, RequestResult res);
   uint64_t enqueue(Request r);
@@ -215,6 +217,9 @@ class SketchServer {
   size_t arena_words = 0;
   std::map<size_t, size_t> arena_free;
   std::multimap<size_t, uint32_t*> payload_free;  // pinned payload buffers by capacity (words)
+  // pool buffers submitted without waiting for their DMA (batching): back to payload_free once
+  // the group stream's copies have drained (release_inflight)
+  std::vector<std::pair<size_t, uint32_t*>> payload_inflight;
   std::string last_error;
   // store group over the admitted tenants (batching mode), rebuilt when membership changes
   gs_group* group = nullptr;
