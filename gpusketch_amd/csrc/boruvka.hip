@@ -1172,6 +1172,22 @@ __global__ __launch_bounds__(kQThreads) void g_sample_kernel(
   }
 }
 
+// After the last round: the queried members' labels in member-local ids (the smallest vertex of
+// each component, as the single-store query gives them) and each member's component count (its
+// roots; few, so one atomic each)
+__global__ __launch_bounds__(kQThreads) void g_labels_kernel(const GraphDesc* __restrict__ descs,
+                                                             const uint32_t* __restrict__ vgraph,
+                                                             const uint32_t* __restrict__ vmap,
+                                                             uint32_t G, uint32_t* gstate,
+                                                             uint32_t* comp, uint32_t* ncc) {
+  FOR_SELECTED(V, vmap, gstate, G) {
+    const uint32_t g = vgraph[V];
+    const uint32_t c = comp[V];
+    comp[V] = c - descs[g].base;
+    if (c == V) atomicAdd(&ncc[g], 1u);
+  }
+}
+
 // compress_kernel and finalize_parent_kernel over the queried members' vertices
 __global__ __launch_bounds__(kQThreads) void g_compress_kernel(const uint32_t* __restrict__ vmap,
                                                                uint32_t G,
@@ -1259,9 +1275,11 @@ static void launch_group_round(Group* g, hipStream_t q) {
                      g->d_comp);
 }
 
-// words of d_gstate / h_gstate: the state (3G + 3), the selection (2 per member + 2), then
-// per member whether its vertex sketches XOR to zero (d_gsym points there)
-static size_t gstate_words(uint32_t G) { return 3 * (size_t)G + 3 + 2 * ((size_t)G + 1) + G; }
+// words of d_gstate / h_gstate: the state (3G + 3), the selection (2 per member + 2), per
+// member whether its vertex sketches XOR to zero (d_gsym points there), then per member its
+// component count (g_labels_kernel)
+static size_t gstate_words(uint32_t G) { return 3 * (size_t)G + 3 + 2 * ((size_t)G + 1) + 2 * (size_t)G; }
+static size_t gstate_ncc(uint32_t G) { return 3 * (size_t)G + 3 + 2 * ((size_t)G + 1) + G; }
 
 // the batched query's scratch and its captured round, made once per group
 int prepare_group_query(Group* g) {
@@ -1353,32 +1371,40 @@ int group_query_cc(Group* g, const uint8_t* mask, uint32_t* labels, uint32_t* nu
     GS_HIP(hipGraphLaunch(g->round_exec, q));
     GS_HIP(hipGraphLaunch(g->round_exec, q));
   }
+  // local labels and component counts on the device, then the state, the counts and the labels
+  // of the queried members (one copy per run of consecutive queried members)
+  uint32_t* d_ncc = g->d_gstate + gstate_ncc(G);
+  hipLaunchKernelGGL(g_labels_kernel, grid, block, 0, q, g->d_descs, g->d_vgraph, g->d_vmap, G,
+                     g->d_gstate, g->d_comp, d_ncc);
+  GS_HIP(hipGetLastError());
   GS_HIP(hipMemcpyAsync(g->h_gstate, g->d_gstate, sizeof(uint32_t) * (3 * (size_t)G + 2),
                         hipMemcpyDeviceToHost, q));
-  // labels of the queried members only
-  for (uint32_t i = 0; i < G; i++) {
-    if (mask && !mask[i]) continue;
-    const uint32_t base = g->bases[i], n = g->members[i]->view.n;
-    GS_HIP(hipMemcpyAsync(labels + base, g->d_comp + base, sizeof(uint32_t) * n,
+  GS_HIP(hipMemcpyAsync(g->h_gstate + gstate_ncc(G), d_ncc, sizeof(uint32_t) * G,
+                        hipMemcpyDeviceToHost, q));
+  for (uint32_t i = 0; i < G;) {
+    if (mask && !mask[i]) {
+      i++;
+      continue;
+    }
+    uint32_t j = i + 1;
+    while (j < G && (!mask || mask[j])) j++;
+    const uint32_t b0 = g->bases[i], b1 = g->bases[j - 1] + g->members[j - 1]->view.n;
+    GS_HIP(hipMemcpyAsync(labels + b0, g->d_comp + b0, sizeof(uint32_t) * (b1 - b0),
                           hipMemcpyDeviceToHost, q));
+    i = j;
   }
   GS_HIP(hipStreamSynchronize(q));
   int rc = GS_OK;
   for (uint32_t i = 0; i < G; i++) {
-    const uint32_t base = g->bases[i], n = g->members[i]->view.n;
+    const uint32_t n = g->members[i]->view.n;
     if (mask && !mask[i]) {  // not queried: its labels are left as they are
       if (num_cc) num_cc[i] = n;
       if (rounds) rounds[i] = 0;
       if (status) status[i] = GS_OK;
       continue;
     }
-    uint32_t cc = 0;
-    for (uint32_t v = 0; v < n; v++) {
-      labels[base + v] -= base;  // local smallest vertex of the component
-      cc += labels[base + v] == v;
-    }
     const uint32_t st = g->h_gstate[G + i];
-    if (num_cc) num_cc[i] = cc;
+    if (num_cc) num_cc[i] = g->h_gstate[gstate_ncc(G) + i];
     if (rounds) rounds[i] = g->h_gstate[2 * G + i];
     const int32_t code = st == kGOutOfSamples ? GS_ERR_OUT_OF_SAMPLES
                          : st == kGDone       ? GS_OK
