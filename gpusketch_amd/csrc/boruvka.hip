@@ -1174,17 +1174,27 @@ __global__ __launch_bounds__(kQThreads) void g_sample_kernel(
 
 // After the last round: the queried members' labels in member-local ids (the smallest vertex of
 // each component, as the single-store query gives them) and each member's component count (its
-// roots; few, so one atomic each)
+// roots). The selected vertices are listed member after member, so a wave's lanes mostly share
+// one member: one atomic per member present in the wave, with the count of its roots there.
 __global__ __launch_bounds__(kQThreads) void g_labels_kernel(const GraphDesc* __restrict__ descs,
                                                              const uint32_t* __restrict__ vgraph,
                                                              const uint32_t* __restrict__ vmap,
                                                              uint32_t G, uint32_t* gstate,
                                                              uint32_t* comp, uint32_t* ncc) {
+  const uint32_t lane = threadIdx.x & 63;
   FOR_SELECTED(V, vmap, gstate, G) {
     const uint32_t g = vgraph[V];
     const uint32_t c = comp[V];
     comp[V] = c - descs[g].base;
-    if (c == V) atomicAdd(&ncc[g], 1u);
+    const bool root = c == V;
+    uint64_t pending = __ballot(root);
+    while (pending) {  // wave-uniform: one member per pass
+      const uint32_t l = (uint32_t)__builtin_ctzll(pending);
+      const uint32_t gl = (uint32_t)__shfl((int)g, (int)l);
+      const uint64_t same = __ballot(root && g == gl);
+      if (lane == l) atomicAdd(&ncc[gl], (uint32_t)__builtin_popcountll(same));
+      pending &= ~same;
+    }
   }
 }
 
