@@ -383,6 +383,19 @@ def bench_single(args) -> dict:
     log(f"timed {args.steps} steps in {elapsed:.4f}s ({launches} launches); per launch: grouping "
         f"{grp_ms / launches:.4f} ms, {kname} {agg_ms / launches:.4f} ms")
 
+    # CC query latency on the final store (the plain single-stream path): the first query of a
+    # store allocates its query scratch (reported apart), then args.query_reps timed queries
+    q0 = time.perf_counter()
+    labels, _, _ = store.query_cc()
+    first_ms = (time.perf_counter() - q0) * 1e3
+    lat = []
+    for _ in range(max(1, args.query_reps)):
+        q0 = time.perf_counter()
+        _, ncc, rounds = store.query_cc()
+        lat.append((time.perf_counter() - q0) * 1e3)
+    log(f"CC query: {ncc} components, {rounds} rounds, p50 {np.percentile(lat, 50):.2f} ms, "
+        f"p99 {np.percentile(lat, 99):.2f} ms over {len(lat)} (first query {first_ms:.2f} ms)")
+
     # PCIe-inclusive rate: batches of the same stream handed over from pinned host memory (the
     # DMA runs on the stream's copy stream while the kernels work), into a second store
     pcie = None
@@ -414,19 +427,6 @@ def bench_single(args) -> dict:
                 "input": "pinned-host", "batch": "2^20 uint32 pairs per submission"}
         log(f"PCIe-inclusive {pcie['value']:.3e} edges/s ({pcie['h2d_gb_per_s']:.1f} GB/s of "
             f"pairs) over {timed} pinned host batches")
-
-    # CC query latency on the final store (the plain single-stream path): the first query of a
-    # store allocates its query scratch (reported apart), then args.query_reps timed queries
-    q0 = time.perf_counter()
-    labels, _, _ = store.query_cc()
-    first_ms = (time.perf_counter() - q0) * 1e3
-    lat = []
-    for _ in range(max(1, args.query_reps)):
-        q0 = time.perf_counter()
-        _, ncc, rounds = store.query_cc()
-        lat.append((time.perf_counter() - q0) * 1e3)
-    log(f"CC query: {ncc} components, {rounds} rounds, p50 {np.percentile(lat, 50):.2f} ms, "
-        f"p99 {np.percentile(lat, 99):.2f} ms over {len(lat)} (first query {first_ms:.2f} ms)")
 
     cpu = host_cpu()
     cpu_query = None
@@ -531,7 +531,7 @@ def main() -> None:
                     help="timed steps; a step is one 2^25-edge launch (32 batches of 2^20 pairs)")
     ap.add_argument("--warmup", type=int, default=5,
                     help="untimed steps first (defaults: 5 + 27 steps = the 2^30-update stream)")
-    ap.add_argument("--cpu-batches", type=int, default=64,
+    ap.add_argument("--cpu-batches", type=int, default=512,
                     help="stream prefix (2^20-pair batches) timed on the CPU leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pcie-batches", type=int, default=264,

@@ -464,6 +464,30 @@ static void *cpu_worker(void *arg) {
   return NULL;
 }
 
+/* The directed entries of m undirected pairs (u->v for every pair in order, then v->u) grouped
+ * by source vertex in that order (a counting sort), counts[v] = the entries of v: the update
+ * lists GraphSketchDriver's gutters hand to apply_update_batch (include/graph_sketch_driver.h:
+ * 171-172,204-208) before they are cut into batches. Returns -1 if a vertex id is >= n. */
+int orc_group_by_vertex(const uint32_t *pairs, uint64_t m, uint32_t n, uint64_t *counts,
+                        uint32_t *dst) {
+  memset(counts, 0, sizeof(uint64_t) * n);
+  for (uint64_t i = 0; i < 2 * m; i++) {
+    if (pairs[i] >= n) return -1;
+    counts[pairs[i]]++;
+  }
+  uint64_t *pos = (uint64_t *)malloc(sizeof(uint64_t) * n);
+  if (!pos) return -1;
+  uint64_t s = 0;
+  for (uint32_t v = 0; v < n; v++) {
+    pos[v] = s;
+    s += counts[v];
+  }
+  for (uint64_t i = 0; i < m; i++) dst[pos[pairs[2 * i]]++] = pairs[2 * i + 1];
+  for (uint64_t i = 0; i < m; i++) dst[pos[pairs[2 * i + 1]]++] = pairs[2 * i];
+  free(pos);
+  return 0;
+}
+
 double orc_cpu_baseline(uint64_t *store, uint32_t bpc, uint32_t cols, uint64_t seed,
                         const uint32_t *src, const uint64_t *sizes, const uint64_t *offsets,
                         uint32_t nbatches, const uint32_t *dsts, uint32_t nthreads, int l0) {

@@ -82,6 +82,8 @@ def _load():
     lib.orc_sample.argtypes = [vp, u32, u32, u32, u64, u32, vp]
     lib.orc_sample.restype = i32
     lib.orc_dsu_components.argtypes = [u32, vp, u64, vp]
+    lib.orc_group_by_vertex.argtypes = [vp, u64, u32, vp, vp]
+    lib.orc_group_by_vertex.restype = i32
     lib.orc_boruvka.argtypes = [vp, u32, u32, u32, u32, u64, vp, vp, vp, vp, vp]
     lib.orc_boruvka.restype = i32
     lib.orc_k_forests.argtypes = [vp, u32, u32, u32, u32, u64, u32, vp, vp]
@@ -247,21 +249,21 @@ def cpu_baseline(store, g: Geometry, seed, src, sizes, offsets, dsts, nthreads: 
 def group_by_vertex(pairs: np.ndarray, n: int, batch_size: int):
     """Splits an undirected update list into pre-grouped vertex batches (both directions), each
     at most ``batch_size`` destinations, the shape GraphSketchDriver hands to
-    ``apply_update_batch`` (include/graph_sketch_driver.h:171-172,204-208)."""
+    ``apply_update_batch`` (include/graph_sketch_driver.h:171-172,204-208). The entries of a
+    vertex keep the order u->v of every pair, then v->u (orc_group_by_vertex)."""
     pairs = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
-    src = np.concatenate([pairs[:, 0], pairs[:, 1]])
-    dst = np.concatenate([pairs[:, 1], pairs[:, 0]])
-    order = np.argsort(src, kind="stable")
-    src, dst = src[order], dst[order]
-    counts = np.bincount(src, minlength=n).astype(np.uint64)
-    b_src, b_size, b_off = [], [], []
-    start = 0
-    for v in np.nonzero(counts)[0]:
-        c = int(counts[v])
-        for o in range(0, c, batch_size):
-            b_src.append(v)
-            b_size.append(min(batch_size, c - o))
-            b_off.append(start + o)
-        start += c
-    return (np.array(b_src, dtype=np.uint32), np.array(b_size, dtype=np.uint64),
-            np.array(b_off, dtype=np.uint64), dst.astype(np.uint32))
+    m = len(pairs)
+    counts = np.zeros(n, dtype=np.uint64)
+    dst = np.zeros(2 * m, dtype=np.uint32)
+    if _load().orc_group_by_vertex(_p(pairs), m, n, _p(counts), _p(dst)) != 0:
+        raise ValueError("vertex id out of range")
+    nz = np.nonzero(counts)[0]
+    c = counts[nz].astype(np.int64)
+    k = (c + batch_size - 1) // batch_size            # batches per vertex
+    starts = np.cumsum(c) - c                          # each vertex's first entry
+    first = np.cumsum(k) - k                           # each vertex's first batch
+    j = np.arange(int(k.sum()), dtype=np.int64) - np.repeat(first, k)  # batch index in vertex
+    b_src = np.repeat(nz, k).astype(np.uint32)
+    b_off = (np.repeat(starts, k) + j * batch_size).astype(np.uint64)
+    b_size = np.minimum(batch_size, np.repeat(c, k) - j * batch_size).astype(np.uint64)
+    return b_src, b_size, b_off, dst
