@@ -473,6 +473,17 @@ class StoreGroup:
         pairs = _pairs(pairs)
         _check(lib.gs_group_submit_member_pairs(self.h, member, _p(pairs), pairs.shape[0]))
 
+    def submit_member_pairs_async(self, member: int, pairs) -> None:
+        """gs_group_submit_member_pairs_async: as submit_member_pairs, but a pinned array is read
+        by a DMA that may still run after the call returns; keep it unchanged until
+        wait_copies (or a flush) has returned."""
+        pairs = _pairs(pairs)
+        _check(lib.gs_group_submit_member_pairs_async(self.h, member, _p(pairs), pairs.shape[0]))
+
+    def wait_copies(self) -> None:
+        """gs_stream_wait_copies on the group's stream: every host batch has been read."""
+        _check(lib.gs_stream_wait_copies(self.stream.h))
+
     def fence(self) -> None:
         """gs_group_fence: the next query waits for the updates submitted so far only."""
         _check(lib.gs_group_fence(self.h))

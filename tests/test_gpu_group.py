@@ -256,6 +256,44 @@ def test_member_pairs_rebased_and_checked_on_device():
             st.close()
 
 
+def test_member_pairs_async_dma_after_return():
+    """gs_group_submit_member_pairs_async: many pinned payloads enqueued back to back (their DMAs
+    may still run when the calls return), one pageable payload staged, launches crossing fill
+    buffers (small launch size); after gs_stream_wait_copies the pinned buffers are overwritten
+    and the sketches still equal the oracle's of the pairs as submitted."""
+    rng = np.random.default_rng(29)
+    sizes, seeds = [4000, 2500, 6000], [41, 42, 43]
+    stores = [gs.SketchStore(n, s) for n, s in zip(sizes, seeds)]
+    pinned = []
+    try:
+        with gs.StoreGroup(stores) as grp:
+            grp.stream.set_tuning(launch_edges=1 << 16)
+            sent = [[] for _ in sizes]
+            for k in range(24):
+                i = k % 3
+                p = member_pairs(rng, sizes[i], int(rng.integers(1000, 30000)))
+                buf = gs.PinnedBuffer((len(p), 2))
+                buf.array[:] = p
+                pinned.append(buf)
+                grp.submit_member_pairs_async(i, buf.array)
+                sent[i].append(p)
+            extra = member_pairs(rng, sizes[1], 7000)  # pageable: staged before the call returns
+            grp.submit_member_pairs_async(1, extra)
+            sent[1].append(extra)
+            grp.wait_copies()
+            for buf in pinned:  # the DMAs have read them: the device must not see this
+                buf.array[:] = 0
+            grp.flush()
+            for i, (n, sd) in enumerate(zip(sizes, seeds)):
+                assert np.array_equal(stores[i].export_buckets(),
+                                      oracle_store(n, sd, np.concatenate(sent[i])))
+    finally:
+        for buf in pinned:
+            buf.free()
+        for st in stores:
+            st.close()
+
+
 def test_store_destroy_refused_while_in_group():
     a, b = gs.SketchStore(100, 1), gs.SketchStore(200, 2)
     grp = gs.StoreGroup([a, b])
