@@ -1557,7 +1557,15 @@ uint64_t stream_launch_edges(const Stream* s) {
 // the fill buffers, sized for the stream's launch size (grown on demand, never while pending)
 static int ensure_pending(Stream* s) {
   const uint64_t want = stream_launch_edges(s);
-  if (s->pend_cap >= want && s->d_pend[0]) return ensure_pair_scratch(s, s->ws, s->pend_cap);
+  if (s->pend_cap >= want && s->d_pend[0]) {
+    // both sets follow the stream's current grouping parameters (a retune may need more
+    // bucket histograms than a set was made with)
+    if (s->ws_alt.d_counters) {
+      int rc = ensure_pair_scratch(s, s->ws_alt, s->pend_cap);
+      if (rc) return rc;
+    }
+    return ensure_pair_scratch(s, s->ws, s->pend_cap);
+  }
   GS_CHECK(s->pend_n == 0, GS_ERR_INVALID, "launch size changed with updates pending");
   if (int rc = sync_pipeline(s)) return rc;
   if (s->copy_stream) GS_HIP(hipStreamSynchronize(s->copy_stream));
