@@ -463,7 +463,7 @@ __global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st, uint32_t c
                                                         const uint32_t* __restrict__ comp,
                                                         const uint32_t* __restrict__ vflags,
                                                         const uint32_t* __restrict__ flags,
-                                                        Bucket16* __restrict__ acc,
+                                                        uint64_t* __restrict__ acc, uint64_t plane,
                                                         Bucket16* __restrict__ cpart,
                                                         Bucket16* __restrict__ bpart) {
   if (flags[kFlagDone]) return;
@@ -535,9 +535,9 @@ __global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st, uint32_t c
           __hip_atomic_fetch_xor(&dst->gamma, xg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
       } else if (lj && (xa | xg)) {
-        Bucket16* dst = acc + (size_t)__builtin_amdgcn_readlane((int)r, i) * w + jj;
-        __hip_atomic_fetch_xor(&dst->alpha, xa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_xor(&dst->gamma, xg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint64_t* dst = acc + (size_t)__builtin_amdgcn_readlane((int)r, i) * w + jj;
+        __hip_atomic_fetch_xor(dst, xa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_xor(dst + plane, xg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
   }
@@ -572,7 +572,8 @@ __global__ __launch_bounds__(kQThreads) void big_acc_kernel(const uint32_t* __re
                                                             const Bucket16* __restrict__ cpart,
                                                             const Bucket16* __restrict__ bpart,
                                                             uint32_t nparts, uint32_t w,
-                                                            Bucket16* __restrict__ acc) {
+                                                            uint64_t* __restrict__ acc,
+                                                            uint64_t plane) {
   const uint32_t k = blockIdx.y;
   const uint32_t root = k < (uint32_t)kBig ? flags[kFlagBig + k] : flags[3];
   if (flags[kFlagDone] || root == kNoGiant) return;
@@ -596,7 +597,8 @@ __global__ __launch_bounds__(kQThreads) void big_acc_kernel(const uint32_t* __re
       a ^= sa[q];
       g ^= sg[q];
     }
-    acc[(size_t)root * w + j] = Bucket16{a, g};
+    acc[(size_t)root * w + j] = a;
+    acc[plane + (size_t)root * w + j] = g;
   }
 }
 
@@ -605,7 +607,8 @@ __global__ __launch_bounds__(kQThreads) void big_acc_kernel(const uint32_t* __re
 // time, one bucket per lane.
 __global__ __launch_bounds__(kQThreads) void sample_kernel(StoreView st,
                                                            const uint32_t* __restrict__ comp,
-                                                           Bucket16* __restrict__ acc,
+                                                           uint64_t* __restrict__ acc,
+                                                           uint64_t plane,
                                                            uint32_t* parent, uint32_t* vflags,
                                                            uint32_t* flags, uint32_t* forest,
                                                            uint64_t forest_cap) {
@@ -627,14 +630,19 @@ __global__ __launch_bounds__(kQThreads) void sample_kernel(StoreView st,
     Bucket16 bv[4];
 #pragma unroll
     for (int q = 0; q < 4; q++)
-      bv[q] = lane < w ? acc[(size_t)(v0 + ii[q]) * w + lane] : Bucket16{0, 0};
+      bv[q] = lane < w ? Bucket16{acc[(size_t)(v0 + ii[q]) * w + lane],
+                                  acc[plane + (size_t)(v0 + ii[q]) * w + lane]}
+                       : Bucket16{0, 0};
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       if (q > 0 && ii[q] == ii[0]) break;  // padding of the last group
       const uint32_t root = v0 + ii[q];
       uint64_t idx = 0;
       const int res = wave_sample(bv[q], lane, st.bpc, st.seed, &idx);
-      if (lane < w) acc[(size_t)root * w + lane] = Bucket16{0, 0};
+      if (lane < w) {
+        acc[(size_t)root * w + lane] = 0;
+        acc[plane + (size_t)root * w + lane] = 0;
+      }
       if (lane == ii[q]) {
         my_res = res;
         my_idx = idx;
@@ -647,7 +655,8 @@ __global__ __launch_bounds__(kQThreads) void sample_kernel(StoreView st,
 // sample_kernel for rows of w = cps*B + 1 slots: one root at a time per wave
 __global__ __launch_bounds__(kQThreads) void sample_wide_kernel(StoreView st, uint32_t cps,
                                                                 const uint32_t* __restrict__ comp,
-                                                                Bucket16* __restrict__ acc,
+                                                                uint64_t* __restrict__ acc,
+                                                                uint64_t plane,
                                                                 uint32_t* parent, uint32_t* vflags,
                                                                 uint32_t* flags, uint32_t* forest,
                                                                 uint64_t forest_cap) {
@@ -662,11 +671,11 @@ __global__ __launch_bounds__(kQThreads) void sample_wide_kernel(StoreView st, ui
   while (m) {
     const uint32_t i = (uint32_t)__builtin_ctzll(m);
     m &= m - 1;
-    Bucket16* row = acc + (size_t)(v0 + i) * w;
-    auto load = [&](uint32_t j) -> Bucket16 { return row[j]; };
+    uint64_t* row = acc + (size_t)(v0 + i) * w;
+    auto load = [&](uint32_t j) -> Bucket16 { return Bucket16{row[j], row[plane + j]}; };
     uint64_t idx = 0;
     const int res = wave_sample_wide(load, lane, sw, cps, st.bpc, st.seed, &idx);
-    for (uint32_t j = lane; j < w; j += 64) row[j] = Bucket16{0, 0};
+    for (uint32_t j = lane; j < w; j += 64) row[j] = row[plane + j] = 0;
     if (lane == i) {
       my_res = res;
       my_idx = idx;
@@ -833,20 +842,24 @@ static int boruvka_pass(Store* st, uint32_t col0, uint64_t forest_base, uint32_t
   hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
                      st->d_vflags, st->d_flags, st->num_samples, symmetric);
   GS_HIP(hipGetLastError());
+  // the per-root accumulators as two planes of n x w words (alpha, then gamma): a row of atomics
+  // (or loads) over a root's w slots is w consecutive 8-byte words
+  uint64_t* const acc = reinterpret_cast<uint64_t*>(st->d_acc);
+  const uint64_t plane = (uint64_t)n * w;
   uint32_t batch = std::max(kRoundsAhead, std::min(st->last_rounds + 1, st->num_samples));
   for (uint32_t enqueued = 1;; enqueued += batch, batch = kRoundsAhead) {
     for (uint32_t k = 0; k < batch; k++) {
       hipLaunchKernelGGL(acc_kernel, dim3(g_acc.x, (w + 63) / 64), block,
                          sizeof(Bucket16) * kBig * std::min(64u, w), q, v, cps,
-                         st->d_parent, st->d_comp, st->d_vflags, st->d_flags, st->d_acc,
+                         st->d_parent, st->d_comp, st->d_vflags, st->d_flags, acc, plane,
                          st->d_cpart, st->d_bpart);
       hipLaunchKernelGGL(big_acc_kernel, dim3(w, kBig + 1), dim3(kQThreads), 0, q, st->d_flags,
-                         st->d_cpart, st->d_bpart, nparts, w, st->d_acc);
+                         st->d_cpart, st->d_bpart, nparts, w, acc, plane);
       if (wide)
-        hipLaunchKernelGGL(sample_wide_kernel, g_smp, block, 0, q, v, cps, st->d_comp, st->d_acc,
+        hipLaunchKernelGGL(sample_wide_kernel, g_smp, block, 0, q, v, cps, st->d_comp, acc, plane,
                            st->d_parent, st->d_vflags, st->d_flags, forest, cap);
       else
-        hipLaunchKernelGGL(sample_kernel, g_smp, block, 0, q, v, st->d_comp, st->d_acc,
+        hipLaunchKernelGGL(sample_kernel, g_smp, block, 0, q, v, st->d_comp, acc, plane,
                            st->d_parent, st->d_vflags, st->d_flags, forest, cap);
       hipLaunchKernelGGL(compress_kernel, grid, block, 0, q, n, st->d_parent, st->d_comp,
                          st->d_vflags, st->d_flags, st->num_samples, symmetric);

@@ -238,7 +238,8 @@ struct Store {
   uint32_t* d_parent = nullptr;      // union-find being merged this round
   uint32_t* d_comp = nullptr;        // supernode root of every vertex at round start
   uint32_t* d_vflags = nullptr;      // per-root query state (ZERO / merged / done)
-  Bucket16* d_acc = nullptr;         // per-root XOR of one column + deterministic bucket
+  Bucket16* d_acc = nullptr;         // per-root XOR of one column + deterministic bucket (the
+                                     // query kernels use it as an alpha and a gamma plane)
   Bucket16* d_cpart = nullptr;       // per-wave partial column sums outside the giant supernode
   Bucket16* d_bpart = nullptr;       // per-wave partial column sums of the big supernodes
   // false once the store took updates that need not be symmetric (vertex batches, imports): the
