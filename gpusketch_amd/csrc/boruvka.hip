@@ -1080,7 +1080,8 @@ __global__ __launch_bounds__(kQThreads) void g_acc_kernel(
     const GraphDesc* __restrict__ descs, const uint32_t* __restrict__ vgraph,
     const uint32_t* __restrict__ vmap, uint32_t G, const uint32_t* __restrict__ comp,
     const uint32_t* __restrict__ vflags, const uint32_t* __restrict__ gstate,
-    const uint32_t* __restrict__ giant, Bucket16* __restrict__ acc, uint32_t acc_stride) {
+    const uint32_t* __restrict__ giant, uint64_t* __restrict__ acc, uint64_t plane,
+    uint32_t acc_stride) {
   const uint32_t col = gstate[3 * G];
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t M = sel_count(gstate, G);
@@ -1089,9 +1090,9 @@ __global__ __launch_bounds__(kQThreads) void g_acc_kernel(
   uint64_t sa = 0, sg = 0, ga = 0, gg = 0;
   auto flush_row = [&](uint32_t row, uint32_t w, uint64_t& a, uint64_t& g) {
     if (row != kNoGiant && lane < w && (a | g)) {
-      Bucket16* dst = acc + (size_t)row * acc_stride + lane;
-      __hip_atomic_fetch_xor(&dst->alpha, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_xor(&dst->gamma, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      uint64_t* dst = acc + (size_t)row * acc_stride + lane;
+      __hip_atomic_fetch_xor(dst, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_xor(dst + plane, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     a = g = 0;
   };
@@ -1170,17 +1171,18 @@ __global__ __launch_bounds__(kQThreads) void g_acc_kernel(
 __global__ __launch_bounds__(kQThreads) void g_sample_kernel(
     const GraphDesc* __restrict__ descs, const uint32_t* __restrict__ vgraph,
     const uint32_t* __restrict__ vmap, uint32_t G, const uint32_t* __restrict__ comp,
-    Bucket16* __restrict__ acc, uint32_t acc_stride, uint32_t* parent, uint32_t* vflags,
-    uint32_t* gstate) {
+    uint64_t* __restrict__ acc, uint64_t plane, uint32_t acc_stride, uint32_t* parent,
+    uint32_t* vflags, uint32_t* gstate) {
   FOR_SELECTED(V, vmap, gstate, G) {
   if (comp[V] != V || (vflags[V] & kVDone)) continue;
   const uint32_t g = vgraph[V];
   if (gstate[G + g] != kGActive) continue;
   const GraphDesc d = descs[g];
-  Bucket16* a = acc + (size_t)V * acc_stride;
+  uint64_t* a = acc + (size_t)V * acc_stride;
   uint64_t idx = 0;
-  const int res = sample_column(a[d.bpc], [&](uint32_t k) { return a[k]; }, d.bpc, d.seed, &idx);
-  for (uint32_t j = 0; j <= d.bpc; j++) a[j] = Bucket16{0, 0};
+  auto load = [&](uint32_t k) { return Bucket16{a[k], a[plane + k]}; };
+  const int res = sample_column(load(d.bpc), load, d.bpc, d.seed, &idx);
+  for (uint32_t j = 0; j <= d.bpc; j++) a[j] = a[plane + j] = 0;
   handle_group_sample(res, idx, V, d, parent, vflags, &gstate[g]);
   }
 }
@@ -1286,11 +1288,13 @@ static void launch_group_round(Group* g, hipStream_t q) {
   hipLaunchKernelGGL(g_pick_giant_kernel, dim3((G + 255) / 256), dim3(256), 0, q, g->d_descs, G,
                      g->d_comp, g->d_vflags, g->d_gsym, g->d_gstate, g->d_giant);
   hipLaunchKernelGGL(g_acc_kernel, grid, block, 0, q, g->d_descs, g->d_vgraph, g->d_vmap, G,
-                     g->d_comp, g->d_vflags, g->d_gstate, g->d_giant, g->d_acc,
+                     g->d_comp, g->d_vflags, g->d_gstate, g->d_giant,
+                     reinterpret_cast<uint64_t*>(g->d_acc), (uint64_t)g->n_total * g->acc_stride,
                      (uint32_t)g->acc_stride);
   hipLaunchKernelGGL(g_sample_kernel, grid, block, 0, q, g->d_descs, g->d_vgraph, g->d_vmap, G,
-                     g->d_comp, g->d_acc, (uint32_t)g->acc_stride, g->d_parent, g->d_vflags,
-                     g->d_gstate);
+                     g->d_comp, reinterpret_cast<uint64_t*>(g->d_acc),
+                     (uint64_t)g->n_total * g->acc_stride, (uint32_t)g->acc_stride, g->d_parent,
+                     g->d_vflags, g->d_gstate);
   hipLaunchKernelGGL(g_compress_kernel, grid, block, 0, q, g->d_vmap, G, g->d_gstate, g->d_parent,
                      g->d_comp, g->d_vflags);
   hipLaunchKernelGGL(g_settle_kernel, dim3(1), dim3(1024), 0, q, g->d_descs, G, g->d_gstate);
