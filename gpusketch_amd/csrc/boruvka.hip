@@ -481,10 +481,12 @@ __global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st, uint32_t c
   const uint32_t v0 = gwave * kAccVerts;
   const uint32_t giant = flags[3];
   // the big supernodes' sums of this block: LDS rows of rw slots, one per big supernode
-  extern __shared__ Bucket16 s_big[];
+  // (alpha plane then gamma plane, kBig x rw words each: a row of LDS atomics is consecutive
+  // 8-byte words, no bank conflicts)
+  extern __shared__ uint64_t s_big[];
   __shared__ uint32_t s_bigr[kBig];
   const uint32_t rw = min(64u, w);
-  for (uint32_t i = threadIdx.x; i < kBig * rw; i += blockDim.x) s_big[i] = Bucket16{0, 0};
+  for (uint32_t i = threadIdx.x; i < 2 * kBig * rw; i += blockDim.x) s_big[i] = 0;
   if (threadIdx.x < (uint32_t)kBig) s_bigr[threadIdx.x] = flags[kFlagBig + threadIdx.x];
   __syncthreads();
   const bool split = giant != kNoGiant;
@@ -530,9 +532,9 @@ __global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st, uint32_t c
       const int kbi = __builtin_amdgcn_readlane(kb, i);
       if (kbi >= 0) {
         if (lj && (xa | xg)) {
-          Bucket16* dst = &s_big[(uint32_t)kbi * rw + lane];
-          __hip_atomic_fetch_xor(&dst->alpha, xa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          __hip_atomic_fetch_xor(&dst->gamma, xg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          uint64_t* dst = &s_big[(uint32_t)kbi * rw + lane];
+          __hip_atomic_fetch_xor(dst, xa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_fetch_xor(dst + kBig * rw, xg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
       } else if (lj && (xa | xg)) {
         uint64_t* dst = acc + (size_t)__builtin_amdgcn_readlane((int)r, i) * w + jj;
@@ -561,7 +563,8 @@ __global__ __launch_bounds__(kQThreads) void acc_kernel(StoreView st, uint32_t c
   __syncthreads();  // every wave's LDS sums are in
   for (uint32_t i = threadIdx.x; i < kBig * rw; i += blockDim.x) {
     const uint32_t k = i / rw, sl = blockIdx.y * 64 + (i - k * rw);
-    if (s_bigr[k] != kNoGiant && sl < w) bpart[((size_t)blockIdx.x * kBig + k) * w + sl] = s_big[i];
+    if (s_bigr[k] != kNoGiant && sl < w)
+      bpart[((size_t)blockIdx.x * kBig + k) * w + sl] = Bucket16{s_big[i], s_big[kBig * rw + i]};
   }
 }
 
