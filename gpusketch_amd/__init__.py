@@ -102,6 +102,7 @@ _sigs = {
     "gs_group_submit_pairs_device": ([_vp, _vp, _u64], _i32),
     "gs_group_submit_member_pairs": ([_vp, _u32, _vp, _u64], _i32),
     "gs_group_submit_member_pairs_async": ([_vp, _u32, _vp, _u64], _i32),
+    "gs_group_submit_member_batches_async": ([_vp, _u32, _vp, _vp, _vp], _i32),
     "gs_stream_wait_copies": ([_vp], _i32),
     "gs_group_query_cc": ([_vp, _vp, _vp, _vp, _vp, _vp], _i32),
     "gs_group_fence": ([_vp], _i32),
@@ -479,6 +480,19 @@ class StoreGroup:
         wait_copies (or a flush) has returned."""
         pairs = _pairs(pairs)
         _check(lib.gs_group_submit_member_pairs_async(self.h, member, _p(pairs), pairs.shape[0]))
+
+    def submit_member_batches_async(self, members, batches) -> None:
+        """gs_group_submit_member_batches_async: batch i of `batches` (an (m, 2) uint32 array in
+        member members[i]'s ids) as submit_member_pairs_async, all in one call. Keep pinned
+        arrays unchanged until wait_copies (or a flush) has returned."""
+        batches = [_pairs(b) for b in batches]
+        mem = np.ascontiguousarray(members, dtype=np.uint32)
+        if mem.shape != (len(batches),):
+            raise ValueError("one member per batch")
+        ptrs = np.array([b.ctypes.data for b in batches], dtype=np.uint64)
+        counts = np.array([b.shape[0] for b in batches], dtype=np.uint64)
+        _check(lib.gs_group_submit_member_batches_async(self.h, len(batches), _p(mem), _p(ptrs),
+                                                        _p(counts)))
 
     def wait_copies(self) -> None:
         """gs_stream_wait_copies on the group's stream: every host batch has been read."""

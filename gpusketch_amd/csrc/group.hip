@@ -220,6 +220,24 @@ int gs_group_submit_member_pairs_async(gs_group* h, uint32_t member, const uint3
   return submit_pairs_host(g.s, pairs, nedges, 0, 0, mid, false);
 }
 
+int gs_group_submit_member_batches_async(gs_group* h, uint32_t count, const uint32_t* members,
+                                         const uint32_t* const* pairs, const uint64_t* nedges) {
+  GS_CHECK(h, GS_ERR_INVALID, "null group");
+  Group& g = h->impl;
+  GS_CHECK(count == 0 || (members && pairs && nedges), GS_ERR_INVALID, "null batch arrays");
+  // every argument is checked before anything is enqueued
+  std::vector<MemberIds> mids(count);
+  for (uint32_t i = 0; i < count; i++) {
+    GS_CHECK(members[i] < g.members.size(), GS_ERR_INVALID, "member %u of a group of %zu",
+             members[i], g.members.size());
+    GS_CHECK(pairs[i] || nedges[i] == 0, GS_ERR_INVALID, "null pairs (batch %u)", i);
+    GS_CHECK(nedges[i] < (1ULL << 40), GS_ERR_INVALID, "batch %u too large", i);
+    mids[i].base = g.bases[members[i]];
+    mids[i].n = g.members[members[i]]->view.n;
+  }
+  return submit_member_batches_host(g.s, count, mids.data(), pairs, nedges);
+}
+
 int gs_stream_wait_copies(gs_stream* sh) {
   GS_CHECK(sh, GS_ERR_INVALID, "null stream");
   return wait_copies(&sh->impl);

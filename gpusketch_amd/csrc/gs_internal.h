@@ -337,6 +337,11 @@ struct MemberIds {
 int submit_pairs_host(Stream* s, const uint32_t* pairs, uint64_t nedges, uint32_t min_depth,
                       uint32_t depth_cap, const MemberIds& mid = MemberIds{},
                       bool wait_read = true);
+// submit_pairs_host(s, pairs[i], nedges[i], 0, 0, mids[i], false) for each i in order, with one
+// rebase launch per up to kAppendSegs batches
+constexpr uint32_t kAppendSegs = 64;
+int submit_member_batches_host(Stream* s, uint32_t count, const MemberIds* mids,
+                               const uint32_t* const* pairs, const uint64_t* nedges);
 // every host batch submitted so far has been read by its DMA
 int wait_copies(Stream* s);
 int drain_stream(Stream* s);

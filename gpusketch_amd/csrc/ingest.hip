@@ -143,6 +143,27 @@ __global__ void append_kernel(const uint2* src, uint2* dst, uint64_t nedges, uin
   }
 }
 
+// Member segments copied into a fill buffer by one batched host submission: each segment's
+// pairs (in the member's ids) get the member's base added in place, a pair with an id at or past
+// the member's vertex count becomes (kBadVertex, kBadVertex). One workgroup row per segment.
+struct AppendSegs {
+  uint64_t off[kAppendSegs];  // first pair of the segment in the fill buffer
+  uint32_t m[kAppendSegs];
+  uint32_t base[kAppendSegs];
+  uint32_t n[kAppendSegs];
+};
+
+__global__ void append_segs_kernel(uint2* __restrict__ buf, AppendSegs sg) {
+  const uint32_t k = blockIdx.y;
+  uint2* p = buf + sg.off[k];
+  const uint32_t m = sg.m[k], base = sg.base[k], n = sg.n[k];
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
+    const uint2 uv = p[e];
+    p[e] = uv.x < n && uv.y < n ? make_uint2(uv.x + base, uv.y + base)
+                                : make_uint2(kBadVertex, kBadVertex);
+  }
+}
+
 // The K pairs of this thread in a K1/K3 workgroup (edges e0 + t + k * kPartThreads below e1), all
 // loads issued before any is used, then (groups) the members of their endpoints, also together.
 // ok[k]: the slot holds a pair of the launch; a pair the grouping drops (an endpoint outside the
@@ -1825,6 +1846,70 @@ int submit_pairs_host(Stream* s, const uint32_t* pairs, uint64_t nedges, uint32_
     GS_HIP(q);
   }
   return GS_OK;
+}
+
+// Several members' host batches in one call, without waiting for their DMAs: each chunk is
+// copied into the fill buffer as in submit_pairs_host, but the rebase runs as one append launch
+// per up to kAppendSegs chunks (and before each launch of the pipeline) instead of one per batch.
+int submit_member_batches_host(Stream* s, uint32_t count, const MemberIds* mids,
+                               const uint32_t* const* pairs, const uint64_t* nedges) {
+  std::lock_guard<std::mutex> lk(s->mu);
+  int rc = ensure_pending(s);
+  if (rc) return rc;
+  AppendSegs sg;
+  uint32_t nseg = 0;
+  uint64_t seg_max = 0;
+  auto append = [&]() -> int {
+    if (nseg == 0) return GS_OK;
+    const unsigned gx = (unsigned)std::min<uint64_t>((seg_max + 255) / 256, 1024);
+    hipLaunchKernelGGL(append_segs_kernel, dim3(gx, nseg), dim3(256), 0, s->copy_stream,
+                       reinterpret_cast<uint2*>(s->d_pend[s->pend_buf]), sg);
+    GS_HIP(hipGetLastError());
+    GS_HIP(hipEventRecord(s->copied, s->copy_stream));
+    s->copies_pending = true;
+    nseg = 0;
+    seg_max = 0;
+    return GS_OK;
+  };
+  if ((rc = wait_fill_buffer(s))) return rc;
+  for (uint32_t i = 0; i < count; i++) {
+    if (nedges[i] == 0) continue;
+    hipPointerAttribute_t attr{};
+    bool pinned = false;
+    if (hipPointerGetAttributes(&attr, pairs[i]) == hipSuccess)
+      pinned = attr.type == hipMemoryTypeHost;
+    else
+      (void)hipGetLastError();
+    for (uint64_t e0 = 0; e0 < nedges[i];) {
+      const uint64_t m = std::min(nedges[i] - e0, s->pend_cap - s->pend_n);
+      const size_t bytes = m * 2 * sizeof(uint32_t);
+      const uint32_t* src = pairs[i] + 2 * e0;
+      if (!pinned) {
+        if ((rc = ensure_stage(s, bytes, false))) return rc;
+        std::memcpy(s->h_stage, src, bytes);
+        src = s->h_stage;
+      }
+      GS_HIP(hipMemcpyAsync(s->d_pend[s->pend_buf] + 2 * s->fill_n, src, bytes,
+                            hipMemcpyHostToDevice, s->copy_stream));
+      if (!pinned) {
+        GS_HIP(hipEventRecord(s->stage_free, s->copy_stream));
+        s->stage_pending = true;
+      }
+      sg.off[nseg] = s->fill_n;
+      sg.m[nseg] = (uint32_t)m;
+      sg.base[nseg] = mids[i].base;
+      sg.n[nseg] = mids[i].n;
+      seg_max = std::max(seg_max, m);
+      s->fill_n += m;
+      s->pend_n += m;
+      e0 += m;
+      if (++nseg == kAppendSegs && (rc = append())) return rc;
+      if (s->pend_n == s->pend_cap) {
+        if ((rc = append()) || (rc = process_pending(s)) || (rc = wait_fill_buffer(s))) return rc;
+      }
+    }
+  }
+  return append();
 }
 
 int wait_copies(Stream* s) {

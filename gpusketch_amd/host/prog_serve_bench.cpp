@@ -247,14 +247,6 @@ static bool same_answers(const RunStats& a, const RunStats& b, std::string* why)
   return true;
 }
 
-static uint64_t fnv1a(const uint32_t* p, size_t n) {  // the server's hash of a label array
-  uint64_t h = 1469598103934665603ull;
-  for (size_t i = 0; i < n; i++) {
-    h ^= p[i];
-    h *= 1099511628211ull;
-  }
-  return h;
-}
 
 static uint32_t uf_find(std::vector<uint32_t>& p, uint32_t x) {
   while (p[x] != x) x = p[x] = p[p[x]];
@@ -312,7 +304,7 @@ static Verified verify_answers(const Trace& tr, const std::vector<const RunStats
       }
       cut = std::min<uint64_t>(value, k);
     }
-    const uint64_t hash = fnv1a(lab.data(), n);
+    const uint64_t hash = label_hash(lab.data(), n);
     for (const RunStats* rs : runs) {
       auto it = rs->by_index.find(q);
       if (it == rs->by_index.end()) continue;  // not served in this run (tenant not admitted)

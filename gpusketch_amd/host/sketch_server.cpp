@@ -16,16 +16,6 @@
 #include "gpusketch/device_sketch_store.h"
 #include "gpusketch/sketch.h"
 
-namespace {
-uint64_t fnv1a(const uint32_t* p, size_t n) {
-  uint64_t h = 1469598103934665603ull;
-  for (size_t i = 0; i < n; i++) {
-    h ^= p[i];
-    h *= 1099511628211ull;
-  }
-  return h;
-}
-}  // namespace
 
 SketchServer::SketchServer(ServerConfig c) : cfg(c), t0(std::chrono::steady_clock::now()) {
   worker = std::thread([this] { loop(); });
@@ -484,14 +474,23 @@ void SketchServer::apply_updates(std::vector<Request*>& upd, bool may_wait) {
     // each payload by DMA as it arrived (member ids; rebase and range check on the device),
     // enqueued back to back without waiting for the copies: a pool buffer stays out of the pool
     // until they have drained
-    for (Request* r : upd) {
-      gs_throw_if(gs_group_submit_member_pairs_async(group, (uint32_t)slot[r->tenant], r->pairs,
-                                                     r->nedges));
-      if (r->owned) {
-        std::lock_guard<std::mutex> lk(mu);
-        payload_inflight.emplace_back(r->owned_words, r->owned);
-        r->owned = nullptr;
-      }
+    std::vector<uint32_t> members(upd.size());
+    std::vector<const uint32_t*> payloads(upd.size());
+    std::vector<uint64_t> counts(upd.size());
+    for (size_t i = 0; i < upd.size(); i++) {
+      members[i] = (uint32_t)slot[upd[i]->tenant];
+      payloads[i] = upd[i]->pairs;
+      counts[i] = upd[i]->nedges;
+    }
+    gs_throw_if(gs_group_submit_member_batches_async(group, (uint32_t)upd.size(), members.data(),
+                                                     payloads.data(), counts.data()));
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      for (Request* r : upd)
+        if (r->owned) {
+          payload_inflight.emplace_back(r->owned_words, r->owned);
+          r->owned = nullptr;
+        }
     }
     // the launches are not waited for while more requests are queued (a query waits for its
     // tenant's store); the last step before the queue runs dry waits, and so reports pairs the
@@ -579,7 +578,7 @@ void SketchServer::run_step(std::vector<Request>& step) {
         RequestResult res;
         res.status = status[i];
         res.num_cc = ncc[i];
-        res.partition_hash = fnv1a(labels.data() + base, n);
+        res.partition_hash = label_hash(labels.data() + base, n);
         if (cfg.keep_labels) res.labels.assign(labels.begin() + base, labels.begin() + base + n);
         finish(*r, std::move(res));
       }
@@ -593,7 +592,7 @@ void SketchServer::run_step(std::vector<Request>& step) {
         res.status = gs_query_cc(t.store, labels.data(), &ncc, &rounds, nullptr, 0, nullptr);
         if (res.status != GS_OK && res.status != GS_ERR_OUT_OF_SAMPLES) gs_throw_if(res.status);
         res.num_cc = ncc;
-        res.partition_hash = fnv1a(labels.data(), n);
+        res.partition_hash = label_hash(labels.data(), n);
         if (cfg.keep_labels) res.labels = std::move(labels);
         finish(*r, std::move(res));
       }

@@ -272,6 +272,13 @@ int gs_group_submit_member_pairs(gs_group* g, uint32_t member, const uint32_t* p
  * to back this way. No counterpart in the reference. */
 int gs_group_submit_member_pairs_async(gs_group* g, uint32_t member, const uint32_t* pairs,
                                        uint64_t nedges);
+/* gs_group_submit_member_pairs_async for each i < count in order (batch i: members[i],
+ * pairs[i], nedges[i]), with the per-batch work on the host cut to the DMA: the device rebase and
+ * range check run as one launch per up to 64 batches. Every argument is checked before anything
+ * is enqueued (GS_ERR_INVALID and nothing applied). A serving loop hands a step's payloads over
+ * in one call. No counterpart in the reference. */
+int gs_group_submit_member_batches_async(gs_group* g, uint32_t count, const uint32_t* members,
+                                         const uint32_t* const* pairs, const uint64_t* nedges);
 /* Waits until every host batch submitted on the stream so far has been read by its DMA (the
  * buffers of gs_group_submit_member_pairs_async may then be reused). No counterpart in the
  * reference. */

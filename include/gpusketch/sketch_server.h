@@ -61,11 +61,27 @@ struct RequestResult {
   double done_s = 0;
   int status = 0;            // gs_status of the answer (e.g. out of samples)
   uint32_t num_cc = 0;       // CC_QUERY
-  uint64_t partition_hash = 0;
+  uint64_t partition_hash = 0;  // CC_QUERY: label_hash of the labels
   uint64_t step = 0;         // the scheduling step that was running when it was answered
   uint64_t mincut = 0;       // MINCUT_QUERY: the cut if below k, else k
   std::vector<node_id_t> labels;  // CC_QUERY with keep_labels
 };
+
+// The hash of a label array in RequestResult::partition_hash: FNV-1a over the 32-bit labels in
+// four interleaved lanes (label i into lane i mod 4, so the four multiply chains overlap), the
+// lanes then folded in order by FNV-1a.
+inline uint64_t label_hash(const uint32_t* p, size_t n) {
+  const uint64_t prime = 1099511628211ull;
+  uint64_t h[4] = {1469598103934665603ull, 1469598103934665603ull ^ 1,
+                   1469598103934665603ull ^ 2, 1469598103934665603ull ^ 3};
+  size_t i = 0;
+  for (; i + 4 <= n; i += 4)
+    for (int l = 0; l < 4; l++) h[l] = (h[l] ^ p[i + l]) * prime;
+  for (int l = 0; i < n; i++, l++) h[l] = (h[l] ^ p[i]) * prime;
+  uint64_t out = 1469598103934665603ull ^ n;
+  for (int l = 0; l < 4; l++) out = (out ^ h[l]) * prime;
+  return out;
+}
 
 class SketchServer {
  public:

@@ -294,6 +294,60 @@ def test_member_pairs_async_dma_after_return():
             st.close()
 
 
+def test_member_batches_in_one_call():
+    """gs_group_submit_member_batches_async: 150 payloads of three members in one call (three
+    rebase launches of up to 64 batches, launches crossing fill buffers at a small launch size),
+    pinned and pageable, one holding ids past its member's end (dropped and reported by the next
+    flush). The sketches equal the oracle's of the valid pairs, and a call naming a member past
+    the group's end applies nothing."""
+    rng = np.random.default_rng(31)
+    sizes, seeds = [3000, 700, 5000], [51, 52, 53]
+    stores = [gs.SketchStore(n, s) for n, s in zip(sizes, seeds)]
+    pinned = []
+    try:
+        with gs.StoreGroup(stores) as grp:
+            grp.stream.set_tuning(launch_edges=1 << 16)
+            sent = [[] for _ in sizes]
+            members, batches = [], []
+            for k in range(150):
+                i = int(rng.integers(0, 3))
+                p = member_pairs(rng, sizes[i], int(rng.integers(1, 4000)))
+                if k % 5 == 0:  # pageable: staged
+                    batches.append(p)
+                else:
+                    buf = gs.PinnedBuffer((len(p), 2))
+                    buf.array[:] = p
+                    pinned.append(buf)
+                    batches.append(buf.array)
+                members.append(i)
+                sent[i].append(p)
+            bad = np.array([[0, sizes[1]], [sizes[1] + 9, 2]], dtype=np.uint32)
+            members.insert(70, 1)
+            batches.insert(70, bad)
+            members.append(0)
+            batches.append(np.zeros((0, 2), dtype=np.uint32))  # empty batch
+            grp.submit_member_batches_async(members, batches)
+            grp.wait_copies()
+            for buf in pinned:  # read by the DMAs already
+                buf.array[:] = 0
+            with pytest.raises(gs.GsError, match="2 submitted updates"):
+                grp.flush()
+            for i, (n, sd) in enumerate(zip(sizes, seeds)):
+                assert np.array_equal(stores[i].export_buckets(),
+                                      oracle_store(n, sd, np.concatenate(sent[i])))
+            before = [st.export_buckets() for st in stores]
+            with pytest.raises(gs.GsError, match="member 3"):
+                grp.submit_member_batches_async([0, 3], [sent[0][0], sent[0][0]])
+            grp.flush()
+            for st, b in zip(stores, before):
+                assert np.array_equal(st.export_buckets(), b)
+    finally:
+        for buf in pinned:
+            buf.free()
+        for st in stores:
+            st.close()
+
+
 def test_store_destroy_refused_while_in_group():
     a, b = gs.SketchStore(100, 1), gs.SketchStore(200, 2)
     grp = gs.StoreGroup([a, b])
