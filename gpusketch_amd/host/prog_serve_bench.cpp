@@ -135,6 +135,7 @@ static double pct(std::vector<double> v, double p) {
 }
 
 static uint64_t g_max_step_updates = 0;  // 0: the server's default
+static uint32_t g_launch_edges = 0;  // 0: the server's default
 static std::string g_dump;                 // --dump prefix ("" = none)
 
 static void dump_csv(const RunStats& s, const char* mode) {
@@ -164,6 +165,7 @@ static RunStats run(const Trace& tr, bool batching, uint64_t budget, uint32_t k,
   cfg.batching = batching;
   cfg.memory_budget_bytes = budget;
   if (g_max_step_updates) cfg.max_step_updates = g_max_step_updates;
+  if (g_launch_edges) cfg.launch_edges = g_launch_edges;
   cfg.evict_idle = g_evict;
   cfg.pinned_spill = g_pinned_spill;
   SketchServer srv(cfg);
@@ -366,6 +368,7 @@ int main(int argc, char** argv) {
     else if (a == "--max-edges") max_edges = std::stoull(v);
     else if (a == "--alone-check") alone = std::atoi(v);
     else if (a == "--max-step-updates") g_max_step_updates = std::stoull(v);
+    else if (a == "--launch-edges") g_launch_edges = (uint32_t)std::stoull(v);
     else if (a == "--evict") g_evict = std::atoi(v) != 0;
     else if (a == "--pinned-spill") g_pinned_spill = std::atoi(v) != 0;
     else if (a == "--verify") verify = std::atoi(v);

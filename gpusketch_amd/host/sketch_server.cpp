@@ -432,11 +432,13 @@ void SketchServer::ensure_group() {
     gs_throw_if(gs_group_create(stores.data(), (uint32_t)stores.size(), &group));
     // a step's host batches are copied while the kernels of the launches before them run; the
     // default launch grows with the group's vertex count (to 2^27 edges), which would leave a
-    // step's copies and kernels one after the other, so the loop keeps launches at 2^25
+    // step's copies and kernels one after the other, so the loop keeps launches at
+    // cfg.launch_edges (2^23: a 2^25-update step in four launches, the first three of them
+    // aggregated while the next one's payloads are copied)
     gs_stream* gsm = nullptr;
     gs_throw_if(gs_group_stream(group, &gsm));
     gs_tuning tun{};
-    tun.launch_edges = 1u << 25;
+    tun.launch_edges = cfg.launch_edges;
     gs_throw_if(gs_stream_set_tuning(gsm, &tun));
     // buffers and query scratch now, not in the first step that updates or queries
     gs_throw_if(gs_group_prepare(group));

@@ -33,6 +33,7 @@
 struct ServerConfig {
   uint64_t memory_budget_bytes = 200ull * 1000 * 1000 * 1000;  // sketch memory of admitted tenants
   uint64_t max_step_updates = 1ull << 25;  // updates per device submission when batching
+  uint32_t launch_edges = 1u << 23;  // edges per launch of the group's ingest pipeline
   bool batching = true;
   // a tenant with pending requests that does not fit the budget makes room by spilling idle
   // resident tenants (least recently served first) to host memory; a spilled tenant is restored
