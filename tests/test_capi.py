@@ -63,6 +63,10 @@ def test_invalid_arguments_report_errors():
         gs.geometry_for(1)
     with pytest.raises(gs.GsError):
         gs.gen_rmat(10, 10, 1, a=0.9, b=0.2, c=0.1)
+    # group entry points check their handles before any device call
+    assert gs.lib.gs_group_submit_member_batches_async(None, 0, None, None, None) == gs.GS_ERR_INVALID
+    assert gs.lib.gs_group_submit_member_pairs_async(None, 0, None, 0) == gs.GS_ERR_INVALID
+    assert gs.lib.gs_stream_wait_copies(None) == gs.GS_ERR_INVALID
 
 
 def test_min_cut_matches_stoer_wagner():
