@@ -14,14 +14,18 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PROG = os.path.join(ROOT, "gpusketch_amd", "bin", "serve_bench")
 
 
-# pinned 1: payloads handed over in pinned memory (DMA straight from them); 0: copied at arrival
-@pytest.mark.parametrize("rate,pinned", [(0, 1), (3000, 0)])
-def test_served_equals_plain_and_alone(rate, pinned):
+# pinned 1: payloads handed over in pinned memory (DMA straight from them); 0: copied at arrival.
+# launch: the group's launch size (0: the server's default); 4096 edges splits payloads across
+# fill buffers and a step's submission across many launches.
+@pytest.mark.parametrize("rate,pinned,launch", [(0, 1, 0), (3000, 0, 0), (0, 1, 4096)])
+def test_served_equals_plain_and_alone(rate, pinned, launch):
     if not os.path.exists(PROG):
         pytest.fail("serve_bench is not built; run `python gpusketch_amd/build.py`")
+    extra = ["--launch-edges", str(launch)] if launch else []
     r = subprocess.run([PROG, "--tenants", "10", "--requests", "300", "--rate", str(rate),
-                        "--max-n", "30000", "--max-edges", "20000", "--seed", str(11 + rate),
-                        "--alone-check", "1", "--verify", "1", "--pinned-payloads", str(pinned)],
+                        "--max-n", "30000", "--max-edges", "20000", "--seed", str(11 + rate + launch),
+                        "--alone-check", "1", "--verify", "1", "--pinned-payloads", str(pinned),
+                        *extra],
                        capture_output=True, text=True, timeout=600)
     print(r.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
